@@ -1,0 +1,487 @@
+"""ctypes binding of libbrainmatch.so (the C ABI in include/brainmatch.h).
+
+Host-side mirror of the reference tools' operations for Python callers
+(tests, bench): load graphs, set a matching, score it, evaluate swap and
+3-vertex permutation deltas, accept disjoint swaps, run the search drivers.
+numpy arrays in, numpy arrays out; every non-zero status raises BMError with
+the library's message.  There is no fallback: if the shared library is
+missing, importing this module raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libbrainmatch.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C brain-match_amd")
+
+_lib = C.CDLL(LIB_PATH)
+
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+u8p = C.POINTER(C.c_uint8)
+
+KERNEL_SWAP, KERNEL_ROT3, KERNEL_SCORE = 0, 1, 2
+WEIGHTS_GEOMETRIC, WEIGHTS_LOGNORMAL = 1, 2
+SEARCH_SWAP_WEIGHTED, SEARCH_SWAP_UNIFORM, SEARCH_ROT3 = 0, 1, 2
+
+
+class BMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"brainmatch error {code}: {msg}")
+        self.code = code
+
+
+class _Edges(C.Structure):
+    _fields_ = [("m", C.c_int64), ("src", i32p), ("dst", i32p), ("w", i32p)]
+
+
+class _EdgeList(C.Structure):
+    _fields_ = [("m", C.c_int64), ("n", C.c_int32), ("src", i32p), ("dst", i32p), ("w", i32p)]
+
+
+class _SynthParams(C.Structure):
+    _fields_ = [("n", C.c_int32), ("mean_out_degree", C.c_double), ("alpha", C.c_double),
+                ("weights", C.c_int32), ("modify_frac", C.c_double), ("shuffle_frac", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("count", C.c_int64), ("sum", C.c_int64), ("max", C.c_int64), ("argmax", C.c_int64),
+                ("n_pos", C.c_int64), ("n_zero", C.c_int64)]
+
+
+class _Tuning(C.Structure):
+    _fields_ = [("waves_per_block", C.c_int32), ("lds_slots", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("group_lanes", C.c_int32)]
+
+
+class _SearchOpts(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("seed", C.c_uint64), ("time_limit_s", C.c_double),
+                ("max_iters", C.c_int64), ("out_path", C.c_char_p), ("sync_path", C.c_char_p),
+                ("sync_interval_s", C.c_double), ("verify_every", C.c_int64), ("verbose", C.c_int32),
+                ("stop_flag", C.POINTER(C.c_int)), ("restart_gap", C.c_double)]
+
+
+class _SearchResult(C.Structure):
+    _fields_ = [("initial_score", C.c_int64), ("best_score", C.c_int64), ("final_score", C.c_int64),
+                ("iterations", C.c_int64), ("accepted", C.c_int64), ("evaluations", C.c_int64),
+                ("verify_failures", C.c_int64), ("seconds", C.c_double)]
+
+
+def _sig(name, res, args):
+    f = getattr(_lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_c = C.c_void_p
+_sig("bm_api_version", C.c_int, [])
+_sig("bm_last_error", C.c_char_p, [_c])
+_sig("bm_load_graph_csv", C.c_int, [C.c_char_p, C.POINTER(_EdgeList)])
+_sig("bm_write_graph_csv", C.c_int, [C.c_char_p, C.POINTER(_EdgeList)])
+_sig("bm_edge_list_free", None, [C.POINTER(_EdgeList)])
+_sig("bm_load_mapping_csv", C.c_int, [C.c_char_p, C.c_int32, i32p, i64p])
+_sig("bm_write_mapping_csv", C.c_int, [C.c_char_p, C.c_int32, i32p])
+_sig("bm_synth_pair", C.c_int, [C.POINTER(_SynthParams), C.POINTER(_EdgeList), C.POINTER(_EdgeList), i32p, i32p])
+_sig("bm_greedy_match", C.c_int, [C.c_int32, C.POINTER(_Edges), C.POINTER(_Edges), i32p])
+_sig("bm_create", C.c_int, [C.POINTER(_c), C.c_int32, C.POINTER(_Edges), C.POINTER(_Edges), C.c_int32])
+_sig("bm_destroy", None, [_c])
+_sig("bm_num_vertices", C.c_int32, [_c])
+_sig("bm_num_edges", C.c_int64, [_c, C.c_int])
+_sig("bm_degrees", C.c_int, [_c, C.c_int, i32p, i32p])
+_sig("bm_set_mapping", C.c_int, [_c, i32p])
+_sig("bm_get_mapping", C.c_int, [_c, i32p])
+_sig("bm_score", C.c_int, [_c, i64p])
+_sig("bm_swap_deltas", C.c_int, [_c, i32p, C.c_int64, i64p])
+_sig("bm_swap_deltas_changed", C.c_int, [_c, i32p, C.c_int32, i64p, i32p, i64p])
+_sig("bm_rot3_deltas", C.c_int, [_c, i32p, C.c_int64, i64p, u8p])
+_sig("bm_rot3_deltas_changed", C.c_int, [_c, i32p, C.c_int32, i64p, i32p, i64p])
+_sig("bm_accept_disjoint", C.c_int, [_c, i32p, i64p, C.c_int64, i32p, i64p, i64p])
+_sig("bm_apply_swap", C.c_int, [_c, C.c_int32, C.c_int32])
+_sig("bm_apply_perm3", C.c_int, [_c, C.c_int32, C.c_int32, C.c_int32, C.c_int32])
+_sig("bm_batch_create", C.c_int, [_c, C.c_int64, C.POINTER(_c)])
+_sig("bm_batch_destroy", None, [_c])
+_sig("bm_batch_size", C.c_int64, [_c])
+_sig("bm_batch_set_pairs", C.c_int, [_c, i32p, C.c_int64])
+_sig("bm_batch_set_triples", C.c_int, [_c, i32p, C.c_int64])
+_sig("bm_batch_gen_pairs", C.c_int, [_c, C.c_uint64, C.c_int64])
+_sig("bm_batch_gen_triples", C.c_int, [_c, C.c_uint64, C.c_int64])
+_sig("bm_batch_gen_changed_pairs", C.c_int, [_c, i32p, C.c_int32])
+_sig("bm_batch_gen_changed_triples", C.c_int, [_c, i32p, C.c_int32])
+_sig("bm_batch_eval_swaps", C.c_int, [_c])
+_sig("bm_batch_eval_swaps_uniform", C.c_int, [_c, C.c_uint64, C.c_int64])
+_sig("bm_batch_eval_rot3", C.c_int, [_c])
+_sig("bm_batch_get", C.c_int, [_c, C.c_int64, C.c_int64, i64p, u8p, i32p])
+_sig("bm_batch_stats", C.c_int, [_c, C.POINTER(_Stats)])
+_sig("bm_batch_pick", C.c_int, [_c, C.c_int32, C.c_int64, i64p])
+_sig("bm_batch_accept_disjoint", C.c_int, [_c, C.c_int64, i64p, i64p])
+_sig("bm_timing_enable", C.c_int, [_c, C.c_int])
+_sig("bm_timing_get", C.c_int, [_c, C.c_int32, i64p, C.POINTER(C.c_double)])
+_sig("bm_timing_reset", C.c_int, [_c])
+_sig("bm_sync", C.c_int, [_c])
+_sig("bm_batch_algo_bytes", C.c_int, [_c, C.c_int32, i64p, i64p])
+_sig("bm_uniform_algo_bytes", C.c_int, [_c, C.c_uint64, C.c_int64, i64p, i64p])
+_sig("bm_get_tuning", C.c_int, [_c, C.c_int32, C.POINTER(_Tuning)])
+_sig("bm_set_tuning", C.c_int, [_c, C.c_int32, C.POINTER(_Tuning)])
+_sig("bm_device_info", C.c_int, [_c, C.c_char_p, C.c_size_t])
+_sig("bm_search", C.c_int, [_c, C.POINTER(_SearchOpts), i32p, C.POINTER(_SearchResult)])
+_sig("bm_hillclimb", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int64, C.c_int32, i64p, C.POINTER(_SearchResult)])
+
+
+def _p32(a):
+    return a.ctypes.data_as(i32p) if a is not None else None
+
+
+def _p64(a):
+    return a.ctypes.data_as(i64p) if a is not None else None
+
+
+def _pu8(a):
+    return a.ctypes.data_as(u8p) if a is not None else None
+
+
+def _check(rc, ctx=None):
+    if rc != 0:
+        msg = _lib.bm_last_error(ctx)
+        raise BMError(rc, msg.decode() if msg else "")
+
+
+def api_version():
+    return _lib.bm_api_version()
+
+
+@dataclass
+class Graph:
+    """Edge list of one directed weighted graph (ids 1..n)."""
+    src: np.ndarray
+    dst: np.ndarray
+    w: np.ndarray
+    n: int
+
+    @property
+    def m(self):
+        return int(self.src.shape[0])
+
+    def _edges(self):
+        self.src = np.ascontiguousarray(self.src, dtype=np.int32)
+        self.dst = np.ascontiguousarray(self.dst, dtype=np.int32)
+        self.w = np.ascontiguousarray(self.w, dtype=np.int32)
+        return _Edges(self.m, _p32(self.src), _p32(self.dst), _p32(self.w))
+
+
+def _from_edge_list(el: _EdgeList) -> Graph:
+    m = el.m
+    src = np.ctypeslib.as_array(el.src, shape=(m,)).copy() if m else np.zeros(0, np.int32)
+    dst = np.ctypeslib.as_array(el.dst, shape=(m,)).copy() if m else np.zeros(0, np.int32)
+    w = np.ctypeslib.as_array(el.w, shape=(m,)).copy() if m else np.zeros(0, np.int32)
+    g = Graph(src, dst, w, el.n)
+    _lib.bm_edge_list_free(C.byref(el))
+    return g
+
+
+def load_graph_csv(path) -> Graph:
+    el = _EdgeList()
+    _check(_lib.bm_load_graph_csv(os.fsencode(path), C.byref(el)))
+    return _from_edge_list(el)
+
+
+def write_graph_csv(path, g: Graph):
+    e = g._edges()
+    el = _EdgeList(e.m, g.n, e.src, e.dst, e.w)
+    _check(_lib.bm_write_graph_csv(os.fsencode(path), C.byref(el)))
+
+
+def load_mapping_csv(path, n) -> np.ndarray:
+    pi = np.zeros(n + 1, np.int32)
+    lines = C.c_int64(0)
+    _check(_lib.bm_load_mapping_csv(os.fsencode(path), n, _p32(pi), C.byref(lines)))
+    return pi
+
+
+def write_mapping_csv(path, pi):
+    pi = np.ascontiguousarray(pi, dtype=np.int32)
+    _check(_lib.bm_write_mapping_csv(os.fsencode(path), pi.shape[0] - 1, _p32(pi)))
+
+
+def synth_pair(n, mean_out_degree, weights=WEIGHTS_GEOMETRIC, shuffle_frac=0.3, seed=1, alpha=2.1,
+               modify_frac=0.2):
+    """Returns (male, female, true_perm, pi0)."""
+    p = _SynthParams(n, mean_out_degree, alpha, weights, modify_frac, shuffle_frac, seed)
+    m, f = _EdgeList(), _EdgeList()
+    truth = np.zeros(n + 1, np.int32)
+    pi0 = np.zeros(n + 1, np.int32)
+    _check(_lib.bm_synth_pair(C.byref(p), C.byref(m), C.byref(f), _p32(truth), _p32(pi0)))
+    return _from_edge_list(m), _from_edge_list(f), truth, pi0
+
+
+def greedy_match(n, male: Graph, female: Graph) -> np.ndarray:
+    pi = np.zeros(n + 1, np.int32)
+    em, ef = male._edges(), female._edges()
+    _check(_lib.bm_greedy_match(n, C.byref(em), C.byref(ef), _p32(pi)))
+    return pi
+
+
+class Batch:
+    def __init__(self, ctx: "Context", capacity: int):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(_lib.bm_batch_create(ctx.h, capacity, C.byref(h)), ctx.h)
+        self.h = h
+        self.capacity = capacity
+
+    def close(self):
+        if self.h:
+            _lib.bm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        _check(rc, self.ctx.h)
+
+    @property
+    def size(self):
+        return _lib.bm_batch_size(self.h)
+
+    def set_pairs(self, uv):
+        uv = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1, 2)
+        self._chk(_lib.bm_batch_set_pairs(self.h, _p32(uv), uv.shape[0]))
+
+    def set_triples(self, abc):
+        abc = np.ascontiguousarray(abc, dtype=np.int32).reshape(-1, 3)
+        self._chk(_lib.bm_batch_set_triples(self.h, _p32(abc), abc.shape[0]))
+
+    def gen_pairs(self, seed, k):
+        self._chk(_lib.bm_batch_gen_pairs(self.h, seed, k))
+
+    def gen_triples(self, seed, k):
+        self._chk(_lib.bm_batch_gen_triples(self.h, seed, k))
+
+    def gen_changed_pairs(self, changed):
+        ch = np.ascontiguousarray(changed, dtype=np.int32)
+        self._chk(_lib.bm_batch_gen_changed_pairs(self.h, _p32(ch), ch.shape[0]))
+
+    def gen_changed_triples(self, changed):
+        ch = np.ascontiguousarray(changed, dtype=np.int32)
+        self._chk(_lib.bm_batch_gen_changed_triples(self.h, _p32(ch), ch.shape[0]))
+
+    def eval_swaps(self):
+        self._chk(_lib.bm_batch_eval_swaps(self.h))
+
+    def eval_swaps_uniform(self, seed, k):
+        self._chk(_lib.bm_batch_eval_swaps_uniform(self.h, seed, k))
+
+    def eval_rot3(self):
+        self._chk(_lib.bm_batch_eval_rot3(self.h))
+
+    def deltas(self, first=0, count=None):
+        count = self.size - first if count is None else count
+        d = np.empty(count, np.int64)
+        self._chk(_lib.bm_batch_get(self.h, first, count, _p64(d), None, None))
+        return d
+
+    def perms(self, first=0, count=None):
+        count = self.size - first if count is None else count
+        p = np.empty(count, np.uint8)
+        self._chk(_lib.bm_batch_get(self.h, first, count, None, _pu8(p), None))
+        return p
+
+    def pairs(self, first=0, count=None):
+        count = self.size - first if count is None else count
+        c = np.empty((count, 2), np.int32)
+        self._chk(_lib.bm_batch_get(self.h, first, count, None, None, _p32(c)))
+        return c
+
+    def triples(self, first=0, count=None):
+        count = self.size - first if count is None else count
+        c = np.empty((count, 4), np.int32)
+        self._chk(_lib.bm_batch_get(self.h, first, count, None, None, _p32(c)))
+        return c
+
+    def stats(self):
+        s = _Stats()
+        self._chk(_lib.bm_batch_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _Stats._fields_}
+
+    def pick(self, kind, rank):
+        idx = C.c_int64(-1)
+        self._chk(_lib.bm_batch_pick(self.h, kind, rank, C.byref(idx)))
+        return idx.value
+
+    def accept_disjoint(self, max_accept=0):
+        na, gain = C.c_int64(0), C.c_int64(0)
+        self._chk(_lib.bm_batch_accept_disjoint(self.h, max_accept, C.byref(na), C.byref(gain)))
+        return na.value, gain.value
+
+    def algo_bytes(self, generated):
+        b, d = C.c_int64(0), C.c_int64(0)
+        self._chk(_lib.bm_batch_algo_bytes(self.h, 1 if generated else 0, C.byref(b), C.byref(d)))
+        return b.value, d.value
+
+
+class Context:
+    """Graphs + matching resident on one device (bm_create)."""
+
+    def __init__(self, n, male: Graph, female: Graph, device=0):
+        self.n = int(n)
+        h = C.c_void_p()
+        em, ef = male._edges(), female._edges()
+        _check(_lib.bm_create(C.byref(h), self.n, C.byref(em), C.byref(ef), device))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.bm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        _check(rc, self.h)
+
+    def num_edges(self, female=False):
+        return _lib.bm_num_edges(self.h, 1 if female else 0)
+
+    def degrees(self, female=False):
+        o = np.zeros(self.n + 1, np.int32)
+        i = np.zeros(self.n + 1, np.int32)
+        self._chk(_lib.bm_degrees(self.h, 1 if female else 0, _p32(o), _p32(i)))
+        return o, i
+
+    def set_mapping(self, pi):
+        pi = np.ascontiguousarray(pi, dtype=np.int32)
+        if pi.shape[0] != self.n + 1:
+            raise ValueError(f"mapping must have n+1={self.n + 1} entries")
+        self._chk(_lib.bm_set_mapping(self.h, _p32(pi)))
+
+    def get_mapping(self):
+        pi = np.zeros(self.n + 1, np.int32)
+        self._chk(_lib.bm_get_mapping(self.h, _p32(pi)))
+        return pi
+
+    def score(self):
+        s = C.c_int64(0)
+        self._chk(_lib.bm_score(self.h, C.byref(s)))
+        return s.value
+
+    def swap_deltas(self, uv):
+        uv = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1, 2)
+        d = np.empty(uv.shape[0], np.int64)
+        self._chk(_lib.bm_swap_deltas(self.h, _p32(uv), uv.shape[0], _p64(d)))
+        return d
+
+    def swap_deltas_changed(self, changed):
+        ch = np.ascontiguousarray(changed, dtype=np.int32)
+        c = ch.shape[0]
+        P = c * (c - 1) // 2 + c * (self.n - c)
+        d = np.empty(P, np.int64)
+        uv = np.empty((P, 2), np.int32)
+        cnt = C.c_int64(0)
+        self._chk(_lib.bm_swap_deltas_changed(self.h, _p32(ch), c, _p64(d), _p32(uv), C.byref(cnt)))
+        return d, uv
+
+    def rot3_deltas(self, abc):
+        abc = np.ascontiguousarray(abc, dtype=np.int32).reshape(-1, 3)
+        b = np.empty(abc.shape[0], np.int64)
+        p = np.empty(abc.shape[0], np.uint8)
+        self._chk(_lib.bm_rot3_deltas(self.h, _p32(abc), abc.shape[0], _p64(b), _pu8(p)))
+        return b, p
+
+    def rot3_deltas_changed(self, changed):
+        ch = np.ascontiguousarray(changed, dtype=np.int32)
+        c, n = ch.shape[0], self.n
+        T = c * (c - 1) * (c - 2) // 6 + c * (c - 1) // 2 * (n - c) + (c * (n - c - 1) if n - c - 1 > 0 else 0)
+        b = np.empty(T, np.int64)
+        q = np.empty((T, 4), np.int32)
+        cnt = C.c_int64(0)
+        self._chk(_lib.bm_rot3_deltas_changed(self.h, _p32(ch), c, _p64(b), _p32(q), C.byref(cnt)))
+        return b, q
+
+    def accept_disjoint(self, uv, d):
+        uv = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1, 2)
+        d = np.ascontiguousarray(d, dtype=np.int64)
+        acc = np.empty(max(1, uv.shape[0]), np.int32)
+        na, gain = C.c_int64(0), C.c_int64(0)
+        self._chk(_lib.bm_accept_disjoint(self.h, _p32(uv), _p64(d), uv.shape[0], _p32(acc), C.byref(na),
+                                          C.byref(gain)))
+        return acc[:na.value].copy(), gain.value
+
+    def apply_swap(self, u, v):
+        self._chk(_lib.bm_apply_swap(self.h, u, v))
+
+    def apply_perm3(self, a, b, c, p):
+        self._chk(_lib.bm_apply_perm3(self.h, a, b, c, p))
+
+    def batch(self, capacity):
+        return Batch(self, capacity)
+
+    def timing_enable(self, on=True):
+        self._chk(_lib.bm_timing_enable(self.h, 1 if on else 0))
+
+    def timing(self, kernel):
+        n, ms = C.c_int64(0), C.c_double(0)
+        self._chk(_lib.bm_timing_get(self.h, kernel, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def timing_reset(self):
+        self._chk(_lib.bm_timing_reset(self.h))
+
+    def sync(self):
+        self._chk(_lib.bm_sync(self.h))
+
+    def uniform_algo_bytes(self, seed, k):
+        b, d = C.c_int64(0), C.c_int64(0)
+        self._chk(_lib.bm_uniform_algo_bytes(self.h, seed, k, C.byref(b), C.byref(d)))
+        return b.value, d.value
+
+    def get_tuning(self, kernel):
+        t = _Tuning()
+        self._chk(_lib.bm_get_tuning(self.h, kernel, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _Tuning._fields_}
+
+    def set_tuning(self, kernel, waves_per_block, lds_slots, blocks_per_cu, group_lanes=64):
+        t = _Tuning(waves_per_block, lds_slots, blocks_per_cu, group_lanes)
+        self._chk(_lib.bm_set_tuning(self.h, kernel, C.byref(t)))
+
+    def device_info(self):
+        buf = C.create_string_buffer(256)
+        self._chk(_lib.bm_device_info(self.h, buf, 256))
+        return buf.value.decode()
+
+    def search(self, mode, seed=1, time_limit_s=0.0, max_iters=0, out_path=None, verify_every=64, verbose=0,
+               restart_gap=0.002):
+        o = _SearchOpts(mode, seed, time_limit_s, max_iters, os.fsencode(out_path) if out_path else None, None,
+                        0.0, verify_every, verbose, None, restart_gap)
+        best = np.zeros(self.n + 1, np.int32)
+        r = _SearchResult()
+        self._chk(_lib.bm_search(self.h, C.byref(o), _p32(best), C.byref(r)))
+        return best, {k: getattr(r, k) for k, _ in _SearchResult._fields_}
+
+    def hillclimb(self, seed, rounds, batch, verify=True):
+        trace = np.zeros(rounds + 1, np.int64)
+        r = _SearchResult()
+        self._chk(_lib.bm_hillclimb(self.h, seed, rounds, batch, 1 if verify else 0, _p64(trace), C.byref(r)))
+        return trace, {k: getattr(r, k) for k, _ in _SearchResult._fields_}

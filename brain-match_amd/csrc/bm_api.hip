@@ -1,0 +1,861 @@
+/*
+ * C ABI implementation (include/brainmatch.h): contexts, batches, launches,
+ * timing.  Host logic only; the kernels are in bm_kernels.hip.
+ */
+#include "bm_device.h"
+#include "bm_host.h"
+#include "bm_rng.h"
+
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+extern "C" const char* bm_global_error(void);
+
+struct KTiming {
+    int kernel;
+    hipEvent_t a, b;
+};
+
+struct bm_ctx {
+    int32_t n;
+    int device;
+    int num_cus;
+    char arch[64];
+    hipStream_t stream;
+    bm_graph_host male;     /* host copy (degrees, neighbourhoods) */
+    int64_t female_m;
+    int32_t* f_out_deg;     /* female degrees, n+1 */
+    int32_t* f_in_deg;
+    /* device buffers */
+    int64_t* d_m_out_ptr;
+    int2* d_m_out;
+    int64_t* d_m_in_ptr;
+    int2* d_m_in;
+    uint64_t* d_f_out_desc;
+    int2* d_f_out_tab;
+    uint64_t* d_f_in_desc;
+    int2* d_f_in_tab;
+    int32_t* d_pi;
+    int64_t* d_small;       /* 16 int64 of scratch for scores / flags */
+    int32_t* d_reorder;     /* n ints for the changed-first enumerations */
+    BmLaunch tune[BM_KERNEL_COUNT];
+    int32_t group_lanes[BM_KERNEL_COUNT];
+    int timing_on;
+    std::vector<KTiming>* pending;
+    int64_t launches[BM_KERNEL_COUNT];
+    double total_ms[BM_KERNEL_COUNT];
+    bm_batch* work;         /* internal batch for the host-buffer calls */
+    char err[512];
+};
+
+struct bm_batch {
+    bm_ctx* ctx;
+    int64_t cap;
+    int64_t k;
+    int kind;               /* 0 empty, 1 pairs, 2 triples */
+    int generated;          /* candidates came from a device generator */
+    int evaluated;
+    void* d_cand;           /* cap * 16 bytes (int2 pairs or int4 triples) */
+    int64_t* d_delta;
+    uint8_t* d_perm;
+    int64_t* d_part;
+    int64_t* d_out;         /* 16 int64 */
+    void* d_acc_scratch;
+    size_t acc_scratch_bytes;
+    int64_t* d_acc_idx;
+};
+
+/* ------------------------------------------------------------------ */
+
+static int fail(bm_ctx* c, int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    if (c)
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+    else {
+        char buf[512];
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        bm_set_global_error("%s", buf);
+    }
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail((ctx), BM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                           \
+    } while (0)
+
+static BmDevGraph devgraph(const bm_ctx* c) {
+    BmDevGraph g;
+    g.n = c->n;
+    g.m_out_ptr = c->d_m_out_ptr;
+    g.m_out = c->d_m_out;
+    g.m_in_ptr = c->d_m_in_ptr;
+    g.m_in = c->d_m_in;
+    g.f_out_desc = c->d_f_out_desc;
+    g.f_out_tab = c->d_f_out_tab;
+    g.f_in_desc = c->d_f_in_desc;
+    g.f_in_tab = c->d_f_in_tab;
+    g.pi = c->d_pi;
+    return g;
+}
+
+static void timing_begin(bm_ctx* c, int kernel, KTiming* t) {
+    t->kernel = -1;
+    if (!c->timing_on) return;
+    if (hipEventCreate(&t->a) != hipSuccess) return;
+    if (hipEventCreate(&t->b) != hipSuccess) { (void)hipEventDestroy(t->a); return; }
+    t->kernel = kernel;
+    (void)hipEventRecord(t->a, c->stream);
+}
+
+static void timing_end(bm_ctx* c, KTiming* t) {
+    if (t->kernel < 0) return;
+    (void)hipEventRecord(t->b, c->stream);
+    c->pending->push_back(*t);
+}
+
+static int timing_resolve(bm_ctx* c) {
+    if (c->pending->empty()) return BM_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (KTiming& t : *c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+            c->launches[t.kernel]++;
+            c->total_ms[t.kernel] += ms;
+        }
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+    }
+    c->pending->clear();
+    return BM_OK;
+}
+
+template <typename T>
+static int dalloc(bm_ctx* c, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(c, BM_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", count * sizeof(T), hipGetErrorString(e));
+    return BM_OK;
+}
+
+template <typename T>
+static int h2d(bm_ctx* c, T* d, const void* h, size_t count) {
+    if (count == 0) return BM_OK;
+    HIPCHK(c, hipMemcpyAsync(d, h, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return BM_OK;
+}
+
+/* ------------------------------------------------------------------ */
+
+extern "C" int bm_api_version(void) { return BM_API_VERSION; }
+
+extern "C" const char* bm_last_error(const bm_ctx* ctx) { return ctx ? ctx->err : bm_global_error(); }
+
+extern "C" int32_t bm_num_vertices(const bm_ctx* ctx) { return ctx ? ctx->n : 0; }
+
+extern "C" int64_t bm_num_edges(const bm_ctx* ctx, int female) {
+    if (!ctx) return 0;
+    return female ? ctx->female_m : ctx->male.m;
+}
+
+static void free_ctx(bm_ctx* c) {
+    if (!c) return;
+    if (c->work) bm_batch_destroy(c->work);
+    (void)hipFree(c->d_m_out_ptr);
+    (void)hipFree(c->d_m_out);
+    (void)hipFree(c->d_m_in_ptr);
+    (void)hipFree(c->d_m_in);
+    (void)hipFree(c->d_f_out_desc);
+    (void)hipFree(c->d_f_out_tab);
+    (void)hipFree(c->d_f_in_desc);
+    (void)hipFree(c->d_f_in_tab);
+    (void)hipFree(c->d_pi);
+    (void)hipFree(c->d_small);
+    (void)hipFree(c->d_reorder);
+    if (c->pending) {
+        for (KTiming& t : *c->pending) {
+            (void)hipEventDestroy(t.a);
+            (void)hipEventDestroy(t.b);
+        }
+        delete c->pending;
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    bm_graph_free(&c->male);
+    free(c->f_out_deg);
+    free(c->f_in_deg);
+    free(c);
+}
+
+extern "C" void bm_destroy(bm_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_ctx(ctx);
+}
+
+static void default_tuning(bm_ctx* c) {
+    const int cus = c->num_cus > 0 ? c->num_cus : 256;
+    c->tune[BM_KERNEL_SWAP] = BmLaunch{4, 2048, cus * 2};
+    c->tune[BM_KERNEL_ROT3] = BmLaunch{4, 2048, cus * 2};
+    c->tune[BM_KERNEL_SCORE] = BmLaunch{4, 1024, cus * 4};
+    for (int i = 0; i < BM_KERNEL_COUNT; i++) c->group_lanes[i] = 64;
+}
+
+extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm_edges* female, int32_t device) {
+    if (!out) return fail(nullptr, BM_ERR_ARG, "bm_create: out is NULL");
+    *out = nullptr;
+    if (n < 2) return fail(nullptr, BM_ERR_ARG, "bm_create: n must be >= 2 (got %d)", n);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, BM_ERR_NODEV, "bm_create: no HIP device available");
+    if (device < 0 || device >= ndev) return fail(nullptr, BM_ERR_ARG, "bm_create: device %d out of range", device);
+    bm_ctx* c = (bm_ctx*)calloc(1, sizeof(bm_ctx));
+    if (!c) return fail(nullptr, BM_ERR_NOMEM, "bm_create: out of host memory");
+    c->n = n;
+    c->device = device;
+    c->pending = new std::vector<KTiming>();
+    hipError_t he = hipSetDevice(device);
+    if (he != hipSuccess) { free_ctx(c); return fail(nullptr, BM_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->num_cus = prop.multiProcessorCount;
+        snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
+    }
+    default_tuning(c);
+    he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) { free_ctx(c); return fail(nullptr, BM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)); }
+
+    const char* err = nullptr;
+    int rc = bm_graph_build(n, male, &c->male, &err);
+    if (rc != BM_OK) { free_ctx(c); return fail(nullptr, rc, "bm_create: male graph: %s", err); }
+    bm_graph_host fg;
+    rc = bm_graph_build(n, female, &fg, &err);
+    if (rc != BM_OK) { free_ctx(c); return fail(nullptr, rc, "bm_create: female graph: %s", err); }
+    c->female_m = fg.m;
+    c->f_out_deg = (int32_t*)calloc((size_t)n + 1, sizeof(int32_t));
+    c->f_in_deg = (int32_t*)calloc((size_t)n + 1, sizeof(int32_t));
+    bm_rowhash_host ho, hi;
+    memset(&ho, 0, sizeof ho);
+    memset(&hi, 0, sizeof hi);
+    if (!c->f_out_deg || !c->f_in_deg || bm_rowhash_build(&fg.out, &ho) != BM_OK ||
+        bm_rowhash_build(&fg.in, &hi) != BM_OK) {
+        bm_graph_free(&fg);
+        bm_rowhash_free(&ho);
+        bm_rowhash_free(&hi);
+        free_ctx(c);
+        return fail(nullptr, BM_ERR_NOMEM, "bm_create: out of host memory building female tables");
+    }
+    for (int32_t r = 1; r <= n; r++) {
+        c->f_out_deg[r] = (int32_t)(fg.out.ptr[r + 1] - fg.out.ptr[r]);
+        c->f_in_deg[r] = (int32_t)(fg.in.ptr[r + 1] - fg.in.ptr[r]);
+    }
+    bm_graph_free(&fg);
+
+    const bm_graph_host& mg = c->male;
+    rc = BM_OK;
+    if (rc == BM_OK) rc = dalloc(c, &c->d_m_out_ptr, (size_t)n + 2);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_m_out, (size_t)mg.m);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_m_in_ptr, (size_t)n + 2);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_m_in, (size_t)mg.m);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_f_out_desc, (size_t)n + 1);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_f_out_tab, (size_t)(ho.slots ? ho.slots : 2));
+    if (rc == BM_OK) rc = dalloc(c, &c->d_f_in_desc, (size_t)n + 1);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_f_in_tab, (size_t)(hi.slots ? hi.slots : 2));
+    if (rc == BM_OK) rc = dalloc(c, &c->d_pi, (size_t)n + 1);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_small, 16);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_reorder, (size_t)n);
+    if (rc == BM_OK) rc = h2d(c, c->d_m_out_ptr, mg.out.ptr, (size_t)n + 2);
+    if (rc == BM_OK) rc = h2d(c, c->d_m_out, mg.out.adj, (size_t)mg.m);
+    if (rc == BM_OK) rc = h2d(c, c->d_m_in_ptr, mg.in.ptr, (size_t)n + 2);
+    if (rc == BM_OK) rc = h2d(c, c->d_m_in, mg.in.adj, (size_t)mg.m);
+    if (rc == BM_OK) rc = h2d(c, c->d_f_out_desc, ho.desc, (size_t)n + 1);
+    if (rc == BM_OK) rc = h2d(c, c->d_f_out_tab, ho.tab, (size_t)ho.slots);
+    if (rc == BM_OK) rc = h2d(c, c->d_f_in_desc, hi.desc, (size_t)n + 1);
+    if (rc == BM_OK) rc = h2d(c, c->d_f_in_tab, hi.tab, (size_t)hi.slots);
+    if (rc == BM_OK && hipMemsetAsync(c->d_pi, 0, ((size_t)n + 1) * sizeof(int32_t), c->stream) != hipSuccess)
+        rc = fail(c, BM_ERR_HIP, "hipMemsetAsync failed");
+    if (rc == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, BM_ERR_HIP, "hipStreamSynchronize failed");
+    bm_rowhash_free(&ho);
+    bm_rowhash_free(&hi);
+    if (rc != BM_OK) {
+        bm_set_global_error("bm_create: %s", c->err);
+        free_ctx(c);
+        return rc;
+    }
+    *out = c;
+    return BM_OK;
+}
+
+extern "C" int bm_degrees(const bm_ctx* ctx, int female, int32_t* out_deg, int32_t* in_deg) {
+    if (!ctx) return BM_ERR_ARG;
+    const int32_t n = ctx->n;
+    for (int32_t r = 0; r <= n; r++) {
+        if (female) {
+            if (out_deg) out_deg[r] = r ? ctx->f_out_deg[r] : 0;
+            if (in_deg) in_deg[r] = r ? ctx->f_in_deg[r] : 0;
+        } else {
+            if (out_deg) out_deg[r] = (int32_t)(ctx->male.out.ptr[r + 1] - ctx->male.out.ptr[r]);
+            if (in_deg) in_deg[r] = (int32_t)(ctx->male.in.ptr[r + 1] - ctx->male.in.ptr[r]);
+        }
+    }
+    return BM_OK;
+}
+
+extern "C" int bm_set_mapping(bm_ctx* c, const int32_t* pi) {
+    if (!c || !pi) return fail(c, BM_ERR_ARG, "bm_set_mapping: null argument");
+    for (int32_t i = 1; i <= c->n; i++)
+        if (pi[i] < 0 || pi[i] > c->n)
+            return fail(c, BM_ERR_ARG, "bm_set_mapping: pi[%d] = %d outside 0..%d", i, pi[i], c->n);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_pi + 1, pi + 1, (size_t)c->n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+extern "C" int bm_get_mapping(bm_ctx* c, int32_t* pi) {
+    if (!c || !pi) return fail(c, BM_ERR_ARG, "bm_get_mapping: null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(pi, c->d_pi, ((size_t)c->n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    pi[0] = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_score(bm_ctx* c, int64_t* score) {
+    if (!c || !score) return fail(c, BM_ERR_ARG, "bm_score: null argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemsetAsync(c->d_small, 0, sizeof(int64_t), c->stream));
+    KTiming t;
+    timing_begin(c, BM_KERNEL_SCORE, &t);
+    HIPCHK(c, bm_launch_score(devgraph(c), c->tune[BM_KERNEL_SCORE], c->d_small, c->stream));
+    timing_end(c, &t);
+    HIPCHK(c, hipMemcpyAsync(score, c->d_small, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Batches                                                             */
+/* ------------------------------------------------------------------ */
+
+extern "C" void bm_batch_destroy(bm_batch* b) {
+    if (!b) return;
+    if (b->ctx) {
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+    }
+    (void)hipFree(b->d_cand);
+    (void)hipFree(b->d_delta);
+    (void)hipFree(b->d_perm);
+    (void)hipFree(b->d_part);
+    (void)hipFree(b->d_out);
+    (void)hipFree(b->d_acc_scratch);
+    (void)hipFree(b->d_acc_idx);
+    free(b);
+}
+
+extern "C" int bm_batch_create(bm_ctx* c, int64_t capacity, bm_batch** out) {
+    if (!c || !out || capacity < 1) return fail(c, BM_ERR_ARG, "bm_batch_create: invalid argument");
+    if (capacity > ((int64_t)1 << 31) - 1)
+        return fail(c, BM_ERR_ARG, "bm_batch_create: capacity %lld exceeds 2^31-1", (long long)capacity);
+    *out = nullptr;
+    HIPCHK(c, hipSetDevice(c->device));
+    bm_batch* b = (bm_batch*)calloc(1, sizeof(bm_batch));
+    if (!b) return fail(c, BM_ERR_NOMEM, "bm_batch_create: out of host memory");
+    b->ctx = c;
+    b->cap = capacity;
+    int rc = BM_OK;
+    const size_t nb = (size_t)bm_stats_blocks(capacity);
+    if (rc == BM_OK) rc = dalloc(c, (int4**)&b->d_cand, (size_t)capacity);
+    if (rc == BM_OK) rc = dalloc(c, &b->d_delta, (size_t)capacity);
+    if (rc == BM_OK) rc = dalloc(c, &b->d_perm, (size_t)capacity);
+    if (rc == BM_OK) rc = dalloc(c, &b->d_part, nb * 5);
+    if (rc == BM_OK) rc = dalloc(c, &b->d_out, 16);
+    if (rc != BM_OK) {
+        bm_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return BM_OK;
+}
+
+extern "C" int64_t bm_batch_size(const bm_batch* b) { return b ? b->k : 0; }
+
+extern "C" int bm_batch_set_pairs(bm_batch* b, const int32_t* uv, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (k < 0 || k > b->cap || (k > 0 && !uv)) return fail(c, BM_ERR_ARG, "bm_batch_set_pairs: k=%lld (capacity %lld)", (long long)k, (long long)b->cap);
+    for (int64_t i = 0; i < 2 * k; i++)
+        if (uv[i] < 1 || uv[i] > c->n)
+            return fail(c, BM_ERR_ARG, "bm_batch_set_pairs: vertex %d at position %lld outside 1..%d", uv[i], (long long)i, c->n);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(b->d_cand, uv, (size_t)k * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    b->k = k;
+    b->kind = 1;
+    b->generated = 0;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_set_triples(bm_batch* b, const int32_t* abc, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (k < 0 || k > b->cap || (k > 0 && !abc)) return fail(c, BM_ERR_ARG, "bm_batch_set_triples: invalid k");
+    std::vector<int4> tmp((size_t)k);
+    for (int64_t i = 0; i < k; i++) {
+        const int32_t x = abc[3 * i], y = abc[3 * i + 1], z = abc[3 * i + 2];
+        if (x < 1 || x > c->n || y < 1 || y > c->n || z < 1 || z > c->n || x == y || x == z || y == z)
+            return fail(c, BM_ERR_ARG, "bm_batch_set_triples: triple %lld = (%d,%d,%d) invalid (ids in 1..%d, distinct)",
+                        (long long)i, x, y, z, c->n);
+        tmp[(size_t)i] = make_int4(x, y, z, 0);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(b->d_cand, tmp.data(), (size_t)k * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    b->k = k;
+    b->kind = 2;
+    b->generated = 0;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_gen_pairs(bm_batch* b, uint64_t seed, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (k < 0 || k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_gen_pairs: k=%lld exceeds capacity", (long long)k);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, bm_launch_gen_pairs(c->n, seed, k, (int2*)b->d_cand, c->stream));
+    b->k = k;
+    b->kind = 1;
+    b->generated = 1;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_gen_triples(bm_batch* b, uint64_t seed, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (c->n < 3) return fail(c, BM_ERR_ARG, "bm_batch_gen_triples: n < 3");
+    if (k < 0 || k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_gen_triples: k exceeds capacity");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, bm_launch_gen_triples(c->n, seed, k, (int4*)b->d_cand, c->stream));
+    b->k = k;
+    b->kind = 2;
+    b->generated = 1;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
+/* sortArray + fillArray (src/graph-re-alignment-gpu-swap.cu:296-307, :983-999):
+ * changed ids ascending, then the other ids ascending. */
+static int build_reorder(bm_ctx* c, const int32_t* changed, int32_t cnum, std::vector<int32_t>& re) {
+    const int32_t n = c->n;
+    if (cnum < 0 || cnum > n || (cnum > 0 && !changed)) return fail(c, BM_ERR_ARG, "changed list: invalid count %d", cnum);
+    std::vector<uint8_t> present((size_t)n + 1, 0);
+    re.clear();
+    re.reserve((size_t)n);
+    for (int32_t i = 0; i < cnum; i++) {
+        const int32_t x = changed[i];
+        if (x < 1 || x > n) return fail(c, BM_ERR_ARG, "changed list: vertex %d outside 1..%d", x, n);
+        if (present[x]) return fail(c, BM_ERR_ARG, "changed list: vertex %d repeated", x);
+        present[x] = 1;
+        re.push_back(x);
+    }
+    std::sort(re.begin(), re.end());
+    for (int32_t x = 1; x <= n; x++)
+        if (!present[x]) re.push_back(x);
+    return BM_OK;
+}
+
+extern "C" int bm_batch_gen_changed_pairs(bm_batch* b, const int32_t* changed, int32_t cnum) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    std::vector<int32_t> re;
+    int rc = build_reorder(c, changed, cnum, re);
+    if (rc != BM_OK) return rc;
+    const int64_t P = (int64_t)cnum * (cnum - 1) / 2 + (int64_t)cnum * ((int64_t)c->n - cnum);
+    if (P > b->cap) return fail(c, BM_ERR_ARG, "changed pairs: %lld pairs exceed batch capacity %lld", (long long)P, (long long)b->cap);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_reorder, re.data(), re.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, bm_launch_gen_changed_pairs(c->n, cnum, c->d_reorder, P, (int2*)b->d_cand, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  /* re goes out of scope */
+    b->k = P;
+    b->kind = 1;
+    b->generated = 1;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_gen_changed_triples(bm_batch* b, const int32_t* changed, int32_t cnum) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (cnum > 4096) return fail(c, BM_ERR_ARG, "changed triples: c=%d above 4096", cnum);
+    std::vector<int32_t> re;
+    int rc = build_reorder(c, changed, cnum, re);
+    if (rc != BM_OK) return rc;
+    const int64_t cc = (int64_t)cnum;
+    const int64_t nn = (int64_t)c->n;
+    int64_t T = cc * (cc - 1) * (cc - 2) / 6 + cc * (cc - 1) / 2 * (nn - cc);
+    if (nn - cc - 1 > 0) T += cc * (nn - cc - 1);
+    if (T > b->cap) return fail(c, BM_ERR_ARG, "changed triples: %lld triples exceed batch capacity %lld", (long long)T, (long long)b->cap);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->d_reorder, re.data(), re.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, bm_launch_gen_changed_triples(c->n, cnum, c->d_reorder, T, (int4*)b->d_cand, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    b->k = T;
+    b->kind = 2;
+    b->generated = 1;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_eval_swaps(bm_batch* b) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (b->kind != 1) return fail(c, BM_ERR_STATE, "bm_batch_eval_swaps: batch does not hold pairs");
+    HIPCHK(c, hipSetDevice(c->device));
+    KTiming t;
+    timing_begin(c, BM_KERNEL_SWAP, &t);
+    HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_LIST, (const int2*)b->d_cand, 0, b->k,
+                             b->d_delta, c->stream));
+    timing_end(c, &t);
+    b->evaluated = 1;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_eval_swaps_uniform(bm_batch* b, uint64_t seed, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (k < 0 || k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_eval_swaps_uniform: k exceeds capacity");
+    HIPCHK(c, hipSetDevice(c->device));
+    KTiming t;
+    timing_begin(c, BM_KERNEL_SWAP, &t);
+    HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_UNIFORM, nullptr, seed, k, b->d_delta,
+                             c->stream));
+    timing_end(c, &t);
+    b->k = k;
+    b->kind = 3; /* deltas only: candidates implicit in (seed, index) */
+    b->generated = 1;
+    b->evaluated = 1;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_eval_rot3(bm_batch* b) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (b->kind != 2) return fail(c, BM_ERR_STATE, "bm_batch_eval_rot3: batch does not hold triples");
+    HIPCHK(c, hipSetDevice(c->device));
+    KTiming t;
+    timing_begin(c, BM_KERNEL_ROT3, &t);
+    HIPCHK(c, bm_launch_rot3(devgraph(c), c->tune[BM_KERNEL_ROT3], (const int4*)b->d_cand, b->k, b->d_delta,
+                             b->d_perm, (int4*)b->d_cand, c->stream));
+    timing_end(c, &t);
+    b->evaluated = 1;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_get(bm_batch* b, int64_t first, int64_t count, int64_t* delta, uint8_t* perm, int32_t* cand) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (first < 0 || count < 0 || first + count > b->k) return fail(c, BM_ERR_ARG, "bm_batch_get: slice outside the batch");
+    if (count == 0) return BM_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (delta) {
+        if (!b->evaluated) return fail(c, BM_ERR_STATE, "bm_batch_get: batch not evaluated");
+        HIPCHK(c, hipMemcpyAsync(delta, b->d_delta + first, (size_t)count * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    }
+    if (perm) {
+        if (b->kind != 2) return fail(c, BM_ERR_STATE, "bm_batch_get: perm only for triple batches");
+        HIPCHK(c, hipMemcpyAsync(perm, b->d_perm + first, (size_t)count, hipMemcpyDeviceToHost, c->stream));
+    }
+    if (cand) {
+        if (b->kind == 1) {
+            HIPCHK(c, hipMemcpyAsync(cand, (int2*)b->d_cand + first, (size_t)count * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
+        } else if (b->kind == 2) {
+            HIPCHK(c, hipMemcpyAsync(cand, (int4*)b->d_cand + first, (size_t)count * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+        } else if (b->kind == 3) {
+            return fail(c, BM_ERR_STATE, "bm_batch_get: uniform-evaluated batches do not store candidates");
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+extern "C" int bm_batch_stats(bm_batch* b, bm_stats* out) {
+    if (!b || !out) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (!b->evaluated) return fail(c, BM_ERR_STATE, "bm_batch_stats: batch not evaluated");
+    HIPCHK(c, hipSetDevice(c->device));
+    int64_t h[6];
+    if (b->k == 0) {
+        out->count = 0; out->sum = 0; out->max = INT64_MIN; out->argmax = -1; out->n_pos = 0; out->n_zero = 0;
+        return BM_OK;
+    }
+    HIPCHK(c, bm_launch_stats(b->d_delta, b->k, b->d_part, b->d_out, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h, b->d_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out->count = h[0]; out->sum = h[1]; out->max = h[2]; out->argmax = h[3]; out->n_pos = h[4]; out->n_zero = h[5];
+    return BM_OK;
+}
+
+extern "C" int bm_batch_pick(bm_batch* b, int32_t kind, int64_t rank, int64_t* index) {
+    if (!b || !index) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (!b->evaluated) return fail(c, BM_ERR_STATE, "bm_batch_pick: batch not evaluated");
+    if (kind != 0 && kind != 1) return fail(c, BM_ERR_ARG, "bm_batch_pick: kind must be 0 or 1");
+    *index = -1;
+    if (rank < 0 || b->k == 0) return BM_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, bm_launch_pick(b->d_delta, b->k, kind, rank, b->d_part, b->d_out, c->stream));
+    HIPCHK(c, hipMemcpyAsync(index, b->d_out, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+static int accept_impl(bm_batch* b, int64_t max_accept, int32_t* accepted, int64_t* n_acc, int64_t* gain) {
+    bm_ctx* c = b->ctx;
+    if (b->kind != 1 || !b->evaluated) return fail(c, BM_ERR_STATE, "accept: batch must hold evaluated pairs");
+    if (c->n > 1200000) return fail(c, BM_ERR_ARG, "accept: n=%d above the LDS lock-bitmap limit 1200000", c->n);
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!b->d_acc_scratch) {
+        b->acc_scratch_bytes = bm_accept_scratch_bytes(b->cap);
+        HIPCHK(c, hipMalloc(&b->d_acc_scratch, b->acc_scratch_bytes));
+        HIPCHK(c, hipMalloc((void**)&b->d_acc_idx, (size_t)b->cap * sizeof(int64_t)));
+    }
+    HIPCHK(c, bm_launch_accept(devgraph(c), c->d_pi, (const int2*)b->d_cand, b->d_delta, b->k, max_accept,
+                               b->d_acc_scratch, b->acc_scratch_bytes, b->d_acc_idx, b->d_out, c->stream));
+    int64_t meta[2];
+    HIPCHK(c, hipMemcpyAsync(meta, b->d_out, sizeof meta, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (accepted && meta[0] > 0) {
+        std::vector<int64_t> idx((size_t)meta[0]);
+        HIPCHK(c, hipMemcpy(idx.data(), b->d_acc_idx, (size_t)meta[0] * sizeof(int64_t), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < meta[0]; i++) accepted[i] = (int32_t)idx[(size_t)i];
+    }
+    if (n_acc) *n_acc = meta[0];
+    if (gain) *gain = meta[1];
+    b->evaluated = 0; /* the matching changed: deltas are stale */
+    return BM_OK;
+}
+
+extern "C" int bm_batch_accept_disjoint(bm_batch* b, int64_t max_accept, int64_t* n_accepted, int64_t* gain) {
+    if (!b) return BM_ERR_ARG;
+    return accept_impl(b, max_accept, nullptr, n_accepted, gain);
+}
+
+/* ------------------------------------------------------------------ */
+/* Host-buffer convenience calls                                       */
+/* ------------------------------------------------------------------ */
+
+static int work_batch(bm_ctx* c, int64_t k, bm_batch** out) {
+    if (k < 1) k = 1;
+    if (c->work && c->work->cap >= k) { *out = c->work; return BM_OK; }
+    if (c->work) { bm_batch_destroy(c->work); c->work = nullptr; }
+    int rc = bm_batch_create(c, k, &c->work);
+    if (rc != BM_OK) return rc;
+    *out = c->work;
+    return BM_OK;
+}
+
+extern "C" int bm_swap_deltas(bm_ctx* c, const int32_t* uv, int64_t k, int64_t* delta) {
+    if (!c || k < 0 || (k > 0 && (!uv || !delta))) return fail(c, BM_ERR_ARG, "bm_swap_deltas: invalid argument");
+    if (k == 0) return BM_OK;
+    bm_batch* b;
+    int rc = work_batch(c, k, &b);
+    if (rc == BM_OK) rc = bm_batch_set_pairs(b, uv, k);
+    if (rc == BM_OK) rc = bm_batch_eval_swaps(b);
+    if (rc == BM_OK) rc = bm_batch_get(b, 0, k, delta, nullptr, nullptr);
+    return rc;
+}
+
+extern "C" int bm_swap_deltas_changed(bm_ctx* c, const int32_t* changed, int32_t cnum, int64_t* delta, int32_t* uv,
+                                      int64_t* count) {
+    if (!c) return BM_ERR_ARG;
+    const int64_t P = (int64_t)cnum * (cnum - 1) / 2 + (int64_t)cnum * ((int64_t)c->n - cnum);
+    bm_batch* b;
+    int rc = work_batch(c, P, &b);
+    if (rc == BM_OK) rc = bm_batch_gen_changed_pairs(b, changed, cnum);
+    if (rc == BM_OK) rc = bm_batch_eval_swaps(b);
+    if (rc == BM_OK) rc = bm_batch_get(b, 0, P, delta, nullptr, uv);
+    if (rc == BM_OK && count) *count = P;
+    return rc;
+}
+
+extern "C" int bm_rot3_deltas(bm_ctx* c, const int32_t* abc, int64_t k, int64_t* best, uint8_t* perm) {
+    if (!c || k < 0 || (k > 0 && (!abc || !best))) return fail(c, BM_ERR_ARG, "bm_rot3_deltas: invalid argument");
+    if (k == 0) return BM_OK;
+    bm_batch* b;
+    int rc = work_batch(c, k, &b);
+    if (rc == BM_OK) rc = bm_batch_set_triples(b, abc, k);
+    if (rc == BM_OK) rc = bm_batch_eval_rot3(b);
+    if (rc == BM_OK) rc = bm_batch_get(b, 0, k, best, perm, nullptr);
+    return rc;
+}
+
+extern "C" int bm_rot3_deltas_changed(bm_ctx* c, const int32_t* changed, int32_t cnum, int64_t* best, int32_t* abcp,
+                                      int64_t* count) {
+    if (!c) return BM_ERR_ARG;
+    const int64_t cc = cnum, nn = c->n;
+    int64_t T = cc * (cc - 1) * (cc - 2) / 6 + cc * (cc - 1) / 2 * (nn - cc);
+    if (nn - cc - 1 > 0) T += cc * (nn - cc - 1);
+    bm_batch* b;
+    int rc = work_batch(c, T, &b);
+    if (rc == BM_OK) rc = bm_batch_gen_changed_triples(b, changed, cnum);
+    if (rc == BM_OK) rc = bm_batch_eval_rot3(b);
+    if (rc == BM_OK) rc = bm_batch_get(b, 0, T, best, nullptr, abcp);
+    if (rc == BM_OK && count) *count = T;
+    return rc;
+}
+
+extern "C" int bm_accept_disjoint(bm_ctx* c, const int32_t* uv, const int64_t* delta, int64_t k, int32_t* accepted,
+                                  int64_t* n_accepted, int64_t* gain) {
+    if (!c || k < 0 || (k > 0 && (!uv || !delta))) return fail(c, BM_ERR_ARG, "bm_accept_disjoint: invalid argument");
+    bm_batch* b;
+    int rc = work_batch(c, k, &b);
+    if (rc != BM_OK) return rc;
+    rc = bm_batch_set_pairs(b, uv, k);
+    if (rc != BM_OK) return rc;
+    if (k > 0) HIPCHK(c, hipMemcpyAsync(b->d_delta, delta, (size_t)k * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    b->evaluated = 1;
+    return accept_impl(b, 0, accepted, n_accepted, gain);
+}
+
+extern "C" int bm_apply_swap(bm_ctx* c, int32_t u, int32_t v) {
+    if (!c || u < 1 || u > c->n || v < 1 || v > c->n) return fail(c, BM_ERR_ARG, "bm_apply_swap: vertex outside 1..n");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, bm_launch_apply_swap(c->d_pi, u, v, c->stream));
+    return BM_OK;
+}
+
+extern "C" int bm_apply_perm3(bm_ctx* c, int32_t a, int32_t b, int32_t d, int32_t p) {
+    if (!c || a < 1 || a > c->n || b < 1 || b > c->n || d < 1 || d > c->n || a == b || a == d || b == d || p < 0 || p > 5)
+        return fail(c, BM_ERR_ARG, "bm_apply_perm3: invalid triple or permutation");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, bm_launch_apply_perm3(c->d_pi, a, b, d, p, c->stream));
+    return BM_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Measurement                                                         */
+/* ------------------------------------------------------------------ */
+
+extern "C" int bm_timing_enable(bm_ctx* c, int on) {
+    if (!c) return BM_ERR_ARG;
+    c->timing_on = on ? 1 : 0;
+    return BM_OK;
+}
+
+extern "C" int bm_timing_get(bm_ctx* c, int32_t kernel, int64_t* launches, double* total_ms) {
+    if (!c || kernel < 0 || kernel >= BM_KERNEL_COUNT) return fail(c, BM_ERR_ARG, "bm_timing_get: bad kernel id");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = timing_resolve(c);
+    if (rc != BM_OK) return rc;
+    if (launches) *launches = c->launches[kernel];
+    if (total_ms) *total_ms = c->total_ms[kernel];
+    return BM_OK;
+}
+
+extern "C" int bm_timing_reset(bm_ctx* c) {
+    if (!c) return BM_ERR_ARG;
+    int rc = timing_resolve(c);
+    if (rc != BM_OK) return rc;
+    for (int i = 0; i < BM_KERNEL_COUNT; i++) { c->launches[i] = 0; c->total_ms[i] = 0; }
+    return BM_OK;
+}
+
+extern "C" int bm_sync(bm_ctx* c) {
+    if (!c) return BM_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+static inline int64_t vdeg(const bm_graph_host& g, int32_t x) {
+    return (g.out.ptr[x + 1] - g.out.ptr[x]) + (g.in.ptr[x + 1] - g.in.ptr[x]);
+}
+
+extern "C" int bm_batch_algo_bytes(bm_batch* b, int32_t generated, int64_t* bytes, int64_t* sum_degree) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (b->kind != 1 && b->kind != 2) return fail(c, BM_ERR_STATE, "bm_batch_algo_bytes: batch holds no candidates");
+    const int64_t k = b->k;
+    const int w = b->kind == 1 ? 2 : 4;
+    std::vector<int32_t> h((size_t)k * w);
+    if (k) {
+        HIPCHK(c, hipMemcpy(h.data(), b->d_cand, (size_t)k * w * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    int64_t D = 0;
+    for (int64_t i = 0; i < k; i++) {
+        const int32_t* q = &h[(size_t)i * w];
+        D += vdeg(c->male, q[0]) + vdeg(c->male, q[1]);
+        if (w == 4) D += vdeg(c->male, q[2]);
+    }
+    int64_t B;
+    if (b->kind == 1)
+        B = k * (48 + (generated ? 0 : 8)) + 28 * D;
+    else
+        B = k * (69 + (generated ? 0 : 12)) + 36 * D;
+    if (bytes) *bytes = B;
+    if (sum_degree) *sum_degree = D;
+    return BM_OK;
+}
+
+extern "C" int bm_uniform_algo_bytes(bm_ctx* c, uint64_t seed, int64_t k, int64_t* bytes, int64_t* sum_degree) {
+    if (!c || k < 0) return BM_ERR_ARG;
+    int64_t D = 0;
+    for (int64_t i = 0; i < k; i++) {
+        int32_t u, v;
+        bm_uniform_pair(seed, (uint64_t)i, c->n, &u, &v);
+        D += vdeg(c->male, u) + vdeg(c->male, v);
+    }
+    if (bytes) *bytes = 48 * k + 28 * D;
+    if (sum_degree) *sum_degree = D;
+    return BM_OK;
+}
+
+extern "C" int bm_get_tuning(const bm_ctx* c, int32_t kernel, bm_tuning* t) {
+    if (!c || !t || kernel < 0 || kernel >= BM_KERNEL_COUNT) return BM_ERR_ARG;
+    const int cus = c->num_cus > 0 ? c->num_cus : 256;
+    t->waves_per_block = c->tune[kernel].waves_per_block;
+    t->lds_slots = c->tune[kernel].lds_slots;
+    t->blocks_per_cu = c->tune[kernel].grid_blocks / cus;
+    t->group_lanes = c->group_lanes[kernel];
+    return BM_OK;
+}
+
+extern "C" int bm_set_tuning(bm_ctx* c, int32_t kernel, const bm_tuning* t) {
+    if (!c || !t || kernel < 0 || kernel >= BM_KERNEL_COUNT) return BM_ERR_ARG;
+    const int w = t->waves_per_block;
+    if (w != 1 && w != 2 && w != 4 && w != 8) return fail(c, BM_ERR_ARG, "bm_set_tuning: waves_per_block must be 1,2,4,8");
+    if (t->lds_slots < 2 || (t->lds_slots & 1) || (size_t)w * (size_t)t->lds_slots * 8 > 160 * 1024)
+        return fail(c, BM_ERR_ARG, "bm_set_tuning: lds_slots must be even and waves*slots*8 <= 160 KiB");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(c, BM_ERR_ARG, "bm_set_tuning: blocks_per_cu in 1..64");
+    if (t->group_lanes != 64) return fail(c, BM_ERR_ARG, "bm_set_tuning: group_lanes other than 64 not built");
+    const int cus = c->num_cus > 0 ? c->num_cus : 256;
+    c->tune[kernel] = BmLaunch{w, t->lds_slots, t->blocks_per_cu * cus};
+    c->group_lanes[kernel] = t->group_lanes;
+    return BM_OK;
+}
+
+extern "C" int bm_device_info(bm_ctx* c, char* buf, size_t len) {
+    if (!c || !buf || len == 0) return BM_ERR_ARG;
+    snprintf(buf, len, "%s, %d CUs", c->arch, c->num_cus);
+    return BM_OK;
+}
+
+extern "C" const bm_graph_host* bm_ctx_male(const bm_ctx* c) { return c ? &c->male : nullptr; }
+
+extern "C" int bm_ctx_fail(bm_ctx* c, int code, const char* msg) { return fail(c, code, "%s", msg); }

@@ -1,0 +1,72 @@
+/*
+ * Internal interface between the C ABI (bm_api.hip) and the gfx950 kernels
+ * (bm_kernels.hip).  Device pointers only; every launcher enqueues on the
+ * given stream and returns the launch status.
+ */
+#ifndef BM_DEVICE_H
+#define BM_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+/* Graph and matching as seen by the kernels (passed by value). */
+struct BmDevGraph {
+    int32_t n;
+    const int64_t* m_out_ptr;   /* male CSR row offsets, n+2 */
+    const int2* m_out;          /* {dst, w} */
+    const int64_t* m_in_ptr;    /* male CSC column offsets, n+2 */
+    const int2* m_in;           /* {src, w} */
+    const uint64_t* f_out_desc; /* female out-row hash table descriptors, n+1 */
+    const int2* f_out_tab;
+    const uint64_t* f_in_desc;  /* female in-row hash table descriptors, n+1 */
+    const int2* f_in_tab;
+    const int32_t* pi;          /* matching, n+1 */
+};
+
+struct BmLaunch {
+    int32_t waves_per_block;   /* 1, 2, 4 or 8 */
+    int32_t lds_slots;         /* table slots per wave (8 bytes each) */
+    int32_t grid_blocks;       /* persistent grid size */
+};
+
+/* Candidate sources. */
+enum BmCandMode { BM_CAND_LIST = 0, BM_CAND_UNIFORM = 1 };
+
+hipError_t bm_launch_swap(const BmDevGraph& g, const BmLaunch& L, int mode, const int2* cand,
+                          uint64_t seed, int64_t k, int64_t* delta, hipStream_t s);
+hipError_t bm_launch_rot3(const BmDevGraph& g, const BmLaunch& L, const int4* cand, int64_t k,
+                          int64_t* best, uint8_t* perm, int4* cand_out_perm, hipStream_t s);
+hipError_t bm_launch_score(const BmDevGraph& g, const BmLaunch& L, int64_t* acc, hipStream_t s);
+
+hipError_t bm_launch_gen_pairs(int32_t n, uint64_t seed, int64_t k, int2* cand, hipStream_t s);
+hipError_t bm_launch_gen_triples(int32_t n, uint64_t seed, int64_t k, int4* cand, hipStream_t s);
+/* reorder: c changed ids ascending, then the rest ascending (n entries). */
+hipError_t bm_launch_gen_changed_pairs(int32_t n, int32_t c, const int32_t* reorder, int64_t P,
+                                       int2* cand, hipStream_t s);
+hipError_t bm_launch_gen_changed_triples(int32_t n, int32_t c, const int32_t* reorder, int64_t T,
+                                         int4* cand, hipStream_t s);
+
+/* Validate candidate ids on the device: sets *bad (int32, zeroed by the
+ * caller) to 1 if any id is outside 1..n. */
+hipError_t bm_launch_check_pairs(int32_t n, const int2* cand, int64_t k, int32_t* bad, hipStream_t s);
+hipError_t bm_launch_check_triples(int32_t n, const int4* cand, int64_t k, int32_t* bad, hipStream_t s);
+
+/* Reductions over int64 deltas.  part must hold bm_stats_parts(k) * 6 int64. */
+int64_t bm_stats_blocks(int64_t k);
+hipError_t bm_launch_stats(const int64_t* d, int64_t k, int64_t* part, int64_t* out6, hipStream_t s);
+/* rank-th index with delta > 0 (kind 1) or == 0 (kind 0); out = -1 if none.
+ * cnt must hold bm_stats_blocks(k) int64. */
+hipError_t bm_launch_pick(const int64_t* d, int64_t k, int32_t kind, int64_t rank, int64_t* cnt,
+                          int64_t* out, hipStream_t s);
+
+/* Disjoint acceptance.  Scratch sizes come from bm_accept_scratch_bytes. */
+size_t bm_accept_scratch_bytes(int64_t k);
+hipError_t bm_launch_accept(const BmDevGraph& g, int32_t* pi_rw, const int2* cand, const int64_t* d,
+                            int64_t k, int64_t max_accept, void* scratch, size_t scratch_bytes,
+                            int64_t* out_acc_idx /* k */, int64_t* out_meta /* [n_acc, gain] */,
+                            hipStream_t s);
+
+hipError_t bm_launch_apply_swap(int32_t* pi, int32_t u, int32_t v, hipStream_t s);
+hipError_t bm_launch_apply_perm3(int32_t* pi, int32_t a, int32_t b, int32_t c, int32_t p, hipStream_t s);
+
+#endif

@@ -1,0 +1,345 @@
+/*
+ * brainmatch.h -- C ABI of the MI355X (gfx950) exact swap-delta engine.
+ *
+ * Problem.  Two directed integer-weighted graphs on vertices 1..n, male M and
+ * female F, and a matching pi: 1..n -> 0..n (0 = unmatched).  The alignment
+ * score is
+ *     S(pi) = sum over male edges (u,v,w) of  c(w, pi(u), pi(v)),
+ *     c(w, a, b) = min(w, w_F(a,b)) if F has the edge a->b, else 0,
+ * which is what score-matching computes (src/score-matching.c:558-597).
+ * Every delta returned here is the exact integer S(pi') - S(pi), int64.
+ *
+ * Conventions for every entry point:
+ *   - return 0 (BM_OK) on success, a negative BM_ERR_* code on failure; the
+ *     message is then available from bm_last_error(ctx) (bm_last_error(NULL)
+ *     for a failed bm_create);
+ *   - host pointers are caller-owned and only read/written during the call;
+ *   - a context owns one HIP stream and its device buffers; it is not
+ *     thread-safe (one host thread per context), like the single host thread
+ *     that drives the reference kernels (src/graph-re-alignment-gpu-swap.cu:1002-1350);
+ *   - calls that return host data synchronise the context stream.
+ *
+ * The reference has no library boundary: its tools launch
+ * calculateSwapDeltasKernelQuick / calculateThreeNodeSwapKernel directly with
+ * cudaMemcpy before and after (src/graph-re-alignment-gpu-swap.cu:1164-1195,
+ * src/graph-re-alignment-gpu-3-swap.cu:1128-1195).  Each entry point below
+ * names the reference code it replaces.
+ */
+#ifndef BRAINMATCH_H
+#define BRAINMATCH_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BM_OK 0
+#define BM_ERR_ARG (-1)      /* invalid argument (range, size, null) */
+#define BM_ERR_HIP (-2)      /* HIP runtime failure */
+#define BM_ERR_NOMEM (-3)    /* host or device allocation failed */
+#define BM_ERR_IO (-4)       /* file could not be opened/parsed */
+#define BM_ERR_STATE (-5)    /* call not valid in the current state */
+#define BM_ERR_NODEV (-6)    /* no HIP device */
+
+#define BM_API_VERSION 1
+
+typedef struct bm_ctx bm_ctx;
+typedef struct bm_batch bm_batch;
+
+/* One graph as an edge list.  Vertex ids in 1..n.  Duplicate (src,dst) lines
+ * keep the LAST weight once, as score-matching's add_edge does
+ * (src/score-matching.c:339-359).  Weights are int32; INT32_MIN is rejected. */
+typedef struct bm_edges {
+    int64_t m;
+    const int32_t* src;
+    const int32_t* dst;
+    const int32_t* w;
+} bm_edges;
+
+/* Edge list owned by the library (from the CSV loader or the generator). */
+typedef struct bm_edge_list {
+    int64_t m;
+    int32_t n;          /* largest vertex id seen */
+    int32_t* src;
+    int32_t* dst;
+    int32_t* w;
+} bm_edge_list;
+
+/* ------------------------------------------------------------------ */
+/* Library / device                                                    */
+/* ------------------------------------------------------------------ */
+
+int bm_api_version(void);
+const char* bm_last_error(const bm_ctx* ctx);
+
+/* ------------------------------------------------------------------ */
+/* Files (host only; no device needed)                                 */
+/* ------------------------------------------------------------------ */
+
+/* Graph CSV: one header line, then "src,dst,weight" lines; ids may carry an
+ * 'm'/'f' prefix.  Replaces load_graph_from_csv
+ * (src/graph-re-alignment-gpu-swap.cu:1389-1433) with score-matching's parse
+ * semantics (src/score-matching.c:361-488). */
+int bm_load_graph_csv(const char* path, bm_edge_list* out);
+int bm_write_graph_csv(const char* path, const bm_edge_list* g);
+void bm_edge_list_free(bm_edge_list* g);
+
+/* Matching CSV: header, then "male,female" lines, generic or m/f-prefixed.
+ * pi must hold n+1 ints; unlisted vertices get 0.  Replaces
+ * load_benchmark_mapping (src/graph-re-alignment-gpu-swap.cu:699-746).
+ * Returns BM_ERR_ARG if an id exceeds n. */
+int bm_load_mapping_csv(const char* path, int32_t n, int32_t* pi, int64_t* n_lines);
+/* Writes "Male Node ID,Female Node ID" then "m<u>,f<pi[u]>" for pi[u] != 0.
+ * Replaces write_mapping (src/graph-re-alignment-gpu-swap.cu:681-696). */
+int bm_write_mapping_csv(const char* path, int32_t n, const int32_t* pi);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic workloads (host only)                                     */
+/* ------------------------------------------------------------------ */
+
+#define BM_WEIGHTS_GEOMETRIC 1   /* geometric(p=0.3)+1: failures-before-success + 1, >= 1 */
+#define BM_WEIGHTS_LOGNORMAL 2   /* max(1, round(exp(0.5 + 1.2 Z))) */
+
+typedef struct bm_synth_params {
+    int32_t n;              /* vertices per graph */
+    double mean_out_degree; /* target mean out-degree of the male graph */
+    double alpha;           /* power-law exponent of out-degrees (2.1) */
+    int32_t weights;        /* BM_WEIGHTS_* */
+    double modify_frac;     /* fraction of female edges modified (0.2) */
+    double shuffle_frac;    /* fraction of pi0 entries shuffled among themselves */
+    uint64_t seed;
+} bm_synth_params;
+
+/* Male graph, female graph (male relabelled by a hidden permutation P, then
+ * modify_frac of edges half reweighted / half rewired), the hidden P (n+1
+ * ints, P[0]=0) and pi0 = P with shuffle_frac of its entries shuffled.
+ * No self-loops, no duplicate (src,dst), weights >= 1. */
+int bm_synth_pair(const bm_synth_params* p, bm_edge_list* male, bm_edge_list* female,
+                  int32_t* perm_true, int32_t* pi0);
+
+/* Deterministic greedy initial matching in the manner of greedy-match
+ * (src/greedy-match.c:213-530): features (in, out, 2-hop reach, 2-hop
+ * co-reach), both sides sorted by feature total descending, then each male
+ * vertex takes the unmatched female vertex with the smallest L1 feature
+ * distance.  Ties are broken by the smaller vertex id (the reference's qsort
+ * leaves them to libc).  pi gets n+1 ints. */
+int bm_greedy_match(int32_t n, const bm_edges* male, const bm_edges* female, int32_t* pi);
+
+/* ------------------------------------------------------------------ */
+/* Context: graphs + matching resident in HBM                          */
+/* ------------------------------------------------------------------ */
+
+/* Builds male CSR/CSC and the per-row female hash tables, copies them to
+ * device `device`, and sets pi = identity-free zero matching.  Replaces the
+ * dense (n+1)^2 short matrices and their 1.37 GB H2D copy
+ * (src/graph-re-alignment-gpu-swap.cu:1040-1062). */
+int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm_edges* female,
+              int32_t device);
+void bm_destroy(bm_ctx* ctx);
+
+int32_t bm_num_vertices(const bm_ctx* ctx);
+int64_t bm_num_edges(const bm_ctx* ctx, int female);
+
+/* Per-vertex degree arrays of the male (female=0) or female (female=1)
+ * graph, n+1 entries each (index 0 = 0). */
+int bm_degrees(const bm_ctx* ctx, int female, int32_t* out_deg, int32_t* in_deg);
+
+/* Replace the matching: pi has n+1 entries, pi[0] ignored, values in 0..n.
+ * Replaces the per-iteration cudaMemcpy of the mapping
+ * (src/graph-re-alignment-gpu-swap.cu:1167). */
+int bm_set_mapping(bm_ctx* ctx, const int32_t* pi);
+int bm_get_mapping(bm_ctx* ctx, int32_t* pi);
+
+/* Full score of the current matching on the device.  Replaces
+ * calculate_alignment_score (src/graph-re-alignment-gpu-swap.cu:563-578);
+ * equals score-matching (src/score-matching.c:558-597) exactly, in int64. */
+int bm_score(bm_ctx* ctx, int64_t* score);
+
+/* ------------------------------------------------------------------ */
+/* Host-buffer convenience calls (upload, evaluate, download)          */
+/* ------------------------------------------------------------------ */
+
+/* delta[i] = S(pi o (u_i v_i)) - S(pi) for uv = {u0,v0,u1,v1,...}, k pairs;
+ * u == v gives 0.  Replaces calculate_swap_delta
+ * (src/graph-alignment-optimized.c:321-408) for a whole list. */
+int bm_swap_deltas(bm_ctx* ctx, const int32_t* uv, int64_t k, int64_t* delta);
+
+/* The reference's changed-first enumeration: with changed[0..c) (any order;
+ * sorted internally like sortArray, src/graph-re-alignment-gpu-swap.cu:296-307)
+ * and the remaining vertices ascending (fillArray, :983-999), the
+ * P = c(c-1)/2 + c(n-c) pairs of calculateSwapDeltasKernelQuick
+ * (:141-250) in its output order.  delta gets P entries, uv 2P (may be NULL).
+ * Returns P through *count. */
+int bm_swap_deltas_changed(bm_ctx* ctx, const int32_t* changed, int32_t c,
+                           int64_t* delta, int32_t* uv, int64_t* count);
+
+/* For each triple (a,b,c) of distinct vertices: the best of the five
+ * non-identity reassignments of (pi(a),pi(b),pi(c)), numbered as in
+ * calculateThreeNodeSwapKernel (src/graph-re-alignment-gpu-3-swap.cu:167-174):
+ *   1 (f1,f3,f2)  2 (f2,f1,f3)  3 (f2,f3,f1)  4 (f3,f1,f2)  5 (f3,f2,f1),
+ * best = max(0, max_p delta_p) with the first strictly greater p winning and
+ * perm 0 when none is positive (:176-177, :294-297). */
+int bm_rot3_deltas(bm_ctx* ctx, const int32_t* abc, int64_t k, int64_t* best, uint8_t* perm);
+
+/* The reference's 3-section triple enumeration (:67-150) over changed[0..c)
+ * (sorted internally) and the remaining vertices ascending:
+ * T = C(c,3) + C(c,2)(n-c) + c(n-c-1).  abcp gets (a,b,c,perm) per triple like
+ * node_triplets (may be NULL).  Returns T through *count. */
+int bm_rot3_deltas_changed(bm_ctx* ctx, const int32_t* changed, int32_t c,
+                           int64_t* best, int32_t* abcp, int64_t* count);
+
+/* Deterministic disjoint acceptance over a host candidate list: candidates
+ * with delta > 0 are taken in order (delta desc, index asc); one is accepted
+ * iff neither endpoint lies in the closed 1-hop neighbourhood (in- and
+ * out-edges of M) of an endpoint of an already accepted swap.  Accepted swaps
+ * are applied to the device matching and their deltas (which are additive by
+ * construction) summed.  accepted[] gets candidate indices in acceptance
+ * order (capacity k, may be NULL).  No reference counterpart is exact: the
+ * reference accepts one swap per launch with a full recount
+ * (src/graph-re-alignment-gpu-swap.cu:1220-1249). */
+int bm_accept_disjoint(bm_ctx* ctx, const int32_t* uv, const int64_t* delta, int64_t k,
+                       int32_t* accepted, int64_t* n_accepted, int64_t* gain);
+
+/* Apply one swap / one 3-permutation to the device matching
+ * (src/graph-re-alignment-gpu-3-swap.cu:925-956 for apply_permutation). */
+int bm_apply_swap(bm_ctx* ctx, int32_t u, int32_t v);
+int bm_apply_perm3(bm_ctx* ctx, int32_t a, int32_t b, int32_t c, int32_t perm);
+
+/* ------------------------------------------------------------------ */
+/* Device-resident batches (no host round trip per launch)             */
+/* ------------------------------------------------------------------ */
+
+typedef struct bm_stats {
+    int64_t count;      /* candidates in the batch */
+    int64_t sum;        /* sum of deltas (int64) */
+    int64_t max;        /* largest delta (INT64_MIN for an empty batch) */
+    int64_t argmax;     /* first index attaining max (-1 if empty) */
+    int64_t n_pos;      /* deltas > 0 */
+    int64_t n_zero;     /* deltas == 0 */
+} bm_stats;
+
+int bm_batch_create(bm_ctx* ctx, int64_t capacity, bm_batch** out);
+void bm_batch_destroy(bm_batch* b);
+int64_t bm_batch_size(const bm_batch* b);
+
+/* Fill the batch with candidates. */
+int bm_batch_set_pairs(bm_batch* b, const int32_t* uv, int64_t k);
+int bm_batch_set_triples(bm_batch* b, const int32_t* abc, int64_t k);
+int bm_batch_gen_pairs(bm_batch* b, uint64_t seed, int64_t k);      /* bm_uniform_pair stream */
+int bm_batch_gen_triples(bm_batch* b, uint64_t seed, int64_t k);    /* bm_uniform_triple stream */
+int bm_batch_gen_changed_pairs(bm_batch* b, const int32_t* changed, int32_t c);
+int bm_batch_gen_changed_triples(bm_batch* b, const int32_t* changed, int32_t c);
+
+/* Evaluate.  eval_swaps_uniform generates pair c of stream `seed` inside the
+ * delta kernel (nothing is read for the candidate) and writes the deltas. */
+int bm_batch_eval_swaps(bm_batch* b);
+int bm_batch_eval_swaps_uniform(bm_batch* b, uint64_t seed, int64_t k);
+int bm_batch_eval_rot3(bm_batch* b);
+
+/* Read back a slice [first, first+count).  Any output pointer may be NULL.
+ * cand gets 2 ints per pair batch / 4 ints (a,b,c,perm) per triple batch. */
+int bm_batch_get(bm_batch* b, int64_t first, int64_t count, int64_t* delta,
+                 uint8_t* perm, int32_t* cand);
+
+/* Device reductions over the batch's deltas. */
+int bm_batch_stats(bm_batch* b, bm_stats* out);
+/* Index of the rank-th (0-based, index order) candidate with delta > 0
+ * (kind = 1) or delta == 0 (kind = 0); -1 if there is none. */
+int bm_batch_pick(bm_batch* b, int32_t kind, int64_t rank, int64_t* index);
+/* Disjoint acceptance (as bm_accept_disjoint) over a pair batch, applied to
+ * the context matching.  max_accept <= 0 means unlimited. */
+int bm_batch_accept_disjoint(bm_batch* b, int64_t max_accept, int64_t* n_accepted, int64_t* gain);
+
+/* ------------------------------------------------------------------ */
+/* Measurement                                                         */
+/* ------------------------------------------------------------------ */
+
+#define BM_KERNEL_SWAP 0
+#define BM_KERNEL_ROT3 1
+#define BM_KERNEL_SCORE 2
+#define BM_KERNEL_COUNT 3
+
+/* When enabled, every delta/score launch is bracketed by HIP events on the
+ * context stream; bm_timing_get synchronises and returns the number of
+ * launches and their summed duration in milliseconds. */
+int bm_timing_enable(bm_ctx* ctx, int on);
+int bm_timing_get(bm_ctx* ctx, int32_t kernel, int64_t* launches, double* total_ms);
+int bm_timing_reset(bm_ctx* ctx);
+int bm_sync(bm_ctx* ctx);
+
+/* Algorithmic bytes of a batch (per-candidate byte model, summed exactly from
+ * the degree arrays): pairs 48 + 28*D(u,v) (+8 if read from memory), triples
+ * 69 + 36*D3 (+12 if read from memory). */
+int bm_batch_algo_bytes(bm_batch* b, int32_t generated, int64_t* bytes, int64_t* sum_degree);
+int bm_uniform_algo_bytes(bm_ctx* ctx, uint64_t seed, int64_t k, int64_t* bytes, int64_t* sum_degree);
+
+/* Launch-shape knobs (autotuner).  waves_per_block in {1,2,4,8},
+ * lds_slots per wave (table slots of 8 bytes), blocks_per_cu >= 1. */
+typedef struct bm_tuning {
+    int32_t waves_per_block;
+    int32_t lds_slots;
+    int32_t blocks_per_cu;
+    int32_t group_lanes;  /* lanes cooperating on one candidate: 64, 32 or 16 */
+} bm_tuning;
+int bm_get_tuning(const bm_ctx* ctx, int32_t kernel, bm_tuning* t);
+int bm_set_tuning(bm_ctx* ctx, int32_t kernel, const bm_tuning* t);
+
+/* Device information string ("gfx950, 256 CUs, ...") for logs. */
+int bm_device_info(bm_ctx* ctx, char* buf, size_t len);
+
+/* ------------------------------------------------------------------ */
+/* Search drivers (host loop over the device calls above)              */
+/* ------------------------------------------------------------------ */
+
+#define BM_SEARCH_SWAP_WEIGHTED 0  /* graph-re-alignment-gpu-swap: 100-101 score-weighted pairs */
+#define BM_SEARCH_SWAP_UNIFORM 1   /* graph-re-alignment-gpu: 3-5 uniform pairs */
+#define BM_SEARCH_ROT3 2           /* graph-re-alignment-gpu-3-swap: 3-5 uniform pairs, triples */
+
+typedef struct bm_search_opts {
+    int32_t mode;             /* BM_SEARCH_* */
+    uint64_t seed;
+    double time_limit_s;      /* <= 0: until *stop_flag or max_iters */
+    int64_t max_iters;        /* <= 0: unlimited (kernel launches, as the reference's iter) */
+    const char* out_path;     /* mapping written on every new best (NULL: none) */
+    const char* sync_path;    /* adopted if better every sync_interval_s (NULL: none) */
+    double sync_interval_s;
+    int64_t verify_every;     /* device recount every this many accepted moves (0: never) */
+    int32_t verbose;          /* 0 quiet, 1 [INFO] lines like the reference */
+    volatile int* stop_flag;  /* polled between launches; nonzero stops */
+    double restart_gap;       /* restart from best when (best-cur)/best exceeds this (0.002) */
+} bm_search_opts;
+
+typedef struct bm_search_result {
+    int64_t initial_score;
+    int64_t best_score;
+    int64_t final_score;      /* score of the matching left on the device */
+    int64_t iterations;       /* delta-kernel launches */
+    int64_t accepted;         /* moves applied (positive and zero) */
+    int64_t evaluations;      /* candidates evaluated */
+    int64_t verify_failures;  /* running score != device recount (must stay 0) */
+    double seconds;
+} bm_search_result;
+
+/* The perturb -> evaluate -> accept loop of optimize_mapping
+ * (src/graph-re-alignment-gpu-swap.cu:1002-1350, src/graph-re-alignment-gpu-3-swap.cu:960-1361,
+ * src/graph-re-alignment-gpu.cu:869), with the evaluation, the argmax and the
+ * random positive/zero picks on the device and the running score kept from
+ * the exact deltas instead of a full recount per move.  Starts from the
+ * context matching; best_pi (n+1 ints, may be NULL) receives the best. */
+int bm_search(bm_ctx* ctx, const bm_search_opts* opts, int32_t* best_pi, bm_search_result* res);
+
+/* Greedy disjoint-swap hill climbing: for r in 0..rounds-1, evaluate `batch`
+ * uniform pairs of stream (seed + r), accept a disjoint set (as
+ * bm_batch_accept_disjoint) and apply it.  score_trace (rounds+1 entries, may
+ * be NULL) gets the running score before round 0 and after each round; with
+ * verify != 0 every round is checked against a device recount. */
+int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int32_t verify,
+                 int64_t* score_trace, bm_search_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BRAINMATCH_H */

@@ -1,0 +1,302 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and of the reference code
+compiled into oracle/_ref.  TEST INFRASTRUCTURE ONLY: imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker, never
+by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+REF_LIB = os.path.join(REF_DIR, "libref_gao.so")
+REF_SCORE = os.path.join(REF_DIR, "score-matching")
+REF_GREEDY = os.path.join(REF_DIR, "greedy-match")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run make -C oracle")
+_o = C.CDLL(LIB_PATH)
+
+i32p = C.POINTER(C.c_int32)
+i64p = C.POINTER(C.c_int64)
+vp = C.c_void_p
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+_sig(_o, "or_build", vp, [C.c_int32, C.c_int64, i32p, i32p, i32p, C.c_int64, i32p, i32p, i32p])
+_sig(_o, "or_free", None, [vp])
+_sig(_o, "or_num_edges", C.c_int64, [vp, C.c_int])
+_sig(_o, "or_score", C.c_int64, [vp, i32p])
+_sig(_o, "or_swap_delta", C.c_int64, [vp, i32p, C.c_int32, C.c_int32])
+_sig(_o, "or_swap_deltas", None, [vp, i32p, i32p, C.c_int64, i64p])
+_sig(_o, "or_swap_delta_brute", C.c_int64, [vp, i32p, C.c_int32, C.c_int32])
+_sig(_o, "or_rot3_all", None, [vp, i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int, i64p])
+_sig(_o, "or_rot3", None, [vp, i32p, C.c_int32, C.c_int32, C.c_int32, i64p, i32p])
+_sig(_o, "or_rot3_many", None, [vp, i32p, i32p, C.c_int64, i64p, i32p])
+_sig(_o, "or_quick_enum", C.c_int64, [C.c_int32, i32p, C.c_int32, i32p])
+_sig(_o, "or_three_enum", C.c_int64, [C.c_int32, i32p, C.c_int32, i32p])
+_sig(_o, "or_dense_build", vp, [C.c_int32, C.c_int64, i32p, i32p, i32p, C.c_int64, i32p, i32p, i32p])
+_sig(_o, "or_dense_free", None, [vp])
+_sig(_o, "or_ref_dense_swap", C.c_int32, [vp, i32p, C.c_int32, C.c_int32])
+_sig(_o, "or_ref_dense_quick", C.c_int64, [vp, i32p, i32p, C.c_int32, i32p, i32p])
+_sig(_o, "or_ref_dense_three", None, [vp, i32p, C.c_int32, C.c_int32, C.c_int32, i32p, i32p])
+_sig(_o, "or_ref_dense_three_launch", C.c_int64, [vp, i32p, i32p, C.c_int32, i32p, i32p])
+_sig(_o, "or_lists_build", vp, [C.c_int32, C.c_int64, i32p, i32p, i32p, C.c_int64, i32p, i32p, i32p])
+_sig(_o, "or_lists_free", None, [vp])
+_sig(_o, "or_ref_lists_delta", C.c_int32, [vp, i32p, C.c_int32, C.c_int32])
+_sig(_o, "or_ref_lists_deltas", None, [vp, i32p, i32p, C.c_int64, i32p, C.c_int])
+_sig(_o, "or_max_threads", C.c_int, [])
+_sig(_o, "or_uniform_pairs", None, [C.c_uint64, C.c_int64, C.c_int32, i32p])
+_sig(_o, "or_accept_disjoint", C.c_int64, [vp, i32p, i32p, i64p, C.c_int64, C.c_int64, i64p, i64p])
+_sig(_o, "or_hillclimb", None, [vp, i32p, C.c_uint64, C.c_int64, C.c_int64, i64p])
+
+
+def _a32(x):
+    return np.ascontiguousarray(x, dtype=np.int32)
+
+
+def _p32(a):
+    return a.ctypes.data_as(i32p)
+
+
+def _p64(a):
+    return a.ctypes.data_as(i64p)
+
+
+def _edges(g):
+    s, d, w = _a32(g.src), _a32(g.dst), _a32(g.w)
+    return s, d, w
+
+
+class Oracle:
+    """Exact score / deltas on the CPU (score-matching semantics)."""
+
+    def __init__(self, n, male, female):
+        self.n = int(n)
+        self._keep = _edges(male) + _edges(female)
+        ms, md, mw, fs, fd, fw = self._keep
+        self.h = _o.or_build(self.n, ms.shape[0], _p32(ms), _p32(md), _p32(mw), fs.shape[0], _p32(fs), _p32(fd),
+                             _p32(fw))
+        if not self.h:
+            raise ValueError("oracle: invalid graph")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _o.or_free(self.h)
+            self.h = None
+
+    def num_edges(self, female=False):
+        return _o.or_num_edges(self.h, 1 if female else 0)
+
+    def score(self, pi):
+        pi = _a32(pi)
+        return _o.or_score(self.h, _p32(pi))
+
+    def swap_delta(self, pi, u, v):
+        pi = _a32(pi)
+        return _o.or_swap_delta(self.h, _p32(pi), u, v)
+
+    def swap_deltas(self, pi, uv):
+        pi, uv = _a32(pi), _a32(uv).reshape(-1, 2)
+        out = np.empty(uv.shape[0], np.int64)
+        _o.or_swap_deltas(self.h, _p32(pi), _p32(uv), uv.shape[0], _p64(out))
+        return out
+
+    def swap_delta_brute(self, pi, u, v):
+        pi = _a32(pi)
+        return _o.or_swap_delta_brute(self.h, _p32(pi), u, v)
+
+    def rot3_all(self, pi, a, b, c, brute=False):
+        pi = _a32(pi)
+        d5 = np.empty(5, np.int64)
+        _o.or_rot3_all(self.h, _p32(pi), a, b, c, 1 if brute else 0, _p64(d5))
+        return d5
+
+    def rot3_many(self, pi, abc):
+        pi, abc = _a32(pi), _a32(abc).reshape(-1, 3)
+        best = np.empty(abc.shape[0], np.int64)
+        perm = np.empty(abc.shape[0], np.int32)
+        _o.or_rot3_many(self.h, _p32(pi), _p32(abc), abc.shape[0], _p64(best), _p32(perm))
+        return best, perm
+
+    def accept_disjoint(self, pi, uv, d, max_accept=0):
+        pi = _a32(pi).copy()
+        uv = _a32(uv).reshape(-1, 2)
+        d = np.ascontiguousarray(d, dtype=np.int64)
+        acc = np.empty(max(1, uv.shape[0]), np.int64)
+        gain = C.c_int64(0)
+        na = _o.or_accept_disjoint(self.h, _p32(pi), _p32(uv), _p64(d), uv.shape[0], max_accept, _p64(acc),
+                                   C.byref(gain))
+        return pi, acc[:na].copy(), gain.value
+
+    def hillclimb(self, pi, seed, rounds, batch):
+        pi = _a32(pi).copy()
+        trace = np.empty(rounds + 1, np.int64)
+        _o.or_hillclimb(self.h, _p32(pi), seed, rounds, batch, _p64(trace))
+        return pi, trace
+
+
+def uniform_pairs(seed, k, n):
+    uv = np.empty((k, 2), np.int32)
+    _o.or_uniform_pairs(seed, k, n, _p32(uv))
+    return uv
+
+
+def quick_enum(n, changed):
+    ch = _a32(changed)
+    c = ch.shape[0]
+    P = c * (c - 1) // 2 + c * (n - c)
+    uv = np.empty((P, 2), np.int32)
+    _o.or_quick_enum(n, _p32(ch), c, _p32(uv))
+    return uv
+
+
+def three_enum(n, changed):
+    ch = _a32(changed)
+    c = ch.shape[0]
+    T = c * (c - 1) * (c - 2) // 6 + c * (c - 1) // 2 * (n - c) + (c * (n - c - 1) if n - c - 1 > 0 else 0)
+    abc = np.empty((T, 3), np.int32)
+    _o.or_three_enum(n, _p32(ch), c, _p32(abc))
+    return abc
+
+
+class DenseRef:
+    """The reference kernels' arithmetic on dense int16 matrices."""
+
+    def __init__(self, n, male, female):
+        self.n = int(n)
+        self._keep = _edges(male) + _edges(female)
+        ms, md, mw, fs, fd, fw = self._keep
+        self.h = _o.or_dense_build(self.n, ms.shape[0], _p32(ms), _p32(md), _p32(mw), fs.shape[0], _p32(fs),
+                                   _p32(fd), _p32(fw))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _o.or_dense_free(self.h)
+            self.h = None
+
+    def swap(self, pi, u, v):
+        pi = _a32(pi)
+        return _o.or_ref_dense_swap(self.h, _p32(pi), u, v)
+
+    def quick(self, pi, changed):
+        pi, ch = _a32(pi), _a32(changed)
+        c = ch.shape[0]
+        P = c * (c - 1) // 2 + c * (self.n - c)
+        d = np.empty(P, np.int32)
+        uv = np.empty((P, 2), np.int32)
+        _o.or_ref_dense_quick(self.h, _p32(pi), _p32(ch), c, _p32(d), _p32(uv))
+        return d, uv
+
+    def three(self, pi, a, b, c):
+        pi = _a32(pi)
+        best, p = C.c_int32(0), C.c_int32(0)
+        _o.or_ref_dense_three(self.h, _p32(pi), a, b, c, C.byref(best), C.byref(p))
+        return best.value, p.value
+
+    def three_launch(self, pi, changed):
+        pi, ch = _a32(pi), _a32(changed)
+        c, n = ch.shape[0], self.n
+        T = c * (c - 1) * (c - 2) // 6 + c * (c - 1) // 2 * (n - c) + (c * (n - c - 1) if n - c - 1 > 0 else 0)
+        d = np.empty(T, np.int32)
+        t4 = np.empty((T, 4), np.int32)
+        _o.or_ref_dense_three_launch(self.h, _p32(pi), _p32(ch), c, _p32(d), _p32(t4))
+        return d, t4
+
+
+class ListsRef:
+    """calculate_swap_delta's arithmetic (edge lists + dense female matrix)."""
+
+    def __init__(self, n, male, female):
+        self.n = int(n)
+        self._keep = _edges(male) + _edges(female)
+        ms, md, mw, fs, fd, fw = self._keep
+        self.h = _o.or_lists_build(self.n, ms.shape[0], _p32(ms), _p32(md), _p32(mw), fs.shape[0], _p32(fs),
+                                   _p32(fd), _p32(fw))
+        if not self.h:
+            raise MemoryError("oracle lists build failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _o.or_lists_free(self.h)
+            self.h = None
+
+    def delta(self, pi, u, v):
+        pi = _a32(pi)
+        return _o.or_ref_lists_delta(self.h, _p32(pi), u, v)
+
+    def deltas(self, pi, uv, threads=0):
+        pi, uv = _a32(pi), _a32(uv).reshape(-1, 2)
+        out = np.empty(uv.shape[0], np.int32)
+        _o.or_ref_lists_deltas(self.h, _p32(pi), _p32(uv), uv.shape[0], _p32(out), threads)
+        return out
+
+
+def max_threads():
+    return _o.or_max_threads()
+
+
+# ---------------------------------------------------------------------------
+# oracle/_ref: the reference's own code, compiled from its sources here
+# ---------------------------------------------------------------------------
+
+def ref_available():
+    return os.path.exists(REF_LIB) and os.path.exists(REF_SCORE)
+
+
+class RefCode:
+    """calculate_swap_delta / calculate_alignment_score of
+    src/graph-alignment-optimized.c, compiled unmodified (ids <= 18524)."""
+
+    def __init__(self, male, female):
+        self.lib = C.CDLL(REF_LIB)
+        _sig(self.lib, "refh_build", vp, [C.c_int64, i32p, i32p, i32p])
+        _sig(self.lib, "refh_free", None, [vp])
+        _sig(self.lib, "refh_score", C.c_int32, [vp, vp, i32p, C.c_int32])
+        _sig(self.lib, "refh_deltas", None, [vp, vp, i32p, C.c_int32, i32p, C.c_int64, i32p, C.c_int])
+        _sig(self.lib, "refh_num_nodes", C.c_int, [])
+        self.cap = self.lib.refh_num_nodes()
+        self._keep = _edges(male) + _edges(female)
+        ms, md, mw, fs, fd, fw = self._keep
+        self.gm = self.lib.refh_build(ms.shape[0], _p32(ms), _p32(md), _p32(mw))
+        self.gf = self.lib.refh_build(fs.shape[0], _p32(fs), _p32(fd), _p32(fw))
+
+    def close(self):
+        if getattr(self, "gm", None):
+            self.lib.refh_free(self.gm)
+            self.lib.refh_free(self.gf)
+            self.gm = self.gf = None
+
+    def __del__(self):
+        self.close()
+
+    def score(self, pi):
+        pi = _a32(pi)
+        return self.lib.refh_score(self.gm, self.gf, _p32(pi), pi.shape[0] - 1)
+
+    def deltas(self, pi, uv, threads=0):
+        pi, uv = _a32(pi), _a32(uv).reshape(-1, 2)
+        out = np.empty(uv.shape[0], np.int32)
+        self.lib.refh_deltas(self.gm, self.gf, _p32(pi), pi.shape[0] - 1, _p32(uv), uv.shape[0], _p32(out), threads)
+        return out
+
+
+def ref_score_matching(g1_csv, g2_csv, matching_csv):
+    """Runs the compiled score-matching and returns its integer score (the
+    only stdout line that is a bare integer, src/score-matching.c:685)."""
+    r = subprocess.run([REF_SCORE, g1_csv, g2_csv, matching_csv], capture_output=True, text=True, check=True)
+    for line in r.stdout.splitlines():
+        s = line.strip()
+        if s.lstrip("-").isdigit():
+            return int(s)
+    raise RuntimeError("score-matching printed no score:\n" + r.stdout[-2000:])
