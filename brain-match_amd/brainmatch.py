@@ -60,7 +60,7 @@ class _Stats(C.Structure):
 
 class _Tuning(C.Structure):
     _fields_ = [("waves_per_block", C.c_int32), ("lds_slots", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("group_lanes", C.c_int32)]
+                ("group_lanes", C.c_int32), ("unroll", C.c_int32)]
 
 
 class _SearchOpts(C.Structure):
@@ -462,8 +462,8 @@ class Context:
         self._chk(_lib.bm_get_tuning(self.h, kernel, C.byref(t)))
         return {k: getattr(t, k) for k, _ in _Tuning._fields_}
 
-    def set_tuning(self, kernel, waves_per_block, lds_slots, blocks_per_cu, group_lanes=64):
-        t = _Tuning(waves_per_block, lds_slots, blocks_per_cu, group_lanes)
+    def set_tuning(self, kernel, waves_per_block, lds_slots, blocks_per_cu, group_lanes=64, unroll=8):
+        t = _Tuning(waves_per_block, lds_slots, blocks_per_cu, group_lanes, unroll)
         self._chk(_lib.bm_set_tuning(self.h, kernel, C.byref(t)))
 
     def device_info(self):

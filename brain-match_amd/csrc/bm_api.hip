@@ -42,6 +42,11 @@ struct bm_ctx {
     int2* d_f_out_tab;
     uint64_t* d_f_in_desc;
     int2* d_f_in_tab;
+    int64_t* d_f_out_ptr;   /* female CSR / CSC rows */
+    int2* d_f_out;
+    int64_t* d_f_in_ptr;
+    int2* d_f_in;
+    int32_t compact;
     int32_t* d_pi;
     int64_t* d_small;       /* 16 int64 of scratch for scores / flags */
     int32_t* d_reorder;     /* n ints for the changed-first enumerations */
@@ -107,7 +112,12 @@ static BmDevGraph devgraph(const bm_ctx* c) {
     g.f_out_tab = c->d_f_out_tab;
     g.f_in_desc = c->d_f_in_desc;
     g.f_in_tab = c->d_f_in_tab;
+    g.f_out_ptr = c->d_f_out_ptr;
+    g.f_out = c->d_f_out;
+    g.f_in_ptr = c->d_f_in_ptr;
+    g.f_in = c->d_f_in;
     g.pi = c->d_pi;
+    g.compact = c->compact;
     return g;
 }
 
@@ -182,6 +192,10 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_f_out_tab);
     (void)hipFree(c->d_f_in_desc);
     (void)hipFree(c->d_f_in_tab);
+    (void)hipFree(c->d_f_out_ptr);
+    (void)hipFree(c->d_f_out);
+    (void)hipFree(c->d_f_in_ptr);
+    (void)hipFree(c->d_f_in);
     (void)hipFree(c->d_pi);
     (void)hipFree(c->d_small);
     (void)hipFree(c->d_reorder);
@@ -208,9 +222,11 @@ extern "C" void bm_destroy(bm_ctx* ctx) {
 
 static void default_tuning(bm_ctx* c) {
     const int cus = c->num_cus > 0 ? c->num_cus : 256;
-    c->tune[BM_KERNEL_SWAP] = BmLaunch{4, 2048, cus * 2};
-    c->tune[BM_KERNEL_ROT3] = BmLaunch{4, 2048, cus * 2};
-    c->tune[BM_KERNEL_SCORE] = BmLaunch{4, 1024, cus * 4};
+    /* slots per wave; 32 waves per CU fit 160 KiB when slots * slot bytes <= 5 KiB */
+    const int s = c->compact ? 1280 : 640;
+    c->tune[BM_KERNEL_SWAP] = BmLaunch{4, s, 8, cus, 8};
+    c->tune[BM_KERNEL_ROT3] = BmLaunch{4, s, 8, cus, 8};
+    c->tune[BM_KERNEL_SCORE] = BmLaunch{4, s, 8, cus, 8};
     for (int i = 0; i < BM_KERNEL_COUNT; i++) c->group_lanes[i] = 64;
 }
 
@@ -234,7 +250,6 @@ extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm
         c->num_cus = prop.multiProcessorCount;
         snprintf(c->arch, sizeof c->arch, "%s", prop.gcnArchName);
     }
-    default_tuning(c);
     he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (he != hipSuccess) { free_ctx(c); return fail(nullptr, BM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(he)); }
 
@@ -258,11 +273,15 @@ extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm
         free_ctx(c);
         return fail(nullptr, BM_ERR_NOMEM, "bm_create: out of host memory building female tables");
     }
+    /* Compact LDS slots need keys below 2^17 and female weights in 0..32767. */
+    c->compact = n < (1 << 17);
     for (int32_t r = 1; r <= n; r++) {
         c->f_out_deg[r] = (int32_t)(fg.out.ptr[r + 1] - fg.out.ptr[r]);
         c->f_in_deg[r] = (int32_t)(fg.in.ptr[r + 1] - fg.in.ptr[r]);
     }
-    bm_graph_free(&fg);
+    for (int64_t e = 0; e < fg.m && c->compact; e++)
+        if (fg.out.adj[e].y < 0 || fg.out.adj[e].y > 32767) c->compact = 0;
+    default_tuning(c);
 
     const bm_graph_host& mg = c->male;
     rc = BM_OK;
@@ -274,6 +293,10 @@ extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm
     if (rc == BM_OK) rc = dalloc(c, &c->d_f_out_tab, (size_t)(ho.slots ? ho.slots : 2));
     if (rc == BM_OK) rc = dalloc(c, &c->d_f_in_desc, (size_t)n + 1);
     if (rc == BM_OK) rc = dalloc(c, &c->d_f_in_tab, (size_t)(hi.slots ? hi.slots : 2));
+    if (rc == BM_OK) rc = dalloc(c, &c->d_f_out_ptr, (size_t)n + 2);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_f_out, (size_t)fg.m);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_f_in_ptr, (size_t)n + 2);
+    if (rc == BM_OK) rc = dalloc(c, &c->d_f_in, (size_t)fg.m);
     if (rc == BM_OK) rc = dalloc(c, &c->d_pi, (size_t)n + 1);
     if (rc == BM_OK) rc = dalloc(c, &c->d_small, 16);
     if (rc == BM_OK) rc = dalloc(c, &c->d_reorder, (size_t)n);
@@ -285,11 +308,16 @@ extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm
     if (rc == BM_OK) rc = h2d(c, c->d_f_out_tab, ho.tab, (size_t)ho.slots);
     if (rc == BM_OK) rc = h2d(c, c->d_f_in_desc, hi.desc, (size_t)n + 1);
     if (rc == BM_OK) rc = h2d(c, c->d_f_in_tab, hi.tab, (size_t)hi.slots);
+    if (rc == BM_OK) rc = h2d(c, c->d_f_out_ptr, fg.out.ptr, (size_t)n + 2);
+    if (rc == BM_OK) rc = h2d(c, c->d_f_out, fg.out.adj, (size_t)fg.m);
+    if (rc == BM_OK) rc = h2d(c, c->d_f_in_ptr, fg.in.ptr, (size_t)n + 2);
+    if (rc == BM_OK) rc = h2d(c, c->d_f_in, fg.in.adj, (size_t)fg.m);
     if (rc == BM_OK && hipMemsetAsync(c->d_pi, 0, ((size_t)n + 1) * sizeof(int32_t), c->stream) != hipSuccess)
         rc = fail(c, BM_ERR_HIP, "hipMemsetAsync failed");
     if (rc == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, BM_ERR_HIP, "hipStreamSynchronize failed");
     bm_rowhash_free(&ho);
     bm_rowhash_free(&hi);
+    bm_graph_free(&fg);
     if (rc != BM_OK) {
         bm_set_global_error("bm_create: %s", c->err);
         free_ctx(c);
@@ -813,26 +841,13 @@ extern "C" int bm_batch_algo_bytes(bm_batch* b, int32_t generated, int64_t* byte
     return BM_OK;
 }
 
-extern "C" int bm_uniform_algo_bytes(bm_ctx* c, uint64_t seed, int64_t k, int64_t* bytes, int64_t* sum_degree) {
-    if (!c || k < 0) return BM_ERR_ARG;
-    int64_t D = 0;
-    for (int64_t i = 0; i < k; i++) {
-        int32_t u, v;
-        bm_uniform_pair(seed, (uint64_t)i, c->n, &u, &v);
-        D += vdeg(c->male, u) + vdeg(c->male, v);
-    }
-    if (bytes) *bytes = 48 * k + 28 * D;
-    if (sum_degree) *sum_degree = D;
-    return BM_OK;
-}
-
 extern "C" int bm_get_tuning(const bm_ctx* c, int32_t kernel, bm_tuning* t) {
     if (!c || !t || kernel < 0 || kernel >= BM_KERNEL_COUNT) return BM_ERR_ARG;
-    const int cus = c->num_cus > 0 ? c->num_cus : 256;
     t->waves_per_block = c->tune[kernel].waves_per_block;
     t->lds_slots = c->tune[kernel].lds_slots;
-    t->blocks_per_cu = c->tune[kernel].grid_blocks / cus;
+    t->blocks_per_cu = c->tune[kernel].blocks_per_cu;
     t->group_lanes = c->group_lanes[kernel];
+    t->unroll = c->tune[kernel].unroll;
     return BM_OK;
 }
 
@@ -840,12 +855,14 @@ extern "C" int bm_set_tuning(bm_ctx* c, int32_t kernel, const bm_tuning* t) {
     if (!c || !t || kernel < 0 || kernel >= BM_KERNEL_COUNT) return BM_ERR_ARG;
     const int w = t->waves_per_block;
     if (w != 1 && w != 2 && w != 4 && w != 8) return fail(c, BM_ERR_ARG, "bm_set_tuning: waves_per_block must be 1,2,4,8");
-    if (t->lds_slots < 2 || (t->lds_slots & 1) || (size_t)w * (size_t)t->lds_slots * 8 > 160 * 1024)
-        return fail(c, BM_ERR_ARG, "bm_set_tuning: lds_slots must be even and waves*slots*8 <= 160 KiB");
+    const size_t sb = c->compact ? 4 : 8;
+    if (t->lds_slots < 4 || (t->lds_slots & 3) || (size_t)w * (size_t)t->lds_slots * sb > 160 * 1024)
+        return fail(c, BM_ERR_ARG, "bm_set_tuning: lds_slots must be a multiple of 4 and waves*slots*slot bytes <= 160 KiB");
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(c, BM_ERR_ARG, "bm_set_tuning: blocks_per_cu in 1..64");
     if (t->group_lanes != 64) return fail(c, BM_ERR_ARG, "bm_set_tuning: group_lanes other than 64 not built");
     const int cus = c->num_cus > 0 ? c->num_cus : 256;
-    c->tune[kernel] = BmLaunch{w, t->lds_slots, t->blocks_per_cu * cus};
+    if (t->unroll != 2 && t->unroll != 4 && t->unroll != 8) return fail(c, BM_ERR_ARG, "bm_set_tuning: unroll must be 2, 4 or 8");
+    c->tune[kernel] = BmLaunch{w, t->lds_slots, t->blocks_per_cu, cus, t->unroll};
     c->group_lanes[kernel] = t->group_lanes;
     return BM_OK;
 }

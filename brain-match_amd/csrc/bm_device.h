@@ -20,13 +20,23 @@ struct BmDevGraph {
     const int2* f_out_tab;
     const uint64_t* f_in_desc;  /* female in-row hash table descriptors, n+1 */
     const int2* f_in_tab;
+    const int64_t* f_out_ptr;   /* female CSR (sorted rows), n+2 */
+    const int2* f_out;          /* {dst, w} */
+    const int64_t* f_in_ptr;    /* female CSC, n+2 */
+    const int2* f_in;           /* {src, w} */
     const int32_t* pi;          /* matching, n+1 */
+    int32_t compact;            /* 1: LDS slots are (key << 15) | w (n < 2^17, weights 0..32767) */
 };
+
+/* Bytes per LDS table slot for this graph (4 compact, 8 otherwise). */
+size_t bm_slot_bytes(const BmDevGraph& g);
 
 struct BmLaunch {
     int32_t waves_per_block;   /* 1, 2, 4 or 8 */
     int32_t lds_slots;         /* table slots per wave (8 bytes each) */
-    int32_t grid_blocks;       /* persistent grid size */
+    int32_t blocks_per_cu;     /* persistent grid: blocks per CU (capped by occupancy) */
+    int32_t num_cus;
+    int32_t unroll;            /* list entries per lane in flight (2, 4 or 8) */
 };
 
 /* Candidate sources. */

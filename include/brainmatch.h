@@ -276,12 +276,15 @@ int bm_batch_algo_bytes(bm_batch* b, int32_t generated, int64_t* bytes, int64_t*
 int bm_uniform_algo_bytes(bm_ctx* ctx, uint64_t seed, int64_t k, int64_t* bytes, int64_t* sum_degree);
 
 /* Launch-shape knobs (autotuner).  waves_per_block in {1,2,4,8},
- * lds_slots per wave (table slots of 8 bytes), blocks_per_cu >= 1. */
+ * lds_slots per wave (table slots of 4 bytes for graphs with n < 2^17 and
+ * female weights in 0..32767, else 8 bytes; a multiple of 4), blocks_per_cu
+ * >= 1 (capped at launch by the occupancy the kernel reaches). */
 typedef struct bm_tuning {
     int32_t waves_per_block;
     int32_t lds_slots;
     int32_t blocks_per_cu;
-    int32_t group_lanes;  /* lanes cooperating on one candidate: 64, 32 or 16 */
+    int32_t group_lanes;  /* lanes cooperating on one candidate (64) */
+    int32_t unroll;       /* list entries per lane in flight: 2, 4 or 8 (swap kernel) */
 } bm_tuning;
 int bm_get_tuning(const bm_ctx* ctx, int32_t kernel, bm_tuning* t);
 int bm_set_tuning(bm_ctx* ctx, int32_t kernel, const bm_tuning* t);

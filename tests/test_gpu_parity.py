@@ -177,7 +177,7 @@ def test_global_table_path_matches(c1, bm, orc):
     saved = {k: ctx.get_tuning(k) for k in (bm.KERNEL_SWAP, bm.KERNEL_ROT3, bm.KERNEL_SCORE)}
     try:
         for k in saved:
-            ctx.set_tuning(k, 4, 2, 1)
+            ctx.set_tuning(k, 4, 4, 1)
         uv = orc.uniform_pairs(6, 8192, n)
         assert np.array_equal(ctx.swap_deltas(uv), o.swap_deltas(pi, uv))
         assert ctx.score() == o.score(pi)

@@ -1,0 +1,216 @@
+"""CPU checks that pin the oracle: brute force, closed forms, the reference's
+own compiled code (oracle/_ref) and the committed golden vectors.  No GPU."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import make_pair, quirky_pair
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_local_swap_delta_equals_brute_force(bm, orc):
+    male, female, pi = quirky_pair(bm)
+    o = orc.Oracle(60, male, female)
+    for u in range(1, 61):
+        for v in range(1, 61):
+            assert o.swap_delta(pi, u, v) == o.swap_delta_brute(pi, u, v)
+
+
+def test_swap_delta_antisymmetry_and_symmetry(bm, orc):
+    male, female, truth, pi0 = make_pair(bm)
+    o = orc.Oracle(300, male, female)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        u, v = (int(x) for x in rng.choice(np.arange(1, 301), 2, replace=False))
+        d = o.swap_delta(pi0, u, v)
+        assert d == o.swap_delta(pi0, v, u)
+        q = pi0.copy()
+        q[u], q[v] = q[v], q[u]
+        assert o.swap_delta(q, u, v) == -d
+        assert o.score(q) - o.score(pi0) == d
+
+
+def test_rot3_local_equals_brute_force(bm, orc):
+    male, female, pi = quirky_pair(bm, seed=9)
+    o = orc.Oracle(60, male, female)
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        a, b, c = (int(x) for x in rng.choice(np.arange(1, 61), 3, replace=False))
+        assert np.array_equal(o.rot3_all(pi, a, b, c), o.rot3_all(pi, a, b, c, brute=True))
+
+
+def test_rot3_cycle_cubed_is_identity(bm, orc):
+    """p=3 and p=4 are the two 3-cycles: applying one three times restores pi,
+    and the three deltas along the way sum to zero."""
+    male, female, truth, pi0 = make_pair(bm, seed=4)
+    o = orc.Oracle(300, male, female)
+    a, b, c = 10, 20, 30
+    q = pi0.copy()
+    total = 0
+    for _ in range(3):
+        total += o.rot3_all(q, a, b, c)[2]
+        f = (q[a], q[b], q[c])
+        q[a], q[b], q[c] = f[1], f[2], f[0]
+    assert np.array_equal(q, pi0) and total == 0
+
+
+def test_score_closed_forms(bm, orc):
+    male, female, truth, pi0 = make_pair(bm, seed=6)
+    n = 300
+    ident = np.arange(n + 1, dtype=np.int32)
+    assert orc.Oracle(n, male, male).score(ident) == int(male.w.sum())
+    assert orc.Oracle(n, male, female).score(np.zeros(n + 1, np.int32)) == 0
+    empty = bm.Graph(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), n)
+    assert orc.Oracle(n, empty, female).score(pi0) == 0
+    # unrelabelled female with the true permutation: every unmodified edge scores fully
+    o = orc.Oracle(n, male, female)
+    assert o.score(truth) > o.score(pi0)
+
+
+def test_duplicate_lines_last_weight_wins(bm, orc):
+    m = bm.Graph(np.array([1, 1, 2], np.int32), np.array([2, 2, 3], np.int32), np.array([5, 2, 4], np.int32), 3)
+    f = bm.Graph(np.array([1, 2, 2], np.int32), np.array([2, 3, 3], np.int32), np.array([9, 1, 7], np.int32), 3)
+    o = orc.Oracle(3, m, f)
+    assert o.num_edges() == 2 and o.num_edges(female=True) == 2
+    assert o.score(np.array([0, 1, 2, 3], np.int32)) == min(2, 9) + min(4, 7)
+
+
+def test_handmade_known_answer(bm, orc):
+    """M = {1->2:3, 2->3:5, 3->1:2, 1->3:4}, F = {1->2:1, 2->3:7, 3->1:2, 2->1:4}:
+    identity scores 1+5+2+0 = 8; swapping 1 and 2 scores 7 (delta -1)."""
+    m = bm.Graph(np.array([1, 2, 3, 1], np.int32), np.array([2, 3, 1, 3], np.int32), np.array([3, 5, 2, 4], np.int32), 3)
+    f = bm.Graph(np.array([1, 2, 3, 2], np.int32), np.array([2, 3, 1, 1], np.int32), np.array([1, 7, 2, 4], np.int32), 3)
+    o = orc.Oracle(3, m, f)
+    ident = np.array([0, 1, 2, 3], np.int32)
+    assert o.score(ident) == 8
+    assert o.swap_delta(ident, 1, 2) == -1
+    assert o.score(np.array([0, 2, 1, 3], np.int32)) == 7
+
+
+def test_enumerations_are_complete_and_ordered(orc):
+    n = 40
+    changed = np.array([17, 3, 25, 9], np.int32)
+    uv = orc.quick_enum(n, changed)
+    c = len(changed)
+    assert uv.shape[0] == c * (c - 1) // 2 + c * (n - c)
+    s = sorted(changed)
+    assert [tuple(x) for x in uv[:6]] == [(s[0], s[1]), (s[0], s[2]), (s[0], s[3]), (s[1], s[2]), (s[1], s[3]),
+                                           (s[2], s[3])]
+    rest = [x for x in range(1, n + 1) if x not in s]
+    assert list(uv[6:6 + len(rest), 1]) == rest and (uv[6:6 + len(rest), 0] == s[0]).all()
+    abc = orc.three_enum(n, changed)
+    T = 4 + 6 * (n - c) + c * (n - c - 1)
+    assert abc.shape[0] == T
+    assert len({tuple(sorted(t)) for t in abc[:4]}) == 4
+    # third section: changed vertex with two adjacent unchanged ones
+    third = abc[4 + 6 * (n - c):]
+    assert (np.searchsorted(rest, third[:, 2]) == np.searchsorted(rest, third[:, 1]) + 1).all()
+
+
+def test_dense_restatement_matches_exact_without_quirks(bm, orc):
+    male, female, truth, pi0 = make_pair(bm, seed=8)
+    n = 300
+    o, D = orc.Oracle(n, male, female), orc.DenseRef(n, male, female)
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        u, v = (int(x) for x in rng.choice(np.arange(1, n + 1), 2, replace=False))
+        assert D.swap(pi0, u, v) == o.swap_delta(pi0, u, v)
+    for _ in range(100):
+        a, b, c = (int(x) for x in rng.choice(np.arange(1, n + 1), 3, replace=False))
+        best, p = D.three(pi0, a, b, c)
+        d5 = o.rot3_all(pi0, a, b, c)
+        assert best == max(0, d5.max())
+        assert p == (0 if best == 0 else int(np.argmax(d5)) + 1)
+
+
+def test_dense_restatement_skips_self_loops(bm, orc):
+    """The reference kernels skip i == m1 (src/graph-re-alignment-gpu-swap.cu:199),
+    so a self-loop's change is missing from their delta; the exact delta has it."""
+    m = bm.Graph(np.array([1, 1], np.int32), np.array([1, 2], np.int32), np.array([5, 1], np.int32), 3)
+    f = bm.Graph(np.array([2, 1], np.int32), np.array([2, 2], np.int32), np.array([5, 1], np.int32), 3)
+    pi = np.array([0, 1, 2, 3], np.int32)
+    o, D = orc.Oracle(3, m, f), orc.DenseRef(3, m, f)
+    exact = o.swap_delta_brute(pi, 1, 2)
+    assert o.swap_delta(pi, 1, 2) == exact == 4   # self-loop 1->1 now maps onto f2->f2 (w 5): +5, edge 1->2: -1
+    assert D.swap(pi, 1, 2) == -1                  # the reference kernel sees only the 1->2 edge
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "libref_gao.so")),
+                    reason="oracle/_ref not built (reference sources absent)")
+def test_oracle_matches_compiled_reference(bm, orc):
+    male, female, truth, pi0 = make_pair(bm, n=500, deg=15.0, seed=12)
+    ref = orc.RefCode(male, female)
+    o = orc.Oracle(500, male, female)
+    uv = orc.uniform_pairs(3, 20000, 500)
+    assert np.array_equal(ref.deltas(pi0, uv).astype(np.int64), o.swap_deltas(pi0, uv))
+    assert ref.score(pi0) == o.score(pi0)
+    lists = orc.ListsRef(500, male, female)
+    assert np.array_equal(lists.deltas(pi0, uv), ref.deltas(pi0, uv))
+    ref.close()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "libref_gao.so")),
+                    reason="oracle/_ref not built (reference sources absent)")
+def test_lists_restatement_matches_reference_on_quirks(bm, orc):
+    """calculate_swap_delta's self-loop / duplicate behaviour is reproduced by
+    the restatement (both differ from the exact delta on such graphs)."""
+    male, female, pi = quirky_pair(bm, seed=5)
+    male.w = np.abs(male.w) + 1
+    female.w = np.abs(female.w) + 1
+    ref = orc.RefCode(male, female)
+    lists = orc.ListsRef(60, male, female)
+    uv = np.array([(u, v) for u in range(1, 61) for v in range(1, 61) if u != v], np.int32)
+    assert np.array_equal(lists.deltas(pi, uv), ref.deltas(pi, uv))
+    ref.close()
+
+
+def _golden_files():
+    return sorted(glob.glob(os.path.join(GOLDEN, "synth_*.npz")) + glob.glob(os.path.join(GOLDEN, "quirky_*.npz")))
+
+
+@pytest.mark.parametrize("path", _golden_files(), ids=lambda p: os.path.basename(p))
+def test_golden_vectors(bm, orc, path):
+    """Fixtures written from the reference binaries (tests/golden/make_golden.py):
+    the oracle reproduces score-matching's integer, and on quirk-free pairs the
+    reference's calculate_swap_delta; the lists restatement reproduces the
+    reference delta on every pair (quirks included)."""
+    z = np.load(path, allow_pickle=False)
+    n = int(z["n"])
+    male = bm.Graph(z["m_src"], z["m_dst"], z["m_w"], n)
+    female = bm.Graph(z["f_src"], z["f_dst"], z["f_w"], n)
+    o = orc.Oracle(n, male, female)
+    pi = z["pi"]
+    assert o.score(pi) == int(z["score_matching"])
+    d = o.swap_deltas(pi, z["pairs"])
+    if bool(z["quirk_free"]):
+        assert np.array_equal(d, z["ref_deltas"].astype(np.int64))
+    else:
+        assert not np.array_equal(d, z["ref_deltas"].astype(np.int64))
+        brute = np.array([o.swap_delta_brute(pi, int(u), int(v)) for u, v in z["pairs"][:400]])
+        assert np.array_equal(d[:400], brute)
+    lists = orc.ListsRef(n, male, female)
+    assert np.array_equal(lists.deltas(pi, z["pairs"]), z["ref_deltas"])
+
+
+def test_golden_csv_loader_matches_score_matching(bm, orc):
+    """Prefixed ids, a duplicate line (last weight wins) and a malformed line:
+    the library's loaders plus the oracle give score-matching's integer."""
+    d = os.path.join(GOLDEN, "csv")
+    male = bm.load_graph_csv(os.path.join(d, "male.csv"))
+    female = bm.load_graph_csv(os.path.join(d, "female.csv"))
+    n = max(male.n, female.n)
+    pi = bm.load_mapping_csv(os.path.join(d, "matching.csv"), n)
+    with open(os.path.join(d, "score_matching.txt")) as f:
+        expected = int(f.read().strip())
+    assert orc.Oracle(n, male, female).score(pi) == expected
+
+
+def test_golden_greedy_match(bm):
+    z = np.load(os.path.join(GOLDEN, "greedy_0.npz"), allow_pickle=False)
+    n = int(z["n"])
+    male = bm.Graph(z["m_src"], z["m_dst"], z["m_w"], n)
+    female = bm.Graph(z["f_src"], z["f_dst"], z["f_w"], n)
+    assert np.array_equal(bm.greedy_match(n, male, female), z["ref_matching"])
