@@ -5,20 +5,27 @@ Workload (the full-size single-GPU swap-delta configuration): a synthetic
 male/female pair with n = 18,524 vertices and ~4M directed edges per graph
 (power-law out-degrees, alpha 2.1; lognormal(0.5, 1.2) integer weights >= 1;
 female = male relabelled with 20% of edges reweighted/rewired), initial
-matching = the greedy matching of the pair.  One step evaluates the exact
-swap delta of 2^28 uniform candidate pairs, generated inside the delta kernel
-from a counter-based stream (a new stream per step), writing 2^28 int64
-deltas to HBM.  The 3-vertex permutation kernel is timed on 2^26 uniform
-triples in the same run and reported under "rot3".
+matching = the greedy matching of the pair.  One step evaluates the exact swap
+delta of 2^28 uniform candidate pairs, generated inside the delta kernel from a
+counter-based stream (a new stream per step), writing 2^28 int64 deltas to HBM.
 
-value = candidates / second over the K timed steps (graph, matching and
-output buffers already resident in HBM).  roofline.achieved = algorithmic
-bytes per launch (48 + 28*D(u,v) per pair, summed exactly) / mean launch time
-measured with HIP events on the library's stream.  cpu_baseline = the
-reference's own calculate_swap_delta (src/graph-alignment-optimized.c:321-408,
-compiled from its sources into oracle/_ref) over the first candidates of the
-last timed stream, OpenMP dynamic schedule, on this host; its deltas are
-compared with the device's.
+Two exact evaluation strategies are timed; both give identical deltas:
+  gain  -- each step rebuilds the (n+1)^2 int32 placement-gain table for the
+           current matching, then evaluates every pair from four table entries
+           plus an exact correction for adjacent pairs (default headline);
+  scan  -- each pair scans its four male lists against LDS-staged female rows
+           (one wavefront per pair).
+The 3-vertex permutation kernels are timed on 2^26 uniform triples in the
+same run and reported under "rot3".
+
+value = candidates / second over the K timed steps of the headline strategy
+(graph, matching and output buffers already resident in HBM; the table build is
+inside every step).  roofline = the step's dominant kernel: algorithmic bytes
+per launch / mean launch time from HIP events on the library stream.
+cpu_baseline = the reference's own calculate_swap_delta
+(src/graph-alignment-optimized.c:321-408, compiled from its sources into
+oracle/_ref) over the first candidates of the last timed stream, OpenMP
+dynamic schedule, on this host; its deltas are compared with the device's.
 """
 import argparse
 import json
@@ -38,6 +45,39 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def gain_build_bytes(ctx, pi):
+    """Algorithmic bytes of one table build: every male edge read from both
+    directions (8 B) with pi of its other endpoint (4 B), every (male edge,
+    female-row entry) pair it expands into (8 B per entry), and the table write
+    ((n+1)^2 * 4 B)."""
+    import numpy as np
+    mo, mi = ctx.degrees(female=False)
+    fo, fi = ctx.degrees(female=True)
+    pi = np.asarray(pi, dtype=np.int64)
+    # out-edge u->x expands over the female in-row of pi(x): count by x (in-degree of x)
+    entries = int((mi.astype(np.int64) * fi[pi]).sum() + (mo.astype(np.int64) * fo[pi]).sum())
+    m = int(mo.sum())
+    return 2 * m * 12 + 8 * entries + (ctx.n + 1) ** 2 * 4, entries
+
+
+def gain_lookup_bytes(k):
+    """Per pair: pi(u), pi(v) (8 B), four table entries (16 B), the two male
+    adjacency probes (2 x 8 B), two self-loop words (8 B), delta out (8 B)."""
+    return 56 * k
+
+
+def timed(ctx, bm, kernel_ids, fn, steps):
+    ctx.sync()
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        fn(i)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    kt = {kid: ctx.timing(kid) for kid in kernel_ids}
+    return wall, kt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,19 +85,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--candidates", type=int, default=1 << 28)
     ap.add_argument("--triples", type=int, default=1 << 26)
+    ap.add_argument("--scan-steps", type=int, default=2, help="timed steps of the scan strategy")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rot3", action="store_true")
+    ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--headline", choices=["gain", "scan"], default="gain")
     ap.add_argument("--pi0", choices=["greedy", "shuffle"], default="greedy")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
-              "the multi-GPU path is not built -- see the sharding paragraph of the design notes.", file=sys.stderr)
+              "the multi-GPU path is not built.", file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
 
     import brainmatch as bm
 
@@ -76,55 +117,104 @@ def main():
     ctx.set_mapping(pi0)
     score0 = ctx.score()
     log(f"device {ctx.device_info()}; initial score {score0}")
-
     K = args.candidates
     batch = ctx.batch(K)
     ctx.timing_enable(True)
-    for i in range(args.warmup):
-        batch.eval_swaps_uniform(1000 + i, K)
-    ctx.sync()
-    ctx.timing_reset()
-
     seeds = [2000 + i for i in range(args.steps)]
-    ctx.sync()
-    t0 = time.perf_counter()
-    for s in seeds:
-        batch.eval_swaps_uniform(s, K)
-    ctx.sync()
-    t1 = time.perf_counter()
-    launches, total_ms = ctx.timing(bm.KERNEL_SWAP)
-    wall = t1 - t0
-    ms_per_step = 1000.0 * wall / args.steps
-    value = K * args.steps / wall
-    avg_launch_s = total_ms / 1000.0 / max(1, launches)
-    log(f"swap: {args.steps} x {K} candidates in {wall:.3f}s -> {value:.4g} deltas/s "
-        f"(kernel {1000 * avg_launch_s:.1f} ms/launch over {launches} launches)")
 
-    t = time.time()
-    algo = [ctx.uniform_algo_bytes(s, K) for s in seeds]
-    bytes_per_launch = sum(b for b, _ in algo) / len(algo)
-    mean_D = sum(d for _, d in algo) / len(algo) / K
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    log(f"algorithmic bytes/launch {bytes_per_launch:.4g} (mean D {mean_D:.1f}) -> {achieved:.1f} GB/s "
-        f"({time.time() - t:.1f}s to sum)")
+    # ---------------- gain-table strategy ----------------
+    gain = None
+    if args.headline == "gain" or not args.no_scan:
+        ctx.gain_enable(True)
+        for i in range(args.warmup):
+            ctx.gain_rebuild()
+            batch.eval_swaps_uniform(1000 + i, K)
+
+        def gain_step(i):
+            ctx.gain_rebuild()
+            batch.eval_swaps_uniform(seeds[i], K)
+
+        wall, kt = timed(ctx, bm, (bm.KERNEL_GAIN_BUILD, bm.KERNEL_GAIN_SWAP), gain_step, args.steps)
+        nb, msb = kt[bm.KERNEL_GAIN_BUILD]
+        nl, msl = kt[bm.KERNEL_GAIN_SWAP]
+        build_s = msb / 1000 / max(1, nb)
+        look_s = msl / 1000 / max(1, nl)
+        bb, entries = gain_build_bytes(ctx, pi0)
+        lb = gain_lookup_bytes(K)
+        gain = {
+            "value": K * args.steps / wall,
+            "ms_per_step": 1000 * wall / args.steps,
+            "build_ms": 1000 * build_s,
+            "lookup_ms": 1000 * look_s,
+            "table_bytes": (N_VERTICES + 1) ** 2 * 4,
+            "build_roofline": {"bound": "hbm", "achieved": bb / build_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": bb / build_s / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": bb,
+                               "expanded_entries": entries},
+            "lookup_roofline": {"bound": "hbm", "achieved": lb / look_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": lb / look_s / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": lb},
+        }
+        log(f"gain: {args.steps} x {K} pairs in {wall:.3f}s -> {gain['value']:.4g} deltas/s "
+            f"(build {1000 * build_s:.2f} ms, lookups {1000 * look_s:.2f} ms per step)")
+        gain_deltas_tail = batch.deltas(0, 1 << 16)
+        ctx.gain_enable(False)
+
+    # ---------------- scan strategy ----------------
+    scan = None
+    if args.headline == "scan" or not args.no_scan:
+        steps = args.steps if args.headline == "scan" else args.scan_steps
+        sseeds = seeds[-steps:]
+        for i in range(min(args.warmup, 1) if args.headline != "scan" else args.warmup):
+            batch.eval_swaps_uniform(1000 + i, K)
+        wall, kt = timed(ctx, bm, (bm.KERNEL_SWAP,), lambda i: batch.eval_swaps_uniform(sseeds[i], K), steps)
+        nl, ms = kt[bm.KERNEL_SWAP]
+        launch_s = ms / 1000 / max(1, nl)
+        algo = [ctx.uniform_algo_bytes(s, K) for s in sseeds]
+        bytes_per_launch = sum(b for b, _ in algo) / len(algo)
+        mean_D = sum(d for _, d in algo) / len(algo) / K
+        scan = {
+            "value": K * steps / wall,
+            "steps": steps,
+            "ms_per_step": 1000 * wall / steps,
+            "roofline": {"bound": "hbm", "achieved": bytes_per_launch / launch_s / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": bytes_per_launch / launch_s / 1e9 / HBM_PEAK_GBS,
+                         "algo_bytes_per_launch": bytes_per_launch, "mean_D": mean_D, "launch_ms": 1000 * launch_s},
+        }
+        log(f"scan: {steps} x {K} pairs in {wall:.3f}s -> {scan['value']:.4g} deltas/s "
+            f"({scan['roofline']['achieved']:.0f} GB/s algorithmic)")
+        if gain is not None:
+            same = bool((batch.deltas(0, 1 << 16) == gain_deltas_tail).all())
+            scan["matches_gain_path"] = same
+            log(f"scan and gain deltas identical on the last stream's first 65536 pairs: {same}")
+
+    head = gain if args.headline == "gain" else scan
     stats = batch.stats()
 
     traffic = None
-    prof = os.path.join(ROOT, "profiles", "swap_delta_n18524_traffic.json")
+    prof = os.path.join(ROOT, "profiles", f"{args.headline}_swap_n18524_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             p = json.load(f)
-        if p.get("candidates") == K and p.get("n") == N_VERTICES and p.get("pi0") == args.pi0:
+        if p.get("candidates") == K and p.get("pi0") == args.pi0:
             traffic = p.get("hbm_bytes_per_launch")
+
+    if args.headline == "gain":
+        dom = "lookup" if gain["lookup_ms"] >= gain["build_ms"] else "build"
+        rf = dict(gain[dom + "_roofline"])
+        rf["kernel"] = "k_gain_swap<uniform>" if dom == "lookup" else "k_gain_build"
+        rf["traffic"] = traffic
+    else:
+        rf = dict(scan["roofline"])
+        rf["kernel"] = "k_swap<uniform>"
+        rf["traffic"] = traffic
 
     result = {
         "metric": "exact swap-deltas/s",
-        "value": value,
+        "value": head["value"],
         "unit": "swap-deltas/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": args.steps if args.headline == "gain" else scan["steps"],
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -132,6 +222,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "swap-delta n=18524 ~4M edges/graph, 2^28 uniform pairs per step",
+            "strategy": args.headline,
             "n": N_VERTICES,
             "male_edges": male.m,
             "female_edges": female.m,
@@ -141,48 +232,14 @@ def main():
             "weights": "lognormal(0.5,1.2) rounded >=1",
             "parallelism": "single GPU",
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": "k_swap<uniform>",
-            "algo_bytes_per_launch": bytes_per_launch,
-            "mean_D": mean_D,
-            "launch_ms": 1000 * avg_launch_s,
-        },
+        "roofline": rf,
+        "gain_table": gain,
+        "scan_kernel": scan,
         "delta_stats_last_step": {k: stats[k] for k in ("sum", "max", "n_pos", "n_zero")},
     }
 
     if not args.no_rot3:
-        T = args.triples
-        b3 = ctx.batch(T)
-        b3.gen_triples(3000, T)
-        b3.eval_rot3()
-        ctx.sync()
-        ctx.timing_reset()
-        tr0 = time.perf_counter()
-        for i in range(args.steps):
-            b3.gen_triples(4000 + i, T)
-            b3.eval_rot3()
-        ctx.sync()
-        tr1 = time.perf_counter()
-        n3, ms3 = ctx.timing(bm.KERNEL_ROT3)
-        l3 = ms3 / 1000.0 / max(1, n3)
-        ab3, d3 = b3.algo_bytes(generated=False)
-        result["rot3"] = {
-            "triples_per_step": T,
-            "value": T / l3,
-            "unit": "triples/s (5 permutations each)",
-            "launch_ms": 1000 * l3,
-            "wall_s_incl_generation": tr1 - tr0,
-            "roofline": {"bound": "hbm", "achieved": ab3 / l3 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ab3 / l3 / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": ab3, "mean_D3": d3 / T},
-        }
-        log(f"rot3: {T} triples in {1000 * l3:.1f} ms/launch -> {T / l3:.4g} triples/s")
-        b3.close()
+        result["rot3"] = bench_rot3(ctx, bm, args)
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args, male, female, pi0, batch, seeds[-1], K, N_VERTICES)
@@ -193,6 +250,36 @@ def main():
     print(json.dumps(result), flush=True)
 
 
+def bench_rot3(ctx, bm, args):
+    T = args.triples
+    b3 = ctx.batch(T)
+    out = {"triples_per_step": T, "unit": "triples/s (5 permutations each)"}
+    for mode in ("gain", "scan"):
+        ctx.gain_enable(mode == "gain")
+        b3.gen_triples(3000, T)
+        b3.eval_rot3()
+        kid = bm.KERNEL_GAIN_ROT3 if mode == "gain" else bm.KERNEL_ROT3
+        steps = args.steps if mode == "gain" else max(1, args.scan_steps)
+
+        def step(i):
+            b3.gen_triples(4000 + i, T)
+            b3.eval_rot3()
+
+        wall, kt = timed(ctx, bm, (kid,), step, steps)
+        n3, ms3 = kt[kid]
+        l3 = ms3 / 1000.0 / max(1, n3)
+        ab3, d3 = b3.algo_bytes(generated=False)
+        if mode == "gain":
+            ab3 = T * (12 + 36 + 3 * 8 + 6 * 8 + 12 + 9)   # triple, 9 table entries, 3 self words, 6 adjacency probes, out
+        out[mode] = {"value": T / l3, "launch_ms": 1000 * l3,
+                     "roofline": {"bound": "hbm", "achieved": ab3 / l3 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": ab3 / l3 / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": ab3}}
+        log(f"rot3 {mode}: {T} triples in {1000 * l3:.1f} ms/launch -> {T / l3:.4g} triples/s")
+    ctx.gain_enable(False)
+    b3.close()
+    return out
+
+
 def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
     """The reference CPU arithmetic on a bounded sample of the same stream."""
     import numpy as np
@@ -201,17 +288,15 @@ def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
     import oracle as orc
 
     threads = max(1, min(16, os.cpu_count() or 1))
-    uv_all = None
     if orc.ref_available() and n <= 18524:
         kind = "reference"
         runner = orc.RefCode(male, female)
-        run = lambda pi, uv: runner.deltas(pi, uv, threads)  # noqa: E731
         what = "src/graph-alignment-optimized.c calculate_swap_delta compiled from the reference sources"
     else:
         kind = "port"
         runner = orc.ListsRef(n, male, female)
-        run = lambda pi, uv: runner.deltas(pi, uv, threads)  # noqa: E731
         what = "oracle restatement of calculate_swap_delta (edge lists + dense int16 female matrix)"
+    run = lambda pi, uv: runner.deltas(pi, uv, threads)  # noqa: E731
     calib = 1 << 16
     uv_all = orc.uniform_pairs(seed, calib, n)
     t = time.perf_counter()

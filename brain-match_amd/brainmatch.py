@@ -29,6 +29,7 @@ i64p = C.POINTER(C.c_int64)
 u8p = C.POINTER(C.c_uint8)
 
 KERNEL_SWAP, KERNEL_ROT3, KERNEL_SCORE = 0, 1, 2
+KERNEL_GAIN_BUILD, KERNEL_GAIN_SWAP, KERNEL_GAIN_ROT3, KERNEL_GAIN_UPDATE = 3, 4, 5, 6
 WEIGHTS_GEOMETRIC, WEIGHTS_LOGNORMAL = 1, 2
 SEARCH_SWAP_WEIGHTED, SEARCH_SWAP_UNIFORM, SEARCH_ROT3 = 0, 1, 2
 
@@ -135,6 +136,11 @@ _sig("bm_set_tuning", C.c_int, [_c, C.c_int32, C.POINTER(_Tuning)])
 _sig("bm_device_info", C.c_int, [_c, C.c_char_p, C.c_size_t])
 _sig("bm_search", C.c_int, [_c, C.POINTER(_SearchOpts), i32p, C.POINTER(_SearchResult)])
 _sig("bm_hillclimb", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int64, C.c_int32, i64p, C.POINTER(_SearchResult)])
+_sig("bm_gain_enable", C.c_int, [_c, C.c_int])
+_sig("bm_gain_enabled", C.c_int, [_c])
+_sig("bm_gain_rebuild", C.c_int, [_c])
+_sig("bm_gain_table_bytes", C.c_int64, [C.c_int32])
+_sig("bm_gain_row", C.c_int, [_c, C.c_int32, i32p])
 
 
 def _p32(a):
@@ -479,6 +485,21 @@ class Context:
         r = _SearchResult()
         self._chk(_lib.bm_search(self.h, C.byref(o), _p32(best), C.byref(r)))
         return best, {k: getattr(r, k) for k, _ in _SearchResult._fields_}
+
+    def gain_enable(self, on=True):
+        """Switch evaluation to the placement-gain table (or back to edge scans)."""
+        self._chk(_lib.bm_gain_enable(self.h, 1 if on else 0))
+
+    def gain_enabled(self):
+        return bool(_lib.bm_gain_enabled(self.h))
+
+    def gain_rebuild(self):
+        self._chk(_lib.bm_gain_rebuild(self.h))
+
+    def gain_row(self, u):
+        row = np.zeros(self.n + 1, np.int32)
+        self._chk(_lib.bm_gain_row(self.h, u, _p32(row)))
+        return row
 
     def hillclimb(self, seed, rounds, batch, verify=True):
         trace = np.zeros(rounds + 1, np.int64)

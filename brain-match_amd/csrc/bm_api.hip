@@ -57,6 +57,19 @@ struct bm_ctx {
     int64_t launches[BM_KERNEL_COUNT];
     double total_ms[BM_KERNEL_COUNT];
     bm_batch* work;         /* internal batch for the host-buffer calls */
+    /* placement-gain table (bm_gain.hip) */
+    int gain_on;
+    int32_t* h_m_self;      /* host: self-loop weights or INT32_MIN */
+    int32_t* h_f_self;
+    int32_t max_abs_fw;     /* largest |female weight| */
+    int32_t* d_G;
+    uint64_t* d_m_out_desc;
+    int2* d_m_out_tab;
+    int32_t* d_m_self;
+    int32_t* d_f_self;
+    int4* d_moves;
+    int64_t moves_cap;
+    int64_t* d_nmoves;
     char err[512];
 };
 
@@ -207,10 +220,58 @@ static void free_ctx(bm_ctx* c) {
         delete c->pending;
     }
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    (void)hipFree(c->d_G);
+    (void)hipFree(c->d_m_out_desc);
+    (void)hipFree(c->d_m_out_tab);
+    (void)hipFree(c->d_m_self);
+    (void)hipFree(c->d_f_self);
+    (void)hipFree(c->d_moves);
+    (void)hipFree(c->d_nmoves);
     bm_graph_free(&c->male);
     free(c->f_out_deg);
     free(c->f_in_deg);
+    free(c->h_m_self);
+    free(c->h_f_self);
     free(c);
+}
+
+static BmGain devgain(const bm_ctx* c) {
+    BmGain G;
+    G.G = c->d_G;
+    G.stride = (int64_t)c->n + 1;
+    G.m_out_desc = c->d_m_out_desc;
+    G.m_out_tab = c->d_m_out_tab;
+    G.m_self = c->d_m_self;
+    G.f_self = c->d_f_self;
+    return G;
+}
+
+static int gain_rebuild(bm_ctx* c) {
+    KTiming t;
+    timing_begin(c, BM_KERNEL_GAIN_BUILD, &t);
+    HIPCHK(c, bm_launch_gain_build(devgraph(c), devgain(c), c->num_cus, c->stream));
+    timing_end(c, &t);
+    return BM_OK;
+}
+
+/* Grow the device move list to hold `need` moves. */
+static int gain_moves_cap(bm_ctx* c, int64_t need) {
+    if (c->moves_cap >= need) return BM_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->d_moves);
+    c->d_moves = nullptr;
+    c->moves_cap = 0;
+    HIPCHK(c, hipMalloc((void**)&c->d_moves, (size_t)need * sizeof(int4)));
+    c->moves_cap = need;
+    return BM_OK;
+}
+
+static int gain_patch(bm_ctx* c, const int64_t* d_count, int64_t max_moves) {
+    KTiming t;
+    timing_begin(c, BM_KERNEL_GAIN_UPDATE, &t);
+    HIPCHK(c, bm_launch_gain_update(devgraph(c), devgain(c), c->d_moves, d_count, max_moves, c->stream));
+    timing_end(c, &t);
+    return BM_OK;
 }
 
 extern "C" void bm_destroy(bm_ctx* ctx) {
@@ -282,6 +343,28 @@ extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm
     for (int64_t e = 0; e < fg.m && c->compact; e++)
         if (fg.out.adj[e].y < 0 || fg.out.adj[e].y > 32767) c->compact = 0;
     default_tuning(c);
+    /* self-loop weights of both graphs and the largest |female weight| (gain table) */
+    c->h_m_self = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    c->h_f_self = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    if (!c->h_m_self || !c->h_f_self) {
+        bm_rowhash_free(&ho);
+        bm_rowhash_free(&hi);
+        bm_graph_free(&fg);
+        free_ctx(c);
+        return fail(nullptr, BM_ERR_NOMEM, "bm_create: out of host memory");
+    }
+    for (int32_t r = 0; r <= n; r++) c->h_m_self[r] = c->h_f_self[r] = INT32_MIN;
+    for (int32_t r = 1; r <= n; r++) {
+        for (int64_t e = c->male.out.ptr[r]; e < c->male.out.ptr[r + 1]; e++)
+            if (c->male.out.adj[e].x == r) c->h_m_self[r] = c->male.out.adj[e].y;
+        for (int64_t e = fg.out.ptr[r]; e < fg.out.ptr[r + 1]; e++)
+            if (fg.out.adj[e].x == r) c->h_f_self[r] = fg.out.adj[e].y;
+    }
+    c->max_abs_fw = 0;
+    for (int64_t e = 0; e < fg.m; e++) {
+        const int32_t a = fg.out.adj[e].y < 0 ? -fg.out.adj[e].y : fg.out.adj[e].y;
+        if (a > c->max_abs_fw) c->max_abs_fw = a;
+    }
 
     const bm_graph_host& mg = c->male;
     rc = BM_OK;
@@ -349,6 +432,10 @@ extern "C" int bm_set_mapping(bm_ctx* c, const int32_t* pi) {
             return fail(c, BM_ERR_ARG, "bm_set_mapping: pi[%d] = %d outside 0..%d", i, pi[i], c->n);
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(c->d_pi + 1, pi + 1, (size_t)c->n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (c->gain_on) {
+        int rc = gain_rebuild(c);
+        if (rc != BM_OK) return rc;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return BM_OK;
 }
@@ -556,9 +643,15 @@ extern "C" int bm_batch_eval_swaps(bm_batch* b) {
     if (b->kind != 1) return fail(c, BM_ERR_STATE, "bm_batch_eval_swaps: batch does not hold pairs");
     HIPCHK(c, hipSetDevice(c->device));
     KTiming t;
-    timing_begin(c, BM_KERNEL_SWAP, &t);
-    HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_LIST, (const int2*)b->d_cand, 0, b->k,
-                             b->d_delta, c->stream));
+    if (c->gain_on) {
+        timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
+        HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_LIST, (const int2*)b->d_cand, 0, b->k,
+                                      b->d_delta, c->stream));
+    } else {
+        timing_begin(c, BM_KERNEL_SWAP, &t);
+        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_LIST, (const int2*)b->d_cand, 0, b->k,
+                                 b->d_delta, c->stream));
+    }
     timing_end(c, &t);
     b->evaluated = 1;
     return BM_OK;
@@ -570,9 +663,15 @@ extern "C" int bm_batch_eval_swaps_uniform(bm_batch* b, uint64_t seed, int64_t k
     if (k < 0 || k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_eval_swaps_uniform: k exceeds capacity");
     HIPCHK(c, hipSetDevice(c->device));
     KTiming t;
-    timing_begin(c, BM_KERNEL_SWAP, &t);
-    HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_UNIFORM, nullptr, seed, k, b->d_delta,
-                             c->stream));
+    if (c->gain_on) {
+        timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
+        HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_UNIFORM, nullptr, seed, k, b->d_delta,
+                                      c->stream));
+    } else {
+        timing_begin(c, BM_KERNEL_SWAP, &t);
+        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_UNIFORM, nullptr, seed, k, b->d_delta,
+                                 c->stream));
+    }
     timing_end(c, &t);
     b->k = k;
     b->kind = 3; /* deltas only: candidates implicit in (seed, index) */
@@ -587,9 +686,15 @@ extern "C" int bm_batch_eval_rot3(bm_batch* b) {
     if (b->kind != 2) return fail(c, BM_ERR_STATE, "bm_batch_eval_rot3: batch does not hold triples");
     HIPCHK(c, hipSetDevice(c->device));
     KTiming t;
-    timing_begin(c, BM_KERNEL_ROT3, &t);
-    HIPCHK(c, bm_launch_rot3(devgraph(c), c->tune[BM_KERNEL_ROT3], (const int4*)b->d_cand, b->k, b->d_delta,
-                             b->d_perm, (int4*)b->d_cand, c->stream));
+    if (c->gain_on) {
+        timing_begin(c, BM_KERNEL_GAIN_ROT3, &t);
+        HIPCHK(c, bm_launch_gain_rot3(devgraph(c), devgain(c), (const int4*)b->d_cand, b->k, b->d_delta, b->d_perm,
+                                      (int4*)b->d_cand, c->stream));
+    } else {
+        timing_begin(c, BM_KERNEL_ROT3, &t);
+        HIPCHK(c, bm_launch_rot3(devgraph(c), c->tune[BM_KERNEL_ROT3], (const int4*)b->d_cand, b->k, b->d_delta,
+                                 b->d_perm, (int4*)b->d_cand, c->stream));
+    }
     timing_end(c, &t);
     b->evaluated = 1;
     return BM_OK;
@@ -663,8 +768,17 @@ static int accept_impl(bm_batch* b, int64_t max_accept, int32_t* accepted, int64
         HIPCHK(c, hipMalloc(&b->d_acc_scratch, b->acc_scratch_bytes));
         HIPCHK(c, hipMalloc((void**)&b->d_acc_idx, (size_t)b->cap * sizeof(int64_t)));
     }
+    if (c->gain_on) {
+        int rc = gain_moves_cap(c, 2 * (b->k > 0 ? b->k : 1));
+        if (rc != BM_OK) return rc;
+    }
     HIPCHK(c, bm_launch_accept(devgraph(c), c->d_pi, (const int2*)b->d_cand, b->d_delta, b->k, max_accept,
-                               b->d_acc_scratch, b->acc_scratch_bytes, b->d_acc_idx, b->d_out, c->stream));
+                               b->d_acc_scratch, b->acc_scratch_bytes, b->d_acc_idx, b->d_out,
+                               c->gain_on ? c->d_moves : nullptr, c->stream));
+    if (c->gain_on && b->k > 0) {
+        int rc = gain_patch(c, b->d_out + 2, 2 * b->k);
+        if (rc != BM_OK) return rc;
+    }
     int64_t meta[2];
     HIPCHK(c, hipMemcpyAsync(meta, b->d_out, sizeof meta, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -764,7 +878,13 @@ extern "C" int bm_accept_disjoint(bm_ctx* c, const int32_t* uv, const int64_t* d
 extern "C" int bm_apply_swap(bm_ctx* c, int32_t u, int32_t v) {
     if (!c || u < 1 || u > c->n || v < 1 || v > c->n) return fail(c, BM_ERR_ARG, "bm_apply_swap: vertex outside 1..n");
     HIPCHK(c, hipSetDevice(c->device));
+    if (c->gain_on) {
+        int rc = gain_moves_cap(c, 3);
+        if (rc != BM_OK) return rc;
+        HIPCHK(c, bm_launch_moves_from_swap(c->d_pi, u, v, c->d_moves, c->d_nmoves, c->stream));
+    }
     HIPCHK(c, bm_launch_apply_swap(c->d_pi, u, v, c->stream));
+    if (c->gain_on) return gain_patch(c, c->d_nmoves, 2);
     return BM_OK;
 }
 
@@ -772,7 +892,100 @@ extern "C" int bm_apply_perm3(bm_ctx* c, int32_t a, int32_t b, int32_t d, int32_
     if (!c || a < 1 || a > c->n || b < 1 || b > c->n || d < 1 || d > c->n || a == b || a == d || b == d || p < 0 || p > 5)
         return fail(c, BM_ERR_ARG, "bm_apply_perm3: invalid triple or permutation");
     HIPCHK(c, hipSetDevice(c->device));
+    if (c->gain_on) {
+        int rc = gain_moves_cap(c, 3);
+        if (rc != BM_OK) return rc;
+        HIPCHK(c, bm_launch_moves_from_perm3(c->d_pi, a, b, d, p, c->d_moves, c->d_nmoves, c->stream));
+    }
     HIPCHK(c, bm_launch_apply_perm3(c->d_pi, a, b, d, p, c->stream));
+    if (c->gain_on) return gain_patch(c, c->d_nmoves, 3);
+    return BM_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Placement-gain table                                                */
+/* ------------------------------------------------------------------ */
+
+extern "C" int64_t bm_gain_table_bytes(int32_t n) {
+    if (n < 1) return 0;
+    return ((int64_t)n + 1) * ((int64_t)n + 1) * (int64_t)sizeof(int32_t);
+}
+
+extern "C" int bm_gain_enabled(const bm_ctx* c) { return c ? c->gain_on : 0; }
+
+extern "C" int bm_gain_enable(bm_ctx* c, int on) {
+    if (!c) return BM_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!on) {
+        if (!c->gain_on) return BM_OK;
+        (void)hipFree(c->d_G);
+        c->d_G = nullptr;
+        c->gain_on = 0;
+        return BM_OK;
+    }
+    if (c->gain_on) return BM_OK;
+    /* every table entry must fit int32: |G(u,f)| <= sum over u's edges of max(|w|, max|w_F|) */
+    const bm_graph_host& mg = c->male;
+    for (int32_t u = 1; u <= c->n; u++) {
+        int64_t bound = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            const bm_csr_host& L = pass ? mg.in : mg.out;
+            for (int64_t e = L.ptr[u]; e < L.ptr[u + 1]; e++) {
+                const int64_t w = L.adj[e].y < 0 ? -(int64_t)L.adj[e].y : L.adj[e].y;
+                bound += w > c->max_abs_fw ? w : c->max_abs_fw;
+            }
+        }
+        if (bound > INT32_MAX)
+            return fail(c, BM_ERR_ARG, "bm_gain_enable: vertex %d's incident weights can exceed int32 in the table", u);
+    }
+    const size_t bytes = (size_t)bm_gain_table_bytes(c->n);
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+    if (bytes + ((size_t)1 << 30) > free_b)
+        return fail(c, BM_ERR_NOMEM, "bm_gain_enable: table needs %zu bytes, %zu free on the device", bytes, free_b);
+    if (!c->d_m_out_desc) {
+        bm_rowhash_host hm;
+        memset(&hm, 0, sizeof hm);
+        if (bm_rowhash_build(&mg.out, &hm) != BM_OK) return fail(c, BM_ERR_NOMEM, "bm_gain_enable: out of host memory");
+        int rc = dalloc(c, &c->d_m_out_desc, (size_t)c->n + 1);
+        if (rc == BM_OK) rc = dalloc(c, &c->d_m_out_tab, (size_t)(hm.slots ? hm.slots : 2));
+        if (rc == BM_OK) rc = dalloc(c, &c->d_m_self, (size_t)c->n + 1);
+        if (rc == BM_OK) rc = dalloc(c, &c->d_f_self, (size_t)c->n + 1);
+        if (rc == BM_OK) rc = dalloc(c, &c->d_nmoves, 1);
+        if (rc == BM_OK) rc = h2d(c, c->d_m_out_desc, hm.desc, (size_t)c->n + 1);
+        if (rc == BM_OK) rc = h2d(c, c->d_m_out_tab, hm.tab, (size_t)hm.slots);
+        if (rc == BM_OK) rc = h2d(c, c->d_m_self, c->h_m_self, (size_t)c->n + 1);
+        if (rc == BM_OK) rc = h2d(c, c->d_f_self, c->h_f_self, (size_t)c->n + 1);
+        if (rc == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, BM_ERR_HIP, "sync failed");
+        bm_rowhash_free(&hm);
+        if (rc != BM_OK) return rc;
+    }
+    int rc = dalloc(c, &c->d_G, bytes / sizeof(int32_t));
+    if (rc != BM_OK) return rc;
+    /* row 0 is never written by the build; clear it once */
+    HIPCHK(c, hipMemsetAsync(c->d_G, 0, ((size_t)c->n + 1) * sizeof(int32_t), c->stream));
+    c->gain_on = 1;
+    rc = gain_rebuild(c);
+    if (rc != BM_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return BM_OK;
+}
+
+extern "C" int bm_gain_rebuild(bm_ctx* c) {
+    if (!c) return BM_ERR_ARG;
+    if (!c->gain_on) return fail(c, BM_ERR_STATE, "bm_gain_rebuild: gain table not enabled");
+    HIPCHK(c, hipSetDevice(c->device));
+    return gain_rebuild(c);
+}
+
+extern "C" int bm_gain_row(bm_ctx* c, int32_t u, int32_t* row) {
+    if (!c || !row || u < 0 || u > c->n) return fail(c, BM_ERR_ARG, "bm_gain_row: invalid argument");
+    if (!c->gain_on) return fail(c, BM_ERR_STATE, "bm_gain_row: gain table not enabled");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(row, c->d_G + (int64_t)u * ((int64_t)c->n + 1), ((size_t)c->n + 1) * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return BM_OK;
 }
 

@@ -31,6 +31,28 @@ struct BmDevGraph {
 /* Bytes per LDS table slot for this graph (4 compact, 8 otherwise). */
 size_t bm_slot_bytes(const BmDevGraph& g);
 
+/* Placement-gain table (bm_gain.hip). */
+struct BmGain {
+    int32_t* G;                 /* (n+1) x stride int32, row-major; row 0 / column 0 zero */
+    int64_t stride;             /* n + 1 */
+    const uint64_t* m_out_desc; /* male out-row hash tables (adjacency tests) */
+    const int2* m_out_tab;
+    const int32_t* m_self;      /* male self-loop weight per vertex or BM_ABSENT */
+    const int32_t* f_self;      /* female self-loop weight per vertex or BM_ABSENT */
+};
+
+hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s);
+hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int4* moves, const int64_t* count,
+                                 int64_t max_moves, hipStream_t s);
+hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, uint64_t seed,
+                               int64_t k, int64_t* delta, hipStream_t s);
+hipError_t bm_launch_gain_rot3(const BmDevGraph& g, const BmGain& G, const int4* cand, int64_t k, int64_t* best,
+                               uint8_t* perm, int4* cand_out_perm, hipStream_t s);
+hipError_t bm_launch_moves_from_swap(const int32_t* pi_before, int32_t u, int32_t v, int4* moves, int64_t* count,
+                                     hipStream_t s);
+hipError_t bm_launch_moves_from_perm3(const int32_t* pi_before, int32_t a, int32_t b, int32_t c, int32_t p,
+                                      int4* moves, int64_t* count, hipStream_t s);
+
 struct BmLaunch {
     int32_t waves_per_block;   /* 1, 2, 4 or 8 */
     int32_t lds_slots;         /* table slots per wave (8 bytes each) */
@@ -73,8 +95,8 @@ hipError_t bm_launch_pick(const int64_t* d, int64_t k, int32_t kind, int64_t ran
 size_t bm_accept_scratch_bytes(int64_t k);
 hipError_t bm_launch_accept(const BmDevGraph& g, int32_t* pi_rw, const int2* cand, const int64_t* d,
                             int64_t k, int64_t max_accept, void* scratch, size_t scratch_bytes,
-                            int64_t* out_acc_idx /* k */, int64_t* out_meta /* [n_acc, gain] */,
-                            hipStream_t s);
+                            int64_t* out_acc_idx /* k */, int64_t* out_meta /* [n_acc, gain, n_moves] */,
+                            int4* out_moves /* 2k {x, old, new, 0}, may be NULL */, hipStream_t s);
 
 hipError_t bm_launch_apply_swap(int32_t* pi, int32_t u, int32_t v, hipStream_t s);
 hipError_t bm_launch_apply_perm3(int32_t* pi, int32_t a, int32_t b, int32_t c, int32_t p, hipStream_t s);

@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(kAccThreads) k_accept_greedy(BmDevGraph g, int
                                                                const int64_t* __restrict__ d,
                                                                const int64_t* __restrict__ order, int64_t npos,
                                                                int64_t max_accept, int64_t* __restrict__ acc_idx,
-                                                               int64_t* __restrict__ meta) {
+                                                               int64_t* __restrict__ meta, int4* __restrict__ moves) {
     extern __shared__ uint32_t lock[];
     __shared__ int s_first;
     __shared__ int32_t s_u, s_v;
@@ -380,11 +380,15 @@ __global__ void __launch_bounds__(kAccThreads) k_accept_greedy(BmDevGraph g, int
     const int64_t nacc = s_nacc;
     for (int64_t a = t; a < nacc; a += kAccThreads) {
         const int2 p = cand[acc_idx[a]];
-        const int32_t fu = pi[p.x];
-        pi[p.x] = pi[p.y];
+        const int32_t fu = pi[p.x], fv = pi[p.y];
+        if (moves) {
+            moves[2 * a] = make_int4(p.x, fu, fv, 0);
+            moves[2 * a + 1] = make_int4(p.y, fv, fu, 0);
+        }
+        pi[p.x] = fv;
         pi[p.y] = fu;
     }
-    if (t == 0) { meta[0] = nacc; meta[1] = s_gain; }
+    if (t == 0) { meta[0] = nacc; meta[1] = s_gain; meta[2] = 2 * nacc; }
 }
 
 __global__ void k_apply_swap(int32_t* pi, int32_t u, int32_t v) {
@@ -491,9 +495,9 @@ size_t bm_accept_scratch_bytes(int64_t k) {
 
 hipError_t bm_launch_accept(const BmDevGraph& g, int32_t* pi_rw, const int2* cand, const int64_t* d, int64_t k,
                             int64_t max_accept, void* scratch, size_t scratch_bytes, int64_t* out_acc_idx,
-                            int64_t* out_meta, hipStream_t s) {
+                            int64_t* out_meta, int4* out_moves, hipStream_t s) {
     if (k <= 0) {
-        hipError_t e = hipMemsetAsync(out_meta, 0, 2 * sizeof(int64_t), s);
+        hipError_t e = hipMemsetAsync(out_meta, 0, 3 * sizeof(int64_t), s);
         return e;
     }
     char* p = (char*)scratch;
@@ -531,7 +535,7 @@ hipError_t bm_launch_accept(const BmDevGraph& g, int32_t* pi_rw, const int2* can
     if (e != hipSuccess) return e;
     if (max_accept <= 0) max_accept = INT64_MAX;
     hipLaunchKernelGGL(k_accept_greedy, dim3(1), dim3(kAccThreads), lds, s, g, pi_rw, cand, d, sidx, npos,
-                       max_accept, out_acc_idx, out_meta);
+                       max_accept, out_acc_idx, out_meta, out_moves);
     return hipGetLastError();
 }
 

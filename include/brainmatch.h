@@ -253,13 +253,40 @@ int bm_batch_pick(bm_batch* b, int32_t kind, int64_t rank, int64_t* index);
 int bm_batch_accept_disjoint(bm_batch* b, int64_t max_accept, int64_t* n_accepted, int64_t* gain);
 
 /* ------------------------------------------------------------------ */
+/* Placement-gain table (second evaluation strategy, same results)     */
+/* ------------------------------------------------------------------ */
+
+/* G(u, f) = score of u's incident male edges (self-loops excluded) if u alone
+ * were placed on female f, an (n+1)^2 int32 table in HBM (1.37 GB at
+ * n = 18,524).  While enabled, every swap / 3-permutation evaluation reads it
+ * (four table entries plus an exact correction for edges inside the moved set
+ * and self-loops) instead of scanning the candidate's edges; bm_set_mapping
+ * rebuilds it and bm_apply_swap, bm_apply_perm3 and the disjoint acceptance
+ * patch the rows of the moved vertices' neighbours.  Results are identical to
+ * the scan kernels.  The reference's closest relative is the n x n int16
+ * swap-delta table of src/best-swap/score-lookup.c:246-293 (patched after each
+ * swap, :129-174).  Fails with BM_ERR_NOMEM when the table does not fit the
+ * device and with BM_ERR_ARG when a vertex's incident weights could overflow
+ * int32. */
+int bm_gain_enable(bm_ctx* ctx, int on);
+int bm_gain_enabled(const bm_ctx* ctx);
+int bm_gain_rebuild(bm_ctx* ctx);
+int64_t bm_gain_table_bytes(int32_t n);
+/* Copy row u (n+1 entries) of the table to the host. */
+int bm_gain_row(bm_ctx* ctx, int32_t u, int32_t* row);
+
+/* ------------------------------------------------------------------ */
 /* Measurement                                                         */
 /* ------------------------------------------------------------------ */
 
 #define BM_KERNEL_SWAP 0
 #define BM_KERNEL_ROT3 1
 #define BM_KERNEL_SCORE 2
-#define BM_KERNEL_COUNT 3
+#define BM_KERNEL_GAIN_BUILD 3
+#define BM_KERNEL_GAIN_SWAP 4
+#define BM_KERNEL_GAIN_ROT3 5
+#define BM_KERNEL_GAIN_UPDATE 6
+#define BM_KERNEL_COUNT 7
 
 /* When enabled, every delta/score launch is bracketed by HIP events on the
  * context stream; bm_timing_get synchronises and returns the number of

@@ -739,3 +739,24 @@ void or_hillclimb(const or_graph* g, int32_t* pi, uint64_t seed, int64_t rounds,
     free(d);
     free(acc);
 }
+
+/* Row u of the placement-gain table, straight from its definition:
+ * row[f] = sum over male edges u->x (x != u) of c(w, f, pi(x))
+ *        + sum over male edges x->u (x != u) of c(w, pi(x), f),   f = 0..n.
+ * O(n * deg(u)) female lookups; for checking small graphs only. */
+void or_gain_row(const or_graph* g, const int32_t* pi, int32_t u, int32_t* row) {
+    for (int32_t f = 0; f <= g->n; f++) {
+        int64_t s = 0;
+        for (int64_t p = g->M.out_off[u]; p < g->M.out_off[u + 1]; p++) {
+            const int64_t e = g->M.out_e[p];
+            const int32_t x = og_dst(&g->M, e);
+            if (x != u) s += contrib(&g->F, g->M.w[e], f, pi[x]);
+        }
+        for (int64_t p = g->M.in_off[u]; p < g->M.in_off[u + 1]; p++) {
+            const int64_t e = g->M.in_e[p];
+            const int32_t x = og_src(&g->M, e);
+            if (x != u) s += contrib(&g->F, g->M.w[e], pi[x], f);
+        }
+        row[f] = (int32_t)s;
+    }
+}

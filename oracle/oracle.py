@@ -300,3 +300,14 @@ def ref_score_matching(g1_csv, g2_csv, matching_csv):
         if s.lstrip("-").isdigit():
             return int(s)
     raise RuntimeError("score-matching printed no score:\n" + r.stdout[-2000:])
+
+
+_sig(_o, "or_gain_row", None, [vp, i32p, C.c_int32, i32p])
+
+
+def gain_row(o: "Oracle", pi, u):
+    """Row u of the placement-gain table from its definition (small graphs)."""
+    pi = _a32(pi)
+    row = np.empty(o.n + 1, np.int32)
+    _o.or_gain_row(o.h, _p32(pi), u, _p32(row))
+    return row

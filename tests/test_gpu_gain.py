@@ -1,0 +1,159 @@
+"""Device parity of the placement-gain evaluation path: table rows against
+their definition, deltas against the oracle, and the incremental patches
+against a fresh rebuild after every kind of move."""
+import numpy as np
+import pytest
+
+from conftest import quirky_pair
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g1(bm, orc):
+    male, female, truth, pi0 = bm.synth_pair(2000, 20.0, bm.WEIGHTS_GEOMETRIC, 0.3, seed=1)
+    ctx = bm.Context(2000, male, female)
+    ctx.set_mapping(pi0)
+    ctx.gain_enable(True)
+    o = orc.Oracle(2000, male, female)
+    yield dict(male=male, female=female, pi0=pi0, ctx=ctx, o=o, n=2000)
+    ctx.close()
+
+
+def test_rows_match_definition(g1, orc):
+    ctx, o, pi = g1["ctx"], g1["o"], g1["pi0"]
+    assert ctx.gain_enabled()
+    rng = np.random.default_rng(0)
+    for u in [1, 2, 1999, 2000] + list(rng.integers(1, 2001, 12)):
+        assert np.array_equal(ctx.gain_row(int(u)), orc.gain_row(o, pi, int(u)))
+    assert not ctx.gain_row(0).any()
+
+
+def test_swap_deltas_match_oracle(g1, orc):
+    ctx, o, pi, n = g1["ctx"], g1["o"], g1["pi0"], g1["n"]
+    uv = orc.uniform_pairs(5, 1 << 16, n)
+    assert np.array_equal(ctx.swap_deltas(uv), o.swap_deltas(pi, uv))
+    b = ctx.batch(1 << 15)
+    b.eval_swaps_uniform(123, 1 << 15)
+    assert np.array_equal(b.deltas(), o.swap_deltas(pi, orc.uniform_pairs(123, 1 << 15, n)))
+    b.close()
+    assert ctx.swap_deltas(np.array([[3, 3]]))[0] == 0
+
+
+def test_rot3_matches_oracle(g1, orc):
+    ctx, o, pi, n = g1["ctx"], g1["o"], g1["pi0"], g1["n"]
+    rng = np.random.default_rng(3)
+    abc = np.stack([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(3000)]).astype(np.int32)
+    best, perm = ctx.rot3_deltas(abc)
+    rb, rp = o.rot3_many(pi, abc)
+    assert np.array_equal(best, rb) and np.array_equal(perm.astype(np.int32), rp)
+
+
+def test_changed_enumerations_match_scan_path(g1, orc):
+    ctx, n = g1["ctx"], g1["n"]
+    changed = np.array([17, 400, 1999, 3, 1200], np.int32)
+    d_gain, uv = ctx.swap_deltas_changed(changed)
+    b_gain, q = ctx.rot3_deltas_changed(changed)
+    ctx.gain_enable(False)
+    try:
+        d_scan, uv2 = ctx.swap_deltas_changed(changed)
+        b_scan, q2 = ctx.rot3_deltas_changed(changed)
+    finally:
+        ctx.gain_enable(True)
+    assert np.array_equal(uv, uv2) and np.array_equal(d_gain, d_scan)
+    assert np.array_equal(q, q2) and np.array_equal(b_gain, b_scan)
+
+
+def test_patches_equal_rebuild(g1, orc):
+    """apply_swap, apply_perm3 and disjoint acceptance patch the table; a full
+    rebuild of the resulting matching must give the same rows."""
+    ctx, o, n = g1["ctx"], g1["o"], g1["n"]
+    pi0 = g1["pi0"]
+    ctx.set_mapping(pi0)
+    ctx.apply_swap(10, 20)
+    ctx.apply_swap(20, 1500)
+    ctx.apply_perm3(7, 8, 9, 4)
+    ctx.apply_perm3(30, 10, 1999, 1)
+    uv = orc.uniform_pairs(41, 20000, n)
+    d = ctx.swap_deltas(uv)
+    acc, gain = ctx.accept_disjoint(uv, d)
+    assert gain > 0 and len(acc) > 1
+    pi = ctx.get_mapping()
+    rows = list(range(1, n + 1, 37)) + [10, 20, 1500, 7, 8, 9, 30, 1999]
+    patched = {u: ctx.gain_row(u) for u in rows}
+    ctx.gain_rebuild()
+    for u in rows:
+        assert np.array_equal(patched[u], ctx.gain_row(u)), u
+        assert np.array_equal(patched[u], orc.gain_row(o, pi, u)), u
+    uv2 = orc.uniform_pairs(42, 8192, n)
+    assert np.array_equal(ctx.swap_deltas(uv2), o.swap_deltas(pi, uv2))
+    ctx.set_mapping(pi0)
+
+
+def test_hillclimb_with_gain_matches_replay(g1, orc):
+    ctx, o, pi = g1["ctx"], g1["o"], g1["pi0"]
+    ctx.set_mapping(pi)
+    trace, res = ctx.hillclimb(seed=1000, rounds=25, batch=4096, verify=True)
+    pi_o, trace_o = o.hillclimb(pi, 1000, 25, 4096)
+    assert res["verify_failures"] == 0
+    assert np.array_equal(trace, trace_o)
+    assert np.array_equal(ctx.get_mapping(), pi_o)
+    ctx.set_mapping(pi)
+
+
+def test_search_drivers_with_gain(g1, bm, orc):
+    ctx, o, pi = g1["ctx"], g1["o"], g1["pi0"]
+    for mode in (bm.SEARCH_SWAP_UNIFORM, bm.SEARCH_ROT3):
+        ctx.set_mapping(pi)
+        best, res = ctx.search(mode, seed=9, max_iters=80, verify_every=1)
+        assert res["verify_failures"] == 0
+        assert o.score(best) == res["best_score"] >= res["initial_score"]
+    ctx.set_mapping(pi)
+
+
+def test_quirky_graph_gain_exact(bm, orc):
+    male, female, pi = quirky_pair(bm, seed=11)
+    n = 60
+    o = orc.Oracle(n, male, female)
+    with bm.Context(n, male, female) as ctx:
+        ctx.set_mapping(pi)
+        ctx.gain_enable(True)
+        for u in range(1, n + 1):
+            assert np.array_equal(ctx.gain_row(u), orc.gain_row(o, pi, u))
+        uv = np.array([(u, v) for u in range(1, n + 1) for v in range(1, n + 1)], np.int32)
+        brute = np.array([o.swap_delta_brute(pi, int(u), int(v)) for u, v in uv])
+        assert np.array_equal(ctx.swap_deltas(uv), brute)
+        rng = np.random.default_rng(2)
+        abc = np.stack([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(400)]).astype(np.int32)
+        best, perm = ctx.rot3_deltas(abc)
+        for i, (a, b_, c) in enumerate(abc):
+            d5 = o.rot3_all(pi, int(a), int(b_), int(c), brute=True)
+            assert best[i] == max(0, d5.max())
+        # patches on a graph with self-loops and u<->v edges
+        ctx.apply_swap(1, 2)
+        ctx.apply_perm3(3, 4, 5, 3)
+        q = ctx.get_mapping()
+        for u in range(1, n + 1):
+            assert np.array_equal(ctx.gain_row(u), orc.gain_row(o, q, u))
+
+
+def test_full_size_gain_equals_scan(bm, orc):
+    """n=18,524, ~4M edges: the table path and the scan path agree on 2^20
+    device candidates; 4096 of them are checked against the oracle."""
+    n = 18524
+    male, female, truth, pi0 = bm.synth_pair(n, 4e6 / n, bm.WEIGHTS_LOGNORMAL, 0.3, seed=2)
+    with bm.Context(n, male, female) as ctx:
+        ctx.set_mapping(pi0)
+        k = 1 << 20
+        b = ctx.batch(k)
+        b.eval_swaps_uniform(99, k)
+        d_scan = b.deltas()
+        ctx.gain_enable(True)
+        b.eval_swaps_uniform(99, k)
+        d_gain = b.deltas()
+        assert np.array_equal(d_scan, d_gain)
+        o = orc.Oracle(n, male, female)
+        uv = orc.uniform_pairs(99, k, n)
+        idx = np.random.default_rng(1).choice(k, 4096, replace=False)
+        assert np.array_equal(d_gain[idx], o.swap_deltas(pi0, uv[idx]))
+        b.close()
