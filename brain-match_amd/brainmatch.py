@@ -506,3 +506,108 @@ class Context:
         r = _SearchResult()
         self._chk(_lib.bm_hillclimb(self.h, seed, rounds, batch, 1 if verify else 0, _p64(trace), C.byref(r)))
         return trace, {k: getattr(r, k) for k, _ in _SearchResult._fields_}
+
+
+class _AnnealParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("chains_per_instance", C.c_int32), ("t0", C.c_double),
+                ("decay", C.c_double), ("p_rot3", C.c_double), ("traj_cap", C.c_int32)]
+
+
+_sig("bm_anneal_create", C.c_int, [C.POINTER(_c), C.POINTER(_c), C.c_int32, C.POINTER(_AnnealParams)])
+_sig("bm_anneal_destroy", None, [_c])
+_sig("bm_anneal_num_chains", C.c_int64, [_c])
+_sig("bm_anneal_run", C.c_int, [_c, C.c_int64, C.c_int32])
+_sig("bm_anneal_state", C.c_int, [_c, i64p, i64p, i64p, i64p])
+_sig("bm_anneal_mapping", C.c_int, [_c, C.c_int64, C.c_int32, i32p])
+_sig("bm_anneal_trajectory", C.c_int, [_c, i64p, C.POINTER(C.c_int32)])
+_sig("bm_anneal_verify", C.c_int, [_c, i64p])
+_sig("bm_anneal_timing", C.c_int, [_c, i64p, C.POINTER(C.c_double)])
+_sig("bm_anneal_algo_bytes", C.c_int, [_c, i64p, i64p])
+_sig("bm_anneal_last_error", C.c_char_p, [_c])
+_sig("bm_anneal_schedule", C.c_int, [_c, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)])
+
+
+class Anneal:
+    """Annealing chains over one or more contexts (instances) on one device:
+    chains_per_instance chains per context, all advanced by one kernel launch
+    replayed from a HIP graph (bm_anneal_* in include/brainmatch.h)."""
+
+    def __init__(self, contexts, chains_per_instance=1, seed=1, t0=50.0, decay=0.9999, p_rot3=0.0, traj_cap=0):
+        self.contexts = list(contexts)
+        self.seed = seed
+        self.ns = [c.n for c in self.contexts]
+        self.per_instance = chains_per_instance
+        self.traj_cap = traj_cap
+        arr = (_c * len(self.contexts))(*[c.h for c in self.contexts])
+        p = _AnnealParams(seed, chains_per_instance, t0, decay, p_rot3, traj_cap)
+        self.h = _c()
+        rc = _lib.bm_anneal_create(C.byref(self.h), arr, len(self.contexts), C.byref(p))
+        if rc != 0:
+            msg = _lib.bm_anneal_last_error(None)
+            raise BMError(rc, msg.decode() if msg else "")
+        self.chains = _lib.bm_anneal_num_chains(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.bm_anneal_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = _lib.bm_anneal_last_error(self.h)
+            raise BMError(rc, msg.decode() if msg else "")
+
+    def run(self, steps, launches=1):
+        self._chk(_lib.bm_anneal_run(self.h, steps, launches))
+
+    def state(self):
+        k = self.chains
+        s, b, st, a = (np.zeros(k, np.int64) for _ in range(4))
+        self._chk(_lib.bm_anneal_state(self.h, _p64(s), _p64(b), _p64(st), _p64(a)))
+        return dict(score=s, best=b, steps=st, accepted=a)
+
+    def instance_of(self, chain):
+        return chain // self.per_instance
+
+    def mapping(self, chain, best=False):
+        pi = np.zeros(self.ns[self.instance_of(chain)] + 1, np.int32)
+        self._chk(_lib.bm_anneal_mapping(self.h, chain, 1 if best else 0, _p32(pi)))
+        return pi
+
+    def trajectory(self):
+        t = np.zeros((self.chains, max(1, self.traj_cap)), np.int64)
+        cnt = C.c_int32(0)
+        self._chk(_lib.bm_anneal_trajectory(self.h, _p64(t), C.byref(cnt)))
+        return t[:, :cnt.value].copy()
+
+    def verify(self):
+        bad = C.c_int64(0)
+        self._chk(_lib.bm_anneal_verify(self.h, C.byref(bad)))
+        return bad.value
+
+    def timing(self):
+        n, ms = C.c_int64(0), C.c_double(0)
+        self._chk(_lib.bm_anneal_timing(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def algo_bytes(self):
+        b, s = C.c_int64(0), C.c_int64(0)
+        self._chk(_lib.bm_anneal_algo_bytes(self.h, C.byref(b), C.byref(s)))
+        return b.value, s.value
+
+    def schedule(self):
+        b, l, p = C.c_double(0), C.c_double(0), C.c_uint64(0)
+        self._chk(_lib.bm_anneal_schedule(self.h, C.byref(b), C.byref(l), C.byref(p)))
+        return b.value, l.value, p.value

@@ -1,0 +1,372 @@
+/*
+ * Annealing chains (include/brainmatch.h, bm_anneal_*): host side.
+ *
+ * Layout in device memory:
+ *   insts   one BmDevGraph per instance (the contexts' resident graphs, not
+ *           copied);
+ *   chains  one BmChain per chain (running score, best score, step and
+ *           acceptance counters, stream seed, offset of its matchings);
+ *   pool    per chain the current matching and the best-so-far copy,
+ *           2 * (n_i + 1) int32 each, chains of one instance contiguous;
+ *   traj    chains x traj_cap int64 score samples, one per launch.
+ * A launch advances every chain by a fixed number of steps (k_anneal in
+ * bm_delta.hip); it is captured once into a HIP graph and replayed.  The
+ * chain state lives only on the device, so a replay needs no host input.
+ */
+#include "bm_device.h"
+#include "bm_host.h"
+#include "bm_rng.h"
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+extern "C" const char* bm_global_error(void);
+
+struct bm_anneal {
+    int device;
+    hipStream_t stream;
+    int32_t n_inst;
+    int64_t per_inst;
+    int64_t n_chains;
+    std::vector<bm_ctx*> ctxs;
+    std::vector<BmDevGraph> graphs;   /* host copies of the instance graphs */
+    std::vector<int64_t> pi_off;      /* per chain */
+    BmDevGraph* d_insts;
+    BmChain* d_chains;
+    int32_t* d_pool;
+    int64_t* d_traj;
+    int64_t* d_scores;                /* n_chains, for bm_anneal_verify */
+    BmAnnealSched sc;
+    BmLaunch L;
+    int compact;
+    dim3 grid, block;
+    size_t lds;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int64_t graph_steps;
+    int64_t launches;
+    double total_ms;
+    uint64_t seed;
+    double p_rot3;
+    char err[512];
+};
+
+static int afail(bm_anneal* a, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (a)
+        snprintf(a->err, sizeof a->err, "%s", buf);
+    else
+        bm_set_global_error("%s", buf);
+    return code;
+}
+
+#define AHIP(a, expr)                                                                                        \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            return afail((a), BM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                         __LINE__);                                                                          \
+    } while (0)
+
+static void free_anneal(bm_anneal* a) {
+    if (!a) return;
+    (void)hipSetDevice(a->device);
+    if (a->stream) (void)hipStreamSynchronize(a->stream);
+    if (a->exec) (void)hipGraphExecDestroy(a->exec);
+    if (a->graph) (void)hipGraphDestroy(a->graph);
+    (void)hipFree(a->d_insts);
+    (void)hipFree(a->d_chains);
+    (void)hipFree(a->d_pool);
+    (void)hipFree(a->d_traj);
+    (void)hipFree(a->d_scores);
+    if (a->stream) (void)hipStreamDestroy(a->stream);
+    delete a;
+}
+
+extern "C" void bm_anneal_destroy(bm_anneal* a) { free_anneal(a); }
+
+extern "C" const char* bm_anneal_last_error(const bm_anneal* a) {
+    return a ? a->err : bm_global_error();
+}
+
+extern "C" int64_t bm_anneal_num_chains(const bm_anneal* a) { return a ? a->n_chains : 0; }
+
+extern "C" int bm_anneal_create(bm_anneal** out, bm_ctx* const* ctxs, int32_t n_inst, const bm_anneal_params* p) {
+    if (!out) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: out is NULL");
+    *out = nullptr;
+    if (!ctxs || !p || n_inst < 1) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: need >= 1 context and params");
+    if (p->chains_per_instance < 1) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: chains_per_instance must be >= 1");
+    if (!(p->t0 > 0.0) || !isfinite(p->t0)) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: t0 must be finite and > 0");
+    if (!(p->decay > 0.0 && p->decay <= 1.0)) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: decay must be in (0, 1]");
+    if (!(p->p_rot3 >= 0.0 && p->p_rot3 <= 1.0)) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: p_rot3 must be in [0, 1]");
+    if (p->traj_cap < 0) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: traj_cap must be >= 0");
+    for (int32_t i = 0; i < n_inst; i++)
+        if (!ctxs[i]) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: context %d is NULL", i);
+    const int dev = bm_ctx_device(ctxs[0]);
+    for (int32_t i = 1; i < n_inst; i++)
+        if (bm_ctx_device(ctxs[i]) != dev)
+            return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: all contexts must be on one device");
+
+    bm_anneal* a = new bm_anneal();
+    memset(a->err, 0, sizeof a->err);
+    a->device = dev;
+    a->n_inst = n_inst;
+    a->per_inst = p->chains_per_instance;
+    a->n_chains = (int64_t)n_inst * p->chains_per_instance;
+    a->seed = p->seed;
+    a->p_rot3 = p->p_rot3;
+    a->ctxs.assign(ctxs, ctxs + n_inst);
+    int rc = BM_OK;
+#define ACHK(expr)                                                                                    \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            rc = afail(nullptr, e_ == hipErrorOutOfMemory ? BM_ERR_NOMEM : BM_ERR_HIP, "%s failed: %s", \
+                       #expr, hipGetErrorString(e_));                                                 \
+            free_anneal(a);                                                                           \
+            return rc;                                                                                \
+        }                                                                                             \
+    } while (0)
+    ACHK(hipSetDevice(dev));
+    ACHK(hipStreamCreate(&a->stream));
+
+    /* initial scores: one device recount per instance, on its own stream */
+    std::vector<int64_t> s0(n_inst);
+    for (int32_t i = 0; i < n_inst; i++) {
+        int r = bm_score(ctxs[i], &s0[i]);
+        if (r != BM_OK) {
+            rc = afail(nullptr, r, "bm_anneal_create: score of instance %d failed: %s", i, bm_last_error(ctxs[i]));
+            free_anneal(a);
+            return rc;
+        }
+    }
+
+    a->compact = 1;
+    for (int32_t i = 0; i < n_inst; i++) {
+        a->graphs.push_back(bm_ctx_devgraph(ctxs[i]));
+        a->compact &= a->graphs.back().compact ? 1 : 0;
+    }
+    /* launch shape: the swap kernel's tuning of the first context, the LDS
+     * slice kept at the same byte size when the slot format widens */
+    a->L = bm_ctx_launch(ctxs[0], BM_KERNEL_SWAP);
+    if (!a->compact && a->graphs[0].compact) a->L.lds_slots /= 2;
+
+    int64_t pool = 0;
+    a->pi_off.resize((size_t)a->n_chains);
+    std::vector<BmChain> h((size_t)a->n_chains);
+    for (int32_t i = 0; i < n_inst; i++) {
+        const int64_t len = (int64_t)a->graphs[i].n + 1;
+        for (int64_t j = 0; j < a->per_inst; j++) {
+            const int64_t c = (int64_t)i * a->per_inst + j;
+            a->pi_off[c] = pool;
+            BmChain& ch = h[c];
+            memset(&ch, 0, sizeof ch);
+            ch.score = s0[i];
+            ch.best = s0[i];
+            ch.pi_off = pool;
+            ch.seed = bm_rand_at(p->seed, (uint64_t)c);
+            ch.inst = i;
+            ch.at_best = 1;
+            pool += 2 * len;
+        }
+    }
+    ACHK(hipMalloc((void**)&a->d_insts, sizeof(BmDevGraph) * (size_t)n_inst));
+    ACHK(hipMalloc((void**)&a->d_chains, sizeof(BmChain) * (size_t)a->n_chains));
+    ACHK(hipMalloc((void**)&a->d_pool, sizeof(int32_t) * (size_t)pool));
+    ACHK(hipMalloc((void**)&a->d_scores, sizeof(int64_t) * (size_t)a->n_chains));
+    if (p->traj_cap > 0) ACHK(hipMalloc((void**)&a->d_traj, sizeof(int64_t) * (size_t)a->n_chains * p->traj_cap));
+    ACHK(hipMemcpyAsync(a->d_insts, a->graphs.data(), sizeof(BmDevGraph) * (size_t)n_inst, hipMemcpyHostToDevice,
+                        a->stream));
+    ACHK(hipMemcpyAsync(a->d_chains, h.data(), sizeof(BmChain) * (size_t)a->n_chains, hipMemcpyHostToDevice,
+                        a->stream));
+    for (int32_t i = 0; i < n_inst; i++) {
+        const size_t bytes = sizeof(int32_t) * ((size_t)a->graphs[i].n + 1);
+        for (int64_t j = 0; j < a->per_inst; j++) {
+            const int64_t off = a->pi_off[(size_t)i * a->per_inst + j];
+            ACHK(hipMemcpyAsync(a->d_pool + off, a->graphs[i].pi, bytes, hipMemcpyDeviceToDevice, a->stream));
+            ACHK(hipMemcpyAsync(a->d_pool + off + a->graphs[i].n + 1, a->graphs[i].pi, bytes, hipMemcpyDeviceToDevice,
+                                a->stream));
+        }
+    }
+    a->sc.beta0 = 1.0 / p->t0;
+    a->sc.lam = -log(p->decay);
+    a->sc.p_rot3 = (uint64_t)llround(p->p_rot3 * 4294967296.0);
+    a->sc.traj_cap = p->traj_cap;
+    a->sc.pad = 0;
+    ACHK(bm_anneal_grid(a->compact, a->sc.p_rot3 != 0, a->L, a->n_chains, &a->grid, &a->block, &a->lds));
+    ACHK(hipStreamSynchronize(a->stream));
+#undef ACHK
+    *out = a;
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_run(bm_anneal* a, int64_t steps, int32_t launches) {
+    if (!a) return afail(nullptr, BM_ERR_ARG, "bm_anneal_run: NULL object");
+    if (steps < 1 || launches < 0) return afail(a, BM_ERR_ARG, "bm_anneal_run: steps must be >= 1 and launches >= 0");
+    AHIP(a, hipSetDevice(a->device));
+    if (!a->exec || a->graph_steps != steps) {
+        if (a->exec) (void)hipGraphExecDestroy(a->exec);
+        if (a->graph) (void)hipGraphDestroy(a->graph);
+        a->exec = nullptr;
+        a->graph = nullptr;
+        AHIP(a, hipStreamBeginCapture(a->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t le = bm_launch_anneal(a->d_insts, a->compact, a->d_chains, a->n_chains, a->d_pool, steps, a->sc,
+                                         a->d_traj, a->L, a->grid, a->block, a->lds, a->stream);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(a->stream, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return afail(a, BM_ERR_HIP, "bm_anneal_run: capture failed: %s / %s", hipGetErrorString(le),
+                         hipGetErrorString(ce));
+        }
+        a->graph = g;
+        AHIP(a, hipGraphInstantiate(&a->exec, a->graph, nullptr, nullptr, 0));
+        a->graph_steps = steps;
+    }
+    if (launches == 0) return BM_OK;
+    std::vector<hipEvent_t> ev((size_t)launches + 1, nullptr);
+    int rc = BM_OK;
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) { rc = afail(a, BM_ERR_HIP, "bm_anneal_run: hipEventCreate failed"); break; }
+    if (rc == BM_OK) {
+        hipError_t e = hipEventRecord(ev[0], a->stream);
+        for (int32_t t = 0; t < launches && e == hipSuccess; t++) {
+            e = hipGraphLaunch(a->exec, a->stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[(size_t)t + 1], a->stream);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(a->stream);
+        if (e != hipSuccess) rc = afail(a, BM_ERR_HIP, "bm_anneal_run: launch failed: %s", hipGetErrorString(e));
+        if (rc == BM_OK) {
+            for (int32_t t = 0; t < launches; t++) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, ev[t], ev[(size_t)t + 1]) == hipSuccess) a->total_ms += ms;
+            }
+            a->launches += launches;
+        }
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
+}
+
+static int fetch_chains(bm_anneal* a, std::vector<BmChain>& h) {
+    h.resize((size_t)a->n_chains);
+    AHIP(a, hipSetDevice(a->device));
+    AHIP(a, hipMemcpyAsync(h.data(), a->d_chains, sizeof(BmChain) * (size_t)a->n_chains, hipMemcpyDeviceToHost,
+                           a->stream));
+    AHIP(a, hipStreamSynchronize(a->stream));
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_state(bm_anneal* a, int64_t* score, int64_t* best, int64_t* steps, int64_t* accepted) {
+    if (!a) return afail(nullptr, BM_ERR_ARG, "bm_anneal_state: NULL object");
+    std::vector<BmChain> h;
+    int rc = fetch_chains(a, h);
+    if (rc != BM_OK) return rc;
+    for (int64_t c = 0; c < a->n_chains; c++) {
+        if (score) score[c] = h[c].score;
+        if (best) best[c] = h[c].best;
+        if (steps) steps[c] = h[c].step;
+        if (accepted) accepted[c] = h[c].accepted;
+    }
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_mapping(bm_anneal* a, int64_t chain, int32_t which, int32_t* pi) {
+    if (!a || !pi) return afail(a, BM_ERR_ARG, "bm_anneal_mapping: null argument");
+    if (chain < 0 || chain >= a->n_chains || (which != 0 && which != 1))
+        return afail(a, BM_ERR_ARG, "bm_anneal_mapping: chain out of range or which not 0/1");
+    std::vector<BmChain> h;
+    int rc = fetch_chains(a, h);
+    if (rc != BM_OK) return rc;
+    const BmChain& ch = h[(size_t)chain];
+    const int64_t len = (int64_t)a->graphs[ch.inst].n + 1;
+    const int64_t off = ch.pi_off + ((which == 1 && !ch.at_best) ? len : 0);
+    AHIP(a, hipMemcpyAsync(pi, a->d_pool + off, sizeof(int32_t) * (size_t)len, hipMemcpyDeviceToHost, a->stream));
+    AHIP(a, hipStreamSynchronize(a->stream));
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_trajectory(bm_anneal* a, int64_t* traj, int32_t* count) {
+    if (!a) return afail(nullptr, BM_ERR_ARG, "bm_anneal_trajectory: NULL object");
+    std::vector<BmChain> h;
+    int rc = fetch_chains(a, h);
+    if (rc != BM_OK) return rc;
+    const int32_t cnt = a->n_chains ? (h[0].nseg < a->sc.traj_cap ? h[0].nseg : a->sc.traj_cap) : 0;
+    if (count) *count = cnt;
+    if (traj && a->d_traj && a->sc.traj_cap > 0) {
+        AHIP(a, hipMemcpyAsync(traj, a->d_traj, sizeof(int64_t) * (size_t)a->n_chains * a->sc.traj_cap,
+                               hipMemcpyDeviceToHost, a->stream));
+        AHIP(a, hipStreamSynchronize(a->stream));
+    }
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_verify(bm_anneal* a, int64_t* mismatches) {
+    if (!a || !mismatches) return afail(a, BM_ERR_ARG, "bm_anneal_verify: null argument");
+    std::vector<BmChain> h;
+    int rc = fetch_chains(a, h);
+    if (rc != BM_OK) return rc;
+    AHIP(a, hipMemsetAsync(a->d_scores, 0, sizeof(int64_t) * (size_t)a->n_chains, a->stream));
+    for (int64_t c = 0; c < a->n_chains; c++) {
+        BmDevGraph g = a->graphs[h[c].inst];
+        g.pi = a->d_pool + h[c].pi_off;
+        AHIP(a, bm_launch_score(g, bm_ctx_launch(a->ctxs[h[c].inst], BM_KERNEL_SCORE), a->d_scores + c, a->stream));
+    }
+    std::vector<int64_t> s((size_t)a->n_chains);
+    AHIP(a, hipMemcpyAsync(s.data(), a->d_scores, sizeof(int64_t) * (size_t)a->n_chains, hipMemcpyDeviceToHost,
+                           a->stream));
+    AHIP(a, hipStreamSynchronize(a->stream));
+    int64_t bad = 0;
+    for (int64_t c = 0; c < a->n_chains; c++) bad += s[c] != h[c].score;
+    *mismatches = bad;
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_timing(bm_anneal* a, int64_t* launches, double* total_ms) {
+    if (!a) return afail(nullptr, BM_ERR_ARG, "bm_anneal_timing: NULL object");
+    if (launches) *launches = a->launches;
+    if (total_ms) *total_ms = a->total_ms;
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_algo_bytes(bm_anneal* a, int64_t* bytes, int64_t* steps) {
+    if (!a) return afail(nullptr, BM_ERR_ARG, "bm_anneal_algo_bytes: NULL object");
+    std::vector<BmChain> h;
+    int rc = fetch_chains(a, h);
+    if (rc != BM_OK) return rc;
+    std::vector<const bm_graph_host*> g((size_t)a->n_inst);
+    for (int32_t i = 0; i < a->n_inst; i++) g[i] = bm_ctx_male(a->ctxs[i]);
+    std::vector<int32_t> inst((size_t)a->n_chains);
+    std::vector<uint64_t> seeds((size_t)a->n_chains);
+    std::vector<int64_t> st((size_t)a->n_chains);
+    int64_t S = 0;
+    for (int64_t c = 0; c < a->n_chains; c++) {
+        inst[c] = h[c].inst;
+        seeds[c] = h[c].seed;
+        st[c] = h[c].step;
+        S += h[c].step;
+    }
+    const int64_t B = bm_chain_algo_bytes(g.data(), inst.data(), seeds.data(), st.data(), a->n_chains, a->sc.p_rot3);
+    if (bytes) *bytes = B;
+    if (steps) *steps = S;
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_schedule(const bm_anneal* a, double* beta0, double* lam, uint64_t* p_rot3) {
+    if (!a) return BM_ERR_ARG;
+    if (beta0) *beta0 = a->sc.beta0;
+    if (lam) *lam = a->sc.lam;
+    if (p_rot3) *p_rot3 = a->sc.p_rot3;
+    return BM_OK;
+}

@@ -1088,4 +1088,10 @@ extern "C" int bm_device_info(bm_ctx* c, char* buf, size_t len) {
 
 extern "C" const bm_graph_host* bm_ctx_male(const bm_ctx* c) { return c ? &c->male : nullptr; }
 
+BmDevGraph bm_ctx_devgraph(const bm_ctx* c) { return devgraph(c); }
+int bm_ctx_device(const bm_ctx* c) { return c->device; }
+int bm_ctx_num_cus(const bm_ctx* c) { return c->num_cus; }
+hipStream_t bm_ctx_stream(const bm_ctx* c) { return c->stream; }
+BmLaunch bm_ctx_launch(const bm_ctx* c, int kernel) { return c->tune[kernel]; }
+
 extern "C" int bm_ctx_fail(bm_ctx* c, int code, const char* msg) { return fail(c, code, "%s", msg); }

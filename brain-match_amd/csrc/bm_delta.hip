@@ -23,6 +23,7 @@
  * src/graph-re-alignment-gpu-3-swap.cu:186-248); here the work is
  * O(deg) per candidate and exact for any int32 weights.
  */
+#include "bm_detexp.h"
 #include "bm_device.h"
 #include "bm_devutil.h"
 #include "bm_rng.h"
@@ -341,6 +342,36 @@ __device__ __forceinline__ int64_t swap_scan(const int2* __restrict__ adj, const
     return acc;
 }
 
+/* Exact delta of swapping the images of u != v (f1 = pi(u), f2 = pi(v)) in
+ * the matching g.pi, computed by the whole wave; every lane returns it. */
+template <bool COMPACT, int U>
+__device__ __forceinline__ int64_t swap_delta_wave(const BmDevGraph& g, int32_t u, int32_t v, int32_t f1,
+                                                   int32_t f2, typename Slot<COMPACT>::T* lds, uint32_t cap,
+                                                   int lane) {
+    const int64_t ou0 = uni64(g.m_out_ptr[u]), ou1 = uni64(g.m_out_ptr[u + 1]);
+    const int64_t ov0 = uni64(g.m_out_ptr[v]), ov1 = uni64(g.m_out_ptr[v + 1]);
+    const int64_t iu0 = uni64(g.m_in_ptr[u]), iu1 = uni64(g.m_in_ptr[u + 1]);
+    const int64_t iv0 = uni64(g.m_in_ptr[v]), iv1 = uni64(g.m_in_ptr[v + 1]);
+    int64_t acc;
+    {
+        Tab t[2] = {make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f1),
+                    make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f2)};
+        place<2>(t, cap);
+        build<COMPACT, 2, U>(lds, t, g.f_out, lane);
+        acc = swap_scan<COMPACT, true, U>(g.m_out, g.pi, ou0, ou1 - ou0, ov0, ov1 - ov0, lds, t[0], t[1], u, v, f1,
+                                       f2, lane);
+    }
+    {
+        Tab t[2] = {make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f1),
+                    make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f2)};
+        place<2>(t, cap);
+        build<COMPACT, 2, U>(lds, t, g.f_in, lane);
+        acc += swap_scan<COMPACT, false, U>(g.m_in, g.pi, iu0, iu1 - iu0, iv0, iv1 - iv0, lds, t[0], t[1], u, v,
+                                         f1, f2, lane);
+    }
+    return wave_sum64(acc);
+}
+
 template <int MODE, bool COMPACT, int U>
 __global__ void __launch_bounds__(512) k_swap(BmDevGraph g, const int2* __restrict__ cand, uint64_t seed,
                                              int64_t k, int64_t* __restrict__ delta, uint32_t cap) {
@@ -367,28 +398,7 @@ __global__ void __launch_bounds__(512) k_swap(BmDevGraph g, const int2* __restri
             continue;
         }
         const int32_t f1 = uni(g.pi[u]), f2 = uni(g.pi[v]);
-        const int64_t ou0 = uni64(g.m_out_ptr[u]), ou1 = uni64(g.m_out_ptr[u + 1]);
-        const int64_t ov0 = uni64(g.m_out_ptr[v]), ov1 = uni64(g.m_out_ptr[v + 1]);
-        const int64_t iu0 = uni64(g.m_in_ptr[u]), iu1 = uni64(g.m_in_ptr[u + 1]);
-        const int64_t iv0 = uni64(g.m_in_ptr[v]), iv1 = uni64(g.m_in_ptr[v + 1]);
-        int64_t acc;
-        {
-            Tab t[2] = {make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f1),
-                        make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f2)};
-            place<2>(t, cap);
-            build<COMPACT, 2, U>(lds, t, g.f_out, lane);
-            acc = swap_scan<COMPACT, true, U>(g.m_out, g.pi, ou0, ou1 - ou0, ov0, ov1 - ov0, lds, t[0], t[1], u, v, f1,
-                                           f2, lane);
-        }
-        {
-            Tab t[2] = {make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f1),
-                        make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f2)};
-            place<2>(t, cap);
-            build<COMPACT, 2, U>(lds, t, g.f_in, lane);
-            acc += swap_scan<COMPACT, false, U>(g.m_in, g.pi, iu0, iu1 - iu0, iv0, iv1 - iv0, lds, t[0], t[1], u, v,
-                                             f1, f2, lane);
-        }
-        acc = wave_sum64(acc);
+        const int64_t acc = swap_delta_wave<COMPACT, U>(g, u, v, f1, f2, lds, cap, lane);
         if (lane == 0) delta[c] = acc;
     }
 }
@@ -467,6 +477,65 @@ __device__ __forceinline__ void rot3_scan(const int2* __restrict__ adj, const in
     }
 }
 
+/* Exact deltas d[p], p = 1..5, of the five non-identity permutations of the
+ * images f of the distinct vertices vtx in g.pi, computed by the whole wave;
+ * every lane returns them (d[0] = 0). */
+template <bool COMPACT>
+__device__ __forceinline__ void rot3_delta_wave(const BmDevGraph& g, const int32_t* vtx, const int32_t* f,
+                                                typename Slot<COMPACT>::T* lds, uint32_t cap, int lane,
+                                                int64_t* d) {
+    Rot3Acc A;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) A.s[i][j] = 0;
+#pragma unroll
+    for (int p = 0; p < 6; p++) A.in[p] = 0;
+    {
+        Tab T[3] = {make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f[0]),
+                    make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f[1]),
+                    make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f[2])};
+        place<3>(T, cap);
+        build<COMPACT, 3>(lds, T, g.f_out, lane);
+        int64_t b[3], len[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            b[i] = uni64(g.m_out_ptr[vtx[i]]);
+            len[i] = uni64(g.m_out_ptr[vtx[i] + 1]) - b[i];
+        }
+        rot3_scan<COMPACT, true>(g.m_out, g.pi, b, len, lds, T, vtx, f, A, lane);
+    }
+    {
+        Tab T[3] = {make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[0]),
+                    make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[1]),
+                    make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[2])};
+        place<3>(T, cap);
+        build<COMPACT, 3>(lds, T, g.f_in, lane);
+        int64_t b[3], len[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            b[i] = uni64(g.m_in_ptr[vtx[i]]);
+            len[i] = uni64(g.m_in_ptr[vtx[i] + 1]) - b[i];
+        }
+        rot3_scan<COMPACT, false>(g.m_in, g.pi, b, len, lds, T, vtx, f, A, lane);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) A.s[i][j] = wave_sum64(A.s[i][j]);
+    d[0] = 0;
+#pragma unroll
+    for (int p = 1; p <= 5; p++) {
+        int64_t x = wave_sum64(A.in[p]);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int ni = perm_of(p, i);
+            x += sel3(ni, A.s[i][0], A.s[i][1], A.s[i][2]) - A.s[i][i];
+        }
+        d[p] = x;
+    }
+}
+
 template <bool COMPACT>
 __global__ void __launch_bounds__(512) k_rot3(BmDevGraph g, int4* __restrict__ cand, int64_t k,
                                              int64_t* __restrict__ best, uint8_t* __restrict__ perm,
@@ -482,60 +551,14 @@ __global__ void __launch_bounds__(512) k_rot3(BmDevGraph g, int4* __restrict__ c
         const int4 q = cand[c];
         const int32_t vtx[3] = {uni(q.x), uni(q.y), uni(q.z)};
         const int32_t f[3] = {uni(g.pi[vtx[0]]), uni(g.pi[vtx[1]]), uni(g.pi[vtx[2]])};
-        Rot3Acc A;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) A.s[i][j] = 0;
-#pragma unroll
-        for (int p = 0; p < 6; p++) A.in[p] = 0;
-        {
-            Tab T[3] = {make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f[0]),
-                        make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f[1]),
-                        make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f[2])};
-            place<3>(T, cap);
-            build<COMPACT, 3>(lds, T, g.f_out, lane);
-            int64_t b[3], len[3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                b[i] = uni64(g.m_out_ptr[vtx[i]]);
-                len[i] = uni64(g.m_out_ptr[vtx[i] + 1]) - b[i];
-            }
-            rot3_scan<COMPACT, true>(g.m_out, g.pi, b, len, lds, T, vtx, f, A, lane);
-        }
-        {
-            Tab T[3] = {make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[0]),
-                        make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[1]),
-                        make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[2])};
-            place<3>(T, cap);
-            build<COMPACT, 3>(lds, T, g.f_in, lane);
-            int64_t b[3], len[3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                b[i] = uni64(g.m_in_ptr[vtx[i]]);
-                len[i] = uni64(g.m_in_ptr[vtx[i] + 1]) - b[i];
-            }
-            rot3_scan<COMPACT, false>(g.m_in, g.pi, b, len, lds, T, vtx, f, A, lane);
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-#pragma unroll
-            for (int j = 0; j < 3; j++) A.s[i][j] = wave_sum64(A.s[i][j]);
-#pragma unroll
-        for (int p = 1; p <= 5; p++) A.in[p] = wave_sum64(A.in[p]);
+        int64_t d5[6];
+        rot3_delta_wave<COMPACT>(g, vtx, f, lds, cap, lane, d5);
         if (lane == 0) {
             int64_t bd = 0;
             int bp = 0;
 #pragma unroll
-            for (int p = 1; p <= 5; p++) {
-                int64_t d = A.in[p];
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    const int ni = perm_of(p, i);
-                    d += sel3(ni, A.s[i][0], A.s[i][1], A.s[i][2]) - A.s[i][i];
-                }
-                if (d > bd) { bd = d; bp = p; }
-            }
+            for (int p = 1; p <= 5; p++)
+                if (d5[p] > bd) { bd = d5[p]; bp = p; }
             best[c] = bd;
             if (perm) perm[c] = (uint8_t)bp;
             if (write_perm_into_cand) cand[c].w = bp;
@@ -582,6 +605,116 @@ __global__ void __launch_bounds__(512) k_score(BmDevGraph g, int64_t* __restrict
     }
     acc = wave_sum64(acc);
     if (lane == 0 && acc != 0) atomicAdd(reinterpret_cast<unsigned long long*>(acc_out), (unsigned long long)acc);
+}
+
+/* ------------------------------------------------------------------ */
+/* Annealing chains                                                    */
+/* ------------------------------------------------------------------ */
+
+/* One wave per chain.  Step s of a chain with stream seed q:
+ *   kind:  r_k = rand(q2, s); a 3-vertex move if lo32(r_k) < p_rot3 (n >= 3);
+ *   swap:  (u, v) = uniform pair s of stream q0, delta exact;
+ *   3-move: (a, b, c) = uniform triple s of stream q3, permutation
+ *          p = 1 + range(hi32(r_k), 5), delta of that permutation exact;
+ *   accept if delta >= 0, else if U < exp(delta * beta_s) with
+ *          U = (rand(q1, s) >> 11) * 2^-53 and beta_s = beta0 * exp(s * lam),
+ *          exp being the deterministic bm_det_exp;
+ * with q_i = rand(q, i).  The best matching is kept lazily: a chain standing
+ * on its best score copies the matching aside only when it accepts a move
+ * that lowers the score (the best is the latest matching reaching the
+ * maximum).  The Metropolis rule and the geometric schedule follow
+ * simulated_annealing (src/best-swap/score-lookup.c:345-400); its delta
+ * comes from a lookup table and its stream from xoshiro256**, here the
+ * delta is recomputed exactly every step and the streams are counter-based. */
+template <bool COMPACT, int U, bool ROT>
+__global__ void __launch_bounds__(512) k_anneal(const BmDevGraph* __restrict__ insts, BmChain* __restrict__ chains,
+                                               int64_t n_chains, int32_t* __restrict__ pool, int64_t steps,
+                                               BmAnnealSched sc, int64_t* __restrict__ traj, uint32_t cap) {
+    typedef typename Slot<COMPACT>::T S;
+    extern __shared__ uint4 smem[];
+    const int lane = threadIdx.x & (BM_WAVE - 1);
+    const int wave = threadIdx.x / BM_WAVE;
+    const int wpb = blockDim.x / BM_WAVE;
+    S* lds = reinterpret_cast<S*>(smem) + (size_t)wave * cap;
+    const int64_t nw = (int64_t)gridDim.x * wpb;
+    for (int64_t c = (int64_t)blockIdx.x * wpb + wave; c < n_chains; c += nw) {
+        const BmChain ch = chains[c];
+        BmDevGraph g = insts[uni(ch.inst)];
+        const int32_t n = g.n;
+        int32_t* pi = pool + uni64(ch.pi_off);
+        int32_t* pib = pi + ((int64_t)n + 1);
+        g.pi = pi;
+        const uint64_t q = (uint64_t)uni64((int64_t)ch.seed);
+        const uint64_t q0 = bm_rand_at(q, 0), q1 = bm_rand_at(q, 1), q2 = bm_rand_at(q, 2), q3 = bm_rand_at(q, 3);
+        int64_t score = uni64(ch.score), best = uni64(ch.best), nacc = uni64(ch.accepted);
+        int at_best = uni(ch.at_best);
+        const int64_t s0 = uni64(ch.step);
+        for (int64_t s = s0; s < s0 + steps; s++) {
+            const uint64_t rk = bm_rand_at(q2, (uint64_t)s);
+            /* ROT = false: swap-only chains (p_rot3 == 0), built without the
+             * 3-vertex path and its registers */
+            const bool rot = ROT && n >= 3 && (uint64_t)(uint32_t)rk < sc.p_rot3;
+            int32_t vtx[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+            int64_t d;
+            int p = 0;
+            if (!rot) {
+                bm_uniform_pair(q0, (uint64_t)s, n, &vtx[0], &vtx[1]);
+                f[0] = uni(pi[vtx[0]]);
+                f[1] = uni(pi[vtx[1]]);
+                d = swap_delta_wave<COMPACT, U>(g, vtx[0], vtx[1], f[0], f[1], lds, cap, lane);
+            } else {
+                bm_uniform_triple(q3, (uint64_t)s, n, &vtx[0], &vtx[1], &vtx[2]);
+#pragma unroll
+                for (int i = 0; i < 3; i++) f[i] = uni(pi[vtx[i]]);
+                int64_t d5[6];
+                rot3_delta_wave<COMPACT>(g, vtx, f, lds, cap, lane, d5);
+                p = 1 + (int)bm_range32((uint32_t)(rk >> 32), 5u);
+                d = d5[1];
+#pragma unroll
+                for (int j = 2; j <= 5; j++)
+                    if (p == j) d = d5[j];
+            }
+            bool accept = d >= 0;
+            if (!accept) {
+                const double beta = sc.beta0 * bm_det_exp((double)s * sc.lam);
+                const double x = (double)d * beta;
+                const double r = (double)(bm_rand_at(q1, (uint64_t)s) >> 11) * 0x1p-53;
+                accept = r < bm_det_exp(x);
+            }
+            if (!accept) continue;
+            if (at_best && d < 0) {
+                for (int64_t i = lane; i <= n; i += BM_WAVE) pib[i] = pi[i];
+                at_best = 0;
+            }
+            if (lane == 0) {
+                if (!rot) {
+                    pi[vtx[0]] = f[1];
+                    pi[vtx[1]] = f[0];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 3; i++) pi[vtx[i]] = sel3(perm_of(p, i), f[0], f[1], f[2]);
+                }
+            }
+            __threadfence_block();
+            score += d;
+            nacc++;
+            if (score >= best) {
+                best = score;
+                at_best = 1;
+            }
+        }
+        if (lane == 0) {
+            BmChain o = ch;
+            o.score = score;
+            o.best = best;
+            o.accepted = nacc;
+            o.step = s0 + steps;
+            o.at_best = at_best;
+            if (traj && o.nseg < sc.traj_cap) traj[c * sc.traj_cap + o.nseg] = score;
+            o.nseg++;
+            chains[c] = o;
+        }
+    }
 }
 
 template <typename K>
@@ -651,6 +784,31 @@ hipError_t bm_launch_rot3(const BmDevGraph& g, const BmLaunch& L, const int4* ca
         hipLaunchKernelGGL(k_rot3<false>, grid, block, lds, s, g, const_cast<int4*>(cand), k, best, perm,
                            cand_out_perm ? 1 : 0, (uint32_t)L.lds_slots);
     }
+    return hipGetLastError();
+}
+
+hipError_t bm_anneal_grid(int compact, int rot, const BmLaunch& L, int64_t n_chains, dim3* grid, dim3* block,
+                          size_t* lds) {
+    if (compact)
+        return rot ? launch_delta(k_anneal<true, 4, true>, L, n_chains, 4, nullptr, grid, block, lds)
+                   : launch_delta(k_anneal<true, 4, false>, L, n_chains, 4, nullptr, grid, block, lds);
+    return rot ? launch_delta(k_anneal<false, 4, true>, L, n_chains, 8, nullptr, grid, block, lds)
+               : launch_delta(k_anneal<false, 4, false>, L, n_chains, 8, nullptr, grid, block, lds);
+}
+
+hipError_t bm_launch_anneal(const BmDevGraph* insts, int compact, BmChain* chains, int64_t n_chains,
+                            int32_t* pool, int64_t steps, const BmAnnealSched& sc, int64_t* traj,
+                            const BmLaunch& L, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
+    if (n_chains <= 0 || steps <= 0) return hipSuccess;
+    const uint32_t cap = (uint32_t)L.lds_slots;
+#define BM_ANNEAL_LAUNCH(C, R) \
+    hipLaunchKernelGGL((k_anneal<C, 4, R>), grid, block, lds, s, insts, chains, n_chains, pool, steps, sc, traj, cap)
+    if (compact) {
+        if (sc.p_rot3) BM_ANNEAL_LAUNCH(true, true); else BM_ANNEAL_LAUNCH(true, false);
+    } else {
+        if (sc.p_rot3) BM_ANNEAL_LAUNCH(false, true); else BM_ANNEAL_LAUNCH(false, false);
+    }
+#undef BM_ANNEAL_LAUNCH
     return hipGetLastError();
 }
 

@@ -70,6 +70,35 @@ hipError_t bm_launch_rot3(const BmDevGraph& g, const BmLaunch& L, const int4* ca
                           int64_t* best, uint8_t* perm, int4* cand_out_perm, hipStream_t s);
 hipError_t bm_launch_score(const BmDevGraph& g, const BmLaunch& L, int64_t* acc, hipStream_t s);
 
+/* Annealing chains (k_anneal in bm_delta.hip): per-chain state in device memory. */
+struct BmChain {
+    int64_t score;      /* score of the current matching */
+    int64_t best;       /* best score reached */
+    int64_t step;       /* steps taken */
+    int64_t accepted;   /* accepted moves */
+    int64_t pi_off;     /* current matching at pool + pi_off, best-so-far copy right after it */
+    uint64_t seed;      /* stream seed of the chain */
+    int32_t inst;       /* graph (instance) index */
+    int32_t at_best;    /* 1: the current matching is the best one (the copy is stale) */
+    int32_t nseg;       /* launches run (trajectory entries) */
+    int32_t pad;
+};
+
+struct BmAnnealSched {
+    double beta0;       /* inverse initial temperature */
+    double lam;         /* beta_s = beta0 * exp(s * lam) */
+    uint64_t p_rot3;    /* probability of a 3-vertex move, in units of 2^-32 (0..2^32) */
+    int32_t traj_cap;   /* trajectory entries kept per chain */
+    int32_t pad;
+};
+
+/* Persistent grid for n_chains chains (one wave each); computed once. */
+hipError_t bm_anneal_grid(int compact, int rot, const BmLaunch& L, int64_t n_chains, dim3* grid, dim3* block, size_t* lds);
+/* Advances every chain by `steps` steps.  insts: device array of graphs. */
+hipError_t bm_launch_anneal(const BmDevGraph* insts, int compact, BmChain* chains, int64_t n_chains,
+                            int32_t* pool, int64_t steps, const BmAnnealSched& sc, int64_t* traj,
+                            const BmLaunch& L, dim3 grid, dim3 block, size_t lds, hipStream_t s);
+
 hipError_t bm_launch_gen_pairs(int32_t n, uint64_t seed, int64_t k, int2* cand, hipStream_t s);
 hipError_t bm_launch_gen_triples(int32_t n, uint64_t seed, int64_t k, int4* cand, hipStream_t s);
 /* reorder: c changed ids ascending, then the rest ascending (n entries). */
@@ -97,6 +126,14 @@ hipError_t bm_launch_accept(const BmDevGraph& g, int32_t* pi_rw, const int2* can
                             int64_t k, int64_t max_accept, void* scratch, size_t scratch_bytes,
                             int64_t* out_acc_idx /* k */, int64_t* out_meta /* [n_acc, gain, n_moves] */,
                             int4* out_moves /* 2k {x, old, new, 0}, may be NULL */, hipStream_t s);
+
+/* Context internals for other host translation units (bm_api.hip). */
+struct bm_ctx;
+BmDevGraph bm_ctx_devgraph(const bm_ctx* c);
+int bm_ctx_device(const bm_ctx* c);
+int bm_ctx_num_cus(const bm_ctx* c);
+hipStream_t bm_ctx_stream(const bm_ctx* c);
+BmLaunch bm_ctx_launch(const bm_ctx* c, int kernel);
 
 hipError_t bm_launch_apply_swap(int32_t* pi, int32_t u, int32_t v, hipStream_t s);
 hipError_t bm_launch_apply_perm3(int32_t* pi, int32_t a, int32_t b, int32_t c, int32_t p, hipStream_t s);

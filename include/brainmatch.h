@@ -368,6 +368,62 @@ int bm_search(bm_ctx* ctx, const bm_search_opts* opts, int32_t* best_pi, bm_sear
 int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int32_t verify,
                  int64_t* score_trace, bm_search_result* res);
 
+/* ------------------------------------------------------------------ */
+/* Annealing chains (many independent chains, batched over instances)  */
+/* ------------------------------------------------------------------ */
+
+/* Simulated annealing as simulated_annealing / probabilistic_swaps in
+ * src/best-swap/score-lookup.c:300-400: a uniform random move is accepted
+ * when delta >= 0 or when U < exp(delta / T_s), with T_s = t0 * decay^s.
+ * Differences: the delta is recomputed exactly every step (no int16 lookup
+ * table to patch), moves are swaps or, with probability p_rot3, one of the
+ * five permutations of a uniform vertex triple, the random streams are
+ * counter-based per chain, and exp is a deterministic function with the same
+ * bits on the host and the device, so that any chain can be replayed exactly
+ * on the CPU.  One chain per wavefront; all chains of all instances advance
+ * in one kernel launch, replayed from a captured HIP graph. */
+typedef struct bm_anneal bm_anneal;
+
+typedef struct bm_anneal_params {
+    uint64_t seed;
+    int32_t chains_per_instance;  /* >= 1 */
+    double t0;                    /* initial temperature, > 0 (50) */
+    double decay;                 /* per-step factor in (0, 1] (0.9999) */
+    double p_rot3;                /* probability of a 3-vertex move in [0, 1] */
+    int32_t traj_cap;             /* score samples kept per chain (one per launch) */
+} bm_anneal_params;
+
+/* One instance per context (all on the same device); every chain of instance
+ * i starts from context i's current matching.  Chain j of instance i has the
+ * global index i * chains_per_instance + j and the stream seed
+ * bm_rand_at(seed, global index). */
+int bm_anneal_create(bm_anneal** out, bm_ctx* const* ctxs, int32_t n_instances, const bm_anneal_params* p);
+void bm_anneal_destroy(bm_anneal* a);
+int64_t bm_anneal_num_chains(const bm_anneal* a);
+/* Advance every chain by steps_per_launch steps, `launches` times (one
+ * trajectory sample per chain per launch). */
+int bm_anneal_run(bm_anneal* a, int64_t steps_per_launch, int32_t launches);
+/* Per-chain state; every array has num_chains entries and may be NULL. */
+int bm_anneal_state(bm_anneal* a, int64_t* score, int64_t* best, int64_t* steps, int64_t* accepted);
+/* Current (which = 0) or best (which = 1) matching of a chain (n+1 ints). */
+int bm_anneal_mapping(bm_anneal* a, int64_t chain, int32_t which, int32_t* pi);
+/* Trajectory: traj[chain * traj_cap + t] = score after launch t (t < count). */
+int bm_anneal_trajectory(bm_anneal* a, int64_t* traj, int32_t* count);
+/* Device recount of every chain's current matching; *mismatches gets the
+ * number of chains whose running score differs (must be 0). */
+int bm_anneal_verify(bm_anneal* a, int64_t* mismatches);
+/* Launches run and their summed duration (HIP events around each launch). */
+int bm_anneal_timing(bm_anneal* a, int64_t* launches, double* total_ms);
+/* Algorithmic bytes of the chain steps run so far, from the byte model of
+ * the swap and triple kernels (replays every chain's proposals on the host). */
+int bm_anneal_algo_bytes(bm_anneal* a, int64_t* bytes, int64_t* steps);
+/* Last error of an annealing object (or of the last failed create). */
+const char* bm_anneal_last_error(const bm_anneal* a);
+/* The schedule as the kernels use it, for exact replays: beta_s =
+ * beta0 * exp(s * lam) with beta0 = 1 / t0 and lam = -ln(decay); a move is a
+ * 3-vertex move when 32 random bits are below p_rot3 (round(p * 2^32)). */
+int bm_anneal_schedule(const bm_anneal* a, double* beta0, double* lam, uint64_t* p_rot3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,18 @@
  *                               OpenMP loop with schedule(dynamic) like
  *                               src/re-alignment-parallel.c:601, used as the
  *                               timed CPU baseline ("port").
+ *  or_gain_row                  a placement-gain table row from its
+ *                               definition.  Pinned: G(u, pi(v)) sums give
+ *                               the exact swap delta (tests).
+ *  or_det_exp                   deterministic exp (Cody & Waite reduction,
+ *                               Taylor polynomial).  Pinned against libm exp
+ *                               to 2 ulp over [-700, 700].
+ *  or_anneal_chain              replay of an annealing chain: the Metropolis
+ *                               rule and geometric schedule of
+ *                               simulated_annealing (src/best-swap/score-lookup.c:345-400)
+ *                               over exact deltas.  Pinned by the score
+ *                               invariant (running score == or_score of the
+ *                               final matching) and a brute-force step check.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -759,4 +771,140 @@ void or_gain_row(const or_graph* g, const int32_t* pi, int32_t u, int32_t* row) 
         }
         row[f] = (int32_t)s;
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* Annealing chains                                                    */
+/* ------------------------------------------------------------------ */
+
+static inline uint64_t rand_at(uint64_t seed, uint64_t c) {
+    return mix64(seed * 0x9E3779B97F4A7C15ULL + (c + 1) * 0xD1B54A32D192ED03ULL);
+}
+
+static inline uint32_t range32(uint32_t bits, uint32_t range) {
+    return (uint32_t)(((uint64_t)bits * (uint64_t)range) >> 32);
+}
+
+/* Triple c of stream seed: a uniform, b uniform != a, d uniform outside {a, b}. */
+void or_uniform_triple(uint64_t seed, uint64_t c, int32_t n, int32_t* x, int32_t* y, int32_t* z) {
+    const uint64_t r0 = rand_at(seed, 2 * c), r1 = rand_at(seed, 2 * c + 1);
+    const uint32_t a = range32((uint32_t)r0, (uint32_t)n);
+    uint32_t b = range32((uint32_t)(r0 >> 32), (uint32_t)(n - 1));
+    uint32_t d = range32((uint32_t)r1, (uint32_t)(n - 2));
+    if (b >= a) b++;
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    if (d >= lo) d++;
+    if (d >= hi) d++;
+    *x = (int32_t)a + 1;
+    *y = (int32_t)b + 1;
+    *z = (int32_t)d + 1;
+}
+
+/* exp(x) for |x| <= 700 from IEEE double +, -, * only (the file is compiled
+ * with -ffp-contract=off): x = k ln2 + r with ln2 split in a 32-bit head
+ * and a tail (Cody & Waite), exp(r) by the degree-13 Taylor polynomial in
+ * Horner form, times 2^k.  The acceptance tests of the annealing chains use
+ * this function so that a replay takes the device's decisions bit for bit;
+ * it is pinned against libm's exp (<= 2 ulp) by the tests. */
+double or_det_exp(double x) {
+    if (x != x) return 1.0;
+    if (x < -700.0) return 0.0;
+    if (x > 700.0) return 1.0e304;
+    const double t = x * 1.4426950408889634;
+    const int k = (int)(t < 0.0 ? t - 0.5 : t + 0.5);
+    const double kd = (double)k;
+    const double r = (x - kd * 6.93147180369123816490e-01) - kd * 1.90821492927058770002e-10;
+    static const double inv_fact[14] = {1.0,
+                                        1.0,
+                                        0.5,
+                                        1.0 / 6.0,
+                                        1.0 / 24.0,
+                                        1.0 / 120.0,
+                                        1.0 / 720.0,
+                                        1.0 / 5040.0,
+                                        1.0 / 40320.0,
+                                        1.0 / 362880.0,
+                                        1.0 / 3628800.0,
+                                        1.0 / 39916800.0,
+                                        1.0 / 479001600.0,
+                                        1.0 / 6227020800.0};
+    double p = inv_fact[13];
+    for (int i = 12; i >= 0; i--) p = p * r + inv_fact[i];
+    if (k < -1022) return 0.0;
+    union {
+        uint64_t u;
+        double d;
+    } two_k;
+    two_k.u = (uint64_t)(k + 1023) << 52;
+    return p * two_k.d;
+}
+
+/* Replays `steps` steps (from step index s0) of one annealing chain with
+ * stream seed q: step s draws its move kind from rand(q2, s) (a 3-vertex move
+ * when the low 32 bits are below p_rot3, a 2^-32 fraction, and n >= 3), a
+ * uniform pair s of stream q0 or a uniform triple s of stream q3 with
+ * permutation 1 + range(high 32 bits, 5), computes the exact delta, and
+ * accepts when delta >= 0 or (rand(q1, s) >> 11) * 2^-53 <
+ * or_det_exp(delta * beta0 * or_det_exp(s * lam)), q_i = rand(q, i).
+ * st = {score, best, accepted, at_best} is updated in place; pi and pib (the
+ * latest matching reaching the best score, valid when at_best == 0) too. */
+void or_anneal_chain(const or_graph* g, int32_t* pi, int32_t* pib, uint64_t q, int64_t s0, int64_t steps,
+                     double beta0, double lam, uint64_t p_rot3, int64_t* st) {
+    const int32_t n = g->n;
+    const uint64_t q0 = rand_at(q, 0), q1 = rand_at(q, 1), q2 = rand_at(q, 2), q3 = rand_at(q, 3);
+    int64_t score = st[0], best = st[1], nacc = st[2];
+    int at_best = (int)st[3];
+    for (int64_t s = s0; s < s0 + steps; s++) {
+        const uint64_t rk = rand_at(q2, (uint64_t)s);
+        const int rot = n >= 3 && (uint64_t)(uint32_t)rk < p_rot3;
+        int32_t v[3];
+        int64_t d;
+        int p = 0;
+        if (!rot) {
+            or_uniform_pair(q0, (uint64_t)s, n, &v[0], &v[1]);
+            d = or_swap_delta(g, pi, v[0], v[1]);
+        } else {
+            or_uniform_triple(q3, (uint64_t)s, n, &v[0], &v[1], &v[2]);
+            int64_t d5[5];
+            or_rot3_all(g, pi, v[0], v[1], v[2], 0, d5);
+            p = 1 + (int)range32((uint32_t)(rk >> 32), 5u);
+            d = d5[p - 1];
+        }
+        int accept = d >= 0;
+        if (!accept) {
+            const double beta = beta0 * or_det_exp((double)s * lam);
+            const double r = (double)(rand_at(q1, (uint64_t)s) >> 11) * (1.0 / 9007199254740992.0);
+            accept = r < or_det_exp((double)d * beta);
+        }
+        if (!accept) continue;
+        if (at_best && d < 0) {
+            memcpy(pib, pi, sizeof(int32_t) * ((size_t)n + 1));
+            at_best = 0;
+        }
+        if (!rot) {
+            const int32_t t = pi[v[0]];
+            pi[v[0]] = pi[v[1]];
+            pi[v[1]] = t;
+        } else {
+            const int32_t f[3] = {pi[v[0]], pi[v[1]], pi[v[2]]};
+            for (int i = 0; i < 3; i++) pi[v[i]] = f[PERM[p][i]];
+        }
+        score += d;
+        nacc++;
+        if (score >= best) {
+            best = score;
+            at_best = 1;
+        }
+    }
+    st[0] = score;
+    st[1] = best;
+    st[2] = nacc;
+    st[3] = at_best;
+}
+
+/* Stream seed of chain `index` of an annealing run seeded `seed`. */
+uint64_t or_chain_seed(uint64_t seed, int64_t index) { return rand_at(seed, (uint64_t)index); }
+
+void or_det_exp_many(const double* x, int64_t k, double* out) {
+    for (int64_t i = 0; i < k; i++) out[i] = or_det_exp(x[i]);
 }

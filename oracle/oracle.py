@@ -311,3 +311,45 @@ def gain_row(o: "Oracle", pi, u):
     row = np.empty(o.n + 1, np.int32)
     _o.or_gain_row(o.h, _p32(pi), u, _p32(row))
     return row
+
+
+_sig(_o, "or_det_exp_many", None, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double)])
+_sig(_o, "or_uniform_triple", None, [C.c_uint64, C.c_uint64, C.c_int32, i32p, i32p, i32p])
+_sig(_o, "or_anneal_chain", None, [vp, i32p, i32p, C.c_uint64, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                   C.c_uint64, i64p])
+_sig(_o, "or_chain_seed", C.c_uint64, [C.c_uint64, C.c_int64])
+
+
+def det_exp(x):
+    """The deterministic exp of the annealing acceptance test, elementwise."""
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    _o.or_det_exp_many(x.ctypes.data_as(C.POINTER(C.c_double)), x.size,
+                       out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def uniform_triple(seed, c, n):
+    a, b, d = C.c_int32(), C.c_int32(), C.c_int32()
+    _o.or_uniform_triple(seed, c, n, C.byref(a), C.byref(b), C.byref(d))
+    return a.value, b.value, d.value
+
+
+def chain_seed(seed, index):
+    return int(_o.or_chain_seed(seed, index))
+
+
+def p_rot3_fraction(p):
+    """Probability of a 3-vertex move as the integer threshold on 32 random bits."""
+    return min(1 << 32, max(0, int(round(p * 4294967296.0))))
+
+
+def anneal_chain(o: "Oracle", pi, score, q, s0, steps, beta0, lam, p_rot3, state=None, pib=None):
+    """Replays steps s0..s0+steps-1 of one annealing chain on the CPU.
+    state = [score, best, accepted, at_best]; returns (pi, pib, state) with pib
+    the latest matching reaching the best score (pi itself when at_best)."""
+    pi = _a32(pi).copy()
+    pib = pi.copy() if pib is None else _a32(pib).copy()
+    st = np.array(state if state is not None else [score, score, 0, 1], np.int64)
+    _o.or_anneal_chain(o.h, _p32(pi), _p32(pib), q, s0, steps, beta0, lam, p_rot3, _p64(st))
+    return pi, (pi.copy() if st[3] else pib), st
