@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--headline", choices=["gain", "scan"], default="gain")
     ap.add_argument("--pi0", choices=["greedy", "shuffle"], default="greedy")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (the known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -99,6 +102,17 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
               "the multi-GPU path is not built.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}), flush=True)
+        return
 
     import brainmatch as bm
 

@@ -141,6 +141,8 @@ _sig("bm_gain_enabled", C.c_int, [_c])
 _sig("bm_gain_rebuild", C.c_int, [_c])
 _sig("bm_gain_table_bytes", C.c_int64, [C.c_int32])
 _sig("bm_gain_row", C.c_int, [_c, C.c_int32, i32p])
+_sig("bm_twohop_set", C.c_int, [_c, C.c_int32, i32p, C.POINTER(C.c_int32)])
+_sig("bm_batch_gen_twohop", C.c_int, [_c, i32p, C.c_int32, C.c_int32, C.c_uint64])
 
 
 def _p32(a):
@@ -283,6 +285,15 @@ class Batch:
     def gen_changed_triples(self, changed):
         ch = np.ascontiguousarray(changed, dtype=np.int32)
         self._chk(_lib.bm_batch_gen_changed_triples(self.h, _p32(ch), ch.shape[0]))
+
+    def gen_twohop(self, per_anchor, seed, anchors=None, n_anchors=None):
+        """2-hop proposal pairs: per_anchor slots per anchor (explicit anchors,
+        or n_anchors drawn from the seed)."""
+        if anchors is not None:
+            a = np.ascontiguousarray(anchors, dtype=np.int32)
+            self._chk(_lib.bm_batch_gen_twohop(self.h, _p32(a), a.shape[0], per_anchor, seed))
+        else:
+            self._chk(_lib.bm_batch_gen_twohop(self.h, None, n_anchors, per_anchor, seed))
 
     def eval_swaps(self):
         self._chk(_lib.bm_batch_eval_swaps(self.h))
@@ -485,6 +496,12 @@ class Context:
         r = _SearchResult()
         self._chk(_lib.bm_search(self.h, C.byref(o), _p32(best), C.byref(r)))
         return best, {k: getattr(r, k) for k, _ in _SearchResult._fields_}
+
+    def twohop_set(self, v):
+        out = np.zeros(self.n, np.int32)
+        cnt = C.c_int32(0)
+        self._chk(_lib.bm_twohop_set(self.h, v, _p32(out), C.byref(cnt)))
+        return out[:cnt.value].copy()
 
     def gain_enable(self, on=True):
         """Switch evaluation to the placement-gain table (or back to edge scans)."""

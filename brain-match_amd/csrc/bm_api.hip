@@ -614,6 +614,62 @@ extern "C" int bm_batch_gen_changed_pairs(bm_batch* b, const int32_t* changed, i
     return BM_OK;
 }
 
+static int twohop_lds_ok(bm_ctx* c) {
+    if (bm_twohop_lds_bytes(c->n) > 150 * 1024)
+        return fail(c, BM_ERR_ARG, "2-hop sets: n=%d needs a %zu-byte LDS bitmap (limit 150 KiB)", c->n,
+                    bm_twohop_lds_bytes(c->n));
+    return BM_OK;
+}
+
+extern "C" int bm_twohop_set(bm_ctx* c, int32_t v, int32_t* out, int32_t* count) {
+    if (!c || !out || !count) return fail(c, BM_ERR_ARG, "bm_twohop_set: null argument");
+    if (v < 1 || v > c->n) return fail(c, BM_ERR_ARG, "bm_twohop_set: vertex %d outside 1..%d", v, c->n);
+    int rc = twohop_lds_ok(c);
+    if (rc != BM_OK) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    int32_t* d_v = reinterpret_cast<int32_t*>(c->d_small);          /* d_small[0] */
+    int32_t* d_cnt = reinterpret_cast<int32_t*>(c->d_small + 1);    /* d_small[1] */
+    HIPCHK(c, hipMemcpyAsync(d_v, &v, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, bm_launch_twohop(devgraph(c), d_v, 0, 1, 0, nullptr, c->d_reorder, d_cnt, c->num_cus, c->stream));
+    int32_t cnt = 0;
+    HIPCHK(c, hipMemcpyAsync(&cnt, d_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (cnt < 0 || cnt > c->n) return fail(c, BM_ERR_STATE, "bm_twohop_set: bad count %d", cnt);
+    if (cnt) HIPCHK(c, hipMemcpy(out, c->d_reorder, (size_t)cnt * sizeof(int32_t), hipMemcpyDeviceToHost));
+    *count = cnt;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_gen_twohop(bm_batch* b, const int32_t* anchors, int32_t n_anchors, int32_t per_anchor,
+                                   uint64_t seed) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (n_anchors < 0 || per_anchor < 1) return fail(c, BM_ERR_ARG, "bm_batch_gen_twohop: need n_anchors >= 0, per_anchor >= 1");
+    const int64_t k = (int64_t)n_anchors * per_anchor;
+    if (k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_gen_twohop: %lld pairs exceed batch capacity %lld", (long long)k, (long long)b->cap);
+    int rc = twohop_lds_ok(c);
+    if (rc != BM_OK) return rc;
+    if (anchors)
+        for (int32_t i = 0; i < n_anchors; i++)
+            if (anchors[i] < 1 || anchors[i] > c->n)
+                return fail(c, BM_ERR_ARG, "bm_batch_gen_twohop: anchor %d outside 1..%d", anchors[i], c->n);
+    HIPCHK(c, hipSetDevice(c->device));
+    /* anchors go to the upper half of the candidate buffer (pairs use the first 8 bytes of each 16) */
+    int32_t* d_anch = nullptr;
+    if (anchors && n_anchors) {
+        d_anch = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(b->d_cand) + 8 * (size_t)b->cap);
+        HIPCHK(c, hipMemcpyAsync(d_anch, anchors, sizeof(int32_t) * (size_t)n_anchors, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, bm_launch_twohop(devgraph(c), d_anch, seed, n_anchors, per_anchor, (int2*)b->d_cand, nullptr, nullptr,
+                               c->num_cus, c->stream));
+    if (anchors && n_anchors) HIPCHK(c, hipStreamSynchronize(c->stream));  /* host anchors may go */
+    b->k = k;
+    b->kind = 1;
+    b->generated = 1;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
 extern "C" int bm_batch_gen_changed_triples(bm_batch* b, const int32_t* changed, int32_t cnum) {
     if (!b) return BM_ERR_ARG;
     bm_ctx* c = b->ctx;

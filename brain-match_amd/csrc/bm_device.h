@@ -127,6 +127,15 @@ hipError_t bm_launch_accept(const BmDevGraph& g, int32_t* pi_rw, const int2* can
                             int64_t* out_acc_idx /* k */, int64_t* out_meta /* [n_acc, gain, n_moves] */,
                             int4* out_moves /* 2k {x, old, new, 0}, may be NULL */, hipStream_t s);
 
+/* 2-hop proposals (bm_twohop.hip).  K > 0: pairs[a*K + j] for anchor a
+ * (anchors[a], or drawn from `seed` when anchors is NULL); K == 0: the whole
+ * set of anchors[a] into out_list (n entries) and its size into out_count[a]
+ * (n_anchors must be 1 then). */
+size_t bm_twohop_lds_bytes(int32_t n);
+hipError_t bm_launch_twohop(const BmDevGraph& g, const int32_t* anchors, uint64_t seed, int64_t n_anchors,
+                            int32_t K, int2* pairs, int32_t* out_list, int32_t* out_count, int32_t num_cus,
+                            hipStream_t s);
+
 /* Context internals for other host translation units (bm_api.hip). */
 struct bm_ctx;
 BmDevGraph bm_ctx_devgraph(const bm_ctx* c);

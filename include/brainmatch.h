@@ -232,6 +232,21 @@ int bm_batch_gen_triples(bm_batch* b, uint64_t seed, int64_t k);    /* bm_unifor
 int bm_batch_gen_changed_pairs(bm_batch* b, const int32_t* changed, int32_t c);
 int bm_batch_gen_changed_triples(bm_batch* b, const int32_t* changed, int32_t c);
 
+/* 2-hop proposals.  The 2-hop set of v is collect_neighborhood's
+ * (src/refine-twohop-random.c:582-625): out-neighbours, their out-neighbours,
+ * in-neighbours and their in-neighbours in the male graph, minus v.
+ * bm_twohop_set writes it in ascending order (out: n entries).
+ * bm_batch_gen_twohop fills n_anchors * per_anchor pairs: anchor a is
+ * anchors[a] (or, with anchors NULL, 1 + range(lo32(bm_rand_at(seed, 2a)), n))
+ * and its per_anchor slots hold (anchor, x) for the per_anchor members x of
+ * its set with the smallest hi32(bm_rand_at(bm_rand_at(seed, 2a+1), x)) (ties:
+ * smaller x), in ascending x, i.e. a uniform sample without replacement as
+ * the reference's partial shuffle (:686-691) draws it; slots beyond the set
+ * size hold (anchor, anchor), delta 0.  Needs (n+1)/8 bytes of LDS per
+ * block (n <= 1.2M). */
+int bm_twohop_set(bm_ctx* ctx, int32_t v, int32_t* out, int32_t* count);
+int bm_batch_gen_twohop(bm_batch* b, const int32_t* anchors, int32_t n_anchors, int32_t per_anchor, uint64_t seed);
+
 /* Evaluate.  eval_swaps_uniform generates pair c of stream `seed` inside the
  * delta kernel (nothing is read for the candidate) and writes the deltas. */
 int bm_batch_eval_swaps(bm_batch* b);

@@ -57,6 +57,11 @@
  *                               over exact deltas.  Pinned by the score
  *                               invariant (running score == or_score of the
  *                               final matching) and a brute-force step check.
+ *  or_twohop_set / _pairs       collect_neighborhood's 2-hop set
+ *                               (src/refine-twohop-random.c:582-625) and the
+ *                               K-smallest-key sample of it.  Pinned by set
+ *                               equality with a Python BFS over the raw edge
+ *                               list and by the sample's uniformity.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -907,4 +912,81 @@ uint64_t or_chain_seed(uint64_t seed, int64_t index) { return rand_at(seed, (uin
 
 void or_det_exp_many(const double* x, int64_t k, double* out) {
     for (int64_t i = 0; i < k; i++) out[i] = or_det_exp(x[i]);
+}
+
+/* ------------------------------------------------------------------ */
+/* 2-hop proposals                                                     */
+/* ------------------------------------------------------------------ */
+
+/* collect_neighborhood (src/refine-twohop-random.c:582-625) in the male
+ * graph: out-neighbours of v and theirs, in-neighbours of v and theirs, v
+ * excluded; ascending ids into out (n entries).  Returns the count. */
+int32_t or_twohop_set(const or_graph* g, int32_t v, int32_t* out) {
+    const og_t* M = &g->M;
+    char* mark = (char*)calloc((size_t)g->n + 1, 1);
+    for (int64_t p = M->out_off[v]; p < M->out_off[v + 1]; p++) {
+        const int32_t x = og_dst(M, M->out_e[p]);
+        mark[x] = 1;
+        for (int64_t q = M->out_off[x]; q < M->out_off[x + 1]; q++) mark[og_dst(M, M->out_e[q])] = 1;
+    }
+    for (int64_t p = M->in_off[v]; p < M->in_off[v + 1]; p++) {
+        const int32_t x = og_src(M, M->in_e[p]);
+        mark[x] = 1;
+        for (int64_t q = M->in_off[x]; q < M->in_off[x + 1]; q++) mark[og_src(M, M->in_e[q])] = 1;
+    }
+    mark[v] = 0;
+    int32_t c = 0;
+    for (int32_t x = 1; x <= g->n; x++)
+        if (mark[x]) out[c++] = x;
+    free(mark);
+    return c;
+}
+
+typedef struct { uint32_t key; int32_t x; } kx_t;
+static int cmp_kx(const void* a, const void* b) {
+    const kx_t* p = (const kx_t*)a;
+    const kx_t* q = (const kx_t*)b;
+    if (p->key != q->key) return p->key < q->key ? -1 : 1;
+    return (p->x > q->x) - (p->x < q->x);
+}
+static int cmp_i32(const void* a, const void* b) {
+    const int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* Proposal pairs of a 2-hop batch: anchor a = anchors[a] or drawn from
+ * rand(seed, 2a); its K targets are the set members with the K smallest
+ * (hi32(rand(rand(seed, 2a+1), x)), x), ascending x, padded with (v, v). */
+void or_twohop_pairs(const or_graph* g, const int32_t* anchors, int32_t n_anchors, int32_t K, uint64_t seed,
+                     int32_t* pairs) {
+    int32_t* set = (int32_t*)malloc(sizeof(int32_t) * ((size_t)g->n + 1));
+    kx_t* kx = (kx_t*)malloc(sizeof(kx_t) * ((size_t)g->n + 1));
+    int32_t* take = (int32_t*)malloc(sizeof(int32_t) * ((size_t)g->n + 1));
+    for (int32_t a = 0; a < n_anchors; a++) {
+        const int32_t v = anchors ? anchors[a]
+                                  : 1 + (int32_t)range32((uint32_t)rand_at(seed, 2 * (uint64_t)a), (uint32_t)g->n);
+        const uint64_t q = rand_at(seed, 2 * (uint64_t)a + 1);
+        const int32_t S = or_twohop_set(g, v, set);
+        int32_t t = 0;
+        if (S <= K) {
+            memcpy(take, set, sizeof(int32_t) * (size_t)S);
+            t = S;
+        } else {
+            for (int32_t i = 0; i < S; i++) {
+                kx[i].key = (uint32_t)(rand_at(q, (uint64_t)set[i]) >> 32);
+                kx[i].x = set[i];
+            }
+            qsort(kx, (size_t)S, sizeof(kx_t), cmp_kx);
+            for (int32_t i = 0; i < K; i++) take[i] = kx[i].x;
+            t = K;
+            qsort(take, (size_t)t, sizeof(int32_t), cmp_i32);
+        }
+        for (int32_t j = 0; j < K; j++) {
+            pairs[2 * ((int64_t)a * K + j)] = v;
+            pairs[2 * ((int64_t)a * K + j) + 1] = j < t ? take[j] : v;
+        }
+    }
+    free(set);
+    free(kx);
+    free(take);
 }

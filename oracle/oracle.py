@@ -353,3 +353,25 @@ def anneal_chain(o: "Oracle", pi, score, q, s0, steps, beta0, lam, p_rot3, state
     st = np.array(state if state is not None else [score, score, 0, 1], np.int64)
     _o.or_anneal_chain(o.h, _p32(pi), _p32(pib), q, s0, steps, beta0, lam, p_rot3, _p64(st))
     return pi, (pi.copy() if st[3] else pib), st
+
+
+_sig(_o, "or_twohop_set", C.c_int32, [vp, C.c_int32, i32p])
+_sig(_o, "or_twohop_pairs", None, [vp, i32p, C.c_int32, C.c_int32, C.c_uint64, i32p])
+
+
+def twohop_set(o: "Oracle", v):
+    out = np.empty(o.n + 1, np.int32)
+    c = _o.or_twohop_set(o.h, v, _p32(out))
+    return out[:c].copy()
+
+
+def twohop_pairs(o: "Oracle", per_anchor, seed, anchors=None, n_anchors=None):
+    if anchors is not None:
+        a = _a32(anchors)
+        na = a.shape[0]
+        ap = _p32(a)
+    else:
+        na, ap = n_anchors, None
+    pairs = np.empty((na * per_anchor, 2), np.int32)
+    _o.or_twohop_pairs(o.h, ap, na, per_anchor, seed, _p32(pairs))
+    return pairs
