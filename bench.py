@@ -203,23 +203,24 @@ def main():
     head = gain if args.headline == "gain" else scan
     stats = batch.stats()
 
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", f"{args.headline}_swap_n18524_traffic.json")
+    # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE,
+    # scaled by a 1 GiB copy) for this workload; see profiles/swap_n18524_traffic.md
+    traffic = {}
+    prof = os.path.join(ROOT, "profiles", "swap_n18524_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             p = json.load(f)
         if p.get("candidates") == K and p.get("pi0") == args.pi0:
-            traffic = p.get("hbm_bytes_per_launch")
+            traffic = p.get("hbm_bytes_per_launch", {})
 
     if args.headline == "gain":
         dom = "lookup" if gain["lookup_ms"] >= gain["build_ms"] else "build"
         rf = dict(gain[dom + "_roofline"])
         rf["kernel"] = "k_gain_swap<uniform>" if dom == "lookup" else "k_gain_build"
-        rf["traffic"] = traffic
     else:
         rf = dict(scan["roofline"])
         rf["kernel"] = "k_swap<uniform>"
-        rf["traffic"] = traffic
+    rf["traffic"] = traffic.get(rf["kernel"])
 
     result = {
         "metric": "exact swap-deltas/s",

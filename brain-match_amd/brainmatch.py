@@ -143,6 +143,8 @@ _sig("bm_gain_table_bytes", C.c_int64, [C.c_int32])
 _sig("bm_gain_row", C.c_int, [_c, C.c_int32, i32p])
 _sig("bm_twohop_set", C.c_int, [_c, C.c_int32, i32p, C.POINTER(C.c_int32)])
 _sig("bm_batch_gen_twohop", C.c_int, [_c, i32p, C.c_int32, C.c_int32, C.c_uint64])
+_sig("bm_hillclimb_twohop", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32, i64p,
+                                      C.POINTER(_SearchResult)])
 
 
 def _p32(a):
@@ -518,6 +520,13 @@ class Context:
         self._chk(_lib.bm_gain_row(self.h, u, _p32(row)))
         return row
 
+    def hillclimb_twohop(self, seed, rounds, anchors, per_anchor, verify=True):
+        trace = np.zeros(rounds + 1, np.int64)
+        r = _SearchResult()
+        self._chk(_lib.bm_hillclimb_twohop(self.h, seed, rounds, anchors, per_anchor, 1 if verify else 0,
+                                           _p64(trace), C.byref(r)))
+        return trace, {k: getattr(r, k) for k, _ in _SearchResult._fields_}
+
     def hillclimb(self, seed, rounds, batch, verify=True):
         trace = np.zeros(rounds + 1, np.int64)
         r = _SearchResult()
@@ -599,7 +608,8 @@ class Anneal:
         return chain // self.per_instance
 
     def mapping(self, chain, best=False):
-        pi = np.zeros(self.ns[self.instance_of(chain)] + 1, np.int32)
+        inst = self.instance_of(chain) if 0 <= chain < self.chains else 0   # the library rejects bad chains
+        pi = np.zeros(self.ns[inst] + 1, np.int32)
         self._chk(_lib.bm_anneal_mapping(self.h, chain, 1 if best else 0, _p32(pi)))
         return pi
 

@@ -17,6 +17,11 @@
  *
  * bm_hillclimb is the greedy disjoint-swap climb: each round evaluates a
  * batch of uniform pairs and applies a disjoint set of positive swaps.
+ * bm_hillclimb_twohop does the same with 2-hop proposals, the device-wide
+ * form of refine-twohop-random's loop (src/refine-twohop-random.c:676-760:
+ * random anchors, up to 1000 targets each from the 2-hop set, best delta
+ * applied), which applies a disjoint set of positive swaps per round instead
+ * of the single best one.
  */
 #include "bm_host.h"
 #include "bm_rng.h"
@@ -313,8 +318,15 @@ done:
     return rc;
 }
 
-int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int32_t verify, int64_t* score_trace,
-                 bm_search_result* res) {
+/* Proposal source of a climb round: uniform pairs (batch of them) or 2-hop
+ * pairs (n_anchors anchors x per_anchor targets). */
+typedef struct climb_src {
+    int twohop;
+    int32_t n_anchors, per_anchor;
+} climb_src;
+
+static int climb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, climb_src src, int32_t verify,
+                 int64_t* score_trace, bm_search_result* res) {
     if (!ctx || rounds < 0 || batch < 1 || !res) return BM_ERR_ARG;
     memset(res, 0, sizeof *res);
     bm_batch* b = NULL;
@@ -326,7 +338,11 @@ int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int3
     if (score_trace) score_trace[0] = score;
     const double t0 = now_s();
     for (int64_t r = 0; r < rounds; r++) {
-        if ((rc = bm_batch_gen_pairs(b, seed + (uint64_t)r, batch)) != BM_OK) goto done;
+        if (src.twohop)
+            rc = bm_batch_gen_twohop(b, NULL, src.n_anchors, src.per_anchor, seed + (uint64_t)r);
+        else
+            rc = bm_batch_gen_pairs(b, seed + (uint64_t)r, batch);
+        if (rc != BM_OK) goto done;
         if ((rc = bm_batch_eval_swaps(b)) != BM_OK) goto done;
         int64_t nacc = 0, gain = 0;
         if ((rc = bm_batch_accept_disjoint(b, 0, &nacc, &gain)) != BM_OK) goto done;
@@ -352,4 +368,17 @@ int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int3
 done:
     bm_batch_destroy(b);
     return rc;
+}
+
+int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int32_t verify, int64_t* score_trace,
+                 bm_search_result* res) {
+    climb_src src = {0, 0, 0};
+    return climb(ctx, seed, rounds, batch, src, verify, score_trace, res);
+}
+
+int bm_hillclimb_twohop(bm_ctx* ctx, uint64_t seed, int64_t rounds, int32_t anchors, int32_t per_anchor,
+                        int32_t verify, int64_t* score_trace, bm_search_result* res) {
+    if (anchors < 1 || per_anchor < 1) return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_hillclimb_twohop: anchors and per_anchor must be >= 1");
+    climb_src src = {1, anchors, per_anchor};
+    return climb(ctx, seed, rounds, (int64_t)anchors * per_anchor, src, verify, score_trace, res);
 }

@@ -382,6 +382,12 @@ int bm_search(bm_ctx* ctx, const bm_search_opts* opts, int32_t* best_pi, bm_sear
  * verify != 0 every round is checked against a device recount. */
 int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int32_t verify,
                  int64_t* score_trace, bm_search_result* res);
+/* The same climb over 2-hop proposals: round r evaluates
+ * bm_batch_gen_twohop(NULL, anchors, per_anchor, seed + r) (refine-twohop-random's
+ * proposal, src/refine-twohop-random.c:676-693, with 10 anchors x 1000
+ * targets per thread there). */
+int bm_hillclimb_twohop(bm_ctx* ctx, uint64_t seed, int64_t rounds, int32_t anchors, int32_t per_anchor,
+                        int32_t verify, int64_t* score_trace, bm_search_result* res);
 
 /* ------------------------------------------------------------------ */
 /* Annealing chains (many independent chains, batched over instances)  */

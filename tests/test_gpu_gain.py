@@ -74,11 +74,27 @@ def test_patches_equal_rebuild(g1, orc):
     ctx.apply_swap(20, 1500)
     ctx.apply_perm3(7, 8, 9, 4)
     ctx.apply_perm3(30, 10, 1999, 1)
+    pi_before = ctx.get_mapping()
     uv = orc.uniform_pairs(41, 20000, n)
     d = ctx.swap_deltas(uv)
     acc, gain = ctx.accept_disjoint(uv, d)
-    assert gain > 0 and len(acc) > 1
+    # the graph has hubs adjacent to most vertices, so a uniform batch may
+    # lock everything after one swap; pairs of low-degree vertices give a
+    # batch with many disjoint acceptances
+    od, idg = ctx.degrees()
+    deg = (od + idg)[1:]
+    low = np.nonzero(deg <= np.percentile(deg, 30))[0].astype(np.int32) + 1
+    rng = np.random.default_rng(4)
+    uv2 = np.stack([rng.choice(low, 2, replace=False) for _ in range(4000)]).astype(np.int32)
+    pi_mid = ctx.get_mapping()
+    d2 = ctx.swap_deltas(uv2)
+    acc2, gain2 = ctx.accept_disjoint(uv2, d2)
+    assert gain > 0 and gain2 > 0 and len(acc2) > 10
+    ref_pi, ref_acc, ref_gain = o.accept_disjoint(pi_before, uv, d)
+    assert ref_gain == gain and np.array_equal(ref_acc, acc) and np.array_equal(ref_pi, pi_mid)
+    ref_pi2, ref_acc2, ref_gain2 = o.accept_disjoint(pi_mid, uv2, d2)
     pi = ctx.get_mapping()
+    assert ref_gain2 == gain2 and np.array_equal(ref_acc2, acc2) and np.array_equal(ref_pi2, pi)
     rows = list(range(1, n + 1, 37)) + [10, 20, 1500, 7, 8, 9, 30, 1999]
     patched = {u: ctx.gain_row(u) for u in rows}
     ctx.gain_rebuild()

@@ -64,3 +64,21 @@ def test_bad_anchor_and_capacity_raise(bm):
         with pytest.raises(bm.BMError):
             ctx.twohop_set(101)
         b.close()
+
+
+def test_twohop_climb_matches_replay(bm, orc):
+    male, female, truth, pi0 = make_pair(bm, n=2000, deg=20.0, seed=3)
+    o = orc.Oracle(2000, male, female)
+    with bm.Context(2000, male, female) as ctx:
+        ctx.set_mapping(pi0)
+        trace, res = ctx.hillclimb_twohop(seed=50, rounds=12, anchors=64, per_anchor=100, verify=True)
+        assert res["verify_failures"] == 0 and trace[-1] > trace[0]
+        pi = pi0.copy()
+        ref = [o.score(pi)]
+        for r in range(12):
+            uv = orc.twohop_pairs(o, 100, 50 + r, n_anchors=64)
+            d = o.swap_deltas(pi, uv)
+            pi, _, gain = o.accept_disjoint(pi, uv, d)
+            ref.append(ref[-1] + gain)
+        assert np.array_equal(trace, np.array(ref, np.int64))
+        assert np.array_equal(ctx.get_mapping(), pi)
