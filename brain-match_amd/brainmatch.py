@@ -29,7 +29,7 @@ i64p = C.POINTER(C.c_int64)
 u8p = C.POINTER(C.c_uint8)
 
 KERNEL_SWAP, KERNEL_ROT3, KERNEL_SCORE = 0, 1, 2
-KERNEL_GAIN_BUILD, KERNEL_GAIN_SWAP, KERNEL_GAIN_ROT3, KERNEL_GAIN_UPDATE = 3, 4, 5, 6
+KERNEL_GAIN_BUILD, KERNEL_GAIN_SWAP, KERNEL_GAIN_ROT3, KERNEL_GAIN_UPDATE, KERNEL_AUX = 3, 4, 5, 6, 7
 WEIGHTS_GEOMETRIC, WEIGHTS_LOGNORMAL = 1, 2
 SEARCH_SWAP_WEIGHTED, SEARCH_SWAP_UNIFORM, SEARCH_ROT3 = 0, 1, 2
 
@@ -121,6 +121,8 @@ _sig("bm_batch_gen_changed_triples", C.c_int, [_c, i32p, C.c_int32])
 _sig("bm_batch_eval_swaps", C.c_int, [_c])
 _sig("bm_batch_eval_swaps_uniform", C.c_int, [_c, C.c_uint64, C.c_int64])
 _sig("bm_batch_eval_rot3", C.c_int, [_c])
+_sig("bm_batch_gen_umajor", C.c_int, [_c, C.c_uint64, C.c_int64])
+_sig("bm_batch_eval_swaps_umajor", C.c_int, [_c, C.c_uint64, C.c_int64])
 _sig("bm_batch_get", C.c_int, [_c, C.c_int64, C.c_int64, i64p, u8p, i32p])
 _sig("bm_batch_stats", C.c_int, [_c, C.POINTER(_Stats)])
 _sig("bm_batch_pick", C.c_int, [_c, C.c_int32, C.c_int64, i64p])
@@ -302,6 +304,12 @@ class Batch:
 
     def eval_swaps_uniform(self, seed, k):
         self._chk(_lib.bm_batch_eval_swaps_uniform(self.h, seed, k))
+
+    def gen_umajor(self, seed, k):
+        self._chk(_lib.bm_batch_gen_umajor(self.h, seed, k))
+
+    def eval_swaps_umajor(self, seed, k):
+        self._chk(_lib.bm_batch_eval_swaps_umajor(self.h, seed, k))
 
     def eval_rot3(self):
         self._chk(_lib.bm_batch_eval_rot3(self.h))

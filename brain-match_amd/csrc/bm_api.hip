@@ -70,6 +70,15 @@ struct bm_ctx {
     int4* d_moves;
     int64_t moves_cap;
     int64_t* d_nmoves;
+    int64_t gain_version;   /* bumped by every build and patch of the table */
+    /* u-major evaluation (bm_batch_eval_swaps_umajor) */
+    int32_t* d_um_counts;   /* n+1 */
+    int64_t* d_um_off;      /* n+2 */
+    int32_t* d_GT;          /* transposed table, valid when gt_version == gain_version */
+    int4* d_pis;            /* {pi, G(v, pi(v)), male self-loop, 0} per vertex, same validity */
+    uint32_t* d_f_out_p;    /* packed female rows for the table build (compact graphs) */
+    uint32_t* d_f_in_p;
+    int64_t gt_version;
     char err[512];
 };
 
@@ -227,6 +236,12 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_f_self);
     (void)hipFree(c->d_moves);
     (void)hipFree(c->d_nmoves);
+    (void)hipFree(c->d_um_counts);
+    (void)hipFree(c->d_um_off);
+    (void)hipFree(c->d_GT);
+    (void)hipFree(c->d_pis);
+    (void)hipFree(c->d_f_out_p);
+    (void)hipFree(c->d_f_in_p);
     bm_graph_free(&c->male);
     free(c->f_out_deg);
     free(c->f_in_deg);
@@ -243,6 +258,8 @@ static BmGain devgain(const bm_ctx* c) {
     G.m_out_tab = c->d_m_out_tab;
     G.m_self = c->d_m_self;
     G.f_self = c->d_f_self;
+    G.f_out_p = c->d_f_out_p;
+    G.f_in_p = c->d_f_in_p;
     return G;
 }
 
@@ -251,6 +268,7 @@ static int gain_rebuild(bm_ctx* c) {
     timing_begin(c, BM_KERNEL_GAIN_BUILD, &t);
     HIPCHK(c, bm_launch_gain_build(devgraph(c), devgain(c), c->num_cus, c->stream));
     timing_end(c, &t);
+    c->gain_version++;
     return BM_OK;
 }
 
@@ -271,6 +289,7 @@ static int gain_patch(bm_ctx* c, const int64_t* d_count, int64_t max_moves) {
     timing_begin(c, BM_KERNEL_GAIN_UPDATE, &t);
     HIPCHK(c, bm_launch_gain_update(devgraph(c), devgain(c), c->d_moves, d_count, max_moves, c->stream));
     timing_end(c, &t);
+    c->gain_version++;
     return BM_OK;
 }
 
@@ -701,11 +720,11 @@ extern "C" int bm_batch_eval_swaps(bm_batch* b) {
     KTiming t;
     if (c->gain_on) {
         timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
-        HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_LIST, (const int2*)b->d_cand, 0, b->k,
+        HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_LIST, (const int2*)b->d_cand, nullptr, 0, b->k,
                                       b->d_delta, c->stream));
     } else {
         timing_begin(c, BM_KERNEL_SWAP, &t);
-        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_LIST, (const int2*)b->d_cand, 0, b->k,
+        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_LIST, (const int2*)b->d_cand, nullptr, 0, b->k,
                                  b->d_delta, c->stream));
     }
     timing_end(c, &t);
@@ -721,12 +740,85 @@ extern "C" int bm_batch_eval_swaps_uniform(bm_batch* b, uint64_t seed, int64_t k
     KTiming t;
     if (c->gain_on) {
         timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
-        HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_UNIFORM, nullptr, seed, k, b->d_delta,
+        HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_UNIFORM, nullptr, nullptr, seed, k, b->d_delta,
                                       c->stream));
     } else {
         timing_begin(c, BM_KERNEL_SWAP, &t);
-        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_UNIFORM, nullptr, seed, k, b->d_delta,
+        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_UNIFORM, nullptr, nullptr, seed, k, b->d_delta,
                                  c->stream));
+    }
+    timing_end(c, &t);
+    b->k = k;
+    b->kind = 3; /* deltas only: candidates implicit in (seed, index) */
+    b->generated = 1;
+    b->evaluated = 1;
+    return BM_OK;
+}
+
+/* Offsets of the u-major stream (seed, k) into c->d_um_off. */
+static int umajor_offsets(bm_ctx* c, uint64_t seed, int64_t k) {
+    if (!c->d_um_off) {
+        int rc = dalloc(c, &c->d_um_counts, (size_t)c->n + 1);
+        if (rc == BM_OK) rc = dalloc(c, &c->d_um_off, (size_t)c->n + 2);
+        if (rc != BM_OK) return rc;
+    }
+    KTiming t;
+    timing_begin(c, BM_KERNEL_AUX, &t);
+    HIPCHK(c, bm_launch_umajor_offsets(c->n, seed, k, c->d_um_counts, c->d_um_off, c->num_cus, c->stream));
+    timing_end(c, &t);
+    return BM_OK;
+}
+
+extern "C" int bm_batch_gen_umajor(bm_batch* b, uint64_t seed, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (k < 0 || k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_gen_umajor: k exceeds capacity");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = umajor_offsets(c, seed, k);
+    if (rc != BM_OK) return rc;
+    HIPCHK(c, bm_launch_gen_umajor(c->n, seed, c->d_um_off, k, (int2*)b->d_cand, c->stream));
+    b->k = k;
+    b->kind = 1;
+    b->generated = 1;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (k < 0 || k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_eval_swaps_umajor: k exceeds capacity");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = umajor_offsets(c, seed, k);
+    if (rc != BM_OK) return rc;
+    KTiming t;
+    if (c->gain_on) {
+        const bool staged = bm_gain_umajor_lds(c->n) <= 160 * 1024;
+        if (staged && c->gt_version != c->gain_version) {
+            if (!c->d_GT) {
+                rc = dalloc(c, &c->d_GT, (size_t)bm_gain_table_bytes(c->n) / sizeof(int32_t));
+                if (rc == BM_OK && !c->d_pis) rc = dalloc(c, &c->d_pis, (size_t)c->n + 1);
+                if (rc != BM_OK) return rc;
+            }
+            timing_begin(c, BM_KERNEL_AUX, &t);
+            HIPCHK(c, bm_launch_gain_transpose(devgain(c), c->d_GT, c->num_cus, c->stream));
+            timing_end(c, &t);
+            c->gt_version = c->gain_version;
+        }
+        timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
+        if (staged) {
+            /* pi and the diagonal G(v, pi(v)) are cheap: refreshed every call */
+            HIPCHK(c, bm_launch_gain_pis(devgraph(c), devgain(c), c->d_pis, c->stream));
+            HIPCHK(c, bm_launch_gain_swap_umajor(devgraph(c), devgain(c), c->d_GT, c->d_pis, c->d_um_off, seed,
+                                                 b->d_delta, c->num_cus, c->stream));
+        } else {
+            HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_UMAJOR, nullptr, c->d_um_off, seed, k,
+                                          b->d_delta, c->stream));
+        }
+    } else {
+        timing_begin(c, BM_KERNEL_SWAP, &t);
+        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_UMAJOR, nullptr, c->d_um_off, seed, k,
+                                 b->d_delta, c->stream));
     }
     timing_end(c, &t);
     b->k = k;
@@ -977,7 +1069,10 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         if (!c->gain_on) return BM_OK;
         (void)hipFree(c->d_G);
         c->d_G = nullptr;
+        (void)hipFree(c->d_GT);
+        c->d_GT = nullptr;
         c->gain_on = 0;
+        c->gain_version++;
         return BM_OK;
     }
     if (c->gain_on) return BM_OK;
@@ -1016,6 +1111,13 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         if (rc == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, BM_ERR_HIP, "sync failed");
         bm_rowhash_free(&hm);
         if (rc != BM_OK) return rc;
+    }
+    if (c->compact && !c->d_f_out_p && c->female_m > 0) {
+        int r2 = dalloc(c, &c->d_f_out_p, (size_t)c->female_m);
+        if (r2 == BM_OK) r2 = dalloc(c, &c->d_f_in_p, (size_t)c->female_m);
+        if (r2 != BM_OK) return r2;
+        HIPCHK(c, bm_launch_pack_rows(c->d_f_out, c->female_m, c->d_f_out_p, c->stream));
+        HIPCHK(c, bm_launch_pack_rows(c->d_f_in, c->female_m, c->d_f_in_p, c->stream));
     }
     int rc = dalloc(c, &c->d_G, bytes / sizeof(int32_t));
     if (rc != BM_OK) return rc;

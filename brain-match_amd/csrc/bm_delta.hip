@@ -373,8 +373,9 @@ __device__ __forceinline__ int64_t swap_delta_wave(const BmDevGraph& g, int32_t 
 }
 
 template <int MODE, bool COMPACT, int U>
-__global__ void __launch_bounds__(512) k_swap(BmDevGraph g, const int2* __restrict__ cand, uint64_t seed,
-                                             int64_t k, int64_t* __restrict__ delta, uint32_t cap) {
+__global__ void __launch_bounds__(512) k_swap(BmDevGraph g, const int2* __restrict__ cand,
+                                             const int64_t* __restrict__ off, uint64_t seed, int64_t k,
+                                             int64_t* __restrict__ delta, uint32_t cap) {
     typedef typename Slot<COMPACT>::T S;
     extern __shared__ uint4 smem[];
     const int lane = threadIdx.x & (BM_WAVE - 1);
@@ -388,6 +389,9 @@ __global__ void __launch_bounds__(512) k_swap(BmDevGraph g, const int2* __restri
             const int2 p = cand[c];
             u = p.x;
             v = p.y;
+        } else if (MODE == BM_CAND_UMAJOR) {
+            u = umajor_owner(off, g.n, c);
+            v = bm_umajor_partner(seed, (uint64_t)c, u, g.n);
         } else {
             bm_uniform_pair(seed, (uint64_t)c, g.n, &u, &v);
         }
@@ -743,8 +747,8 @@ hipError_t launch_delta(K kernel, const BmLaunch& L, int64_t work, size_t slot_b
 
 size_t bm_slot_bytes(const BmDevGraph& g) { return g.compact ? 4 : 8; }
 
-hipError_t bm_launch_swap(const BmDevGraph& g, const BmLaunch& L, int mode, const int2* cand, uint64_t seed,
-                          int64_t k, int64_t* delta, hipStream_t s) {
+hipError_t bm_launch_swap(const BmDevGraph& g, const BmLaunch& L, int mode, const int2* cand, const int64_t* off,
+                          uint64_t seed, int64_t k, int64_t* delta, hipStream_t s) {
     if (k <= 0) return hipSuccess;
     dim3 grid, block;
     size_t lds;
@@ -752,13 +756,15 @@ hipError_t bm_launch_swap(const BmDevGraph& g, const BmLaunch& L, int mode, cons
 #define BM_SWAP_CASE(M, C, U)                                                                           \
     e = launch_delta(k_swap<M, C, U>, L, k, C ? 4 : 8, s, &grid, &block, &lds);                         \
     if (e != hipSuccess) return e;                                                                      \
-    hipLaunchKernelGGL((k_swap<M, C, U>), grid, block, lds, s, g, cand, seed, k, delta, (uint32_t)L.lds_slots);
+    hipLaunchKernelGGL((k_swap<M, C, U>), grid, block, lds, s, g, cand, off, seed, k, delta, (uint32_t)L.lds_slots);
 #define BM_SWAP_U(M, C)                             \
     if (L.unroll == 4) { BM_SWAP_CASE(M, C, 4) }     \
     else if (L.unroll == 2) { BM_SWAP_CASE(M, C, 2) } \
     else { BM_SWAP_CASE(M, C, 8) }
     if (mode == BM_CAND_LIST) {
         if (g.compact) { BM_SWAP_U(BM_CAND_LIST, true) } else { BM_SWAP_U(BM_CAND_LIST, false) }
+    } else if (mode == BM_CAND_UMAJOR) {
+        if (g.compact) { BM_SWAP_U(BM_CAND_UMAJOR, true) } else { BM_SWAP_U(BM_CAND_UMAJOR, false) }
     } else {
         if (g.compact) { BM_SWAP_U(BM_CAND_UNIFORM, true) } else { BM_SWAP_U(BM_CAND_UNIFORM, false) }
     }

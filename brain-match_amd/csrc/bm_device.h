@@ -39,13 +39,26 @@ struct BmGain {
     const int2* m_out_tab;
     const int32_t* m_self;      /* male self-loop weight per vertex or BM_ABSENT */
     const int32_t* f_self;      /* female self-loop weight per vertex or BM_ABSENT */
+    const uint32_t* f_out_p;    /* female rows packed (f << 15) | w when the graph is compact, else NULL */
+    const uint32_t* f_in_p;
 };
+
+/* Packs {f, w} rows (f < 2^17, 0 <= w <= 32767) into (f << 15) | w words. */
+hipError_t bm_launch_pack_rows(const int2* in, int64_t m, uint32_t* out, hipStream_t s);
 
 hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int4* moves, const int64_t* count,
                                  int64_t max_moves, hipStream_t s);
-hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, uint64_t seed,
-                               int64_t k, int64_t* delta, hipStream_t s);
+hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, const int64_t* off,
+                               uint64_t seed, int64_t k, int64_t* delta, hipStream_t s);
+/* u-major evaluation: GT = transposed table, pis = {pi, G(v, pi(v)), self-loop, 0}
+ * per vertex, off = stream offsets.  Needs bm_gain_umajor_lds(n) bytes of LDS. */
+size_t bm_gain_umajor_lds(int32_t n);
+hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t* GT, int32_t num_cus, hipStream_t s);
+hipError_t bm_launch_gain_pis(const BmDevGraph& g, const BmGain& G, int4* pis, hipStream_t s);
+hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
+                                      const int64_t* off, uint64_t seed, int64_t* delta, int32_t num_cus,
+                                      hipStream_t s);
 hipError_t bm_launch_gain_rot3(const BmDevGraph& g, const BmGain& G, const int4* cand, int64_t k, int64_t* best,
                                uint8_t* perm, int4* cand_out_perm, hipStream_t s);
 hipError_t bm_launch_moves_from_swap(const int32_t* pi_before, int32_t u, int32_t v, int4* moves, int64_t* count,
@@ -62,10 +75,16 @@ struct BmLaunch {
 };
 
 /* Candidate sources. */
-enum BmCandMode { BM_CAND_LIST = 0, BM_CAND_UNIFORM = 1 };
+enum BmCandMode { BM_CAND_LIST = 0, BM_CAND_UNIFORM = 1, BM_CAND_UMAJOR = 2 };
 
 hipError_t bm_launch_swap(const BmDevGraph& g, const BmLaunch& L, int mode, const int2* cand,
-                          uint64_t seed, int64_t k, int64_t* delta, hipStream_t s);
+                          const int64_t* off, uint64_t seed, int64_t k, int64_t* delta, hipStream_t s);
+
+/* u-major stream (bm_umajor_first / bm_umajor_partner): counts (n+1 int32)
+ * and offsets off[0..n+1] (int64) of k draws; pairs written as a list. */
+hipError_t bm_launch_umajor_offsets(int32_t n, uint64_t seed, int64_t k, int32_t* counts, int64_t* off,
+                                    int32_t num_cus, hipStream_t s);
+hipError_t bm_launch_gen_umajor(int32_t n, uint64_t seed, const int64_t* off, int64_t k, int2* cand, hipStream_t s);
 hipError_t bm_launch_rot3(const BmDevGraph& g, const BmLaunch& L, const int4* cand, int64_t k,
                           int64_t* best, uint8_t* perm, int4* cand_out_perm, hipStream_t s);
 hipError_t bm_launch_score(const BmDevGraph& g, const BmLaunch& L, int64_t* acc, hipStream_t s);

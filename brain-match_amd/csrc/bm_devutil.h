@@ -63,6 +63,17 @@ __host__ __device__ __forceinline__ int perm_of(int p, int i) {
     }
 }
 
+/* Entry i (< off[n+1]) of a u-major pair stream with offsets off[1..n+1]:
+ * the vertex u whose range [off[u], off[u+1]) holds i. */
+__device__ __forceinline__ int32_t umajor_owner(const int64_t* __restrict__ off, int32_t n, int64_t i) {
+    int32_t lo = 1, hi = n;   /* smallest u with off[u + 1] > i */
+    while (lo < hi) {
+        const int32_t m = (lo + hi) >> 1;
+        if (off[m + 1] > i) hi = m; else lo = m + 1;
+    }
+    return lo;
+}
+
 template <typename T>
 __device__ __forceinline__ T sel3(int j, const T& a, const T& b, const T& c) {
     return j == 0 ? a : (j == 1 ? b : c);

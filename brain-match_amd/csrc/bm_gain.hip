@@ -37,8 +37,10 @@ using namespace bmd;
 
 namespace {
 
-constexpr int kBuildThreads = 512;
-constexpr int kBuildWaves = kBuildThreads / BM_WAVE;
+constexpr int kBuildThreads = 1024;   /* build: 16 waves per row */
+constexpr int kBU = 8;                 /* build: female-row entries per lane in flight */
+constexpr int kPatchThreads = 512;
+constexpr int kPatchWaves = kPatchThreads / BM_WAVE;
 
 __device__ __forceinline__ int32_t probe_g(const int2* __restrict__ T, uint32_t bits, int32_t key) {
     if (!bits || key == 0) return BM_ABSENT;
@@ -139,44 +141,89 @@ __device__ __forceinline__ void wave_flat(bool have, int32_t w, int32_t y, const
     wave_sync();
 }
 
+/* Female-row entry {f, w} in either layout: int2, or the packed word
+ * (f << 15) | w used when ids are < 2^17 and weights in 0..32767. */
+template <bool PACKED>
+struct Row;
+template <>
+struct Row<false> {
+    typedef int2 T;
+    static __device__ __forceinline__ int32_t key(T e) { return e.x; }
+    static __device__ __forceinline__ int32_t w(T e) { return e.y; }
+    static __device__ __forceinline__ T none() { return make_int2(0, 0); }
+};
+template <>
+struct Row<true> {
+    typedef uint32_t T;
+    static __device__ __forceinline__ int32_t key(T e) { return (int32_t)(e >> 15); }
+    static __device__ __forceinline__ int32_t w(T e) { return (int32_t)(e & 0x7fffu); }
+    static __device__ __forceinline__ T none() { return 0u; }
+};
+
+/* Row u: every male edge of u (out-edges against the female in-row of the
+ * neighbour's image, in-edges against its out-row) is taken by one wave at a
+ * time from a block-wide counter (so a long female row does not hold up the
+ * block); the wave's lanes stride the row with kBU entries each in flight and
+ * add min(w, fw) into the LDS copy of the row's column window. */
+template <bool PACKED>
 __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGain G, int32_t col0, int32_t ncols) {
+    typedef Row<PACKED> R;
+    typedef typename R::T E;
     extern __shared__ int32_t row[];
-    __shared__ WaveScratch scratch[kBuildWaves];
+    __shared__ int32_t next_edge;
     const int tid = threadIdx.x;
     const int lane = tid & (BM_WAVE - 1);
-    const int wave = tid / BM_WAVE;
     const bool windowed = !(col0 == 0 && (int64_t)ncols >= G.stride);
-    WaveScratch* S = &scratch[wave];
-    auto add = [&](int32_t, int32_t c, int32_t v) { atomicAdd(&row[c], v); };
+    const E* __restrict__ f_in_rows = PACKED ? (const E*)G.f_in_p : (const E*)g.f_in;
+    const E* __restrict__ f_out_rows = PACKED ? (const E*)G.f_out_p : (const E*)g.f_out;
     for (int32_t u = blockIdx.x + 1; u <= g.n; u += gridDim.x) {
         for (int32_t c = tid; c < ncols; c += kBuildThreads) row[c] = 0;
+        if (tid == 0) next_edge = 0;
         __syncthreads();
-        /* out-edges u -> x: female edges f -> pi(x), i.e. the in-row (CSC) of pi(x) */
-        {
-            const int64_t b = g.m_out_ptr[u], e = g.m_out_ptr[u + 1];
-            for (int64_t e0 = b + (int64_t)wave * BM_WAVE; e0 < e; e0 += (int64_t)kBuildWaves * BM_WAVE) {
-                const int64_t i = e0 + lane;
-                bool have = i < e;
-                int2 ed = have ? g.m_out[i] : make_int2(0, 0);
-                have = have && ed.x != u;
-                const int32_t gx = have ? g.pi[ed.x] : 0;
-                have = have && gx != 0;
-                const int64_t rb = have ? g.f_in_ptr[gx] : 0, re = have ? g.f_in_ptr[gx + 1] : 0;
-                wave_flat(have, ed.y, 0, g.f_in, rb, re, col0, ncols, windowed, S, lane, add);
+        const int64_t ob = g.m_out_ptr[u], nout = g.m_out_ptr[u + 1] - ob;
+        const int64_t ib = g.m_in_ptr[u], L = nout + (g.m_in_ptr[u + 1] - ib);
+        for (;;) {
+            int32_t t = 0;
+            if (lane == 0) t = atomicAdd(&next_edge, 1);
+            t = uni(t);
+            if (t >= L) break;
+            const bool outp = t < nout;
+            const int2 ed = outp ? g.m_out[ob + t] : g.m_in[ib + (t - nout)];
+            const int32_t x = uni(ed.x), w = uni(ed.y);
+            if (x == u) continue;                      /* self-loops are not in the table */
+            const int32_t gx = uni(g.pi[x]);
+            if (gx == 0) continue;
+            const int64_t* __restrict__ fptr = outp ? g.f_in_ptr : g.f_out_ptr;
+            const E* __restrict__ frow = outp ? f_in_rows : f_out_rows;
+            int64_t rb = uni64(fptr[gx]), re = uni64(fptr[gx + 1]);
+            if (windowed) {
+                int64_t a = rb, b = re;
+                while (a < b) {
+                    const int64_t m = (a + b) >> 1;
+                    if (R::key(frow[m]) < col0) a = m + 1; else b = m;
+                }
+                const int64_t lo = a;
+                b = re;
+                while (a < b) {
+                    const int64_t m = (a + b) >> 1;
+                    if (R::key(frow[m]) < col0 + ncols) a = m + 1; else b = m;
+                }
+                rb = lo;
+                re = a;
             }
-        }
-        /* in-edges x -> u: female edges pi(x) -> f, i.e. the out-row (CSR) of pi(x) */
-        {
-            const int64_t b = g.m_in_ptr[u], e = g.m_in_ptr[u + 1];
-            for (int64_t e0 = b + (int64_t)wave * BM_WAVE; e0 < e; e0 += (int64_t)kBuildWaves * BM_WAVE) {
-                const int64_t i = e0 + lane;
-                bool have = i < e;
-                int2 ed = have ? g.m_in[i] : make_int2(0, 0);
-                have = have && ed.x != u;
-                const int32_t gx = have ? g.pi[ed.x] : 0;
-                have = have && gx != 0;
-                const int64_t rb = have ? g.f_out_ptr[gx] : 0, re = have ? g.f_out_ptr[gx + 1] : 0;
-                wave_flat(have, ed.y, 0, g.f_out, rb, re, col0, ncols, windowed, S, lane, add);
+            for (int64_t j0 = rb; j0 < re; j0 += (int64_t)BM_WAVE * kBU) {
+                E ent[kBU];
+#pragma unroll
+                for (int k = 0; k < kBU; k++) {
+                    const int64_t j = j0 + lane + (int64_t)BM_WAVE * k;
+                    ent[k] = j < re ? frow[j] : R::none();
+                }
+#pragma unroll
+                for (int k = 0; k < kBU; k++) {
+                    if (j0 + lane + (int64_t)BM_WAVE * k >= re) continue;
+                    const int32_t f = R::key(ent[k]), fw = R::w(ent[k]);
+                    atomicAdd(&row[f - col0], w < fw ? w : fw);
+                }
             }
         }
         __syncthreads();
@@ -186,15 +233,21 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
     }
 }
 
+__global__ void k_pack_rows(const int2* __restrict__ in, int64_t m, uint32_t* __restrict__ out) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
+        out[i] = ((uint32_t)in[i].x << 15) | (uint32_t)in[i].y;
+}
+
 /* Patch the rows of the neighbours of each moved vertex x (pi(x): old -> new).
  * moves[i] = {x, old, new, 0}, *count moves, one block per move.  An in-edge
  * y -> x contributes c(w, f, pi(x)) to G(y, f) over the female in-row of pi(x);
  * an out-edge x -> y contributes c(w, pi(x), f) over the female out-row.  The
  * old female's terms are subtracted and the new one's added; self-loops are
  * not in the table.  Updates of different moves commute (integer atomics). */
-__global__ void __launch_bounds__(kBuildThreads) k_gain_update(BmDevGraph g, BmGain G, const int4* __restrict__ moves,
+__global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmGain G, const int4* __restrict__ moves,
                                                                const int64_t* __restrict__ count) {
-    __shared__ WaveScratch scratch[kBuildWaves];
+    __shared__ WaveScratch scratch[kPatchWaves];
     const int tid = threadIdx.x;
     const int lane = tid & (BM_WAVE - 1);
     const int wave = tid / BM_WAVE;
@@ -218,7 +271,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_update(BmDevGraph g, BmG
                 const int2* fadj = dir ? g.f_out : g.f_in;
                 const int64_t b = mptr[x], e = mptr[x + 1];
                 const int64_t rb = fptr[gx], re = fptr[gx + 1];
-                for (int64_t e0 = b + (int64_t)wave * BM_WAVE; e0 < e; e0 += (int64_t)kBuildWaves * BM_WAVE) {
+                for (int64_t e0 = b + (int64_t)wave * BM_WAVE; e0 < e; e0 += (int64_t)kPatchWaves * BM_WAVE) {
                     const int64_t i = e0 + lane;
                     bool have = i < e;
                     const int2 ed = have ? madj[i] : make_int2(0, 0);
@@ -251,7 +304,8 @@ __device__ __forceinline__ int64_t gain_swap_delta(const BmDevGraph& g, const Bm
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const int2* __restrict__ cand,
-                                                   uint64_t seed, int64_t k, int64_t* __restrict__ delta) {
+                                                   const int64_t* __restrict__ off, uint64_t seed, int64_t k,
+                                                   int64_t* __restrict__ delta) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < k; c += stride) {
         int32_t u, v;
@@ -259,10 +313,183 @@ __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const
             const int2 p = cand[c];
             u = p.x;
             v = p.y;
+        } else if (MODE == BM_CAND_UMAJOR) {
+            u = umajor_owner(off, g.n, c);
+            v = bm_umajor_partner(seed, (uint64_t)c, u, g.n);
         } else {
             bm_uniform_pair(seed, (uint64_t)c, g.n, &u, &v);
         }
         delta[c] = gain_swap_delta(g, G, u, v);
+    }
+}
+
+/* Transposed table: GT(f, u) = G(u, f), 64 x 64 tiles through LDS. */
+__global__ void __launch_bounds__(256) k_gain_transpose(const int32_t* __restrict__ Gt, int32_t* __restrict__ GT,
+                                                        int64_t stride) {
+    __shared__ int32_t tile[64][65];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int64_t tiles = (stride + 63) / 64;
+    for (int64_t t = blockIdx.x; t < tiles * tiles; t += gridDim.x) {
+        const int64_t br = t / tiles, bc = t % tiles;
+        for (int r = ty; r < 64; r += 4) {
+            const int64_t u = br * 64 + r, f = bc * 64 + tx;
+            tile[r][tx] = (u < stride && f < stride) ? Gt[u * stride + f] : 0;
+        }
+        __syncthreads();
+        for (int r = ty; r < 64; r += 4) {
+            const int64_t f = bc * 64 + r, u = br * 64 + tx;
+            if (f < stride && u < stride) GT[f * stride + u] = tile[tx][r];
+        }
+        __syncthreads();
+    }
+}
+
+/* Per vertex v: {pi(v), G(v, pi(v)), male self-loop weight of v, female
+ * self-loop weight of pi(v)} (BM_ABSENT for a missing loop). */
+__global__ void k_gain_pis(BmDevGraph g, BmGain G, int4* __restrict__ pis) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= g.n; v += stride) {
+        const int32_t f = v ? g.pi[v] : 0;
+        pis[v] = make_int4(f, v ? G.G[v * G.stride + f] : 0, v ? G.m_self[v] : BM_ABSENT,
+                           f ? G.f_self[f] : BM_ABSENT);
+    }
+}
+
+/* Four hash-table lookups with their probe sequences interleaved (one
+ * round of loads for all unresolved keys at a time); key 0 or an empty
+ * table gives BM_ABSENT. */
+__device__ __forceinline__ void probe4(const uint64_t* desc, const int2* const* tab, const int32_t* key,
+                                       int32_t* out) {
+    const int2* T[4];
+    uint32_t bits[4], s[4], pend = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        out[j] = BM_ABSENT;
+        bits[j] = (uint32_t)(desc[j] & 63u);
+        T[j] = tab[j] + (desc[j] >> 6);
+        const bool live = bits[j] != 0 && key[j] != 0;
+        s[j] = live ? slot_hash(key[j], bits[j]) : 0u;
+        pend |= (live ? 1u : 0u) << j;
+    }
+    while (pend) {
+        int2 e[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) e[j] = ((pend >> j) & 1u) ? T[j][s[j]] : make_int2(0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (!((pend >> j) & 1u)) continue;
+            if (e[j].x == 0 || e[j].x == key[j]) {
+                if (e[j].x != 0) out[j] = e[j].y;
+                pend &= ~(1u << j);
+            } else {
+                s[j] = (s[j] + 1u) & ((1u << bits[j]) - 1u);
+            }
+        }
+    }
+}
+
+/* u-major lookups: one block per first vertex u of the stream.  The block
+ * stages row G(u, .) (indexed by the partner's female), row GT(pi(u), .) =
+ * G(., pi(u)) (indexed by the partner) and u's out- and in-neighbour bitmaps
+ * in LDS; a pair (u, v) then costs two LDS reads, one gather of v's packed
+ * {pi, G(v, pi(v)), self-loop} and the delta store.  Pairs with an edge
+ * between u and v (bitmap hit) add the exact edge correction through the
+ * hash tables; self-loops add their exact terms. */
+constexpr int kUmThreads = 1024;
+constexpr int kUmStage = 8;    /* row words per thread in flight while staging */
+constexpr int kUmPairs = 4;    /* pairs per thread in flight */
+
+__global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, BmGain G, const int32_t* __restrict__ GT,
+                                                                 const int4* __restrict__ pis,
+                                                                 const int64_t* __restrict__ off, uint64_t seed,
+                                                                 int64_t* __restrict__ delta) {
+    extern __shared__ int32_t sm[];
+    const int32_t n = g.n;
+    const int64_t stride = G.stride;
+    const int32_t words = (n + 1 + 31) >> 5;
+    int32_t* rowG = sm;
+    int32_t* rowT = sm + stride;
+    uint32_t* adj = reinterpret_cast<uint32_t*>(sm + 2 * stride);   /* [0, words): u -> v, [words, 2 words): v -> u */
+    const int tid = threadIdx.x;
+    for (int32_t u = blockIdx.x + 1; u <= n; u += gridDim.x) {
+        const int64_t i0 = off[u], i1 = off[u + 1];
+        if (i0 == i1) continue;   /* block-uniform */
+        const int4 pu = pis[u];
+        const int32_t f1 = pu.x;
+        const int32_t F11 = pu.w;                           /* w_F(f1, f1) */
+        const uint64_t du = G.m_out_desc[u], df1 = f1 ? g.f_out_desc[f1] : 0;
+        const int32_t* __restrict__ Gu = G.G + (int64_t)u * stride;
+        const int32_t* __restrict__ Tu = GT + (int64_t)f1 * stride;
+        for (int64_t c0 = tid; c0 < stride; c0 += (int64_t)kUmThreads * kUmStage) {
+            int32_t a[kUmStage], t[kUmStage];
+#pragma unroll
+            for (int j = 0; j < kUmStage; j++) {
+                const int64_t c = c0 + (int64_t)kUmThreads * j;
+                a[j] = c < stride ? Gu[c] : 0;
+                t[j] = c < stride ? Tu[c] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kUmStage; j++) {
+                const int64_t c = c0 + (int64_t)kUmThreads * j;
+                if (c < stride) {
+                    rowG[c] = a[j];
+                    rowT[c] = t[j];
+                }
+            }
+        }
+        for (int32_t w = tid; w < 2 * words; w += kUmThreads) adj[w] = 0;
+        __syncthreads();
+        for (int64_t e = g.m_out_ptr[u] + tid; e < g.m_out_ptr[u + 1]; e += kUmThreads) {
+            const int32_t x = g.m_out[e].x;
+            atomicOr(&adj[x >> 5], 1u << (x & 31));
+        }
+        for (int64_t e = g.m_in_ptr[u] + tid; e < g.m_in_ptr[u + 1]; e += kUmThreads) {
+            const int32_t x = g.m_in[e].x;
+            atomicOr(&adj[words + (x >> 5)], 1u << (x & 31));
+        }
+        __syncthreads();
+        for (int64_t ib = i0 + tid; ib < i1; ib += (int64_t)kUmThreads * kUmPairs) {
+            int32_t v[kUmPairs];
+            int4 pv[kUmPairs];
+#pragma unroll
+            for (int j = 0; j < kUmPairs; j++) {   /* all gathers in flight before any use */
+                const int64_t i = ib + (int64_t)kUmThreads * j;
+                v[j] = i < i1 ? bm_umajor_partner(seed, (uint64_t)i, u, n) : 0;
+                pv[j] = i < i1 ? pis[v[j]] : make_int4(0, 0, BM_ABSENT, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < kUmPairs; j++) {
+                const int64_t i = ib + (int64_t)kUmThreads * j;
+                if (i >= i1) continue;
+                const int32_t f2 = pv[j].x, x = v[j];
+                const int32_t F22 = pv[j].w;                /* w_F(f2, f2) */
+                int64_t d = (int64_t)rowG[f2] - pu.y + rowT[x] - pv[j].y;
+                const bool uv = (adj[x >> 5] >> (x & 31)) & 1u;
+                const bool vu = (adj[words + (x >> 5)] >> (x & 31)) & 1u;
+                if (uv || vu) {
+                    /* edges between u and v: w_M(u, v), w_M(v, u), w_F(f1, f2), w_F(f2, f1) */
+                    const bool ff = f1 != 0 && f2 != 0;
+                    const uint64_t desc[4] = {du, G.m_out_desc[x], df1, f2 ? g.f_out_desc[f2] : 0};
+                    const int2* tab[4] = {G.m_out_tab, G.m_out_tab, g.f_out_tab, g.f_out_tab};
+                    const int32_t key[4] = {uv ? x : 0, vu ? u : 0, ff ? f2 : 0, ff ? f1 : 0};
+                    int32_t r[4];
+                    probe4(desc, tab, key, r);
+                    const int32_t F12 = r[2], F21 = r[3];
+                    if (uv) {
+                        const int32_t w = r[0];
+                        d += (int64_t)contrib(w, F21) - contrib(w, F22) - contrib(w, F11) + contrib(w, F12);
+                    }
+                    if (vu) {
+                        const int32_t w = r[1];
+                        d += (int64_t)contrib(w, F12) - contrib(w, F11) - contrib(w, F22) + contrib(w, F21);
+                    }
+                }
+                if (pu.z != BM_ABSENT) d += contrib(pu.z, F22) - (int64_t)contrib(pu.z, F11);
+                if (pv[j].z != BM_ABSENT) d += contrib(pv[j].z, F11) - (int64_t)contrib(pv[j].z, F22);
+                delta[i] = d;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -336,17 +563,18 @@ __global__ void k_moves_from_perm3(const int32_t* __restrict__ pi_before, int32_
 /* ------------------------------------------------------------------ */
 
 int32_t bm_gain_chunk_cols(int64_t stride) {
-    const int64_t cap = 36 * 1024;   /* 144 KiB of int32 per block */
+    const int64_t cap = 20000;   /* 80,000 bytes of int32: two blocks per CU (160 KiB of LDS) */
     return (int32_t)(stride < cap ? stride : cap);
 }
 
 hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
     const int32_t chunk = bm_gain_chunk_cols(G.stride);
     const size_t lds = (size_t)chunk * sizeof(int32_t);
-    hipError_t e = hipFuncSetAttribute((const void*)k_gain_build, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const void* kern = G.f_out_p ? (const void*)k_gain_build<true> : (const void*)k_gain_build<false>;
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int occ = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_gain_build, kBuildThreads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kBuildThreads, lds);
     if (e != hipSuccess) return e;
     if (occ < 1) occ = 1;
     int64_t blocks = (int64_t)occ * (num_cus > 0 ? num_cus : 256);
@@ -354,7 +582,12 @@ hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t nu
     if (blocks < 1) blocks = 1;
     for (int64_t c0 = 0; c0 < G.stride; c0 += chunk) {
         const int32_t nc = (int32_t)(G.stride - c0 < chunk ? G.stride - c0 : chunk);
-        hipLaunchKernelGGL(k_gain_build, dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g, G, (int32_t)c0, nc);
+        if (G.f_out_p)
+            hipLaunchKernelGGL(k_gain_build<true>, dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g, G,
+                               (int32_t)c0, nc);
+        else
+            hipLaunchKernelGGL(k_gain_build<false>, dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g, G,
+                               (int32_t)c0, nc);
     }
     return hipGetLastError();
 }
@@ -363,21 +596,60 @@ hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int
                                  int64_t max_moves, hipStream_t s) {
     if (max_moves <= 0) return hipSuccess;
     int64_t blocks = max_moves < 4096 ? max_moves : 4096;
-    hipLaunchKernelGGL(k_gain_update, dim3((unsigned)blocks), dim3(kBuildThreads), 0, s, g, G, moves, count);
+    hipLaunchKernelGGL(k_gain_update, dim3((unsigned)blocks), dim3(kPatchThreads), 0, s, g, G, moves, count);
     return hipGetLastError();
 }
 
-hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, uint64_t seed,
-                               int64_t k, int64_t* delta, hipStream_t s) {
+hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, const int64_t* off,
+                               uint64_t seed, int64_t k, int64_t* delta, hipStream_t s) {
     if (k <= 0) return hipSuccess;
     int64_t blocks = (k + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (mode == BM_CAND_LIST)
-        hipLaunchKernelGGL(k_gain_swap<BM_CAND_LIST>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, seed, k,
-                           delta);
+        hipLaunchKernelGGL(k_gain_swap<BM_CAND_LIST>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, off, seed,
+                           k, delta);
+    else if (mode == BM_CAND_UMAJOR)
+        hipLaunchKernelGGL(k_gain_swap<BM_CAND_UMAJOR>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, off, seed,
+                           k, delta);
     else
-        hipLaunchKernelGGL(k_gain_swap<BM_CAND_UNIFORM>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, seed, k,
-                           delta);
+        hipLaunchKernelGGL(k_gain_swap<BM_CAND_UNIFORM>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, off,
+                           seed, k, delta);
+    return hipGetLastError();
+}
+
+size_t bm_gain_umajor_lds(int32_t n) { return 2 * ((size_t)n + 1) * 4 + 2 * (size_t)((n + 1 + 31) >> 5) * 4; }
+
+hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t* GT, int32_t num_cus, hipStream_t s) {
+    const int64_t tiles = (G.stride + 63) / 64;
+    int64_t blocks = tiles * tiles;
+    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 16;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(k_gain_transpose, dim3((unsigned)blocks), dim3(256), 0, s, G.G, GT, G.stride);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_gain_pis(const BmDevGraph& g, const BmGain& G, int4* pis, hipStream_t s) {
+    int64_t blocks = ((int64_t)g.n + 256) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_gain_pis, dim3((unsigned)blocks), dim3(256), 0, s, g, G, pis);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
+                                      const int64_t* off, uint64_t seed, int64_t* delta, int32_t num_cus,
+                                      hipStream_t s) {
+    const size_t lds = bm_gain_umajor_lds(g.n);
+    hipError_t e = hipFuncSetAttribute((const void*)k_gain_swap_umajor, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    int occ = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_gain_swap_umajor, kUmThreads, lds);
+    if (e != hipSuccess) return e;
+    if (occ < 1) occ = 1;
+    int64_t blocks = (int64_t)occ * (num_cus > 0 ? num_cus : 256);
+    if (blocks > g.n) blocks = g.n;
+    hipLaunchKernelGGL(k_gain_swap_umajor, dim3((unsigned)blocks), dim3(kUmThreads), lds, s, g, G, GT, pis, off, seed,
+                       delta);
     return hipGetLastError();
 }
 
@@ -400,5 +672,13 @@ hipError_t bm_launch_moves_from_swap(const int32_t* pi_before, int32_t u, int32_
 hipError_t bm_launch_moves_from_perm3(const int32_t* pi_before, int32_t a, int32_t b, int32_t c, int32_t p,
                                       int4* moves, int64_t* count, hipStream_t s) {
     hipLaunchKernelGGL(k_moves_from_perm3, dim3(1), dim3(1), 0, s, pi_before, a, b, c, p, moves, count);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_pack_rows(const int2* in, int64_t m, uint32_t* out, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    int64_t blocks = (m + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)blocks), dim3(256), 0, s, in, m, out);
     return hipGetLastError();
 }

@@ -405,6 +405,71 @@ __global__ void k_apply_perm3(int32_t* pi, int32_t a, int32_t b, int32_t c, int3
     pi[c] = f[perm_of(p, 2)];
 }
 
+/* ---- u-major uniform stream (bm_umajor_first / bm_umajor_partner) ---- */
+
+__global__ void k_fill_i32(int32_t* __restrict__ p, int64_t count, int32_t v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
+}
+
+/* counts[u] += #{c : first vertex of draw c is u}; a block-private histogram
+ * in LDS (n+1 ints) when it fits, merged with one global atomic per bin. */
+__global__ void __launch_bounds__(1024) k_umajor_hist(int32_t n, uint64_t seed, int64_t k, int32_t* __restrict__ counts,
+                                                     int use_lds) {
+    extern __shared__ int32_t h[];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (use_lds) {
+        for (int32_t x = threadIdx.x; x <= n; x += blockDim.x) h[x] = 0;
+        __syncthreads();
+        for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < k; c += stride)
+            atomicAdd(&h[bm_umajor_first(seed, (uint64_t)c, n)], 1);
+        __syncthreads();
+        for (int32_t x = threadIdx.x; x <= n; x += blockDim.x)
+            if (h[x]) atomicAdd(&counts[x], h[x]);
+    } else {
+        for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < k; c += stride)
+            atomicAdd(&counts[bm_umajor_first(seed, (uint64_t)c, n)], 1);
+    }
+}
+
+/* off[0] = off[1] = 0, off[u + 1] = off[u] + counts[u] (u = 1..n): one block,
+ * each thread a contiguous run, then a block scan of the run sums. */
+__global__ void __launch_bounds__(1024) k_umajor_offsets(int32_t n, const int32_t* __restrict__ counts,
+                                                        int64_t* __restrict__ off) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t per = ((int64_t)n + 1023) / 1024;
+    const int64_t a = 1 + (int64_t)t * per, b = a + per < (int64_t)n + 1 ? a + per : (int64_t)n + 1;
+    int64_t sum = 0;
+    for (int64_t x = a; x < b; x++) sum += counts[x];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int64_t y = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += y;
+        __syncthreads();
+    }
+    int64_t run = t ? part[t - 1] : 0;
+    if (t == 0) {
+        off[0] = 0;
+        off[1] = 0;
+    }
+    for (int64_t x = a; x < b; x++) {
+        run += counts[x];
+        off[x + 1] = run;
+    }
+}
+
+__global__ void k_gen_umajor(int32_t n, uint64_t seed, const int64_t* __restrict__ off, int64_t k,
+                             int2* __restrict__ cand) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) {
+        const int32_t u = umajor_owner(off, n, i);
+        cand[i] = make_int2(u, bm_umajor_partner(seed, (uint64_t)i, u, n));
+    }
+}
+
 int grid_for(int64_t work, int threads) {
     int64_t b = (work + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -546,5 +611,32 @@ hipError_t bm_launch_apply_swap(int32_t* pi, int32_t u, int32_t v, hipStream_t s
 
 hipError_t bm_launch_apply_perm3(int32_t* pi, int32_t a, int32_t b, int32_t c, int32_t p, hipStream_t s) {
     hipLaunchKernelGGL(k_apply_perm3, dim3(1), dim3(1), 0, s, pi, a, b, c, p);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_umajor_offsets(int32_t n, uint64_t seed, int64_t k, int32_t* counts, int64_t* off,
+                                    int32_t num_cus, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for((int64_t)n + 1, 256)), dim3(256), 0, s, counts, (int64_t)n + 1, 0);
+    const size_t lds = ((size_t)n + 1) * sizeof(int32_t);
+    const int use_lds = lds <= 150 * 1024 ? 1 : 0;
+    if (use_lds) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_umajor_hist, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    int64_t blocks = (k + 1023) / 1024;
+    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256);
+    if (blocks > cap) blocks = cap;
+    if (blocks < 1) blocks = 1;
+    if (k > 0)
+        hipLaunchKernelGGL(k_umajor_hist, dim3((unsigned)blocks), dim3(1024), use_lds ? lds : 0, s, n, seed, k, counts,
+                           use_lds);
+    hipLaunchKernelGGL(k_umajor_offsets, dim3(1), dim3(1024), 0, s, n, counts, off);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_gen_umajor(int32_t n, uint64_t seed, const int64_t* off, int64_t k, int2* cand, hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_umajor, dim3(grid_for(k, 256)), dim3(256), 0, s, n, seed, off, k, cand);
     return hipGetLastError();
 }

@@ -253,6 +253,19 @@ int bm_batch_eval_swaps(bm_batch* b);
 int bm_batch_eval_swaps_uniform(bm_batch* b, uint64_t seed, int64_t k);
 int bm_batch_eval_rot3(bm_batch* b);
 
+/* u-major uniform pairs (bm_umajor_first / bm_umajor_partner in
+ * brain-match_amd/csrc/bm_rng.h): k independent uniform pairs listed grouped
+ * by their first vertex, so an evaluator can visit every first vertex once.
+ * The reference's changed-first enumeration is grouped the same way (all
+ * partners of one changed vertex together, src/graph-re-alignment-gpu-swap.cu:163-195).
+ * gen_umajor writes the pairs into the batch; eval_swaps_umajor evaluates the
+ * same stream without materialising it.  With the gain table enabled and
+ * (n+1) <= 20,000, one block per first vertex stages the two table rows it
+ * needs in LDS (the table's transpose is kept next to it, 2 x (n+1)^2 int32
+ * in all); otherwise each pair is evaluated on its own. */
+int bm_batch_gen_umajor(bm_batch* b, uint64_t seed, int64_t k);
+int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k);
+
 /* Read back a slice [first, first+count).  Any output pointer may be NULL.
  * cand gets 2 ints per pair batch / 4 ints (a,b,c,perm) per triple batch. */
 int bm_batch_get(bm_batch* b, int64_t first, int64_t count, int64_t* delta,
@@ -301,7 +314,8 @@ int bm_gain_row(bm_ctx* ctx, int32_t u, int32_t* row);
 #define BM_KERNEL_GAIN_SWAP 4
 #define BM_KERNEL_GAIN_ROT3 5
 #define BM_KERNEL_GAIN_UPDATE 6
-#define BM_KERNEL_COUNT 7
+#define BM_KERNEL_AUX 7          /* u-major stream offsets, table transpose */
+#define BM_KERNEL_COUNT 8
 
 /* When enabled, every delta/score launch is bracketed by HIP events on the
  * context stream; bm_timing_get synchronises and returns the number of

@@ -62,6 +62,10 @@
  *                               K-smallest-key sample of it.  Pinned by set
  *                               equality with a Python BFS over the raw edge
  *                               list and by the sample's uniformity.
+ *  or_umajor_pairs              the u-major uniform pair stream.  Pinned by
+ *                               its law (first vertices = the uniform pair
+ *                               stream's multiset, partners uniform) and by
+ *                               equality with pairs downloaded from the device.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -989,4 +993,24 @@ void or_twohop_pairs(const or_graph* g, const int32_t* anchors, int32_t n_anchor
     free(set);
     free(kx);
     free(take);
+}
+
+/* u-major uniform pairs: draw c picks its first vertex 1 + range(lo32(rand(seed, c)), n);
+ * entries are grouped by first vertex in ascending order, entry i of vertex u
+ * pairs it with 1 + range(hi32(rand(mix(seed ^ K), i)), n - 1), skipping u.
+ * Writes k pairs; returns 0. */
+int or_umajor_pairs(uint64_t seed, int64_t k, int32_t n, int32_t* uv) {
+    int64_t* cnt = (int64_t*)calloc((size_t)n + 2, sizeof(int64_t));
+    for (int64_t c = 0; c < k; c++) cnt[1 + range32((uint32_t)rand_at(seed, (uint64_t)c), (uint32_t)n)]++;
+    const uint64_t q = mix64(seed ^ 0x6A09E667F3BCC909ULL);
+    int64_t i = 0;
+    for (int32_t u = 1; u <= n; u++)
+        for (int64_t j = 0; j < cnt[u]; j++, i++) {
+            const uint64_t r = rand_at(q, (uint64_t)i);
+            int32_t b = (int32_t)range32((uint32_t)(r >> 32), (uint32_t)(n - 1)) + 1;
+            uv[2 * i] = u;
+            uv[2 * i + 1] = b >= u ? b + 1 : b;
+        }
+    free(cnt);
+    return 0;
 }

@@ -375,3 +375,13 @@ def twohop_pairs(o: "Oracle", per_anchor, seed, anchors=None, n_anchors=None):
     pairs = np.empty((na * per_anchor, 2), np.int32)
     _o.or_twohop_pairs(o.h, ap, na, per_anchor, seed, _p32(pairs))
     return pairs
+
+
+_sig(_o, "or_umajor_pairs", C.c_int, [C.c_uint64, C.c_int64, C.c_int32, i32p])
+
+
+def umajor_pairs(seed, k, n):
+    """The u-major uniform pair stream (k pairs, grouped by first vertex)."""
+    uv = np.empty((k, 2), np.int32)
+    _o.or_umajor_pairs(seed, k, n, _p32(uv))
+    return uv

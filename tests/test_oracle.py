@@ -214,3 +214,22 @@ def test_golden_greedy_match(bm):
     male = bm.Graph(z["m_src"], z["m_dst"], z["m_w"], n)
     female = bm.Graph(z["f_src"], z["f_dst"], z["f_w"], n)
     assert np.array_equal(bm.greedy_match(n, male, female), z["ref_matching"])
+
+
+def test_umajor_stream_law(orc):
+    """Grouped by first vertex, u != v, first vertices distributed like k
+    independent uniform draws, partners uniform over the other vertices."""
+    n, k = 50, 200000
+    uv = orc.umajor_pairs(9, k, n)
+    assert (np.diff(uv[:, 0]) >= 0).all() and (uv[:, 0] != uv[:, 1]).all()
+    assert uv.min() >= 1 and uv.max() <= n
+    cu = np.bincount(uv[:, 0], minlength=n + 1)[1:]
+    cv = np.bincount(uv[:, 1], minlength=n + 1)[1:]
+    e = k / n
+    assert np.abs(cu - e).max() < 6 * np.sqrt(e) and np.abs(cv - e).max() < 6 * np.sqrt(e)
+    # the first vertices are the same multiset as the uniform pair stream's (same draws)
+    assert np.array_equal(np.sort(uv[:, 0]), np.sort(orc.uniform_pairs(9, k, n)[:, 0]))
+    # partner frequencies for one first vertex do not favour any other vertex
+    sub = uv[uv[:, 0] == 7][:, 1]
+    h = np.bincount(sub, minlength=n + 1)[1:]
+    assert h[6] == 0 and np.abs(np.delete(h, 6) - len(sub) / (n - 1)).max() < 6 * np.sqrt(len(sub) / (n - 1))
