@@ -6,25 +6,32 @@ male/female pair with n = 18,524 vertices and ~4M directed edges per graph
 (power-law out-degrees, alpha 2.1; lognormal(0.5, 1.2) integer weights >= 1;
 female = male relabelled with 20% of edges reweighted/rewired), initial
 matching = the greedy matching of the pair.  One step evaluates the exact swap
-delta of 2^28 uniform candidate pairs, generated inside the delta kernel from a
-counter-based stream (a new stream per step), writing 2^28 int64 deltas to HBM.
+delta of 2^28 independent uniform candidate pairs (u != v) against the
+current matching and writes 2^28 int64 deltas to HBM.  The pairs come from a
+counter-based stream generated on the device (a new seed per step); by default
+the stream lists the pairs grouped by first vertex ("umajor", same law as
+independent pairs, see bm_umajor_first in brain-match_amd/csrc/bm_rng.h), or
+in draw order with --stream uniform.
 
-Two exact evaluation strategies are timed; both give identical deltas:
+Two exact evaluation strategies are timed on the same stream; both give
+identical deltas:
   gain  -- each step rebuilds the (n+1)^2 int32 placement-gain table for the
-           current matching, then evaluates every pair from four table entries
-           plus an exact correction for adjacent pairs (default headline);
+           current matching (plus its transpose for the umajor stream), then
+           evaluates every pair from table entries plus an exact correction
+           for adjacent pairs (default headline);
   scan  -- each pair scans its four male lists against LDS-staged female rows
            (one wavefront per pair).
 The 3-vertex permutation kernels are timed on 2^26 uniform triples in the
 same run and reported under "rot3".
 
 value = candidates / second over the K timed steps of the headline strategy
-(graph, matching and output buffers already resident in HBM; the table build is
-inside every step).  roofline = the step's dominant kernel: algorithmic bytes
-per launch / mean launch time from HIP events on the library stream.
+(graph, matching and output buffers resident in HBM; the table build is inside
+every step).  roofline = the step's dominant kernel: algorithmic bytes per
+launch / mean launch time from HIP events on the library stream; traffic = HBM
+bytes per launch measured with rocprofv3 (profiles/swap_n18524_traffic.json).
 cpu_baseline = the reference's own calculate_swap_delta
 (src/graph-alignment-optimized.c:321-408, compiled from its sources into
-oracle/_ref) over the first candidates of the last timed stream, OpenMP
+oracle/_ref) over an evenly strided sample of the last timed stream, OpenMP
 dynamic schedule, on this host; its deltas are compared with the device's.
 """
 import argparse
@@ -45,28 +52,35 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gain_build_bytes(ctx, pi):
-    """Algorithmic bytes of one table build: every male edge read from both
-    directions (8 B) with pi of its other endpoint (4 B), every (male edge,
-    female-row entry) pair it expands into (8 B per entry), and the table write
-    ((n+1)^2 * 4 B)."""
+def packed_rows(n, female):
+    """The table build streams female rows as 4-byte (f << 15) | w words when
+    ids are below 2^17 and weights in 0..32767, else as 8-byte {f, w}."""
+    return n < (1 << 17) and int(female.w.min()) >= 0 and int(female.w.max()) <= 32767
+
+
+def gain_build_bytes(ctx, pi, entry_bytes):
+    """Algorithmic bytes of one table build.  The build streams, for every
+    male edge seen from each endpoint (2m visits), the edge (8 B), pi of the
+    other endpoint (4 B) and the bounds of that vertex's image's female row
+    (16 B), then every entry of that female row (entry_bytes each); it writes
+    the whole table ((n+1)^2 * 4 B)."""
     import numpy as np
     mo, mi = ctx.degrees(female=False)
     fo, fi = ctx.degrees(female=True)
     pi = np.asarray(pi, dtype=np.int64)
-    # out-edge u->x expands over the female in-row of pi(x): count by x (in-degree of x)
+    # out-edge u->x expands over the female in-row of pi(x): counted once per in-edge of x
     entries = int((mi.astype(np.int64) * fi[pi]).sum() + (mo.astype(np.int64) * fo[pi]).sum())
     m = int(mo.sum())
-    return 2 * m * 12 + 8 * entries + (ctx.n + 1) ** 2 * 4, entries
+    return 2 * m * 28 + entry_bytes * entries + (ctx.n + 1) ** 2 * 4, entries
 
 
 def gain_lookup_bytes(k):
     """Per pair: pi(u), pi(v) (8 B), four table entries (16 B), the two male
-    adjacency probes (2 x 8 B), two self-loop words (8 B), delta out (8 B)."""
+    adjacency words (2 x 8 B), two self-loop words (8 B), delta out (8 B)."""
     return 56 * k
 
 
-def timed(ctx, bm, kernel_ids, fn, steps):
+def timed(ctx, kernel_ids, fn, steps):
     ctx.sync()
     ctx.timing_reset()
     t0 = time.perf_counter()
@@ -78,6 +92,13 @@ def timed(ctx, bm, kernel_ids, fn, steps):
     return wall, kt
 
 
+def per_launch_ms(kt, kid, steps=None):
+    n, ms = kt[kid]
+    if steps:
+        return ms / steps
+    return ms / n if n else 0.0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,11 +106,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--candidates", type=int, default=1 << 28)
     ap.add_argument("--triples", type=int, default=1 << 26)
+    ap.add_argument("--stream", choices=["umajor", "uniform"], default="umajor")
     ap.add_argument("--scan-steps", type=int, default=2, help="timed steps of the scan strategy")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rot3", action="store_true")
-    ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--no-scan", action="store_true", help="time only the headline strategy")
     ap.add_argument("--headline", choices=["gain", "scan"], default="gain")
     ap.add_argument("--pi0", choices=["greedy", "shuffle"], default="greedy")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -135,6 +157,10 @@ def main():
     batch = ctx.batch(K)
     ctx.timing_enable(True)
     seeds = [2000 + i for i in range(args.steps)]
+    umajor = args.stream == "umajor"
+    evaluate = batch.eval_swaps_umajor if umajor else batch.eval_swaps_uniform
+    stream_bytes = ctx.umajor_algo_bytes if umajor else ctx.uniform_algo_bytes
+    entry_bytes = 4 if packed_rows(N_VERTICES, female) else 8
 
     # ---------------- gain-table strategy ----------------
     gain = None
@@ -142,34 +168,41 @@ def main():
         ctx.gain_enable(True)
         for i in range(args.warmup):
             ctx.gain_rebuild()
-            batch.eval_swaps_uniform(1000 + i, K)
+            evaluate(1000 + i, K)
 
         def gain_step(i):
             ctx.gain_rebuild()
-            batch.eval_swaps_uniform(seeds[i], K)
+            evaluate(seeds[i], K)
 
-        wall, kt = timed(ctx, bm, (bm.KERNEL_GAIN_BUILD, bm.KERNEL_GAIN_SWAP), gain_step, args.steps)
-        nb, msb = kt[bm.KERNEL_GAIN_BUILD]
-        nl, msl = kt[bm.KERNEL_GAIN_SWAP]
-        build_s = msb / 1000 / max(1, nb)
-        look_s = msl / 1000 / max(1, nl)
-        bb, entries = gain_build_bytes(ctx, pi0)
+        wall, kt = timed(ctx, (bm.KERNEL_GAIN_BUILD, bm.KERNEL_GAIN_SWAP, bm.KERNEL_AUX), gain_step, args.steps)
+        build_s = per_launch_ms(kt, bm.KERNEL_GAIN_BUILD) / 1000
+        look_s = per_launch_ms(kt, bm.KERNEL_GAIN_SWAP) / 1000
+        aux_s = per_launch_ms(kt, bm.KERNEL_AUX, steps=args.steps) / 1000
+        bb, entries = gain_build_bytes(ctx, pi0, entry_bytes)
         lb = gain_lookup_bytes(K)
         gain = {
             "value": K * args.steps / wall,
             "ms_per_step": 1000 * wall / args.steps,
             "build_ms": 1000 * build_s,
             "lookup_ms": 1000 * look_s,
+            "aux_ms": 1000 * aux_s,
             "table_bytes": (N_VERTICES + 1) ** 2 * 4,
             "build_roofline": {"bound": "hbm", "achieved": bb / build_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": bb / build_s / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": bb,
-                               "expanded_entries": entries},
+                               "expanded_entries": entries, "female_entry_bytes": entry_bytes},
             "lookup_roofline": {"bound": "hbm", "achieved": lb / look_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": lb / look_s / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": lb},
         }
-        log(f"gain: {args.steps} x {K} pairs in {wall:.3f}s -> {gain['value']:.4g} deltas/s "
-            f"(build {1000 * build_s:.2f} ms, lookups {1000 * look_s:.2f} ms per step)")
-        gain_deltas_tail = batch.deltas(0, 1 << 16)
+        log(f"gain ({args.stream}): {args.steps} x {K} pairs in {wall:.3f}s -> {gain['value']:.4g} deltas/s "
+            f"(build {1000 * build_s:.2f} ms, lookups {1000 * look_s:.2f} ms, stream/transpose {1000 * aux_s:.2f} ms)")
+        gain_deltas_head = batch.deltas(0, 1 << 16)
+        if umajor:
+            # the same table against the draw-order stream (per-pair table reads)
+            for i in range(args.warmup):
+                batch.eval_swaps_uniform(1000 + i, K)
+            w2, kt2 = timed(ctx, (bm.KERNEL_GAIN_SWAP,), lambda i: batch.eval_swaps_uniform(seeds[i], K), args.steps)
+            gain["uniform_stream_lookup_ms"] = per_launch_ms(kt2, bm.KERNEL_GAIN_SWAP)
+            evaluate(seeds[-1], K)
         ctx.gain_enable(False)
 
     # ---------------- scan strategy ----------------
@@ -178,11 +211,10 @@ def main():
         steps = args.steps if args.headline == "scan" else args.scan_steps
         sseeds = seeds[-steps:]
         for i in range(min(args.warmup, 1) if args.headline != "scan" else args.warmup):
-            batch.eval_swaps_uniform(1000 + i, K)
-        wall, kt = timed(ctx, bm, (bm.KERNEL_SWAP,), lambda i: batch.eval_swaps_uniform(sseeds[i], K), steps)
-        nl, ms = kt[bm.KERNEL_SWAP]
-        launch_s = ms / 1000 / max(1, nl)
-        algo = [ctx.uniform_algo_bytes(s, K) for s in sseeds]
+            evaluate(1000 + i, K)
+        wall, kt = timed(ctx, (bm.KERNEL_SWAP, bm.KERNEL_AUX), lambda i: evaluate(sseeds[i], K), steps)
+        launch_s = per_launch_ms(kt, bm.KERNEL_SWAP) / 1000
+        algo = [stream_bytes(s, K) for s in sseeds]
         bytes_per_launch = sum(b for b, _ in algo) / len(algo)
         mean_D = sum(d for _, d in algo) / len(algo) / K
         scan = {
@@ -193,12 +225,14 @@ def main():
                          "unit": "GB/s", "frac": bytes_per_launch / launch_s / 1e9 / HBM_PEAK_GBS,
                          "algo_bytes_per_launch": bytes_per_launch, "mean_D": mean_D, "launch_ms": 1000 * launch_s},
         }
-        log(f"scan: {steps} x {K} pairs in {wall:.3f}s -> {scan['value']:.4g} deltas/s "
+        log(f"scan ({args.stream}): {steps} x {K} pairs in {wall:.3f}s -> {scan['value']:.4g} deltas/s "
             f"({scan['roofline']['achieved']:.0f} GB/s algorithmic)")
         if gain is not None:
-            same = bool((batch.deltas(0, 1 << 16) == gain_deltas_tail).all())
+            same = bool((batch.deltas(0, 1 << 16) == gain_deltas_head).all())
             scan["matches_gain_path"] = same
             log(f"scan and gain deltas identical on the last stream's first 65536 pairs: {same}")
+            if not same:
+                raise SystemExit("the two exact strategies disagree")
 
     head = gain if args.headline == "gain" else scan
     stats = batch.stats()
@@ -210,16 +244,17 @@ def main():
     if os.path.exists(prof):
         with open(prof) as f:
             p = json.load(f)
-        if p.get("candidates") == K and p.get("pi0") == args.pi0:
+        if p.get("candidates") == K and p.get("pi0") == args.pi0 and p.get("stream") == args.stream:
             traffic = p.get("hbm_bytes_per_launch", {})
 
     if args.headline == "gain":
         dom = "lookup" if gain["lookup_ms"] >= gain["build_ms"] else "build"
         rf = dict(gain[dom + "_roofline"])
-        rf["kernel"] = "k_gain_swap<uniform>" if dom == "lookup" else "k_gain_build"
+        rf["kernel"] = ("k_gain_swap_umajor" if umajor else "k_gain_swap<uniform>") if dom == "lookup" \
+            else "k_gain_build"
     else:
         rf = dict(scan["roofline"])
-        rf["kernel"] = "k_swap<uniform>"
+        rf["kernel"] = "k_swap<umajor>" if umajor else "k_swap<uniform>"
     rf["traffic"] = traffic.get(rf["kernel"])
 
     result = {
@@ -238,6 +273,7 @@ def main():
         "config": {
             "workload": "swap-delta n=18524 ~4M edges/graph, 2^28 uniform pairs per step",
             "strategy": args.headline,
+            "stream": args.stream,
             "n": N_VERTICES,
             "male_edges": male.m,
             "female_edges": female.m,
@@ -280,9 +316,8 @@ def bench_rot3(ctx, bm, args):
             b3.gen_triples(4000 + i, T)
             b3.eval_rot3()
 
-        wall, kt = timed(ctx, bm, (kid,), step, steps)
-        n3, ms3 = kt[kid]
-        l3 = ms3 / 1000.0 / max(1, n3)
+        wall, kt = timed(ctx, (kid,), step, steps)
+        l3 = per_launch_ms(kt, kid) / 1000.0
         ab3, d3 = b3.algo_bytes(generated=False)
         if mode == "gain":
             ab3 = T * (12 + 36 + 3 * 8 + 6 * 8 + 12 + 9)   # triple, 9 table entries, 3 self words, 6 adjacency probes, out
@@ -296,7 +331,8 @@ def bench_rot3(ctx, bm, args):
 
 
 def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
-    """The reference CPU arithmetic on a bounded sample of the same stream."""
+    """The reference CPU arithmetic on a bounded, evenly strided sample of the
+    last timed stream."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -312,18 +348,27 @@ def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
         runner = orc.ListsRef(n, male, female)
         what = "oracle restatement of calculate_swap_delta (edge lists + dense int16 female matrix)"
     run = lambda pi, uv: runner.deltas(pi, uv, threads)  # noqa: E731
+    umajor = args.stream == "umajor"
+
+    def pairs_at(idx):
+        if umajor:
+            return orc.umajor_pairs_at(seed, K, n, idx)
+        return orc.uniform_pairs(seed, int(idx[-1]) + 1, n)[idx]
+
     calib = 1 << 16
-    uv_all = orc.uniform_pairs(seed, calib, n)
+    idx = np.linspace(0, K - 1, calib).astype(np.int64)
+    uv = pairs_at(idx)
     t = time.perf_counter()
-    run(pi0, uv_all)
+    run(pi0, uv)
     rate = calib / max(1e-6, time.perf_counter() - t)
     S = int(min(K, max(calib, rate * args.cpu_seconds)))
     S -= S % 1024
-    uv = orc.uniform_pairs(seed, S, n)
+    idx = (np.arange(S, dtype=np.int64) * (K // S)).astype(np.int64)
+    uv = pairs_at(idx)
     t = time.perf_counter()
     d_cpu = run(pi0, uv)
     el = time.perf_counter() - t
-    d_gpu = batch.deltas(0, S)
+    d_gpu = batch.deltas(0, K)[idx]
     match = bool(np.array_equal(d_cpu.astype(np.int64), d_gpu))
     log(f"cpu baseline ({kind}, {threads} threads): {S} deltas in {el:.2f}s -> {S / el:.4g}/s; "
         f"device deltas identical: {match}")
@@ -332,7 +377,8 @@ def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
         "unit": "swap-deltas/s",
         "cores": threads,
         "kind": kind,
-        "sample": f"first {S} pairs of the last timed step's stream; {what}; OpenMP schedule(dynamic,1024)",
+        "sample": f"{S} pairs evenly strided over the last timed step's {args.stream} stream; {what}; "
+                  f"OpenMP schedule(dynamic,1024)",
         "matches_device": match,
     }
 

@@ -133,6 +133,7 @@ _sig("bm_timing_reset", C.c_int, [_c])
 _sig("bm_sync", C.c_int, [_c])
 _sig("bm_batch_algo_bytes", C.c_int, [_c, C.c_int32, i64p, i64p])
 _sig("bm_uniform_algo_bytes", C.c_int, [_c, C.c_uint64, C.c_int64, i64p, i64p])
+_sig("bm_umajor_algo_bytes", C.c_int, [_c, C.c_uint64, C.c_int64, i64p, i64p])
 _sig("bm_get_tuning", C.c_int, [_c, C.c_int32, C.POINTER(_Tuning)])
 _sig("bm_set_tuning", C.c_int, [_c, C.c_int32, C.POINTER(_Tuning)])
 _sig("bm_device_info", C.c_int, [_c, C.c_char_p, C.c_size_t])
@@ -482,6 +483,11 @@ class Context:
     def uniform_algo_bytes(self, seed, k):
         b, d = C.c_int64(0), C.c_int64(0)
         self._chk(_lib.bm_uniform_algo_bytes(self.h, seed, k, C.byref(b), C.byref(d)))
+        return b.value, d.value
+
+    def umajor_algo_bytes(self, seed, k):
+        b, d = C.c_int64(0), C.c_int64(0)
+        self._chk(_lib.bm_umajor_algo_bytes(self.h, seed, k, C.byref(b), C.byref(d)))
         return b.value, d.value
 
     def get_tuning(self, kernel):

@@ -330,6 +330,8 @@ int bm_sync(bm_ctx* ctx);
  * 69 + 36*D3 (+12 if read from memory). */
 int bm_batch_algo_bytes(bm_batch* b, int32_t generated, int64_t* bytes, int64_t* sum_degree);
 int bm_uniform_algo_bytes(bm_ctx* ctx, uint64_t seed, int64_t k, int64_t* bytes, int64_t* sum_degree);
+/* The same sum over the u-major stream (bm_batch_eval_swaps_umajor). */
+int bm_umajor_algo_bytes(bm_ctx* ctx, uint64_t seed, int64_t k, int64_t* bytes, int64_t* sum_degree);
 
 /* Launch-shape knobs (autotuner).  waves_per_block in {1,2,4,8},
  * lds_slots per wave (table slots of 4 bytes for graphs with n < 2^17 and

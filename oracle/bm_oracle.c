@@ -1014,3 +1014,30 @@ int or_umajor_pairs(uint64_t seed, int64_t k, int32_t n, int32_t* uv) {
     free(cnt);
     return 0;
 }
+
+/* Pairs at the given entry indices of the u-major stream (seed, k). */
+void or_umajor_pairs_at(uint64_t seed, int64_t k, int32_t n, const int64_t* idx, int64_t cnt, int32_t* uv) {
+    int64_t* off = (int64_t*)calloc((size_t)n + 2, sizeof(int64_t));
+    for (int64_t c = 0; c < k; c++) off[1 + range32((uint32_t)rand_at(seed, (uint64_t)c), (uint32_t)n)]++;
+    int64_t run = 0;
+    for (int32_t u = 1; u <= n; u++) {
+        const int64_t x = off[u];
+        off[u] = run;
+        run += x;
+    }
+    off[n + 1] = run;
+    const uint64_t q = mix64(seed ^ 0x6A09E667F3BCC909ULL);
+    for (int64_t j = 0; j < cnt; j++) {
+        const int64_t i = idx[j];
+        int32_t lo = 1, hi = n;
+        while (lo < hi) {
+            const int32_t m = (lo + hi) / 2;
+            if (off[m + 1] > i) hi = m; else lo = m + 1;
+        }
+        const uint64_t r = rand_at(q, (uint64_t)i);
+        const int32_t b = (int32_t)range32((uint32_t)(r >> 32), (uint32_t)(n - 1)) + 1;
+        uv[2 * j] = lo;
+        uv[2 * j + 1] = b >= lo ? b + 1 : b;
+    }
+    free(off);
+}

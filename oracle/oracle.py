@@ -385,3 +385,12 @@ def umajor_pairs(seed, k, n):
     uv = np.empty((k, 2), np.int32)
     _o.or_umajor_pairs(seed, k, n, _p32(uv))
     return uv
+_sig(_o, "or_umajor_pairs_at", None, [C.c_uint64, C.c_int64, C.c_int32, i64p, C.c_int64, i32p])
+
+
+def umajor_pairs_at(seed, k, n, idx):
+    """Pairs at entry indices idx of the u-major stream (seed, k)."""
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    uv = np.empty((idx.shape[0], 2), np.int32)
+    _o.or_umajor_pairs_at(seed, k, n, _p64(idx), idx.shape[0], _p32(uv))
+    return uv
