@@ -652,3 +652,134 @@ class Anneal:
         b, l, p = C.c_double(0), C.c_double(0), C.c_uint64(0)
         self._chk(_lib.bm_anneal_schedule(self.h, C.byref(b), C.byref(l), C.byref(p)))
         return b.value, l.value, p.value
+
+
+PROPOSAL_UNIFORM, PROPOSAL_TWOHOP, PROPOSAL_LEARNED = 0, 1, 2
+
+
+class _ClimbOpts(C.Structure):
+    _fields_ = [("proposal", C.c_int32), ("seed", C.c_uint64), ("time_limit_s", C.c_double),
+                ("max_rounds", C.c_int64), ("per_round", C.c_int64), ("pool", C.c_int64),
+                ("per_anchor", C.c_int32), ("mlp", _c), ("refresh_every", C.c_int32), ("verify", C.c_int32)]
+
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_sig("bm_mlp_num_params", C.c_int32, [])
+_sig("bm_mlp_create", C.c_int, [_c, C.c_uint64, C.POINTER(_c)])
+_sig("bm_mlp_destroy", None, [_c])
+_sig("bm_mlp_get_params", C.c_int, [_c, _fp])
+_sig("bm_mlp_set_params", C.c_int, [_c, _fp])
+_sig("bm_mlp_refresh_features", C.c_int, [_c])
+_sig("bm_mlp_vertex_features", C.c_int, [_c, _fp])
+_sig("bm_mlp_score_pairs", C.c_int, [_c, i32p, C.c_int64, _fp])
+_sig("bm_mlp_gradient", C.c_int, [_c, i32p, _fp, C.c_int64, _fp, _fp])
+_sig("bm_mlp_train", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int32, C.c_float, _fp])
+_sig("bm_batch_gen_learned", C.c_int, [_c, _c, C.c_uint64, C.c_int64, C.c_int64])
+_sig("bm_climb", C.c_int, [_c, C.POINTER(_ClimbOpts), i64p, _dp, C.c_int64, C.POINTER(_SearchResult)])
+
+
+def _pf(a):
+    return a.ctypes.data_as(_fp) if a is not None else None
+
+
+class Mlp:
+    """The learned swap proposal of a context (bm_mlp_* in include/brainmatch.h)."""
+
+    N_PARAMS = int(_lib.bm_mlp_num_params())
+
+    def __init__(self, ctx: "Context", seed=1):
+        self.ctx = ctx
+        self.h = _c()
+        ctx._chk(_lib.bm_mlp_create(ctx.h, seed, C.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.bm_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        self.ctx._chk(rc)
+
+    def params(self):
+        p = np.zeros(self.N_PARAMS, np.float32)
+        self._chk(_lib.bm_mlp_get_params(self.h, _pf(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (self.N_PARAMS,)
+        self._chk(_lib.bm_mlp_set_params(self.h, _pf(p)))
+
+    @staticmethod
+    def unpack(p):
+        """(W1 16x128, b1, W2 128x128, b2, w3, b3) views of a parameter vector."""
+        o = 0
+        out = []
+        for shape in ((16, 128), (128,), (128, 128), (128,), (128,), (1,)):
+            size = int(np.prod(shape))
+            out.append(p[o:o + size].reshape(shape))
+            o += size
+        return out
+
+    def refresh_features(self):
+        self._chk(_lib.bm_mlp_refresh_features(self.h))
+
+    def vertex_features(self):
+        out = np.zeros((self.ctx.n + 1, 8), np.float32)
+        self._chk(_lib.bm_mlp_vertex_features(self.h, _pf(out)))
+        return out
+
+    def score_pairs(self, uv):
+        uv = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1, 2)
+        out = np.zeros(uv.shape[0], np.float32)
+        self._chk(_lib.bm_mlp_score_pairs(self.h, _p32(uv), uv.shape[0], _pf(out)))
+        return out
+
+    def gradient(self, uv, target):
+        uv = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1, 2)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(self.N_PARAMS, np.float32)
+        mse = C.c_float(0)
+        self._chk(_lib.bm_mlp_gradient(self.h, _p32(uv), _pf(t), uv.shape[0], _pf(g), C.byref(mse)))
+        return g, mse.value
+
+    def train(self, seed, steps, batch=1 << 14, lr=1e-3):
+        tr = np.zeros(steps, np.float32)
+        self._chk(_lib.bm_mlp_train(self.h, seed, steps, batch, lr, _pf(tr)))
+        return tr
+
+
+def _batch_gen_learned(self, mlp, seed, pool, top):
+    self._chk(_lib.bm_batch_gen_learned(self.h, mlp.h, seed, pool, top))
+
+
+Batch.gen_learned = _batch_gen_learned
+
+
+def _ctx_climb(self, proposal, seed=1, time_limit_s=0.0, max_rounds=0, per_round=4096, pool=1 << 20,
+               per_anchor=64, mlp=None, refresh_every=1, verify=False, trace_cap=100000):
+    o = _ClimbOpts(proposal, seed, time_limit_s, max_rounds, per_round, pool, per_anchor,
+                   mlp.h if mlp is not None else None, refresh_every, 1 if verify else 0)
+    ts = np.zeros(trace_cap, np.int64)
+    tt = np.zeros(trace_cap, np.float64)
+    r = _SearchResult()
+    self._chk(_lib.bm_climb(self.h, C.byref(o), _p64(ts), tt.ctypes.data_as(_dp), trace_cap, C.byref(r)))
+    res = {k: getattr(r, k) for k, _ in _SearchResult._fields_}
+    n = min(trace_cap, res["iterations"])
+    return ts[:n].copy(), tt[:n].copy(), res
+
+
+Context.climb = _ctx_climb

@@ -1247,6 +1247,16 @@ extern "C" int bm_device_info(bm_ctx* c, char* buf, size_t len) {
 extern "C" const bm_graph_host* bm_ctx_male(const bm_ctx* c) { return c ? &c->male : nullptr; }
 
 BmDevGraph bm_ctx_devgraph(const bm_ctx* c) { return devgraph(c); }
+int bm_ctx_set_error(bm_ctx* c, int code, const char* msg) { return fail(c, code, "%s", msg); }
+int2* bm_batch_pairs_dev(bm_batch* b) { return (int2*)b->d_cand; }
+int64_t* bm_batch_deltas_dev(bm_batch* b) { return b->d_delta; }
+int64_t bm_batch_capacity(const bm_batch* b) { return b->cap; }
+void bm_batch_mark_pairs(bm_batch* b, int64_t k) {
+    b->k = k;
+    b->kind = 1;
+    b->generated = 1;
+    b->evaluated = 0;
+}
 int bm_ctx_device(const bm_ctx* c) { return c->device; }
 int bm_ctx_num_cus(const bm_ctx* c) { return c->num_cus; }
 hipStream_t bm_ctx_stream(const bm_ctx* c) { return c->stream; }

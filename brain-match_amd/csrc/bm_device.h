@@ -155,6 +155,37 @@ hipError_t bm_launch_twohop(const BmDevGraph& g, const int32_t* anchors, uint64_
                             int32_t K, int2* pairs, int32_t* out_list, int32_t* out_count, int32_t num_cus,
                             hipStream_t s);
 
+/* Learned proposal (bm_mlp.hip).  params/grad/adam_m/adam_v: fp32 in the
+ * layout W1 [16][128], b1 [128], W2 [128][128], b2 [128], w3 [128], b3;
+ * W1f/W2f/W2Tf: bf16 weights in MFMA fragment order (4 KB / 32 KB / 32 KB). */
+struct BmMlpDev {
+    float* params;
+    float* grad;
+    float* adam_m;
+    float* adam_v;
+    void* W1f;
+    void* W2f;
+    void* W2Tf;
+};
+int32_t bm_mlp_param_count(void);
+hipError_t bm_launch_vertex_features(const BmDevGraph& g, float* vf, int32_t num_cus, hipStream_t s);
+hipError_t bm_launch_mlp_pack(const BmMlpDev& d, hipStream_t s);
+/* Scores of k pairs (list, or uniform stream `seed` when list is NULL). */
+hipError_t bm_launch_mlp_forward(const BmMlpDev& d, const float* vf, const int2* list, uint64_t seed, int32_t n,
+                                 int64_t k, float* score, int32_t num_cus, hipStream_t s);
+/* Gradient of the mean squared error over k pairs, added into d.grad; the
+ * summed squared error is added into *loss.  act: bm_mlp_train_bytes(k). */
+size_t bm_mlp_train_bytes(int64_t B);
+hipError_t bm_launch_mlp_gradient(const BmMlpDev& d, const float* vf, const int2* list, uint64_t seed, int32_t n,
+                                  int64_t k, const float* target, void* act, float* loss, int32_t num_cus,
+                                  hipStream_t s);
+hipError_t bm_launch_mlp_adam(const BmMlpDev& d, float lr, int64_t step, hipStream_t s);
+hipError_t bm_launch_mlp_targets(const int64_t* delta, int64_t k, float* t, hipStream_t s);
+/* Top `top` of `pool` uniform-stream pairs by score, as a pair list. */
+size_t bm_mlp_rank_scratch(int64_t pool);
+hipError_t bm_launch_mlp_rank(const BmMlpDev& d, const float* vf, uint64_t seed, int32_t n, int64_t pool, int64_t top,
+                              void* scratch, size_t scratch_bytes, int2* out, int32_t num_cus, hipStream_t s);
+
 /* Context internals for other host translation units (bm_api.hip). */
 struct bm_ctx;
 BmDevGraph bm_ctx_devgraph(const bm_ctx* c);
@@ -162,6 +193,13 @@ int bm_ctx_device(const bm_ctx* c);
 int bm_ctx_num_cus(const bm_ctx* c);
 hipStream_t bm_ctx_stream(const bm_ctx* c);
 BmLaunch bm_ctx_launch(const bm_ctx* c, int kernel);
+int bm_ctx_set_error(bm_ctx* c, int code, const char* msg);
+struct bm_batch;
+int2* bm_batch_pairs_dev(bm_batch* b);
+int64_t* bm_batch_deltas_dev(bm_batch* b);
+int64_t bm_batch_capacity(const bm_batch* b);
+/* The batch now holds k device-generated pairs (not yet evaluated). */
+void bm_batch_mark_pairs(bm_batch* b, int64_t k);
 
 hipError_t bm_launch_apply_swap(int32_t* pi, int32_t u, int32_t v, hipStream_t s);
 hipError_t bm_launch_apply_perm3(int32_t* pi, int32_t a, int32_t b, int32_t c, int32_t p, hipStream_t s);

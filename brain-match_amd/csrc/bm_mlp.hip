@@ -1,0 +1,630 @@
+/*
+ * Learned swap proposal (gfx950): a 16 -> 128 -> 128 -> 1 perceptron in BF16
+ * with FP32 accumulation on the matrix cores (v_mfma_f32_16x16x16 bf16), with
+ * hand-written forward and backward kernels.  It scores candidate pairs from
+ * per-vertex features of the current matching; a search sends only the top
+ * candidates of a large pool to the exact delta kernels, so the score stays
+ * exact whatever the network predicts.  There is no counterpart in the
+ * reference (its proposals are uniform, score-weighted or 2-hop:
+ * src/graph-re-alignment-gpu-swap.cu:816-887, src/refine-twohop-random.c:676-693).
+ *
+ * Model (x = the 16 pair features, bf() = rounding to bfloat16, nearest even):
+ *   z1 = bf(x) @ bf(W1) + b1          W1: 16 x 128
+ *   h1 = bf(relu(z1))
+ *   z2 = h1 @ bf(W2) + b2             W2: 128 x 128
+ *   h2 = relu(z2)                     (fp32)
+ *   y  = h2 . w3 + b3                 (fp32)
+ * Loss: mean (y - t)^2 with t = sign(delta) log(1 + |delta|).  Backward:
+ *   dy = 2 (y - t) / B;  dw3 = sum h2 dy;  db3 = sum dy
+ *   g2 = bf(dy w3 * [z2 > 0]);  dW2 = h1^T g2;  db2 = sum g2 (unrounded)
+ *   g1 = bf((g2 @ bf(W2)^T) * [z1 > 0]);  dW1 = bf(x)^T g1;  db1 = sum g1 (unrounded)
+ * All matrix products run on MFMA tiles of 16 x 16 x 16; the weight-gradient
+ * products use activations stored transposed (feature-major) so that every
+ * operand fragment is one 8-byte load.
+ *
+ * Fragment layout of v_mfma_f32_16x16x16 bf16 (64 lanes, l = lane):
+ *   A (16 x 16): lane l holds A[l % 16][4 (l / 16) + i], i = 0..3
+ *   B (16 x 16): lane l holds B[4 (l / 16) + i][l % 16]
+ *   D (16 x 16): lane l holds D[4 (l / 16) + j][l % 16], j = 0..3
+ * Weights are kept in "fragment order": for a K x N matrix W, fragment (s, t)
+ * (rows 16 s.., columns 16 t..) stores lane l's four values contiguously.
+ *
+ * Per-vertex features (8 per vertex; a pair's 16 are u's then v's):
+ *   log1p d_out, log1p d_in, log1p (weight sum), L / (weight sum + 1),
+ *   unmatched-edge fraction, log1p d_out_F(pi), log1p d_in_F(pi), log1p L,
+ * with L = the score carried by the vertex's incident edges under pi and an
+ * edge "unmatched" when its contribution is 0.
+ */
+#include "bm_device.h"
+#include "bm_devutil.h"
+#include "bm_rng.h"
+
+#include <hipcub/hipcub.hpp>
+
+using namespace bmd;
+
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kF = 16;     /* features per pair */
+constexpr int kH = 128;    /* hidden width */
+constexpr int kT = kH / 16;   /* 16-wide tiles per hidden layer */
+
+__device__ __forceinline__ short bf16_of(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (short)(u >> 16);   /* inf / nan: truncate */
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+/* w_F(a, b) through a row hash table (descriptor (offset << 6) | bits). */
+__device__ __forceinline__ int32_t row_w(const uint64_t* __restrict__ desc, const int2* __restrict__ tab, int32_t row,
+                                         int32_t key) {
+    if (row == 0 || key == 0) return BM_ABSENT;
+    const uint64_t d = desc[row];
+    const uint32_t bits = (uint32_t)(d & 63u);
+    if (!bits) return BM_ABSENT;
+    const int2* T = tab + (d >> 6);
+    const uint32_t mask = (1u << bits) - 1u;
+    uint32_t s = slot_hash(key, bits);
+    for (;;) {
+        const int2 e = T[s];
+        if (e.x == 0) return BM_ABSENT;
+        if (e.x == key) return e.y;
+        s = (s + 1u) & mask;
+    }
+}
+
+/* One wave per vertex x: 8 features (two float4) into vf[2x], vf[2x+1]. */
+__global__ void __launch_bounds__(256) k_vertex_features(BmDevGraph g, float4* __restrict__ vf) {
+    const int lane = threadIdx.x & (BM_WAVE - 1);
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / BM_WAVE);
+    for (int64_t xx = (int64_t)blockIdx.x * (blockDim.x / BM_WAVE) + threadIdx.x / BM_WAVE; xx <= g.n; xx += nw) {
+        const int32_t x = (int32_t)xx;
+        if (x == 0) {
+            if (lane == 0) {
+                vf[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+                vf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            continue;
+        }
+        const int32_t fx = uni(g.pi[x]);
+        const int64_t ob = g.m_out_ptr[x], oe = g.m_out_ptr[x + 1];
+        const int64_t ib = g.m_in_ptr[x], ie = g.m_in_ptr[x + 1];
+        int64_t L = 0, wsum = 0, miss = 0;
+        for (int64_t e = ob + lane; e < oe; e += BM_WAVE) {
+            const int2 ed = g.m_out[e];
+            const int32_t fy = ed.x == x ? fx : g.pi[ed.x];
+            const int32_t c = contrib(ed.y, row_w(g.f_out_desc, g.f_out_tab, fx, fy));
+            L += c;
+            wsum += ed.y;
+            miss += c == 0;
+        }
+        for (int64_t e = ib + lane; e < ie; e += BM_WAVE) {
+            const int2 ed = g.m_in[e];
+            if (ed.x == x) continue;   /* a self-loop counted once, with the out-edges */
+            const int32_t c = contrib(ed.y, row_w(g.f_in_desc, g.f_in_tab, fx, g.pi[ed.x]));
+            L += c;
+            wsum += ed.y;
+            miss += c == 0;
+        }
+        L = wave_sum64(L);
+        wsum = wave_sum64(wsum);
+        miss = wave_sum64(miss);
+        if (lane == 0) {
+            const float dout = (float)(oe - ob), din = (float)(ie - ib);
+            const float fo = fx ? (float)(g.f_out_ptr[fx + 1] - g.f_out_ptr[fx]) : 0.f;
+            const float fi = fx ? (float)(g.f_in_ptr[fx + 1] - g.f_in_ptr[fx]) : 0.f;
+            const float ws = (float)(wsum > 0 ? wsum : 0);
+            const float Lf = (float)(L > 0 ? L : 0);
+            vf[2 * xx] = make_float4(log1pf(dout), log1pf(din), log1pf(ws), Lf / (ws + 1.f));
+            vf[2 * xx + 1] = make_float4((float)miss / (dout + din + 1.f), log1pf(fo), log1pf(fi), log1pf(Lf));
+        }
+    }
+}
+
+/* Weights in fragment order: frag[(s * (N / 16) + t) * 64 + l] holds
+ * W[16 s + 4 (l / 16) + i][16 t + l % 16], i = 0..3 (W is K x N, row-major);
+ * with transpose, the fragments of W^T (N x K). */
+__global__ void k_mlp_pack(const float* __restrict__ W, int K, int N, int transpose, v4s* __restrict__ frag) {
+    const int KK = transpose ? N : K, NN = transpose ? K : N;   /* shape of the packed matrix */
+    const int total = (KK / 16) * (NN / 16) * 64;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
+        const int l = q & 63, f = q >> 6;
+        const int s = f / (NN / 16), t = f % (NN / 16);
+        v4s v;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = 16 * s + 4 * (l / 16) + i, c = 16 * t + (l % 16);
+            v[i] = bf16_of(transpose ? W[c * N + r] : W[r * N + c]);
+        }
+        frag[q] = v;
+    }
+}
+
+struct MlpDev {
+    const v4s* W1f;     /* 1 x 8 fragments of W1 (16 x 128) */
+    const v4s* W2f;     /* 8 x 8 fragments of W2 (128 x 128) */
+    const v4s* W2Tf;    /* 8 x 8 fragments of W2^T */
+    const float* b1;
+    const float* b2;
+    const float* w3;
+    const float* b3;
+};
+
+/* Candidate pair p of a proposal source. */
+struct PairSrc {
+    const int2* list;   /* explicit pairs, or NULL for the uniform stream */
+    uint64_t seed;
+    int32_t n;
+};
+
+__device__ __forceinline__ int2 pair_at(const PairSrc& ps, int64_t p) {
+    if (ps.list) return ps.list[p];
+    int32_t u, v;
+    bm_uniform_pair(ps.seed, (uint64_t)p, ps.n, &u, &v);
+    return make_int2(u, v);
+}
+
+constexpr int kFwdThreads = 256;
+constexpr int kFwdWaves = kFwdThreads / BM_WAVE;
+constexpr int kHPad = kH + 8;   /* padded bf16 row of a staged 16 x 128 tile */
+
+struct FwdShared {
+    v4s W1[kT * 64];
+    v4s W2[kT * kT * 64];
+    float b1[kH], b2[kH], w3[kH];
+    short Ht[kFwdWaves][16][kHPad];
+};
+
+__device__ __forceinline__ void stage_weights(FwdShared& S, const MlpDev& M) {
+    for (int q = threadIdx.x; q < kT * 64; q += blockDim.x) S.W1[q] = M.W1f[q];
+    for (int q = threadIdx.x; q < kT * kT * 64; q += blockDim.x) S.W2[q] = M.W2f[q];
+    for (int q = threadIdx.x; q < kH; q += blockDim.x) {
+        S.b1[q] = M.b1[q];
+        S.b2[q] = M.b2[q];
+        S.w3[q] = M.w3[q];
+    }
+}
+
+/* Input fragment of a 16-pair tile: lane l holds features 4 (l / 16) .. +3
+ * of pair l % 16 (u's features 0..7, v's 8..15). */
+__device__ __forceinline__ v4s input_frag(const float4* __restrict__ vf, int2 uv, bool valid, int lane) {
+    const int q = lane / 16;
+    float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) f = vf[2 * (int64_t)(q < 2 ? uv.x : uv.y) + (q & 1)];
+    v4s a;
+    a[0] = bf16_of(f.x);
+    a[1] = bf16_of(f.y);
+    a[2] = bf16_of(f.z);
+    a[3] = bf16_of(f.w);
+    return a;
+}
+
+/* Layer 1 for one tile: z1 (D layout, kept for the backward mask) and h1 in
+ * the wave's LDS tile; returns nothing, z1 in z1[t]. */
+__device__ __forceinline__ void layer1(const FwdShared& S, short (*Ht)[kHPad], v4s xa, v4f* z1, int lane) {
+#pragma unroll
+    for (int t = 0; t < kT; t++) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma(xa, S.W1[t * 64 + lane], acc);
+        const int col = 16 * t + (lane % 16);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            acc[j] += S.b1[col];
+            Ht[4 * (lane / 16) + j][col] = bf16_of(acc[j] > 0.f ? acc[j] : 0.f);
+        }
+        z1[t] = acc;
+    }
+}
+
+/* A fragments (8 k-steps) of the wave's staged 16 x 128 bf16 tile. */
+__device__ __forceinline__ void load_a(const short (*Ht)[kHPad], v4s* a, int lane) {
+#pragma unroll
+    for (int s = 0; s < kT; s++) a[s] = *reinterpret_cast<const v4s*>(&Ht[lane % 16][16 * s + 4 * (lane / 16)]);
+}
+
+/* Forward: scores of k pairs (one wave per 16-pair tile). */
+__global__ void __launch_bounds__(kFwdThreads) k_mlp_forward(const float4* __restrict__ vf, PairSrc ps, int64_t k,
+                                                             MlpDev M, float* __restrict__ score) {
+    extern __shared__ uint4 smem[];
+    FwdShared& S = *reinterpret_cast<FwdShared*>(smem);
+    stage_weights(S, M);
+    __syncthreads();
+    const int lane = threadIdx.x & (BM_WAVE - 1), wave = threadIdx.x / BM_WAVE;
+    short(*Ht)[kHPad] = S.Ht[wave];
+    const float b3 = M.b3[0];
+    const int64_t tiles = (k + 15) / 16;
+    for (int64_t tile = (int64_t)blockIdx.x * kFwdWaves + wave; tile < tiles; tile += (int64_t)gridDim.x * kFwdWaves) {
+        const int64_t p = tile * 16 + (lane % 16);
+        const bool valid = p < k;
+        const int2 uv = valid ? pair_at(ps, p) : make_int2(0, 0);
+        const v4s xa = input_frag(vf, uv, valid, lane);
+        v4f z1[kT];
+        layer1(S, Ht, xa, z1, lane);
+        wave_sync();
+        v4s a[kT];
+        load_a(Ht, a, lane);
+        float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < kT; t++) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < kT; s++) acc = mfma(a[s], S.W2[(s * kT + t) * 64 + lane], acc);
+            const int col = 16 * t + (lane % 16);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float h = acc[j] + S.b2[col];
+                part[j] += (h > 0.f ? h : 0.f) * S.w3[col];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) part[j] += __shfl_xor(part[j], o, BM_WAVE);
+        if ((lane % 16) == 0) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int64_t q = tile * 16 + 4 * (lane / 16) + j;
+                if (q < k) score[q] = part[j] + b3;
+            }
+        }
+        wave_sync();
+    }
+}
+
+/* Training pass over a batch of B pairs (B a multiple of 16): forward, loss,
+ * and the backward through the activations.  Stores bf(x)^T (16 x B), h1^T,
+ * g1^T and g2^T (128 x B) for the weight-gradient products and accumulates
+ * the bias gradients, dw3 and the summed squared error with atomics. */
+struct TrainBuf {
+    short* XT;      /* 16 x B */
+    short* H1T;     /* 128 x B */
+    short* G1T;     /* 128 x B */
+    short* G2T;     /* 128 x B */
+    float* grad;    /* gradient in the parameter layout */
+    float* loss;    /* summed squared error */
+    int64_t B;
+};
+
+constexpr int kOffW1 = 0, kOffB1 = kF * kH, kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH, kOffW3 = kOffB2 + kH,
+              kOffB3 = kOffW3 + kH, kParams = kOffB3 + 1;
+
+struct TrainShared {
+    FwdShared f;
+    v4s W2T[kT * kT * 64];
+};
+
+__global__ void __launch_bounds__(kFwdThreads) k_mlp_train_act(const float4* __restrict__ vf, PairSrc ps, int64_t k,
+                                                               const float* __restrict__ target, MlpDev M, TrainBuf T) {
+    extern __shared__ uint4 smem[];
+    TrainShared& S = *reinterpret_cast<TrainShared*>(smem);
+    stage_weights(S.f, M);
+    for (int q = threadIdx.x; q < kT * kT * 64; q += blockDim.x) S.W2T[q] = M.W2Tf[q];
+    __syncthreads();
+    const int lane = threadIdx.x & (BM_WAVE - 1), wave = threadIdx.x / BM_WAVE;
+    short(*Ht)[kHPad] = S.f.Ht[wave];
+    const float b3 = M.b3[0];
+    const float inv = 2.f / (float)k;
+    const int64_t B = T.B;
+    const int64_t tiles = B / 16;
+    for (int64_t tile = (int64_t)blockIdx.x * kFwdWaves + wave; tile < tiles; tile += (int64_t)gridDim.x * kFwdWaves) {
+        const int64_t base = tile * 16;
+        const int64_t p = base + (lane % 16);
+        const bool valid = p < k;
+        const int2 uv = valid ? pair_at(ps, p) : make_int2(0, 0);
+        const v4s xa = input_frag(vf, uv, valid, lane);
+#pragma unroll
+        for (int i = 0; i < 4; i++) T.XT[(int64_t)(4 * (lane / 16) + i) * B + p] = xa[i];
+        v4f z1[kT];
+        layer1(S.f, Ht, xa, z1, lane);
+        wave_sync();
+        v4s a[kT];
+        load_a(Ht, a, lane);
+        /* h1^T: rows 4 (l/16) + j of column c are 4 consecutive pairs */
+#pragma unroll
+        for (int t = 0; t < kT; t++) {
+            const int col = 16 * t + (lane % 16);
+            v4s h;
+#pragma unroll
+            for (int j = 0; j < 4; j++) h[j] = Ht[4 * (lane / 16) + j][col];
+            *reinterpret_cast<v4s*>(&T.H1T[(int64_t)col * B + base + 4 * (lane / 16)]) = h;
+        }
+        v4f z2[kT];
+        float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < kT; t++) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < kT; s++) acc = mfma(a[s], S.f.W2[(s * kT + t) * 64 + lane], acc);
+            const int col = 16 * t + (lane % 16);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                acc[j] += S.f.b2[col];
+                part[j] += (acc[j] > 0.f ? acc[j] : 0.f) * S.f.w3[col];
+            }
+            z2[t] = acc;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) part[j] += __shfl_xor(part[j], o, BM_WAVE);
+        /* dy for this lane's four rows (every lane of a 16-lane group has the sums) */
+        float dy[4], se = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t q = base + 4 * (lane / 16) + j;
+            const bool ok = q < k;
+            const float y = part[j] + b3;
+            const float r = ok ? y - target[q] : 0.f;
+            dy[j] = r * inv;
+            se += r * r;
+        }
+        if ((lane % 16) == 0) atomicAdd(T.loss, se);
+        /* g2 = bf(dy w3 [z2 > 0]); dw3 += h2 dy; db2 += dy w3 [z2 > 0] */
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < kT; t++) {
+            const int col = 16 * t + (lane % 16);
+            float gsum = 0.f, wsum = 0.f;
+            v4s gv;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float h = z2[t][j] > 0.f ? z2[t][j] : 0.f;
+                const float g = z2[t][j] > 0.f ? dy[j] * S.f.w3[col] : 0.f;
+                gsum += g;
+                wsum += h * dy[j];
+                gv[j] = bf16_of(g);
+                Ht[4 * (lane / 16) + j][col] = gv[j];
+            }
+            *reinterpret_cast<v4s*>(&T.G2T[(int64_t)col * B + base + 4 * (lane / 16)]) = gv;
+            gsum += __shfl_xor(gsum, 16, BM_WAVE);
+            gsum += __shfl_xor(gsum, 32, BM_WAVE);
+            wsum += __shfl_xor(wsum, 16, BM_WAVE);
+            wsum += __shfl_xor(wsum, 32, BM_WAVE);
+            if (lane < 16) {
+                atomicAdd(&T.grad[kOffB2 + col], gsum);
+                atomicAdd(&T.grad[kOffW3 + col], wsum);
+            }
+        }
+        {
+            float d = dy[0] + dy[1] + dy[2] + dy[3];
+            d += __shfl_xor(d, 16, BM_WAVE);
+            d += __shfl_xor(d, 32, BM_WAVE);
+            if (lane == 0) atomicAdd(&T.grad[kOffB3], d);
+        }
+        wave_sync();
+        /* g1 = bf((g2 @ W2^T) [z1 > 0]) */
+        v4s ga[kT];
+        load_a(Ht, ga, lane);
+#pragma unroll
+        for (int t = 0; t < kT; t++) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < kT; s++) acc = mfma(ga[s], S.W2T[(s * kT + t) * 64 + lane], acc);
+            const int col = 16 * t + (lane % 16);
+            float gsum = 0.f;
+            v4s gv;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float g = z1[t][j] > 0.f ? acc[j] : 0.f;
+                gsum += g;
+                gv[j] = bf16_of(g);
+            }
+            *reinterpret_cast<v4s*>(&T.G1T[(int64_t)col * B + base + 4 * (lane / 16)]) = gv;
+            gsum += __shfl_xor(gsum, 16, BM_WAVE);
+            gsum += __shfl_xor(gsum, 32, BM_WAVE);
+            if (lane < 16) atomicAdd(&T.grad[kOffB1 + col], gsum);
+        }
+        wave_sync();
+    }
+}
+
+/* Weight gradients: dW1 = bf(x)^T g1 (16 x 128), dW2 = h1^T g2 (128 x 128).
+ * Work item = (output tile, slice of the batch); one wave each, summed into
+ * the gradient with atomics. */
+constexpr int kWgTiles = 1 * kT + kT * kT;
+
+__global__ void __launch_bounds__(256) k_mlp_wgrad(TrainBuf T, int32_t slices) {
+    const int lane = threadIdx.x & (BM_WAVE - 1);
+    const int64_t items = (int64_t)kWgTiles * slices;
+    const int64_t B = T.B;
+    const int64_t steps = B / 16;
+    for (int64_t it = (int64_t)blockIdx.x * (blockDim.x / BM_WAVE) + threadIdx.x / BM_WAVE; it < items;
+         it += (int64_t)gridDim.x * (blockDim.x / BM_WAVE)) {
+        const int tile = (int)(it % kWgTiles), slice = (int)(it / kWgTiles);
+        const bool first = tile < kT;                     /* dW1 tiles, else dW2 */
+        const int mt = first ? 0 : (tile - kT) / kT, nt = first ? tile : (tile - kT) % kT;
+        const short* AT = first ? T.XT : T.H1T;           /* rows = input features / neurons */
+        const short* BT = first ? T.G1T : T.G2T;          /* rows = output neurons */
+        const int64_t s0 = steps * slice / slices, s1 = steps * (slice + 1) / slices;
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        const int64_t arow = (int64_t)(16 * mt + lane % 16) * B, brow = (int64_t)(16 * nt + lane % 16) * B;
+        for (int64_t s = s0; s < s1; s++) {
+            const int64_t kk = 16 * s + 4 * (lane / 16);
+            const v4s a = *reinterpret_cast<const v4s*>(&AT[arow + kk]);
+            const v4s b = *reinterpret_cast<const v4s*>(&BT[brow + kk]);
+            acc = mfma(a, b, acc);
+        }
+        const int off = first ? kOffW1 : kOffW2;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = 16 * mt + 4 * (lane / 16) + j, c = 16 * nt + lane % 16;
+            atomicAdd(&T.grad[off + r * kH + c], acc[j]);
+        }
+    }
+}
+
+/* Adam on the fp32 master parameters; clears the gradient. */
+__global__ void k_mlp_adam(float* __restrict__ p, float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                           int32_t count, float lr, float b1c, float b2c) {
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = beta1 * m[i] + (1.f - beta1) * gi;
+        const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / b1c) / (sqrtf(vi / b2c) + eps);
+        g[i] = 0.f;
+    }
+}
+
+/* Regression target t = sign(d) log(1 + |d|) of exact int64 deltas. */
+__global__ void k_mlp_targets(const int64_t* __restrict__ d, int64_t k, float* __restrict__ t) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t)gridDim.x * blockDim.x) {
+        const double x = (double)d[i];
+        t[i] = (float)(x < 0 ? -log1p(-x) : log1p(x));
+    }
+}
+
+__global__ void k_iota(int32_t* __restrict__ p, int64_t k) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (int32_t)i;
+}
+
+/* Pairs of the top-ranked stream indices into a pair list. */
+__global__ void k_gather_stream_pairs(uint64_t seed, int32_t n, const int32_t* __restrict__ idx, int64_t top,
+                                      int2* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < top; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t u, v;
+        bm_uniform_pair(seed, (uint64_t)idx[i], n, &u, &v);
+        out[i] = make_int2(u, v);
+    }
+}
+
+int grid_of(int64_t work, int threads, int64_t cap = 65536) {
+    int64_t b = (work + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+MlpDev mlpdev(const BmMlpDev& d) {
+    MlpDev M;
+    M.W1f = reinterpret_cast<const v4s*>(d.W1f);
+    M.W2f = reinterpret_cast<const v4s*>(d.W2f);
+    M.W2Tf = reinterpret_cast<const v4s*>(d.W2Tf);
+    M.b1 = d.params + kOffB1;
+    M.b2 = d.params + kOffB2;
+    M.w3 = d.params + kOffW3;
+    M.b3 = d.params + kOffB3;
+    return M;
+}
+
+}  // namespace
+
+int32_t bm_mlp_param_count(void) { return kParams; }
+
+hipError_t bm_launch_vertex_features(const BmDevGraph& g, float* vf, int32_t num_cus, hipStream_t s) {
+    int64_t blocks = ((int64_t)g.n + 1 + 3) / 4;
+    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 8;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(k_vertex_features, dim3((unsigned)blocks), dim3(256), 0, s, g, reinterpret_cast<float4*>(vf));
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_mlp_pack(const BmMlpDev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(8), dim3(256), 0, s, d.params + kOffW1, kF, kH, 0,
+                       reinterpret_cast<v4s*>(d.W1f));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, s, d.params + kOffW2, kH, kH, 0,
+                       reinterpret_cast<v4s*>(d.W2f));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, s, d.params + kOffW2, kH, kH, 1,
+                       reinterpret_cast<v4s*>(d.W2Tf));
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_mlp_forward(const BmMlpDev& d, const float* vf, const int2* list, uint64_t seed, int32_t n,
+                                 int64_t k, float* score, int32_t num_cus, hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    PairSrc ps{list, seed, n};
+    const int64_t tiles = (k + 15) / 16;
+    int64_t blocks = (tiles + kFwdWaves - 1) / kFwdWaves;
+    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 4;
+    if (blocks > cap) blocks = cap;
+    hipError_t e = hipFuncSetAttribute((const void*)k_mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)sizeof(FwdShared));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)blocks), dim3(kFwdThreads), sizeof(FwdShared), s,
+                       reinterpret_cast<const float4*>(vf), ps, k, mlpdev(d), score);
+    return hipGetLastError();
+}
+
+size_t bm_mlp_train_bytes(int64_t B) { return (size_t)B * (kF + 3 * kH) * sizeof(short); }
+
+hipError_t bm_launch_mlp_gradient(const BmMlpDev& d, const float* vf, const int2* list, uint64_t seed, int32_t n,
+                                  int64_t k, const float* target, void* act, float* loss, int32_t num_cus,
+                                  hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    TrainBuf T;
+    T.B = (k + 15) / 16 * 16;
+    T.XT = reinterpret_cast<short*>(act);
+    T.H1T = T.XT + (size_t)kF * T.B;
+    T.G1T = T.H1T + (size_t)kH * T.B;
+    T.G2T = T.G1T + (size_t)kH * T.B;
+    T.grad = d.grad;
+    T.loss = loss;
+    PairSrc ps{list, seed, n};
+    const int64_t tiles = T.B / 16;
+    int64_t blocks = (tiles + kFwdWaves - 1) / kFwdWaves;
+    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 2;
+    if (blocks > cap) blocks = cap;
+    hipError_t e = hipFuncSetAttribute((const void*)k_mlp_train_act, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)sizeof(TrainShared));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_train_act, dim3((unsigned)blocks), dim3(kFwdThreads), sizeof(TrainShared), s,
+                       reinterpret_cast<const float4*>(vf), ps, k, target, mlpdev(d), T);
+    int32_t slices = (int32_t)(tiles / 64);
+    if (slices < 1) slices = 1;
+    if (slices > 64) slices = 64;
+    const int64_t items = (int64_t)kWgTiles * slices;
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3((unsigned)grid_of(items, 4)), dim3(256), 0, s, T, slices);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_mlp_adam(const BmMlpDev& d, float lr, int64_t step, hipStream_t s) {
+    const float b1c = 1.f - powf(0.9f, (float)step), b2c = 1.f - powf(0.999f, (float)step);
+    hipLaunchKernelGGL(k_mlp_adam, dim3(grid_of(kParams, 256)), dim3(256), 0, s, d.params, d.grad, d.adam_m,
+                       d.adam_v, kParams, lr, b1c, b2c);
+    return bm_launch_mlp_pack(d, s);
+}
+
+hipError_t bm_launch_mlp_targets(const int64_t* delta, int64_t k, float* t, hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_targets, dim3(grid_of(k, 256)), dim3(256), 0, s, delta, k, t);
+    return hipGetLastError();
+}
+
+size_t bm_mlp_rank_scratch(int64_t pool) {
+    size_t temp = 0;
+    (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp, (const float*)nullptr, (float*)nullptr,
+                                                       (const int32_t*)nullptr, (int32_t*)nullptr, (int)pool);
+    return temp + 256 + (size_t)pool * (2 * sizeof(float) + 2 * sizeof(int32_t));
+}
+
+hipError_t bm_launch_mlp_rank(const BmMlpDev& d, const float* vf, uint64_t seed, int32_t n, int64_t pool, int64_t top,
+                              void* scratch, size_t scratch_bytes, int2* out, int32_t num_cus, hipStream_t s) {
+    if (pool <= 0 || top <= 0) return hipSuccess;
+    char* p = static_cast<char*>(scratch);
+    float* sc = reinterpret_cast<float*>(p);
+    float* sc_sorted = sc + pool;
+    int32_t* idx = reinterpret_cast<int32_t*>(sc_sorted + pool);
+    int32_t* idx_sorted = idx + pool;
+    char* temp = reinterpret_cast<char*>(idx_sorted + pool);
+    temp += (256 - (reinterpret_cast<uintptr_t>(temp) & 255)) & 255;
+    size_t temp_bytes = scratch_bytes - (size_t)(temp - p);
+    hipError_t e = bm_launch_mlp_forward(d, vf, nullptr, seed, n, pool, sc, num_cus, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_iota, dim3(grid_of(pool, 256)), dim3(256), 0, s, idx, pool);
+    e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, sc, sc_sorted, idx, idx_sorted, (int)pool, 0,
+                                                     32, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gather_stream_pairs, dim3(grid_of(top, 256)), dim3(256), 0, s, seed, n, idx_sorted, top, out);
+    return hipGetLastError();
+}

@@ -382,3 +382,72 @@ int bm_hillclimb_twohop(bm_ctx* ctx, uint64_t seed, int64_t rounds, int32_t anch
     climb_src src = {1, anchors, per_anchor};
     return climb(ctx, seed, rounds, (int64_t)anchors * per_anchor, src, verify, score_trace, res);
 }
+
+/* Disjoint-swap climb with a chosen proposal source.  Round r: candidates
+ * from stream bm_rand_at(seed, r) -- per_round uniform pairs, or 2-hop pairs
+ * (per_round / per_anchor anchors), or the per_round best-scored of `pool`
+ * uniform pairs under the learned proposal -- are evaluated exactly and a
+ * disjoint set of positive swaps is applied.  trace_score / trace_time
+ * (trace_cap entries each, may be NULL) get the score and elapsed seconds
+ * after each round. */
+int bm_climb(bm_ctx* ctx, const bm_climb_opts* o, int64_t* trace_score, double* trace_time, int64_t trace_cap,
+             bm_search_result* res) {
+    if (!ctx || !o || !res) return BM_ERR_ARG;
+    memset(res, 0, sizeof *res);
+    if (o->per_round < 1 || (o->time_limit_s <= 0 && o->max_rounds <= 0))
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: per_round >= 1 and a time or round limit are required");
+    if (o->proposal == BM_PROPOSAL_LEARNED && (!o->mlp || o->pool < o->per_round || o->refresh_every < 1))
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: the learned proposal needs mlp, pool >= per_round, refresh_every >= 1");
+    if (o->proposal == BM_PROPOSAL_TWOHOP && (o->per_anchor < 1 || o->per_round < o->per_anchor))
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: the 2-hop proposal needs 1 <= per_anchor <= per_round");
+    if (o->proposal < BM_PROPOSAL_UNIFORM || o->proposal > BM_PROPOSAL_LEARNED)
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: unknown proposal");
+    bm_batch* b = NULL;
+    int rc = bm_batch_create(ctx, o->per_round, &b);
+    if (rc != BM_OK) return rc;
+    int64_t score = 0;
+    if ((rc = bm_score(ctx, &score)) != BM_OK) goto done;
+    res->initial_score = score;
+    const double t0 = now_s();
+    for (int64_t r = 0;; r++) {
+        if (o->max_rounds > 0 && r >= o->max_rounds) break;
+        if (o->time_limit_s > 0 && now_s() - t0 >= o->time_limit_s) break;
+        const uint64_t sd = bm_rand_at(o->seed, (uint64_t)r);
+        if (o->proposal == BM_PROPOSAL_UNIFORM) {
+            rc = bm_batch_gen_pairs(b, sd, o->per_round);
+        } else if (o->proposal == BM_PROPOSAL_TWOHOP) {
+            rc = bm_batch_gen_twohop(b, NULL, (int32_t)(o->per_round / o->per_anchor), o->per_anchor, sd);
+        } else {
+            if (r % o->refresh_every == 0 && (rc = bm_mlp_refresh_features(o->mlp)) != BM_OK) goto done;
+            rc = bm_batch_gen_learned(b, o->mlp, sd, o->pool, o->per_round);
+        }
+        if (rc != BM_OK) goto done;
+        if ((rc = bm_batch_eval_swaps(b)) != BM_OK) goto done;
+        int64_t nacc = 0, gain = 0;
+        if ((rc = bm_batch_accept_disjoint(b, 0, &nacc, &gain)) != BM_OK) goto done;
+        score += gain;
+        res->accepted += nacc;
+        res->evaluations += bm_batch_size(b);
+        res->iterations++;
+        if (o->verify) {
+            int64_t s = 0;
+            if ((rc = bm_score(ctx, &s)) != BM_OK) goto done;
+            if (s != score) {
+                res->verify_failures++;
+                score = s;
+            }
+        }
+        if (r < trace_cap) {
+            if (trace_score) trace_score[r] = score;
+            if (trace_time) trace_time[r] = now_s() - t0;
+        }
+    }
+    res->seconds = now_s() - t0;
+    res->best_score = score;
+    if ((rc = bm_score(ctx, &res->final_score)) != BM_OK) goto done;
+    if (res->final_score != score) res->verify_failures++;
+    rc = BM_OK;
+done:
+    bm_batch_destroy(b);
+    return rc;
+}

@@ -406,6 +406,67 @@ int bm_hillclimb_twohop(bm_ctx* ctx, uint64_t seed, int64_t rounds, int32_t anch
                         int32_t verify, int64_t* score_trace, bm_search_result* res);
 
 /* ------------------------------------------------------------------ */
+/* Learned swap proposal                                               */
+/* ------------------------------------------------------------------ */
+
+/* A 16 -> 128 -> 128 -> 1 perceptron (ReLU) in BF16 with FP32 accumulation on
+ * the matrix cores, trained on the GPU with its own forward and backward
+ * kernels on (candidate, exact delta) samples that the context's delta
+ * kernels produce, regressing t = sign(delta) log(1 + |delta|).  A pair's 16
+ * inputs are 8 per-vertex features of u then of v under the current matching:
+ * log1p out-degree, log1p in-degree, log1p weight sum, carried score / (weight
+ * sum + 1), fraction of incident edges contributing 0, log1p out- and
+ * in-degree of the female image, log1p carried score.  It only ranks
+ * candidates: every proposed swap is still judged by the exact delta kernels.
+ * The reference has no learned component; its proposals are uniform,
+ * score-weighted (src/graph-re-alignment-gpu-swap.cu:816-887) or 2-hop
+ * (src/refine-twohop-random.c:676-693).  Parameter layout (float):
+ * W1 [16][128], b1 [128], W2 [128][128], b2 [128], w3 [128], b3. */
+typedef struct bm_mlp bm_mlp;
+#define BM_MLP_FEATURES 16
+#define BM_MLP_HIDDEN 128
+int32_t bm_mlp_num_params(void);                     /* 18,817 */
+int bm_mlp_create(bm_ctx* ctx, uint64_t seed, bm_mlp** out);   /* He-normal init, zero biases */
+void bm_mlp_destroy(bm_mlp* m);
+int bm_mlp_get_params(bm_mlp* m, float* params);
+int bm_mlp_set_params(bm_mlp* m, const float* params);   /* also resets the optimiser */
+/* Recompute the per-vertex features for the context's current matching. */
+int bm_mlp_refresh_features(bm_mlp* m);
+int bm_mlp_vertex_features(bm_mlp* m, float* out);       /* (n+1) x 8 floats */
+int bm_mlp_score_pairs(bm_mlp* m, const int32_t* uv, int64_t k, float* out);
+/* Gradient of mean (y - target)^2 over the k pairs (no update); mse may be NULL. */
+int bm_mlp_gradient(bm_mlp* m, const int32_t* uv, const float* target, int64_t k, float* grad, float* mse);
+/* `steps` Adam steps (lr, betas 0.9 / 0.999) on batches of uniform pairs of
+ * stream bm_rand_at(seed, step) labelled by the context's exact deltas;
+ * mse_trace (steps floats, may be NULL) gets each batch's loss. */
+int bm_mlp_train(bm_mlp* m, uint64_t seed, int64_t steps, int32_t batch, float lr, float* mse_trace);
+/* The `top` best-scored of `pool` uniform pairs of stream `seed` into the
+ * batch as a pair list (descending score). */
+int bm_batch_gen_learned(bm_batch* b, bm_mlp* m, uint64_t seed, int64_t pool, int64_t top);
+
+#define BM_PROPOSAL_UNIFORM 0
+#define BM_PROPOSAL_TWOHOP 1
+#define BM_PROPOSAL_LEARNED 2
+
+typedef struct bm_climb_opts {
+    int32_t proposal;         /* BM_PROPOSAL_* */
+    uint64_t seed;
+    double time_limit_s;      /* <= 0: none (then max_rounds > 0) */
+    int64_t max_rounds;       /* <= 0: none */
+    int64_t per_round;        /* pairs evaluated exactly per round */
+    int64_t pool;             /* learned: uniform pairs ranked per round */
+    int32_t per_anchor;       /* 2-hop: targets per anchor */
+    bm_mlp* mlp;              /* learned */
+    int32_t refresh_every;    /* learned: rounds between feature refreshes */
+    int32_t verify;           /* device recount after every round */
+} bm_climb_opts;
+
+/* Disjoint-swap climb with one proposal source, for comparing proposals at
+ * equal wall clock (see bm_batch_accept_disjoint for the acceptance). */
+int bm_climb(bm_ctx* ctx, const bm_climb_opts* opts, int64_t* trace_score, double* trace_time, int64_t trace_cap,
+             bm_search_result* res);
+
+/* ------------------------------------------------------------------ */
 /* Annealing chains (many independent chains, batched over instances)  */
 /* ------------------------------------------------------------------ */
 
