@@ -120,6 +120,10 @@ def main():
                     help="instead of the swap-delta benchmark: train the learned proposal, then compare the exact "
                          "scores reached in SECONDS by uniform, 2-hop and learned proposals (one JSON line)")
     ap.add_argument("--train-samples", type=int, default=50_000_000)
+    ap.add_argument("--anneal", type=int, default=0, metavar="LAUNCHES",
+                    help="instead of the swap-delta benchmark: the batched-annealing workload (64 instances, "
+                         "n in 1k..20k, ~200 n edges), LAUNCHES replays of 1000 steps per chain (one JSON line)")
+    ap.add_argument("--chains-per-instance", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -146,6 +150,8 @@ def main():
 
     if args.proposals > 0:
         return bench_proposals(args, bm)
+    if args.anneal > 0:
+        return bench_anneal(args, bm)
 
     t = time.time()
     male, female, truth, pi_shuffled = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3,
@@ -363,6 +369,85 @@ def bench_proposals(args, bm):
     }
     mlp.close()
     ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+def bench_anneal(args, bm):
+    """Config C3: 64 independent synthetic problems of uneven size (n uniform in
+    1k..20k, ~200 n edges, lognormal weights, 30% of the true matching
+    shuffled), chains_per_instance annealing chains each (T_s = 50 * 0.9999^s,
+    swaps and 3-vertex moves), all advanced by one kernel launch replayed
+    from a HIP graph; the launch shape comes from the autotuner.  Reports
+    chain steps/s (one exact delta per step), the algorithmic GB/s of the
+    steps, the device recount check and a CPU replay of one chain."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    rng = np.random.default_rng(64)
+    sizes = [int(x) for x in rng.integers(1000, 20001, 64)]
+    t = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        pairs = list(ex.map(lambda a: bm.synth_pair(a[1], 200.0, bm.WEIGHTS_LOGNORMAL, 0.3, seed=1000 + a[0]),
+                            enumerate(sizes)))
+    ctxs = []
+    for (male, female, truth, pi0), n in zip(pairs, sizes):
+        c = bm.Context(n, male, female)
+        c.set_mapping(pi0)
+        ctxs.append(c)
+    edges = sum(p[0].m for p in pairs)
+    log(f"64 instances, n {min(sizes)}..{max(sizes)}, {edges} male edges in all ({time.perf_counter() - t:.1f}s)")
+    steps = 1000
+    ann = bm.Anneal(ctxs, chains_per_instance=args.chains_per_instance, seed=5, t0=50.0, decay=0.9999, p_rot3=0.1,
+                    traj_cap=args.anneal)
+    best, trials = ann.autotune(steps=200)
+    log(f"autotuner: {len(trials)} shapes, best {best} ({trials[:, 3].min():.2f} ms for 200 steps; "
+        f"slowest {trials[:, 3].max():.2f} ms)")
+    init = ann.state()["score"].copy()
+    ann.run(steps, 1)                      # capture + first launch (warm-up, part of the chains' history)
+    n0, ms0 = ann.timing()
+    ann.run(steps, args.anneal - 1) if args.anneal > 1 else None
+    n1, ms1 = ann.timing()
+    st = ann.state()
+    timed_launches = n1 - n0
+    timed_ms = ms1 - ms0
+    chains = ann.chains
+    rate = chains * steps * timed_launches / (timed_ms / 1000) if timed_launches else 0.0
+    algo, total_steps = ann.algo_bytes()
+    bad = ann.verify()
+    # CPU replay of chain 0 of the smallest instance, all steps
+    i_small = int(np.argmin(sizes))
+    c = i_small * args.chains_per_instance
+    male, female, truth, pi0 = pairs[i_small]
+    o = orc.Oracle(sizes[i_small], male, female)
+    beta0, lam, p3 = ann.schedule()
+    pi, _, rs = orc.anneal_chain(o, pi0, o.score(pi0), orc.chain_seed(5, c), 0, steps * args.anneal, beta0, lam, p3)
+    replay_ok = bool(rs[0] == st["score"][c] and rs[1] == st["best"][c] and np.array_equal(pi, ann.mapping(c)))
+    gain = (st["best"] - init).reshape(64, -1).max(axis=1)
+    log(f"{chains} chains x {steps * args.anneal} steps: {rate:.4g} steps/s over {timed_launches} timed launches; "
+        f"recount mismatches {bad}; CPU replay of chain {c} (n={sizes[i_small]}) identical: {replay_ok}")
+    result = {
+        "metric": "annealing chain steps/s (exact deltas), 64 batched instances",
+        "value": rate,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "int64 deltas, fp64 acceptance",
+        "data": "synthetic",
+        "config": {"workload": "64 instances, n uniform 1k..20k, ~200n edges, T_s = 50 * 0.9999^s",
+                   "sizes": sizes, "male_edges": edges, "chains_per_instance": args.chains_per_instance,
+                   "chains": chains, "steps_per_launch": steps, "launches": args.anneal, "p_rot3": 0.1},
+        "autotuner": {"best": best, "trials": trials.tolist()},
+        "ms_per_launch": timed_ms / max(1, timed_launches),
+        "algorithmic_GBps": algo / max(1, total_steps) * rate / 1e9,
+        "recount_mismatches": int(bad),
+        "cpu_replay_identical": replay_ok,
+        "best_gain_per_instance": {"min": int(gain.min()), "median": int(np.median(gain)), "max": int(gain.max())},
+    }
+    ann.close()
+    for c in ctxs:
+        c.close()
     print(json.dumps(result), flush=True)
 
 

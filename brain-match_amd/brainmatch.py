@@ -565,6 +565,11 @@ _sig("bm_anneal_timing", C.c_int, [_c, i64p, C.POINTER(C.c_double)])
 _sig("bm_anneal_algo_bytes", C.c_int, [_c, i64p, i64p])
 _sig("bm_anneal_last_error", C.c_char_p, [_c])
 _sig("bm_anneal_schedule", C.c_int, [_c, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)])
+_sig("bm_anneal_reset", C.c_int, [_c])
+_sig("bm_anneal_get_tuning", C.c_int, [_c, C.POINTER(_Tuning)])
+_sig("bm_anneal_set_tuning", C.c_int, [_c, C.POINTER(_Tuning)])
+_sig("bm_anneal_autotune", C.c_int, [_c, C.c_int64, C.POINTER(_Tuning), C.POINTER(C.c_double), C.c_int32,
+                                     C.POINTER(C.c_int32)])
 
 
 class Anneal:
@@ -647,6 +652,26 @@ class Anneal:
         b, s = C.c_int64(0), C.c_int64(0)
         self._chk(_lib.bm_anneal_algo_bytes(self.h, C.byref(b), C.byref(s)))
         return b.value, s.value
+
+    def reset(self):
+        self._chk(_lib.bm_anneal_reset(self.h))
+
+    def tuning(self):
+        t = _Tuning()
+        self._chk(_lib.bm_anneal_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _Tuning._fields_}
+
+    def set_tuning(self, waves_per_block, lds_slots, blocks_per_cu):
+        t = _Tuning(waves_per_block, lds_slots, blocks_per_cu, 64, 4)
+        self._chk(_lib.bm_anneal_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, steps=100):
+        best = _Tuning()
+        trials = np.zeros((64, 4), np.float64)
+        nt = C.c_int32(0)
+        self._chk(_lib.bm_anneal_autotune(self.h, steps, C.byref(best), trials.ctypes.data_as(C.POINTER(C.c_double)),
+                                          64, C.byref(nt)))
+        return {k: getattr(best, k) for k, _ in _Tuning._fields_}, trials[:nt.value].copy()
 
     def schedule(self):
         b, l, p = C.c_double(0), C.c_double(0), C.c_uint64(0)

@@ -37,6 +37,9 @@ struct bm_anneal {
     std::vector<bm_ctx*> ctxs;
     std::vector<BmDevGraph> graphs;   /* host copies of the instance graphs */
     std::vector<int64_t> pi_off;      /* per chain */
+    std::vector<BmChain> h_init;      /* chain states at creation (for bm_anneal_reset) */
+    int32_t* d_init;                  /* each instance's initial matching, concatenated */
+    std::vector<int64_t> init_off;    /* per instance offset into d_init */
     BmDevGraph* d_insts;
     BmChain* d_chains;
     int32_t* d_pool;
@@ -89,6 +92,7 @@ static void free_anneal(bm_anneal* a) {
     (void)hipFree(a->d_pool);
     (void)hipFree(a->d_traj);
     (void)hipFree(a->d_scores);
+    (void)hipFree(a->d_init);
     if (a->stream) (void)hipStreamDestroy(a->stream);
     delete a;
 }
@@ -180,6 +184,16 @@ extern "C" int bm_anneal_create(bm_anneal** out, bm_ctx* const* ctxs, int32_t n_
             pool += 2 * len;
         }
     }
+    int64_t init_len = 0;
+    for (int32_t i = 0; i < n_inst; i++) {
+        a->init_off.push_back(init_len);
+        init_len += (int64_t)a->graphs[i].n + 1;
+    }
+    ACHK(hipMalloc((void**)&a->d_init, sizeof(int32_t) * (size_t)init_len));
+    for (int32_t i = 0; i < n_inst; i++)
+        ACHK(hipMemcpyAsync(a->d_init + a->init_off[i], a->graphs[i].pi, sizeof(int32_t) * ((size_t)a->graphs[i].n + 1),
+                            hipMemcpyDeviceToDevice, a->stream));
+    a->h_init = h;
     ACHK(hipMalloc((void**)&a->d_insts, sizeof(BmDevGraph) * (size_t)n_inst));
     ACHK(hipMalloc((void**)&a->d_chains, sizeof(BmChain) * (size_t)a->n_chains));
     ACHK(hipMalloc((void**)&a->d_pool, sizeof(int32_t) * (size_t)pool));
@@ -368,5 +382,131 @@ extern "C" int bm_anneal_schedule(const bm_anneal* a, double* beta0, double* lam
     if (beta0) *beta0 = a->sc.beta0;
     if (lam) *lam = a->sc.lam;
     if (p_rot3) *p_rot3 = a->sc.p_rot3;
+    return BM_OK;
+}
+
+/* Chains back to their creation state (matchings, scores, counters). */
+static int reset_chains(bm_anneal* a) {
+    AHIP(a, hipSetDevice(a->device));
+    AHIP(a, hipMemcpyAsync(a->d_chains, a->h_init.data(), sizeof(BmChain) * (size_t)a->n_chains, hipMemcpyHostToDevice,
+                           a->stream));
+    for (int64_t c = 0; c < a->n_chains; c++) {
+        const int32_t i = a->h_init[c].inst;
+        const size_t len = (size_t)a->graphs[i].n + 1;
+        AHIP(a, hipMemcpyAsync(a->d_pool + a->pi_off[c], a->d_init + a->init_off[i], len * sizeof(int32_t),
+                               hipMemcpyDeviceToDevice, a->stream));
+        AHIP(a, hipMemcpyAsync(a->d_pool + a->pi_off[c] + len, a->d_init + a->init_off[i], len * sizeof(int32_t),
+                               hipMemcpyDeviceToDevice, a->stream));
+    }
+    AHIP(a, hipStreamSynchronize(a->stream));
+    a->launches = 0;
+    a->total_ms = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_reset(bm_anneal* a) {
+    if (!a) return afail(nullptr, BM_ERR_ARG, "bm_anneal_reset: NULL object");
+    return reset_chains(a);
+}
+
+extern "C" int bm_anneal_get_tuning(const bm_anneal* a, bm_tuning* t) {
+    if (!a || !t) return BM_ERR_ARG;
+    t->waves_per_block = a->L.waves_per_block;
+    t->lds_slots = a->L.lds_slots;
+    t->blocks_per_cu = a->L.blocks_per_cu;
+    t->group_lanes = 64;
+    t->unroll = 4;
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_set_tuning(bm_anneal* a, const bm_tuning* t) {
+    if (!a || !t) return afail(a, BM_ERR_ARG, "bm_anneal_set_tuning: null argument");
+    const int w = t->waves_per_block;
+    const size_t sb = a->compact ? 4 : 8;
+    if ((w != 1 && w != 2 && w != 4 && w != 8) || t->lds_slots < 4 || (t->lds_slots & 3) ||
+        (size_t)w * (size_t)t->lds_slots * sb > 160 * 1024 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64)
+        return afail(a, BM_ERR_ARG, "bm_anneal_set_tuning: waves 1/2/4/8, lds_slots a multiple of 4 within 160 KiB, "
+                                    "blocks_per_cu 1..64");
+    BmLaunch L = a->L;
+    L.waves_per_block = w;
+    L.lds_slots = t->lds_slots;
+    L.blocks_per_cu = t->blocks_per_cu;
+    dim3 grid, block;
+    size_t lds;
+    AHIP(a, hipSetDevice(a->device));
+    AHIP(a, bm_anneal_grid(a->compact, a->sc.p_rot3 != 0, L, a->n_chains, &grid, &block, &lds));
+    a->L = L;
+    a->grid = grid;
+    a->block = block;
+    a->lds = lds;
+    if (a->exec) (void)hipGraphExecDestroy(a->exec);
+    if (a->graph) (void)hipGraphDestroy(a->graph);
+    a->exec = nullptr;
+    a->graph = nullptr;
+    return BM_OK;
+}
+
+/* Times one launch of `steps` steps for every launch shape in a small grid
+ * (waves per block x table slots per wave x blocks per CU), keeps the fastest,
+ * and resets the chains.  trials (may be NULL) gets up to max_trials records
+ * {waves, slots, blocks_per_cu, ms}. */
+extern "C" int bm_anneal_autotune(bm_anneal* a, int64_t steps, bm_tuning* best, double* trials, int32_t max_trials,
+                                  int32_t* n_trials) {
+    if (!a || steps < 1) return afail(a, BM_ERR_ARG, "bm_anneal_autotune: invalid argument");
+    const int waves[4] = {1, 2, 4, 8};
+    const int slots_c[3] = {640, 1280, 2560}, slots_w[3] = {320, 640, 1280};
+    const int bpc[2] = {4, 16};
+    bm_tuning bt;
+    memset(&bt, 0, sizeof bt);
+    double best_ms = 1e300;
+    int32_t nt = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    AHIP(a, hipSetDevice(a->device));
+    AHIP(a, hipEventCreate(&e0));
+    AHIP(a, hipEventCreate(&e1));
+    int rc = BM_OK;
+    for (int wi = 0; wi < 4 && rc == BM_OK; wi++)
+        for (int si = 0; si < 3 && rc == BM_OK; si++)
+            for (int bi = 0; bi < 2 && rc == BM_OK; bi++) {
+                bm_tuning t;
+                t.waves_per_block = waves[wi];
+                t.lds_slots = a->compact ? slots_c[si] : slots_w[si];
+                t.blocks_per_cu = bpc[bi];
+                t.group_lanes = 64;
+                t.unroll = 4;
+                if ((size_t)t.waves_per_block * t.lds_slots * (a->compact ? 4 : 8) > 160 * 1024) continue;
+                if ((rc = bm_anneal_set_tuning(a, &t)) != BM_OK) break;
+                if ((rc = reset_chains(a)) != BM_OK) break;
+                hipError_t e = hipEventRecord(e0, a->stream);
+                if (e == hipSuccess)
+                    e = bm_launch_anneal(a->d_insts, a->compact, a->d_chains, a->n_chains, a->d_pool, steps, a->sc,
+                                         nullptr, a->L, a->grid, a->block, a->lds, a->stream);
+                if (e == hipSuccess) e = hipEventRecord(e1, a->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) {
+                    rc = afail(a, BM_ERR_HIP, "bm_anneal_autotune: trial launch failed: %s", hipGetErrorString(e));
+                    break;
+                }
+                if (trials && nt < max_trials) {
+                    trials[4 * nt] = t.waves_per_block;
+                    trials[4 * nt + 1] = t.lds_slots;
+                    trials[4 * nt + 2] = t.blocks_per_cu;
+                    trials[4 * nt + 3] = ms;
+                }
+                nt++;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    bt = t;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != BM_OK) return rc;
+    if (n_trials) *n_trials = nt;
+    if ((rc = bm_anneal_set_tuning(a, &bt)) != BM_OK) return rc;
+    if ((rc = reset_chains(a)) != BM_OK) return rc;
+    if (best) *best = bt;
     return BM_OK;
 }

@@ -515,6 +515,18 @@ int bm_anneal_timing(bm_anneal* a, int64_t* launches, double* total_ms);
 /* Algorithmic bytes of the chain steps run so far, from the byte model of
  * the swap and triple kernels (replays every chain's proposals on the host). */
 int bm_anneal_algo_bytes(bm_anneal* a, int64_t* bytes, int64_t* steps);
+/* Chains back to their state at creation. */
+int bm_anneal_reset(bm_anneal* a);
+/* Launch shape of the chain kernel (waves_per_block, lds_slots, blocks_per_cu
+ * as in bm_set_tuning; the default is the first context's swap-kernel shape). */
+int bm_anneal_get_tuning(const bm_anneal* a, bm_tuning* t);
+int bm_anneal_set_tuning(bm_anneal* a, const bm_tuning* t);
+/* Autotuner: times one launch of `steps` steps for each shape of a grid of
+ * waves per block {1,2,4,8} x table slots per wave x blocks per CU {4,16},
+ * keeps the fastest and resets the chains.  trials (4 doubles per trial:
+ * waves, slots, blocks_per_cu, ms; up to max_trials) may be NULL. */
+int bm_anneal_autotune(bm_anneal* a, int64_t steps, bm_tuning* best, double* trials, int32_t max_trials,
+                       int32_t* n_trials);
 /* Last error of an annealing object (or of the last failed create). */
 const char* bm_anneal_last_error(const bm_anneal* a);
 /* The schedule as the kernels use it, for exact replays: beta_s =

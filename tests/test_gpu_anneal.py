@@ -153,3 +153,24 @@ def test_bad_parameters_raise(bm):
                 ann.run(0, 1)
             with pytest.raises(bm.BMError):
                 ann.mapping(5)
+
+
+def test_autotune_then_replay(bm, orc):
+    """The autotuner times several launch shapes, keeps the fastest and
+    resets the chains, so a run after it replays exactly from the start."""
+    ctx, o, pi0 = _mk(bm, orc, 1200, 15.0, 13)
+    try:
+        with bm.Anneal([ctx], chains_per_instance=8, seed=17, t0=10.0, decay=0.999, p_rot3=0.2, traj_cap=2) as ann:
+            best, trials = ann.autotune(steps=50)
+            assert len(trials) >= 12 and (trials[:, 3] > 0).all()
+            assert ann.tuning()["waves_per_block"] == best["waves_per_block"]
+            st = ann.state()
+            assert (st["steps"] == 0).all() and (st["accepted"] == 0).all()
+            ann.run(150, 2)
+            assert ann.verify() == 0
+            replay_all(orc, [o], ann, [pi0], [150, 150])
+            ann.reset()
+            assert (ann.state()["steps"] == 0).all()
+            assert np.array_equal(ann.mapping(0), pi0)
+    finally:
+        ctx.close()
