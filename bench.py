@@ -124,6 +124,10 @@ def main():
                     help="instead of the swap-delta benchmark: the batched-annealing workload (64 instances, "
                          "n in 1k..20k, ~200 n edges), LAUNCHES replays of 1000 steps per chain (one JSON line)")
     ap.add_argument("--chains-per-instance", type=int, default=48)
+    ap.add_argument("--scale", type=float, default=0.0, metavar="SECONDS",
+                    help="instead of the swap-delta benchmark: the n=100,000 ~20M-edge pair with 40%% of the matching "
+                         "shuffled: delta throughput of both strategies, then learned vs uniform climbs and "
+                         "swap + 3-vertex annealing for SECONDS each (one JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -152,6 +156,8 @@ def main():
         return bench_proposals(args, bm)
     if args.anneal > 0:
         return bench_anneal(args, bm)
+    if args.scale > 0:
+        return bench_scale(args, bm)
 
     t = time.time()
     male, female, truth, pi_shuffled = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3,
@@ -448,6 +454,86 @@ def bench_anneal(args, bm):
     ann.close()
     for c in ctxs:
         c.close()
+    print(json.dumps(result), flush=True)
+
+
+def bench_scale(args, bm):
+    """Config C5: n = 100,000, ~20M directed edges, lognormal weights, 40% of
+    the true matching shuffled.  Exact swap-delta throughput of both
+    strategies on 2^26 u-major pairs (the gain table is 40 GB here, built in
+    column chunks, with per-pair lookups), then, for SECONDS each: a uniform
+    and a learned-proposal climb (4096 exact candidates per round), and
+    annealing chains with swaps and 3-vertex moves (uniform proposals)."""
+    n = 100_000
+    t = time.perf_counter()
+    male, female, truth, pi0 = bm.synth_pair(n, 200.0, bm.WEIGHTS_LOGNORMAL, 0.4, seed=5)
+    ctx = bm.Context(n, male, female)
+    ctx.set_mapping(pi0)
+    score0 = ctx.score()
+    log(f"scale pair: n={n}, {male.m} male edges, initial score {score0} ({time.perf_counter() - t:.1f}s)")
+    K = 1 << 26
+    b = ctx.batch(K)
+    ctx.timing_enable(True)
+    out = {"initial_score": score0, "male_edges": male.m}
+    b.eval_swaps_umajor(1, 1 << 20)
+    wall, kt = timed(ctx, (bm.KERNEL_SWAP,), lambda i: b.eval_swaps_umajor(77, K), 1)
+    algo, _ = ctx.umajor_algo_bytes(77, K)
+    ms = per_launch_ms(kt, bm.KERNEL_SWAP)
+    out["scan"] = {"deltas_per_s": K / wall, "launch_ms": ms, "algorithmic_GBps": algo / ms / 1e6}
+    head = b.deltas(0, 1 << 16)
+    ctx.gain_enable(True)
+    ctx.gain_rebuild()
+    b.eval_swaps_umajor(77, K)
+    wall, kt = timed(ctx, (bm.KERNEL_GAIN_BUILD, bm.KERNEL_GAIN_SWAP, bm.KERNEL_AUX),
+                     lambda i: (ctx.gain_rebuild(), b.eval_swaps_umajor(77, K)), 2)
+    same = bool((b.deltas(0, 1 << 16) == head).all())
+    out["gain"] = {"deltas_per_s": K * 2 / wall, "ms_per_step": 1000 * wall / 2,
+                   "build_ms": per_launch_ms(kt, bm.KERNEL_GAIN_BUILD),
+                   "lookup_ms": per_launch_ms(kt, bm.KERNEL_GAIN_SWAP), "matches_scan": same,
+                   "table_GB": (n + 1) ** 2 * 4 / 1e9}
+    log(f"scale deltas: scan {out['scan']['deltas_per_s']:.4g}/s, gain {out['gain']['deltas_per_s']:.4g}/s "
+        f"(build {out['gain']['build_ms']:.1f} ms, lookups {out['gain']['lookup_ms']:.1f} ms); identical: {same}")
+    ctx.gain_enable(False)
+    b.close()
+    ctx.timing_enable(False)
+    mlp = bm.Mlp(ctx, seed=2)
+    mlp.refresh_features()
+    t = time.perf_counter()
+    losses = mlp.train(seed=12, steps=300, batch=1 << 16, lr=1e-3)
+    out["train"] = {"samples": 300 << 16, "seconds": time.perf_counter() - t,
+                    "mse_first": float(losses[:10].mean()), "mse_last": float(losses[-10:].mean())}
+    for name, prop, kw in (("uniform", bm.PROPOSAL_UNIFORM, {}),
+                           ("learned", bm.PROPOSAL_LEARNED, {"pool": 1 << 20, "mlp": mlp, "refresh_every": 1})):
+        ctx.set_mapping(pi0)
+        tr, tt, res = ctx.climb(prop, seed=31, time_limit_s=args.scale, per_round=4096, **kw)
+        marks = {}
+        for frac in (0.1, 0.25, 0.5, 1.0):
+            i = int(np.searchsorted(tt, frac * args.scale))
+            marks[f"{frac * args.scale:g}s"] = int(tr[min(i, len(tr) - 1)]) if len(tr) else score0
+        out["climb_" + name] = {"final_score": res["final_score"], "rounds": res["iterations"],
+                                "accepted": res["accepted"], "verify_failures": res["verify_failures"],
+                                "score_at": marks}
+        log(f"climb {name}: {score0} -> {res['final_score']} in {res['seconds']:.0f}s")
+    mlp.close()
+    ctx.set_mapping(pi0)
+    ann = bm.Anneal([ctx], chains_per_instance=2048, seed=9, t0=50.0, decay=0.9999, p_rot3=0.25)
+    t = time.perf_counter()
+    launches = 0
+    while time.perf_counter() - t < args.scale:
+        ann.run(500, 1)
+        launches += 1
+    el = time.perf_counter() - t
+    st = ann.state()
+    out["anneal"] = {"chains": ann.chains, "steps_per_chain": int(st["steps"][0]), "seconds": el,
+                     "steps_per_s": ann.chains * 500 * launches / el, "best_score": int(st["best"].max()),
+                     "median_best": int(np.median(st["best"])), "recount_mismatches": int(ann.verify())}
+    log(f"anneal: {out['anneal']['steps_per_s']:.4g} steps/s, best {out['anneal']['best_score']}")
+    ann.close()
+    ctx.close()
+    result = {"metric": f"exact score at {args.scale:g} s on the n=100,000 pair, learned vs uniform proposals",
+              "value": out["climb_learned"]["final_score"], "unit": "score", "n_gpus": 1, "higher_is_better": True,
+              "data": "synthetic", "config": {"workload": "n=100000, ~20M edges, 40% shuffled", "seconds": args.scale},
+              "results": out}
     print(json.dumps(result), flush=True)
 
 
