@@ -60,12 +60,19 @@ def packed_rows(n, female):
     return n < (1 << 17) and int(female.w.min()) >= 0 and int(female.w.max()) <= 32767
 
 
+def gain_stride(n):
+    """Row stride of the gain table: n + 1 rounded up to 32 int32."""
+    return (n + 1 + 31) // 32 * 32
+
+
 def gain_build_bytes(ctx, pi, entry_bytes):
-    """Algorithmic bytes of one table build.  The build streams, for every
-    male edge seen from each endpoint (2m visits), the edge (8 B), pi of the
-    other endpoint (4 B) and the bounds of that vertex's image's female row
-    (16 B), then every entry of that female row (entry_bytes each); it writes
-    the whole table ((n+1)^2 * 4 B)."""
+    """Compulsory HBM bytes of one table build (work list + rows): for every
+    male edge seen from each endpoint (2m visits) the edge (8 B), pi of the
+    other endpoint (4 B) and the bounds of its image's female row (16 B); every
+    female row entry once, both directions (entry_bytes each); the table write
+    ((n+1) rows x stride x 4 B).  The build expands a female row once per male
+    edge that lands on it ("expanded_entries"); those re-reads are served by
+    L2 / MALL or HBM and appear in the measured traffic, not here."""
     import numpy as np
     mo, mi = ctx.degrees(female=False)
     fo, fi = ctx.degrees(female=True)
@@ -73,13 +80,20 @@ def gain_build_bytes(ctx, pi, entry_bytes):
     # out-edge u->x expands over the female in-row of pi(x): counted once per in-edge of x
     entries = int((mi.astype(np.int64) * fi[pi]).sum() + (mo.astype(np.int64) * fo[pi]).sum())
     m = int(mo.sum())
-    return 2 * m * 28 + entry_bytes * entries + (ctx.n + 1) ** 2 * 4, entries
+    mf = int(fo.sum())
+    return 2 * m * 28 + 2 * mf * entry_bytes + (ctx.n + 1) * gain_stride(ctx.n) * 4, entries
 
 
-def gain_lookup_bytes(k):
-    """Per pair: pi(u), pi(v) (8 B), four table entries (16 B), the two male
-    adjacency words (2 x 8 B), two self-loop words (8 B), delta out (8 B)."""
-    return 56 * k
+def gain_lookup_bytes(ctx, k):
+    """Compulsory HBM bytes of one u-major lookup launch: per first vertex its
+    two staged rows (G(u, .) and GT(pi(u), .), 2 x stride x 4 B); the packed
+    per-vertex words once ((n+1) x 16 B); the male lists and female rows read
+    for the adjacency bitmaps (each once: 2m x 8 B + 2 m_F x 8 B); the stream
+    offsets ((n+2) x 8 B); the int64 deltas (8 k)."""
+    mo, _ = ctx.degrees(female=False)
+    fo, _ = ctx.degrees(female=True)
+    n = ctx.n
+    return n * 2 * gain_stride(n) * 4 + (n + 1) * 16 + 16 * int(mo.sum()) + 16 * int(fo.sum()) + (n + 2) * 8 + 8 * k
 
 
 def timed(ctx, kernel_ids, fn, steps):
@@ -200,14 +214,14 @@ def main():
         look_s = per_launch_ms(kt, bm.KERNEL_GAIN_SWAP) / 1000
         aux_s = per_launch_ms(kt, bm.KERNEL_AUX, steps=args.steps) / 1000
         bb, entries = gain_build_bytes(ctx, pi0, entry_bytes)
-        lb = gain_lookup_bytes(K)
+        lb = gain_lookup_bytes(ctx, K)
         gain = {
             "value": K * args.steps / wall,
             "ms_per_step": 1000 * wall / args.steps,
             "build_ms": 1000 * build_s,
             "lookup_ms": 1000 * look_s,
             "aux_ms": 1000 * aux_s,
-            "table_bytes": (N_VERTICES + 1) ** 2 * 4,
+            "table_bytes": (N_VERTICES + 1) * gain_stride(N_VERTICES) * 4,
             "build_roofline": {"bound": "hbm", "achieved": bb / build_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": bb / build_s / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": bb,
                                "expanded_entries": entries, "female_entry_bytes": entry_bytes},
@@ -272,7 +286,7 @@ def main():
         dom = "lookup" if gain["lookup_ms"] >= gain["build_ms"] else "build"
         rf = dict(gain[dom + "_roofline"])
         rf["kernel"] = ("k_gain_swap_umajor" if umajor else "k_gain_swap<uniform>") if dom == "lookup" \
-            else "k_gain_build"
+            else "k_gain_edge_desc+k_gain_build"
     else:
         rf = dict(scan["roofline"])
         rf["kernel"] = "k_swap<umajor>" if umajor else "k_swap<uniform>"

@@ -78,6 +78,7 @@ struct bm_ctx {
     int4* d_pis;            /* {pi, G(v, pi(v)), male self-loop, 0} per vertex, same validity */
     uint32_t* d_f_out_p;    /* packed female rows for the table build (compact graphs) */
     uint32_t* d_f_in_p;
+    int4* d_edesc;          /* table-build work list, 2 x male edges entries */
     int64_t gt_version;
     char err[512];
 };
@@ -242,6 +243,7 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_pis);
     (void)hipFree(c->d_f_out_p);
     (void)hipFree(c->d_f_in_p);
+    (void)hipFree(c->d_edesc);
     bm_graph_free(&c->male);
     free(c->f_out_deg);
     free(c->f_in_deg);
@@ -253,13 +255,15 @@ static void free_ctx(bm_ctx* c) {
 static BmGain devgain(const bm_ctx* c) {
     BmGain G;
     G.G = c->d_G;
-    G.stride = (int64_t)c->n + 1;
+    G.stride = bm_gain_stride(c->n);
     G.m_out_desc = c->d_m_out_desc;
     G.m_out_tab = c->d_m_out_tab;
     G.m_self = c->d_m_self;
     G.f_self = c->d_f_self;
     G.f_out_p = c->d_f_out_p;
     G.f_in_p = c->d_f_in_p;
+    G.edesc = c->d_edesc;
+    G.build_unroll = c->tune[BM_KERNEL_GAIN_BUILD].unroll;
     return G;
 }
 
@@ -307,6 +311,7 @@ static void default_tuning(bm_ctx* c) {
     c->tune[BM_KERNEL_SWAP] = BmLaunch{4, s, 8, cus, 8};
     c->tune[BM_KERNEL_ROT3] = BmLaunch{4, s, 8, cus, 8};
     c->tune[BM_KERNEL_SCORE] = BmLaunch{4, s, 8, cus, 8};
+    c->tune[BM_KERNEL_GAIN_BUILD] = BmLaunch{4, s, 8, cus, 8};   /* only unroll is used */
     for (int i = 0; i < BM_KERNEL_COUNT; i++) c->group_lanes[i] = 64;
 }
 
@@ -801,7 +806,7 @@ extern "C" int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k)
                 if (rc != BM_OK) return rc;
             }
             timing_begin(c, BM_KERNEL_AUX, &t);
-            HIPCHK(c, bm_launch_gain_transpose(devgain(c), c->d_GT, c->num_cus, c->stream));
+            HIPCHK(c, bm_launch_gain_transpose(devgain(c), c->n + 1, c->d_GT, c->num_cus, c->stream));
             timing_end(c, &t);
             c->gt_version = c->gain_version;
         }
@@ -1056,7 +1061,7 @@ extern "C" int bm_apply_perm3(bm_ctx* c, int32_t a, int32_t b, int32_t d, int32_
 
 extern "C" int64_t bm_gain_table_bytes(int32_t n) {
     if (n < 1) return 0;
-    return ((int64_t)n + 1) * ((int64_t)n + 1) * (int64_t)sizeof(int32_t);
+    return ((int64_t)n + 1) * bm_gain_stride(n) * (int64_t)sizeof(int32_t);
 }
 
 extern "C" int bm_gain_enabled(const bm_ctx* c) { return c ? c->gain_on : 0; }
@@ -1071,6 +1076,8 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         c->d_G = nullptr;
         (void)hipFree(c->d_GT);
         c->d_GT = nullptr;
+        (void)hipFree(c->d_edesc);
+        c->d_edesc = nullptr;
         c->gain_on = 0;
         c->gain_version++;
         return BM_OK;
@@ -1093,7 +1100,8 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
     const size_t bytes = (size_t)bm_gain_table_bytes(c->n);
     size_t free_b = 0, total_b = 0;
     HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
-    if (bytes + ((size_t)1 << 30) > free_b)
+    const size_t list_bytes = (size_t)(2 * c->male.m + 1) * sizeof(int4);
+    if (bytes + list_bytes + ((size_t)1 << 30) > free_b)
         return fail(c, BM_ERR_NOMEM, "bm_gain_enable: table needs %zu bytes, %zu free on the device", bytes, free_b);
     if (!c->d_m_out_desc) {
         bm_rowhash_host hm;
@@ -1119,10 +1127,13 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         HIPCHK(c, bm_launch_pack_rows(c->d_f_out, c->female_m, c->d_f_out_p, c->stream));
         HIPCHK(c, bm_launch_pack_rows(c->d_f_in, c->female_m, c->d_f_in_p, c->stream));
     }
-    int rc = dalloc(c, &c->d_G, bytes / sizeof(int32_t));
+    /* one spare entry: the build's look-ahead reads entry base(u) of a row with no edges */
+    int rc = dalloc(c, &c->d_edesc, (size_t)(2 * c->male.m + 1));
+    if (rc != BM_OK) return rc;
+    rc = dalloc(c, &c->d_G, bytes / sizeof(int32_t));
     if (rc != BM_OK) return rc;
     /* row 0 is never written by the build; clear it once */
-    HIPCHK(c, hipMemsetAsync(c->d_G, 0, ((size_t)c->n + 1) * sizeof(int32_t), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_G, 0, (size_t)bm_gain_stride(c->n) * sizeof(int32_t), c->stream));
     c->gain_on = 1;
     rc = gain_rebuild(c);
     if (rc != BM_OK) return rc;
@@ -1141,7 +1152,7 @@ extern "C" int bm_gain_row(bm_ctx* c, int32_t u, int32_t* row) {
     if (!c || !row || u < 0 || u > c->n) return fail(c, BM_ERR_ARG, "bm_gain_row: invalid argument");
     if (!c->gain_on) return fail(c, BM_ERR_STATE, "bm_gain_row: gain table not enabled");
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipMemcpyAsync(row, c->d_G + (int64_t)u * ((int64_t)c->n + 1), ((size_t)c->n + 1) * sizeof(int32_t),
+    HIPCHK(c, hipMemcpyAsync(row, c->d_G + (int64_t)u * bm_gain_stride(c->n), ((size_t)c->n + 1) * sizeof(int32_t),
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return BM_OK;
@@ -1232,7 +1243,12 @@ extern "C" int bm_set_tuning(bm_ctx* c, int32_t kernel, const bm_tuning* t) {
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(c, BM_ERR_ARG, "bm_set_tuning: blocks_per_cu in 1..64");
     if (t->group_lanes != 64) return fail(c, BM_ERR_ARG, "bm_set_tuning: group_lanes other than 64 not built");
     const int cus = c->num_cus > 0 ? c->num_cus : 256;
-    if (t->unroll != 2 && t->unroll != 4 && t->unroll != 8) return fail(c, BM_ERR_ARG, "bm_set_tuning: unroll must be 2, 4 or 8");
+    if (kernel == BM_KERNEL_GAIN_BUILD) {
+        if (t->unroll != 8 && t->unroll != 16 && t->unroll != 32)
+            return fail(c, BM_ERR_ARG, "bm_set_tuning: table-build unroll must be 8, 16 or 32");
+    } else if (t->unroll != 2 && t->unroll != 4 && t->unroll != 8) {
+        return fail(c, BM_ERR_ARG, "bm_set_tuning: unroll must be 2, 4 or 8");
+    }
     c->tune[kernel] = BmLaunch{w, t->lds_slots, t->blocks_per_cu, cus, t->unroll};
     c->group_lanes[kernel] = t->group_lanes;
     return BM_OK;

@@ -34,18 +34,28 @@ size_t bm_slot_bytes(const BmDevGraph& g);
 /* Placement-gain table (bm_gain.hip). */
 struct BmGain {
     int32_t* G;                 /* (n+1) x stride int32, row-major; row 0 / column 0 zero */
-    int64_t stride;             /* n + 1 */
+    int64_t stride;             /* row stride in int32: n + 1 rounded up to 32 (rows start on 128-byte lines) */
     const uint64_t* m_out_desc; /* male out-row hash tables (adjacency tests) */
     const int2* m_out_tab;
     const int32_t* m_self;      /* male self-loop weight per vertex or BM_ABSENT */
     const int32_t* f_self;      /* female self-loop weight per vertex or BM_ABSENT */
     const uint32_t* f_out_p;    /* female rows packed (f << 15) | w when the graph is compact, else NULL */
     const uint32_t* f_in_p;
+    int4* edesc;                /* build work list: one entry per male edge slot (2 x male edges + 1) */
+    int32_t build_unroll;       /* build: female-row entries per lane in flight (8, 16 or 32) */
 };
+
+/* Row stride of the gain table and its transpose: n + 1 rounded up to a
+ * multiple of 32 int32 (128 bytes), so every row starts on a cache line and
+ * 16-byte accesses are aligned.  Both tables hold n + 1 rows; the padding
+ * columns of G are zero. */
+int64_t bm_gain_stride(int32_t n);
 
 /* Packs {f, w} rows (f < 2^17, 0 <= w <= 32767) into (f << 15) | w words. */
 hipError_t bm_launch_pack_rows(const int2* in, int64_t m, uint32_t* out, hipStream_t s);
 
+/* Table build: per column chunk, the work list (k_gain_edge_desc) then the
+ * rows.  G.edesc must hold 2 x male edges + 1 entries. */
 hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int4* moves, const int64_t* count,
                                  int64_t max_moves, hipStream_t s);
@@ -54,7 +64,7 @@ hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, c
 /* u-major evaluation: GT = transposed table, pis = {pi, G(v, pi(v)), self-loop, 0}
  * per vertex, off = stream offsets.  Needs bm_gain_umajor_lds(n) bytes of LDS. */
 size_t bm_gain_umajor_lds(int32_t n);
-hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t* GT, int32_t num_cus, hipStream_t s);
+hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t nrows, int32_t* GT, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_pis(const BmDevGraph& g, const BmGain& G, int4* pis, hipStream_t s);
 hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
                                       const int64_t* off, uint64_t seed, int64_t* delta, int32_t num_cus,

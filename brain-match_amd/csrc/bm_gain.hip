@@ -22,12 +22,15 @@
  * of the moved vertices' neighbours, and the table is built and patched on the
  * device.
  *
- * Build: one block per male row; a column chunk of the row is accumulated in
- * LDS with atomic adds and written out coalesced.  A row's work is the list of
- * (male edge, female-row entry) pairs, flattened per wave (64 edges at a time,
- * prefix sum of their female-row lengths) so lanes stay busy for short and
- * long female rows alike.  Lookups: one thread per candidate.  Patches after
- * moves: the same flattening, with global integer atomic adds.
+ * Build: a work list first gives every male edge the female row it expands
+ * (one lane per edge, so the dependent loads of all edges overlap); then one
+ * block per male row accumulates a column chunk of the row in LDS with atomic
+ * adds, its waves taking the row's edges one at a time with the next edge's
+ * entry already loading, and writes the chunk out coalesced.  Lookups: one
+ * thread per candidate, or one block per first vertex of a u-major stream.
+ * Patches after moves: (male edge, female-row entry) pairs flattened per wave
+ * (64 edges at a time, prefix sum of their female-row lengths), with global
+ * integer atomic adds.
  */
 #include "bm_device.h"
 #include "bm_devutil.h"
@@ -38,7 +41,6 @@ using namespace bmd;
 namespace {
 
 constexpr int kBuildThreads = 1024;   /* build: 16 waves per row */
-constexpr int kBU = 8;                 /* build: female-row entries per lane in flight */
 constexpr int kPatchThreads = 512;
 constexpr int kPatchWaves = kPatchThreads / BM_WAVE;
 
@@ -160,12 +162,67 @@ struct Row<true> {
     static __device__ __forceinline__ T none() { return 0u; }
 };
 
-/* Row u: every male edge of u (out-edges against the female in-row of the
- * neighbour's image, in-edges against its out-row) is taken by one wave at a
- * time from a block-wide counter (so a long female row does not hold up the
- * block); the wave's lanes stride the row with kBU entries each in flight and
- * add min(w, fw) into the LDS copy of the row's column window. */
+/* Work list of the build, one entry per male edge slot of row u (out-edges,
+ * then in-edges, at base(u) = m_out_ptr[u] + m_in_ptr[u]): the female row the
+ * edge expands (the in-row of the neighbour's image for an out-edge, its
+ * out-row for an in-edge) clipped to the column window, as {start lo,
+ * start hi | in-edge flag << 31, length, w}; length 0 for self-loops and
+ * unmatched neighbours.
+ * One wave per male vertex, one lane per edge, so the dependent loads (edge,
+ * pi of the neighbour, row bounds, window search) of all edges overlap. */
 template <bool PACKED>
+__global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, int32_t col0, int32_t ncols) {
+    typedef Row<PACKED> R;
+    typedef typename R::T E;
+    const int lane = threadIdx.x & (BM_WAVE - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / BM_WAVE;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / BM_WAVE;
+    const bool windowed = !(col0 == 0 && (int64_t)ncols >= G.stride);
+    const E* __restrict__ f_in_rows = PACKED ? (const E*)G.f_in_p : (const E*)g.f_in;
+    const E* __restrict__ f_out_rows = PACKED ? (const E*)G.f_out_p : (const E*)g.f_out;
+    for (int64_t uu = wave + 1; uu <= g.n; uu += nwaves) {
+        const int32_t u = (int32_t)uu;
+        const int64_t ob = g.m_out_ptr[u], nout = g.m_out_ptr[u + 1] - ob;
+        const int64_t ib = g.m_in_ptr[u], L = nout + (g.m_in_ptr[u + 1] - ib);
+        int4* out = G.edesc + ob + ib;
+        for (int64_t t = lane; t < L; t += BM_WAVE) {
+            const bool outp = t < nout;
+            const int2 ed = outp ? g.m_out[ob + t] : g.m_in[ib + (t - nout)];
+            const int32_t gx = ed.x == u ? 0 : g.pi[ed.x];   /* self-loops are not in the table */
+            const E* __restrict__ frow = outp ? f_in_rows : f_out_rows;
+            int64_t rb = 0, re = 0;
+            if (gx != 0) {
+                const int64_t* fptr = outp ? g.f_in_ptr : g.f_out_ptr;
+                rb = fptr[gx];
+                re = fptr[gx + 1];
+                if (windowed) {
+                    int64_t a = rb, b = re;
+                    while (a < b) {
+                        const int64_t m = (a + b) >> 1;
+                        if (R::key(frow[m]) < col0) a = m + 1; else b = m;
+                    }
+                    const int64_t lo = a;
+                    b = re;
+                    while (a < b) {
+                        const int64_t m = (a + b) >> 1;
+                        if (R::key(frow[m]) < col0 + ncols) a = m + 1; else b = m;
+                    }
+                    rb = lo;
+                    re = a;
+                }
+            }
+            const uint32_t hi = (uint32_t)((uint64_t)rb >> 32) | (outp ? 0u : 0x80000000u);
+            out[t] = make_int4((int32_t)(uint32_t)rb, (int32_t)hi, (int32_t)(re - rb), ed.y);
+        }
+    }
+}
+
+/* Row u: the edges of u's work list are taken one at a time by the block's
+ * waves from an LDS counter (so a long female row does not hold up the
+ * block), with the next edge's entry loaded while the current row streams;
+ * the wave's lanes stride the row with U entries each in flight and add
+ * min(w, fw) into the LDS copy of the row's column window. */
+template <bool PACKED, int U>
 __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGain G, int32_t col0, int32_t ncols) {
     typedef Row<PACKED> R;
     typedef typename R::T E;
@@ -173,58 +230,59 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
     __shared__ int32_t next_edge;
     const int tid = threadIdx.x;
     const int lane = tid & (BM_WAVE - 1);
-    const bool windowed = !(col0 == 0 && (int64_t)ncols >= G.stride);
+    constexpr int32_t kStep = BM_WAVE * U;
     const E* __restrict__ f_in_rows = PACKED ? (const E*)G.f_in_p : (const E*)g.f_in;
     const E* __restrict__ f_out_rows = PACKED ? (const E*)G.f_out_p : (const E*)g.f_out;
     for (int32_t u = blockIdx.x + 1; u <= g.n; u += gridDim.x) {
         for (int32_t c = tid; c < ncols; c += kBuildThreads) row[c] = 0;
         if (tid == 0) next_edge = 0;
         __syncthreads();
-        const int64_t ob = g.m_out_ptr[u], nout = g.m_out_ptr[u + 1] - ob;
-        const int64_t ib = g.m_in_ptr[u], L = nout + (g.m_in_ptr[u + 1] - ib);
-        for (;;) {
-            int32_t t = 0;
-            if (lane == 0) t = atomicAdd(&next_edge, 1);
-            t = uni(t);
-            if (t >= L) break;
-            const bool outp = t < nout;
-            const int2 ed = outp ? g.m_out[ob + t] : g.m_in[ib + (t - nout)];
-            const int32_t x = uni(ed.x), w = uni(ed.y);
-            if (x == u) continue;                      /* self-loops are not in the table */
-            const int32_t gx = uni(g.pi[x]);
-            if (gx == 0) continue;
-            const int64_t* __restrict__ fptr = outp ? g.f_in_ptr : g.f_out_ptr;
-            const E* __restrict__ frow = outp ? f_in_rows : f_out_rows;
-            int64_t rb = uni64(fptr[gx]), re = uni64(fptr[gx + 1]);
-            if (windowed) {
-                int64_t a = rb, b = re;
-                while (a < b) {
-                    const int64_t m = (a + b) >> 1;
-                    if (R::key(frow[m]) < col0) a = m + 1; else b = m;
-                }
-                const int64_t lo = a;
-                b = re;
-                while (a < b) {
-                    const int64_t m = (a + b) >> 1;
-                    if (R::key(frow[m]) < col0 + ncols) a = m + 1; else b = m;
-                }
-                rb = lo;
-                re = a;
-            }
-            for (int64_t j0 = rb; j0 < re; j0 += (int64_t)BM_WAVE * kBU) {
-                E ent[kBU];
+        const int64_t base = g.m_out_ptr[u] + g.m_in_ptr[u];
+        const int32_t L = (int32_t)(g.m_out_ptr[u + 1] + g.m_in_ptr[u + 1] - base);
+        const int4* __restrict__ list = G.edesc + base;
+        /* the list index is clamped so the look-ahead load is unconditional
+         * (its value is used only while t < L) */
+        const int32_t last = L > 0 ? L - 1 : 0;
+        int32_t t = 0;
+        if (lane == 0) t = atomicAdd(&next_edge, 1);
+        t = uni(t);
+        int4 d = list[t < last ? t : last];
+        while (t < L) {
+            int32_t tn = 0;
+            if (lane == 0) tn = atomicAdd(&next_edge, 1);
+            tn = uni(tn);
+            const int4 dn = list[tn < last ? tn : last];
+            const uint32_t hi = (uint32_t)uni(d.y);
+            const int64_t start = (int64_t)((uint64_t)(uint32_t)uni(d.x) | ((uint64_t)(hi & 0x7fffffffu) << 32));
+            const E* __restrict__ frow = ((hi >> 31) ? f_out_rows : f_in_rows) + start;
+            const int32_t len = uni(d.z), w = uni(d.w);
+            int32_t j0 = 0;
+            for (; j0 + kStep <= len; j0 += kStep) {
+                E ent[U];
 #pragma unroll
-                for (int k = 0; k < kBU; k++) {
-                    const int64_t j = j0 + lane + (int64_t)BM_WAVE * k;
-                    ent[k] = j < re ? frow[j] : R::none();
-                }
+                for (int k = 0; k < U; k++) ent[k] = frow[j0 + lane + BM_WAVE * k];
 #pragma unroll
-                for (int k = 0; k < kBU; k++) {
-                    if (j0 + lane + (int64_t)BM_WAVE * k >= re) continue;
+                for (int k = 0; k < U; k++) {
                     const int32_t f = R::key(ent[k]), fw = R::w(ent[k]);
                     atomicAdd(&row[f - col0], w < fw ? w : fw);
                 }
             }
+            if (j0 < len) {
+                E ent[U];
+#pragma unroll
+                for (int k = 0; k < U; k++) {
+                    const int32_t j = j0 + lane + BM_WAVE * k;
+                    ent[k] = j < len ? frow[j] : R::none();
+                }
+#pragma unroll
+                for (int k = 0; k < U; k++) {
+                    if (j0 + lane + BM_WAVE * k >= len) continue;
+                    const int32_t f = R::key(ent[k]), fw = R::w(ent[k]);
+                    atomicAdd(&row[f - col0], w < fw ? w : fw);
+                }
+            }
+            t = tn;
+            d = dn;
         }
         __syncthreads();
         int32_t* out = G.G + (int64_t)u * G.stride + col0;
@@ -323,22 +381,42 @@ __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const
     }
 }
 
-/* Transposed table: GT(f, u) = G(u, f), 64 x 64 tiles through LDS. */
+/* Transposed table: GT(f, u) = G(u, f) for f < nrows, with GT(f, u) = 0 in
+ * the padding columns u >= nrows.  64 x 64 tiles through LDS; rows start on
+ * 128-byte lines (stride a multiple of 32), so every global access is an
+ * aligned 16-byte load or store. */
 __global__ void __launch_bounds__(256) k_gain_transpose(const int32_t* __restrict__ Gt, int32_t* __restrict__ GT,
-                                                        int64_t stride) {
+                                                        int64_t nrows, int64_t stride) {
     __shared__ int32_t tile[64][65];
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int64_t tiles = (stride + 63) / 64;
-    for (int64_t t = blockIdx.x; t < tiles * tiles; t += gridDim.x) {
-        const int64_t br = t / tiles, bc = t % tiles;
-        for (int r = ty; r < 64; r += 4) {
-            const int64_t u = br * 64 + r, f = bc * 64 + tx;
-            tile[r][tx] = (u < stride && f < stride) ? Gt[u * stride + f] : 0;
+    const int q = threadIdx.x & 15, r0 = threadIdx.x >> 4;    /* int4 column, first row */
+    const int64_t ut = (stride + 63) / 64, ft = (nrows + 63) / 64;
+    for (int64_t t = blockIdx.x; t < ut * ft; t += gridDim.x) {
+        const int64_t bu = t / ft, bf = t % ft;   /* tile rows u = 64 bu.., columns f = 64 bf.. */
+        const int64_t f0 = bf * 64 + 4 * q;
+        int4 a[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int64_t u = bu * 64 + r0 + 16 * k;
+            a[k] = (u < nrows && f0 < stride) ? *reinterpret_cast<const int4*>(Gt + u * stride + f0)
+                                              : make_int4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int32_t* tr = &tile[r0 + 16 * k][4 * q];
+            tr[0] = a[k].x;
+            tr[1] = a[k].y;
+            tr[2] = a[k].z;
+            tr[3] = a[k].w;
         }
         __syncthreads();
-        for (int r = ty; r < 64; r += 4) {
-            const int64_t f = bc * 64 + r, u = br * 64 + tx;
-            if (f < stride && u < stride) GT[f * stride + u] = tile[tx][r];
+        const int64_t u0 = bu * 64 + 4 * q;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int fl = r0 + 16 * k;
+            const int64_t f = bf * 64 + fl;
+            if (f < nrows && u0 < stride)
+                *reinterpret_cast<int4*>(GT + f * stride + u0) =
+                    make_int4(tile[4 * q][fl], tile[4 * q + 1][fl], tile[4 * q + 2][fl], tile[4 * q + 3][fl]);
         }
         __syncthreads();
     }
@@ -355,97 +433,117 @@ __global__ void k_gain_pis(BmDevGraph g, BmGain G, int4* __restrict__ pis) {
     }
 }
 
-/* Four hash-table lookups with their probe sequences interleaved (one
- * round of loads for all unresolved keys at a time); key 0 or an empty
- * table gives BM_ABSENT. */
-__device__ __forceinline__ void probe4(const uint64_t* desc, const int2* const* tab, const int32_t* key,
-                                       int32_t* out) {
-    const int2* T[4];
-    uint32_t bits[4], s[4], pend = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        out[j] = BM_ABSENT;
-        bits[j] = (uint32_t)(desc[j] & 63u);
-        T[j] = tab[j] + (desc[j] >> 6);
-        const bool live = bits[j] != 0 && key[j] != 0;
-        s[j] = live ? slot_hash(key[j], bits[j]) : 0u;
-        pend |= (live ? 1u : 0u) << j;
-    }
-    while (pend) {
-        int2 e[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) e[j] = ((pend >> j) & 1u) ? T[j][s[j]] : make_int2(0, 0);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if (!((pend >> j) & 1u)) continue;
-            if (e[j].x == 0 || e[j].x == key[j]) {
-                if (e[j].x != 0) out[j] = e[j].y;
-                pend &= ~(1u << j);
-            } else {
-                s[j] = (s[j] + 1u) & ((1u << bits[j]) - 1u);
-            }
-        }
-    }
+/* u-major lookups: one block per first vertex u of the stream.  The block
+ * stages row G(u, .) (indexed by the partner's female) and row GT(pi(u), .) =
+ * G(., pi(u)) (indexed by the partner) in LDS with aligned 16-byte loads, and
+ * four bitmaps over vertex ids with a prefix popcount per 64-bit word: u's
+ * male out- and in-neighbours and f1 = pi(u)'s female out- and in-neighbours.
+ * A pair (u, v) then costs two LDS reads, one gather of v's packed {pi,
+ * G(v, pi(v)), self-loops}, two bitmap reads and the delta store.  For a pair
+ * with an edge between u and v (about 2% at n = 18,524, mean degree 216) the
+ * bit's rank is the entry's position in the sorted CSR row, so each needed
+ * weight (w_M(u, v), w_M(v, u), w_F(f1, f2), w_F(f2, f1)) is one direct load;
+ * self-loops add their exact terms. */
+constexpr int kUmThreads = 1024;
+constexpr int kUmStage = 4;    /* 16-byte row words per thread and row in flight while staging */
+constexpr int kUmPairs = 4;    /* pairs per thread in flight */
+
+/* Rank of x in a bitmap (uint32 words) with prefix counts per 64-bit word. */
+__device__ __forceinline__ int32_t bit_rank(const uint32_t* bm, const uint16_t* pre, int32_t x) {
+    const int32_t w = x >> 6;
+    const uint64_t word = (uint64_t)bm[2 * w] | ((uint64_t)bm[2 * w + 1] << 32);
+    return (int32_t)pre[w] + __popcll(word & ((1ull << (x & 63)) - 1ull));
 }
 
-/* u-major lookups: one block per first vertex u of the stream.  The block
- * stages row G(u, .) (indexed by the partner's female), row GT(pi(u), .) =
- * G(., pi(u)) (indexed by the partner) and u's out- and in-neighbour bitmaps
- * in LDS; a pair (u, v) then costs two LDS reads, one gather of v's packed
- * {pi, G(v, pi(v)), self-loop} and the delta store.  Pairs with an edge
- * between u and v (bitmap hit) add the exact edge correction through the
- * hash tables; self-loops add their exact terms. */
-constexpr int kUmThreads = 1024;
-constexpr int kUmStage = 8;    /* row words per thread in flight while staging */
-constexpr int kUmPairs = 4;    /* pairs per thread in flight */
+__device__ __forceinline__ bool bit_test(const uint32_t* bm, int32_t x) { return (bm[x >> 5] >> (x & 31)) & 1u; }
 
 __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, BmGain G, const int32_t* __restrict__ GT,
                                                                  const int4* __restrict__ pis,
                                                                  const int64_t* __restrict__ off, uint64_t seed,
                                                                  int64_t* __restrict__ delta) {
-    extern __shared__ int32_t sm[];
+    extern __shared__ int4 sm4[];
     const int32_t n = g.n;
-    const int64_t stride = G.stride;
-    const int32_t words = (n + 1 + 31) >> 5;
-    int32_t* rowG = sm;
-    int32_t* rowT = sm + stride;
-    uint32_t* adj = reinterpret_cast<uint32_t*>(sm + 2 * stride);   /* [0, words): u -> v, [words, 2 words): v -> u */
+    const int64_t stride = G.stride;                   /* multiple of 32 */
+    const int32_t s4 = (int32_t)(stride >> 2);
+    const int32_t w64 = (n + 1 + 63) >> 6;             /* prefix entries per bitmap */
+    int32_t* rowG = reinterpret_cast<int32_t*>(sm4);
+    int32_t* rowT = rowG + stride;
+    uint32_t* bits = reinterpret_cast<uint32_t*>(rowT + stride);   /* 4 bitmaps of 2 * w64 words */
+    uint32_t* mo = bits;                               /* u -> v in M */
+    uint32_t* mi = bits + 2 * w64;                     /* v -> u in M */
+    uint32_t* fo = bits + 4 * w64;                     /* f1 -> f in F */
+    uint32_t* fi = bits + 6 * w64;                     /* f -> f1 in F */
+    uint16_t* pre = reinterpret_cast<uint16_t*>(bits + 8 * w64);   /* 4 x w64 prefix counts */
     const int tid = threadIdx.x;
+    const int lane = tid & (BM_WAVE - 1), wave = tid / BM_WAVE;
     for (int32_t u = blockIdx.x + 1; u <= n; u += gridDim.x) {
         const int64_t i0 = off[u], i1 = off[u + 1];
         if (i0 == i1) continue;   /* block-uniform */
         const int4 pu = pis[u];
         const int32_t f1 = pu.x;
         const int32_t F11 = pu.w;                           /* w_F(f1, f1) */
-        const uint64_t du = G.m_out_desc[u], df1 = f1 ? g.f_out_desc[f1] : 0;
-        const int32_t* __restrict__ Gu = G.G + (int64_t)u * stride;
-        const int32_t* __restrict__ Tu = GT + (int64_t)f1 * stride;
-        for (int64_t c0 = tid; c0 < stride; c0 += (int64_t)kUmThreads * kUmStage) {
-            int32_t a[kUmStage], t[kUmStage];
+        const int4* __restrict__ Gu = reinterpret_cast<const int4*>(G.G + (int64_t)u * stride);
+        const int4* __restrict__ Tu = reinterpret_cast<const int4*>(GT + (int64_t)f1 * stride);
+        int4* sG = reinterpret_cast<int4*>(rowG);
+        int4* sT = reinterpret_cast<int4*>(rowT);
+        for (int32_t c0 = tid; c0 < s4; c0 += kUmThreads * kUmStage) {
+            int4 a[kUmStage], t[kUmStage];
 #pragma unroll
             for (int j = 0; j < kUmStage; j++) {
-                const int64_t c = c0 + (int64_t)kUmThreads * j;
-                a[j] = c < stride ? Gu[c] : 0;
-                t[j] = c < stride ? Tu[c] : 0;
+                const int32_t c = c0 + kUmThreads * j;
+                a[j] = c < s4 ? Gu[c] : make_int4(0, 0, 0, 0);
+                t[j] = c < s4 ? Tu[c] : make_int4(0, 0, 0, 0);
             }
 #pragma unroll
             for (int j = 0; j < kUmStage; j++) {
-                const int64_t c = c0 + (int64_t)kUmThreads * j;
-                if (c < stride) {
-                    rowG[c] = a[j];
-                    rowT[c] = t[j];
+                const int32_t c = c0 + kUmThreads * j;
+                if (c < s4) {
+                    sG[c] = a[j];
+                    sT[c] = t[j];
                 }
             }
         }
-        for (int32_t w = tid; w < 2 * words; w += kUmThreads) adj[w] = 0;
+        for (int32_t w = tid; w < 8 * w64; w += kUmThreads) bits[w] = 0;
         __syncthreads();
-        for (int64_t e = g.m_out_ptr[u] + tid; e < g.m_out_ptr[u + 1]; e += kUmThreads) {
+        const int64_t mob = g.m_out_ptr[u], mib = g.m_in_ptr[u];
+        for (int64_t e = mob + tid; e < g.m_out_ptr[u + 1]; e += kUmThreads) {
             const int32_t x = g.m_out[e].x;
-            atomicOr(&adj[x >> 5], 1u << (x & 31));
+            atomicOr(&mo[x >> 5], 1u << (x & 31));
         }
-        for (int64_t e = g.m_in_ptr[u] + tid; e < g.m_in_ptr[u + 1]; e += kUmThreads) {
+        for (int64_t e = mib + tid; e < g.m_in_ptr[u + 1]; e += kUmThreads) {
             const int32_t x = g.m_in[e].x;
-            atomicOr(&adj[words + (x >> 5)], 1u << (x & 31));
+            atomicOr(&mi[x >> 5], 1u << (x & 31));
+        }
+        const int64_t fob = f1 ? g.f_out_ptr[f1] : 0, fib = f1 ? g.f_in_ptr[f1] : 0;
+        if (f1) {
+            for (int64_t e = fob + tid; e < g.f_out_ptr[f1 + 1]; e += kUmThreads) {
+                const int32_t y = g.f_out[e].x;
+                atomicOr(&fo[y >> 5], 1u << (y & 31));
+            }
+            for (int64_t e = fib + tid; e < g.f_in_ptr[f1 + 1]; e += kUmThreads) {
+                const int32_t y = g.f_in[e].x;
+                atomicOr(&fi[y >> 5], 1u << (y & 31));
+            }
+        }
+        __syncthreads();
+        if (wave < 4) {   /* exclusive prefix popcounts, one wave per bitmap */
+            const uint32_t* bm = bits + 2 * w64 * wave;
+            uint16_t* pr = pre + w64 * wave;
+            const int32_t per = (w64 + BM_WAVE - 1) / BM_WAVE;
+            const int32_t lo = lane * per, hi = lo + per < w64 ? lo + per : w64;
+            int32_t cnt = 0;
+            for (int32_t w = lo; w < hi; w++) cnt += __popc(bm[2 * w]) + __popc(bm[2 * w + 1]);
+            int32_t incl = cnt;
+#pragma unroll
+            for (int o = 1; o < BM_WAVE; o <<= 1) {
+                const int32_t t = __shfl_up(incl, o, BM_WAVE);
+                if (lane >= o) incl += t;
+            }
+            int32_t run = incl - cnt;
+            for (int32_t w = lo; w < hi; w++) {
+                pr[w] = (uint16_t)run;
+                run += __popc(bm[2 * w]) + __popc(bm[2 * w + 1]);
+            }
         }
         __syncthreads();
         for (int64_t ib = i0 + tid; ib < i1; ib += (int64_t)kUmThreads * kUmPairs) {
@@ -464,25 +562,18 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
                 const int32_t f2 = pv[j].x, x = v[j];
                 const int32_t F22 = pv[j].w;                /* w_F(f2, f2) */
                 int64_t d = (int64_t)rowG[f2] - pu.y + rowT[x] - pv[j].y;
-                const bool uv = (adj[x >> 5] >> (x & 31)) & 1u;
-                const bool vu = (adj[words + (x >> 5)] >> (x & 31)) & 1u;
+                const bool uv = bit_test(mo, x);
+                const bool vu = bit_test(mi, x);
                 if (uv || vu) {
-                    /* edges between u and v: w_M(u, v), w_M(v, u), w_F(f1, f2), w_F(f2, f1) */
-                    const bool ff = f1 != 0 && f2 != 0;
-                    const uint64_t desc[4] = {du, G.m_out_desc[x], df1, f2 ? g.f_out_desc[f2] : 0};
-                    const int2* tab[4] = {G.m_out_tab, G.m_out_tab, g.f_out_tab, g.f_out_tab};
-                    const int32_t key[4] = {uv ? x : 0, vu ? u : 0, ff ? f2 : 0, ff ? f1 : 0};
-                    int32_t r[4];
-                    probe4(desc, tab, key, r);
-                    const int32_t F12 = r[2], F21 = r[3];
-                    if (uv) {
-                        const int32_t w = r[0];
-                        d += (int64_t)contrib(w, F21) - contrib(w, F22) - contrib(w, F11) + contrib(w, F12);
-                    }
-                    if (vu) {
-                        const int32_t w = r[1];
-                        d += (int64_t)contrib(w, F12) - contrib(w, F11) - contrib(w, F22) + contrib(w, F21);
-                    }
+                    /* edges between u and v: w_M(u, v), w_M(v, u), w_F(f1, f2), w_F(f2, f1);
+                     * f1 = 0 leaves fo, fi empty and bit 0 is never set, so an
+                     * unmatched endpoint reads no female weight */
+                    const int32_t wuv = uv ? g.m_out[mob + bit_rank(mo, pre, x)].y : BM_ABSENT;
+                    const int32_t wvu = vu ? g.m_in[mib + bit_rank(mi, pre + w64, x)].y : BM_ABSENT;
+                    const int32_t F12 = bit_test(fo, f2) ? g.f_out[fob + bit_rank(fo, pre + 2 * w64, f2)].y : BM_ABSENT;
+                    const int32_t F21 = bit_test(fi, f2) ? g.f_in[fib + bit_rank(fi, pre + 3 * w64, f2)].y : BM_ABSENT;
+                    if (uv) d += (int64_t)contrib(wuv, F21) - contrib(wuv, F22) - contrib(wuv, F11) + contrib(wuv, F12);
+                    if (vu) d += (int64_t)contrib(wvu, F12) - contrib(wvu, F11) - contrib(wvu, F22) + contrib(wvu, F21);
                 }
                 if (pu.z != BM_ABSENT) d += contrib(pu.z, F22) - (int64_t)contrib(pu.z, F11);
                 if (pv[j].z != BM_ABSENT) d += contrib(pv[j].z, F11) - (int64_t)contrib(pv[j].z, F22);
@@ -563,33 +654,52 @@ __global__ void k_moves_from_perm3(const int32_t* __restrict__ pi_before, int32_
 /* ------------------------------------------------------------------ */
 
 int32_t bm_gain_chunk_cols(int64_t stride) {
-    const int64_t cap = 20000;   /* 80,000 bytes of int32: two blocks per CU (160 KiB of LDS) */
-    return (int32_t)(stride < cap ? stride : cap);
+    /* at most 20,384 columns (81,536 bytes of int32: two blocks per CU in
+     * 160 KiB of LDS), split evenly into multiples of 32 */
+    const int64_t cap = 20384;
+    const int64_t chunks = (stride + cap - 1) / cap;
+    const int64_t w = ((stride + chunks - 1) / chunks + 31) & ~(int64_t)31;
+    return (int32_t)(w < stride ? w : stride);
 }
 
-hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
+template <bool PACKED, int U>
+static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
     const int32_t chunk = bm_gain_chunk_cols(G.stride);
     const size_t lds = (size_t)chunk * sizeof(int32_t);
-    const void* kern = G.f_out_p ? (const void*)k_gain_build<true> : (const void*)k_gain_build<false>;
+    const void* kern = (const void*)k_gain_build<PACKED, U>;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int occ = 0;
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kBuildThreads, lds);
     if (e != hipSuccess) return e;
     if (occ < 1) occ = 1;
-    int64_t blocks = (int64_t)occ * (num_cus > 0 ? num_cus : 256);
+    const int64_t cus = num_cus > 0 ? num_cus : 256;
+    int64_t blocks = (int64_t)occ * cus;
     if (blocks > g.n) blocks = g.n;
     if (blocks < 1) blocks = 1;
+    int64_t dblocks = (g.n + 3) / 4;               /* 4 waves (vertices) per 256-thread block */
+    if (dblocks > cus * 64) dblocks = cus * 64;
+    if (dblocks < 1) dblocks = 1;
     for (int64_t c0 = 0; c0 < G.stride; c0 += chunk) {
         const int32_t nc = (int32_t)(G.stride - c0 < chunk ? G.stride - c0 : chunk);
-        if (G.f_out_p)
-            hipLaunchKernelGGL(k_gain_build<true>, dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g, G,
-                               (int32_t)c0, nc);
-        else
-            hipLaunchKernelGGL(k_gain_build<false>, dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g, G,
-                               (int32_t)c0, nc);
+        hipLaunchKernelGGL(k_gain_edge_desc<PACKED>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc);
+        hipLaunchKernelGGL((k_gain_build<PACKED, U>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g, G,
+                           (int32_t)c0, nc);
     }
     return hipGetLastError();
+}
+
+hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
+    if (!G.edesc) return hipErrorInvalidValue;
+    const bool packed = G.f_out_p != nullptr;
+    switch (G.build_unroll) {
+    case 16:
+        return packed ? launch_build_rows<true, 16>(g, G, num_cus, s) : launch_build_rows<false, 16>(g, G, num_cus, s);
+    case 32:
+        return packed ? launch_build_rows<true, 32>(g, G, num_cus, s) : launch_build_rows<false, 32>(g, G, num_cus, s);
+    default:
+        return packed ? launch_build_rows<true, 8>(g, G, num_cus, s) : launch_build_rows<false, 8>(g, G, num_cus, s);
+    }
 }
 
 hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int4* moves, const int64_t* count,
@@ -617,14 +727,19 @@ hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, c
     return hipGetLastError();
 }
 
-size_t bm_gain_umajor_lds(int32_t n) { return 2 * ((size_t)n + 1) * 4 + 2 * (size_t)((n + 1 + 31) >> 5) * 4; }
+int64_t bm_gain_stride(int32_t n) { return (((int64_t)n + 1) + 31) & ~(int64_t)31; }
 
-hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t* GT, int32_t num_cus, hipStream_t s) {
-    const int64_t tiles = (G.stride + 63) / 64;
-    int64_t blocks = tiles * tiles;
-    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 16;
+size_t bm_gain_umajor_lds(int32_t n) {
+    const size_t w64 = ((size_t)n + 1 + 63) >> 6;
+    return 2 * (size_t)bm_gain_stride(n) * 4 + 4 * w64 * 8 + 4 * w64 * 2;   /* rows, 4 bitmaps, 4 prefix arrays */
+}
+
+hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t nrows, int32_t* GT, int32_t num_cus, hipStream_t s) {
+    if (G.stride & 31) return hipErrorInvalidValue;   /* 16-byte accesses need 128-byte rows */
+    int64_t blocks = ((G.stride + 63) / 64) * (((int64_t)nrows + 63) / 64);
+    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 8;
     if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(k_gain_transpose, dim3((unsigned)blocks), dim3(256), 0, s, G.G, GT, G.stride);
+    hipLaunchKernelGGL(k_gain_transpose, dim3((unsigned)blocks), dim3(256), 0, s, G.G, GT, (int64_t)nrows, G.stride);
     return hipGetLastError();
 }
 

@@ -285,8 +285,8 @@ int bm_batch_accept_disjoint(bm_batch* b, int64_t max_accept, int64_t* n_accepte
 /* ------------------------------------------------------------------ */
 
 /* G(u, f) = score of u's incident male edges (self-loops excluded) if u alone
- * were placed on female f, an (n+1)^2 int32 table in HBM (1.37 GB at
- * n = 18,524).  While enabled, every swap / 3-permutation evaluation reads it
+ * were placed on female f, an int32 table of n+1 rows in HBM, each row padded
+ * to a multiple of 32 entries (1.37 GB at n = 18,524; bm_gain_table_bytes).  While enabled, every swap / 3-permutation evaluation reads it
  * (four table entries plus an exact correction for edges inside the moved set
  * and self-loops) instead of scanning the candidate's edges; bm_set_mapping
  * rebuilds it and bm_apply_swap, bm_apply_perm3 and the disjoint acceptance
@@ -342,7 +342,9 @@ typedef struct bm_tuning {
     int32_t lds_slots;
     int32_t blocks_per_cu;
     int32_t group_lanes;  /* lanes cooperating on one candidate (64) */
-    int32_t unroll;       /* list entries per lane in flight: 2, 4 or 8 (swap kernel) */
+    int32_t unroll;       /* list entries per lane in flight: 2, 4 or 8 (swap kernel);
+                             female-row entries per lane, 8, 16 or 32 (table build,
+                             which uses only this field) */
 } bm_tuning;
 int bm_get_tuning(const bm_ctx* ctx, int32_t kernel, bm_tuning* t);
 int bm_set_tuning(bm_ctx* ctx, int32_t kernel, const bm_tuning* t);
