@@ -53,6 +53,8 @@ int64_t bm_gain_stride(int32_t n);
 
 /* Packs {f, w} rows (f < 2^17, 0 <= w <= 32767) into (f << 15) | w words. */
 hipError_t bm_launch_pack_rows(const int2* in, int64_t m, uint32_t* out, hipStream_t s);
+/* Column-window width of the table build for a row stride. */
+int32_t bm_gain_chunk_cols(int64_t stride);
 
 /* Table build: per column chunk, the work list (k_gain_edge_desc) then the
  * rows.  G.edesc must hold 2 x male edges + 1 entries. */

@@ -144,7 +144,13 @@ __device__ __forceinline__ void wave_flat(bool have, int32_t w, int32_t y, const
 }
 
 /* Female-row entry {f, w} in either layout: int2, or the packed word
- * (f << 15) | w used when ids are < 2^17 and weights in 0..32767. */
+ * (f << 15) | w used when ids are < 2^17 and weights in 0..32767.  load()
+ * reads entry i of a chunk through a buffer resource whose record count is
+ * the chunk's length in bytes: reads past the chunk return zero instead of
+ * touching memory, and every lane's loads share one offset register. */
+typedef int V2i __attribute__((ext_vector_type(2)));
+constexpr int32_t kBufWord3 = 0x00020000;   /* gfx9 raw-buffer resource word 3 */
+
 template <bool PACKED>
 struct Row;
 template <>
@@ -153,6 +159,10 @@ struct Row<false> {
     static __device__ __forceinline__ int32_t key(T e) { return e.x; }
     static __device__ __forceinline__ int32_t w(T e) { return e.y; }
     static __device__ __forceinline__ T none() { return make_int2(0, 0); }
+    static __device__ __forceinline__ T load(__amdgpu_buffer_rsrc_t r, int32_t i) {
+        const V2i v = __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0);
+        return make_int2(v.x, v.y);
+    }
 };
 template <>
 struct Row<true> {
@@ -160,6 +170,9 @@ struct Row<true> {
     static __device__ __forceinline__ int32_t key(T e) { return (int32_t)(e >> 15); }
     static __device__ __forceinline__ int32_t w(T e) { return (int32_t)(e & 0x7fffu); }
     static __device__ __forceinline__ T none() { return 0u; }
+    static __device__ __forceinline__ T load(__amdgpu_buffer_rsrc_t r, int32_t i) {
+        return (T)__builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0);
+    }
 };
 
 /* Work list of the build, one entry per male edge slot of row u (out-edges,
@@ -219,15 +232,33 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
 
 /* Row u: the edges of u's work list are taken one at a time by the block's
  * waves from an LDS counter (so a long female row does not hold up the
- * block), with the next edge's entry loaded while the current row streams;
- * the wave's lanes stride the row with U entries each in flight and add
- * min(w, fw) into the LDS copy of the row's column window. */
-template <bool PACKED, int U>
-__global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGain G, int32_t col0, int32_t ncols) {
+ * block).  A wave walks its edges' female rows in chunks of 64 x U entries
+ * and keeps two chunks in flight: the next chunk (of the same edge or the
+ * next one) is loaded while the current one's min(w, fw) are added into the
+ * LDS copy of the row's column window.  Work-list entries are read through
+ * the constant address space (scalar loads), so taking a new edge does not
+ * wait on the row loads in flight; the entry after next is always loading. */
+/* One work-list entry, read as constant data (a scalar load for a uniform
+ * index).  The host pass of the compiler only type-checks kernel bodies. */
+__device__ __forceinline__ int4 load_entry(const int4* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(4))) const int4 ConstInt4;
+    return *(const ConstInt4*)p;
+#else
+    return *p;
+#endif
+}
+
+template <bool PACKED, int U, bool WINDOWED>
+__global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGain G, int32_t col0_arg, int32_t ncols) {
     typedef Row<PACKED> R;
     typedef typename R::T E;
+    /* the row window sits at LDS address 0 (the edge counter after it), and
+     * a single window starts at column 0, so a column's LDS address is just
+     * 4 f */
     extern __shared__ int32_t row[];
-    __shared__ int32_t next_edge;
+    int32_t& next_edge = row[ncols];
+    const int32_t col0 = WINDOWED ? col0_arg : 0;
     const int tid = threadIdx.x;
     const int lane = tid & (BM_WAVE - 1);
     constexpr int32_t kStep = BM_WAVE * U;
@@ -239,50 +270,104 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
         __syncthreads();
         const int64_t base = g.m_out_ptr[u] + g.m_in_ptr[u];
         const int32_t L = (int32_t)(g.m_out_ptr[u + 1] + g.m_in_ptr[u + 1] - base);
-        const int4* __restrict__ list = G.edesc + base;
+        const int4* list = G.edesc + base;
         /* the list index is clamped so the look-ahead load is unconditional
-         * (its value is used only while t < L) */
+         * (its value is used only while the index is < L) */
         const int32_t last = L > 0 ? L - 1 : 0;
-        int32_t t = 0;
-        if (lane == 0) t = atomicAdd(&next_edge, 1);
-        t = uni(t);
-        int4 d = list[t < last ? t : last];
-        while (t < L) {
-            int32_t tn = 0;
-            if (lane == 0) tn = atomicAdd(&next_edge, 1);
-            tn = uni(tn);
-            const int4 dn = list[tn < last ? tn : last];
-            const uint32_t hi = (uint32_t)uni(d.y);
-            const int64_t start = (int64_t)((uint64_t)(uint32_t)uni(d.x) | ((uint64_t)(hi & 0x7fffffffu) << 32));
-            const E* __restrict__ frow = ((hi >> 31) ? f_out_rows : f_in_rows) + start;
-            const int32_t len = uni(d.z), w = uni(d.w);
-            int32_t j0 = 0;
-            for (; j0 + kStep <= len; j0 += kStep) {
-                E ent[U];
+        auto grab = [&]() {
+            int32_t x = 0;
+            if (lane == 0) x = atomicAdd(&next_edge, 1);
+            return uni(x);
+        };
+        /* scalar walk state: current edge (row pointer, length, weight, next
+         * offset) and the prefetched next edge */
+        int32_t t = grab();
+        int4 d = load_entry(list + (t < last ? t : last));
+        int32_t tn = grab();
+        int4 dn = load_entry(list + (tn < last ? tn : last));
+        const E* frow = nullptr;
+        int32_t len = 0, w = 0, j0 = 0;
+        auto decode = [&](const int4 e) {
+            const uint32_t hi = (uint32_t)uni(e.y);
+            const int64_t start = (int64_t)((uint64_t)(uint32_t)uni(e.x) | ((uint64_t)(hi & 0x7fffffffu) << 32));
+            frow = ((hi >> 31) ? f_out_rows : f_in_rows) + start;
+            len = uni(e.z);
+            w = uni(e.w);
+            j0 = 0;
+        };
+        if (t < L) decode(d);
+        /* next chunk: pointer, entry count (0 = no more work), weight */
+        auto next_chunk = [&](const E*& p, int32_t& cnt, int32_t& wt) {
+            for (;;) {
+                if (j0 < len) {
+                    p = frow + j0;
+                    cnt = len - j0 < kStep ? len - j0 : kStep;
+                    wt = w;
+                    j0 += kStep;
+                    return;
+                }
+                if (t >= L || tn >= L) {
+                    t = L;
+                    p = f_in_rows;
+                    cnt = 0;
+                    wt = 0;
+                    return;
+                }
+                t = tn;
+                decode(dn);
+                tn = grab();
+                dn = load_entry(list + (tn < last ? tn : last));
+            }
+        };
+        /* Chunk loads are unconditional (buffer loads bounded by the chunk)
+         * and the loop is unrolled twice so the two register sets swap roles
+         * without copies: both keep the compiler's load counting exact, so an
+         * atomic waits only on the older chunk while the newer one loads. */
+        auto load_chunk = [&](E (&r)[U], const E* p, int32_t cnt) {
+            /* cnt = 0 gives a resource of no records: every load returns
+             * zero without touching memory, so the load count never depends
+             * on the branch taken */
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<E*>(p), 0, cnt * (int32_t)sizeof(E), kBufWord3);
 #pragma unroll
-                for (int k = 0; k < U; k++) ent[k] = frow[j0 + lane + BM_WAVE * k];
+            for (int k = 0; k < U; k++) r[k] = R::load(rs, lane + BM_WAVE * k);
+        };
+        /* rowc[f] is column f of the window (the shift is folded into the
+         * LDS base); a full chunk needs no lane mask, the last chunk of an
+         * edge masks the lanes past its end (their zero entries must not
+         * add min(w, 0) for a negative w) */
+        int32_t* const rowc = row - col0;
+        auto add_chunk = [&](const E (&r)[U], int32_t cnt, int32_t wt) {
+            if (cnt == kStep) {
 #pragma unroll
                 for (int k = 0; k < U; k++) {
-                    const int32_t f = R::key(ent[k]), fw = R::w(ent[k]);
-                    atomicAdd(&row[f - col0], w < fw ? w : fw);
+                    const int32_t f = R::key(r[k]), fw = R::w(r[k]);
+                    atomicAdd(&rowc[f], wt < fw ? wt : fw);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < U; k++) {
+                    if (lane + BM_WAVE * k < cnt) {
+                        const int32_t f = R::key(r[k]), fw = R::w(r[k]);
+                        atomicAdd(&rowc[f], wt < fw ? wt : fw);
+                    }
                 }
             }
-            if (j0 < len) {
-                E ent[U];
-#pragma unroll
-                for (int k = 0; k < U; k++) {
-                    const int32_t j = j0 + lane + BM_WAVE * k;
-                    ent[k] = j < len ? frow[j] : R::none();
-                }
-#pragma unroll
-                for (int k = 0; k < U; k++) {
-                    if (j0 + lane + BM_WAVE * k >= len) continue;
-                    const int32_t f = R::key(ent[k]), fw = R::w(ent[k]);
-                    atomicAdd(&row[f - col0], w < fw ? w : fw);
-                }
-            }
-            t = tn;
-            d = dn;
+        };
+        E a[U], b[U];
+        const E* pa;
+        const E* pb;
+        int32_t ca, wa, cb, wb;
+        next_chunk(pa, ca, wa);
+        load_chunk(a, pa, ca);
+        while (ca > 0) {
+            next_chunk(pb, cb, wb);
+            load_chunk(b, pb, cb);
+            add_chunk(a, ca, wa);
+            if (cb == 0) break;
+            next_chunk(pa, ca, wa);
+            load_chunk(a, pa, ca);
+            add_chunk(b, cb, wb);
         }
         __syncthreads();
         int32_t* out = G.G + (int64_t)u * G.stride + col0;
@@ -654,8 +739,8 @@ __global__ void k_moves_from_perm3(const int32_t* __restrict__ pi_before, int32_
 /* ------------------------------------------------------------------ */
 
 int32_t bm_gain_chunk_cols(int64_t stride) {
-    /* at most 20,384 columns (81,536 bytes of int32: two blocks per CU in
-     * 160 KiB of LDS), split evenly into multiples of 32 */
+    /* at most 20,384 columns (81,540 bytes of int32 with the edge counter:
+     * two blocks per CU in 160 KiB of LDS), split evenly into multiples of 32 */
     const int64_t cap = 20384;
     const int64_t chunks = (stride + cap - 1) / cap;
     const int64_t w = ((stride + chunks - 1) / chunks + 31) & ~(int64_t)31;
@@ -665,8 +750,9 @@ int32_t bm_gain_chunk_cols(int64_t stride) {
 template <bool PACKED, int U>
 static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
     const int32_t chunk = bm_gain_chunk_cols(G.stride);
-    const size_t lds = (size_t)chunk * sizeof(int32_t);
-    const void* kern = (const void*)k_gain_build<PACKED, U>;
+    const bool windowed = chunk < G.stride;
+    const size_t lds = ((size_t)chunk + 1) * sizeof(int32_t);   /* row window + edge counter */
+    const void* kern = windowed ? (const void*)k_gain_build<PACKED, U, true> : (const void*)k_gain_build<PACKED, U, false>;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int occ = 0;
@@ -683,8 +769,12 @@ static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_
     for (int64_t c0 = 0; c0 < G.stride; c0 += chunk) {
         const int32_t nc = (int32_t)(G.stride - c0 < chunk ? G.stride - c0 : chunk);
         hipLaunchKernelGGL(k_gain_edge_desc<PACKED>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc);
-        hipLaunchKernelGGL((k_gain_build<PACKED, U>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g, G,
-                           (int32_t)c0, nc);
+        if (windowed)
+            hipLaunchKernelGGL((k_gain_build<PACKED, U, true>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
+                               G, (int32_t)c0, nc);
+        else
+            hipLaunchKernelGGL((k_gain_build<PACKED, U, false>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
+                               G, (int32_t)c0, nc);
     }
     return hipGetLastError();
 }
