@@ -47,6 +47,7 @@ struct bm_ctx {
     int64_t* d_f_in_ptr;
     int2* d_f_in;
     int32_t compact;
+    int32_t row_layout;     /* BM_ROWS_P19 when female weights are in 0..8191, else BM_ROWS_P15 (compact graphs) */
     int32_t* d_pi;
     int64_t* d_small;       /* 16 int64 of scratch for scores / flags */
     int32_t* d_reorder;     /* n ints for the changed-first enumerations */
@@ -262,6 +263,7 @@ static BmGain devgain(const bm_ctx* c) {
     G.f_self = c->d_f_self;
     G.f_out_p = c->d_f_out_p;
     G.f_in_p = c->d_f_in_p;
+    G.row_layout = c->d_f_out_p ? c->row_layout : BM_ROWS_INT2;
     G.edesc = c->d_edesc;
     G.build_unroll = c->tune[BM_KERNEL_GAIN_BUILD].unroll;
     return G;
@@ -1124,8 +1126,11 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         int r2 = dalloc(c, &c->d_f_out_p, (size_t)c->female_m);
         if (r2 == BM_OK) r2 = dalloc(c, &c->d_f_in_p, (size_t)c->female_m);
         if (r2 != BM_OK) return r2;
-        HIPCHK(c, bm_launch_pack_rows(c->d_f_out, c->female_m, c->d_f_out_p, c->stream));
-        HIPCHK(c, bm_launch_pack_rows(c->d_f_in, c->female_m, c->d_f_in_p, c->stream));
+        /* compact graphs have female weights in 0..32767; the narrower weight
+         * field leaves the column's LDS byte address in the low bits */
+        c->row_layout = c->max_abs_fw <= 8191 ? BM_ROWS_P19 : BM_ROWS_P15;
+        HIPCHK(c, bm_launch_pack_rows(c->d_f_out, c->female_m, c->row_layout, c->d_f_out_p, c->stream));
+        HIPCHK(c, bm_launch_pack_rows(c->d_f_in, c->female_m, c->row_layout, c->d_f_in_p, c->stream));
     }
     /* one spare entry: the build's look-ahead reads entry base(u) of a row with no edges */
     int rc = dalloc(c, &c->d_edesc, (size_t)(2 * c->male.m + 1));

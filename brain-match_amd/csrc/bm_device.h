@@ -39,8 +39,9 @@ struct BmGain {
     const int2* m_out_tab;
     const int32_t* m_self;      /* male self-loop weight per vertex or BM_ABSENT */
     const int32_t* f_self;      /* female self-loop weight per vertex or BM_ABSENT */
-    const uint32_t* f_out_p;    /* female rows packed (f << 15) | w when the graph is compact, else NULL */
+    const uint32_t* f_out_p;    /* female rows packed in row_layout when the graph is compact, else NULL */
     const uint32_t* f_in_p;
+    int32_t row_layout;         /* BM_ROWS_*: layout of the rows the table build reads */
     int4* edesc;                /* build work list: one entry per male edge slot (2 x male edges + 1) */
     int32_t build_unroll;       /* build: female-row entries per lane in flight (8, 16 or 32) */
 };
@@ -51,8 +52,14 @@ struct BmGain {
  * columns of G are zero. */
 int64_t bm_gain_stride(int32_t n);
 
-/* Packs {f, w} rows (f < 2^17, 0 <= w <= 32767) into (f << 15) | w words. */
-hipError_t bm_launch_pack_rows(const int2* in, int64_t m, uint32_t* out, hipStream_t s);
+/* Female-row layouts read by the table build: {f, w} int2 rows; packed
+ * (f << 15) | w (ids < 2^17, weights 0..32767); packed (w << 19) | 4 f (ids
+ * < 2^17, weights 0..8191). */
+#define BM_ROWS_INT2 0
+#define BM_ROWS_P15 1
+#define BM_ROWS_P19 2
+/* Packs {f, w} rows into one of the packed layouts. */
+hipError_t bm_launch_pack_rows(const int2* in, int64_t m, int layout, uint32_t* out, hipStream_t s);
 /* Column-window width of the table build for a row stride. */
 int32_t bm_gain_chunk_cols(int64_t stride);
 

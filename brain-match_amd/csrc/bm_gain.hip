@@ -143,37 +143,56 @@ __device__ __forceinline__ void wave_flat(bool have, int32_t w, int32_t y, const
     wave_sync();
 }
 
-/* Female-row entry {f, w} in either layout: int2, or the packed word
- * (f << 15) | w used when ids are < 2^17 and weights in 0..32767.  load()
- * reads entry i of a chunk through a buffer resource whose record count is
- * the chunk's length in bytes: reads past the chunk return zero instead of
- * touching memory, and every lane's loads share one offset register. */
+/* Female-row entry {f, w} in one of three layouts: int2; the packed word
+ * (f << 15) | w (ids < 2^17, weights in 0..32767); or (w << 19) | 4 f (ids
+ * < 2^17, weights in 0..8191), whose low bits are already the byte address
+ * of column f in an LDS row at address 0.  addr() is that byte address,
+ * load() reads entry i of a chunk through a buffer resource whose record
+ * count is the chunk's length in bytes: reads past the chunk return zero
+ * instead of touching memory, and every lane's loads share one offset
+ * register. */
 typedef int V2i __attribute__((ext_vector_type(2)));
 constexpr int32_t kBufWord3 = 0x00020000;   /* gfx9 raw-buffer resource word 3 */
 
-template <bool PACKED>
+template <int LAYOUT>
 struct Row;
 template <>
-struct Row<false> {
+struct Row<BM_ROWS_INT2> {
     typedef int2 T;
     static __device__ __forceinline__ int32_t key(T e) { return e.x; }
+    static __device__ __forceinline__ uint32_t addr(T e) { return (uint32_t)e.x << 2; }
     static __device__ __forceinline__ int32_t w(T e) { return e.y; }
-    static __device__ __forceinline__ T none() { return make_int2(0, 0); }
     static __device__ __forceinline__ T load(__amdgpu_buffer_rsrc_t r, int32_t i) {
         const V2i v = __builtin_amdgcn_raw_buffer_load_b64(r, i * 8, 0, 0);
         return make_int2(v.x, v.y);
     }
 };
 template <>
-struct Row<true> {
+struct Row<BM_ROWS_P15> {
     typedef uint32_t T;
     static __device__ __forceinline__ int32_t key(T e) { return (int32_t)(e >> 15); }
+    static __device__ __forceinline__ uint32_t addr(T e) { return (e >> 13) & 0x7fffcu; }
     static __device__ __forceinline__ int32_t w(T e) { return (int32_t)(e & 0x7fffu); }
-    static __device__ __forceinline__ T none() { return 0u; }
     static __device__ __forceinline__ T load(__amdgpu_buffer_rsrc_t r, int32_t i) {
         return (T)__builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0);
     }
 };
+template <>
+struct Row<BM_ROWS_P19> {
+    typedef uint32_t T;
+    static __device__ __forceinline__ int32_t key(T e) { return (int32_t)((e & 0x7ffffu) >> 2); }
+    static __device__ __forceinline__ uint32_t addr(T e) { return e & 0x7ffffu; }
+    static __device__ __forceinline__ int32_t w(T e) { return (int32_t)(e >> 19); }
+    static __device__ __forceinline__ T load(__amdgpu_buffer_rsrc_t r, int32_t i) {
+        return (T)__builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0);
+    }
+};
+
+/* LDS integer add at a byte address (the build's row window sits at LDS
+ * address 0: the kernel has no static LDS). */
+__device__ __forceinline__ void lds_add(uint32_t byte_addr, int32_t v) {
+    asm volatile("ds_add_u32 %0, %1" ::"v"(byte_addr), "v"(v) : "memory");
+}
 
 /* Work list of the build, one entry per male edge slot of row u (out-edges,
  * then in-edges, at base(u) = m_out_ptr[u] + m_in_ptr[u]): the female row the
@@ -183,9 +202,10 @@ struct Row<true> {
  * unmatched neighbours.
  * One wave per male vertex, one lane per edge, so the dependent loads (edge,
  * pi of the neighbour, row bounds, window search) of all edges overlap. */
-template <bool PACKED>
+template <int LAYOUT>
 __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, int32_t col0, int32_t ncols) {
-    typedef Row<PACKED> R;
+    typedef Row<LAYOUT> R;
+    constexpr bool PACKED = LAYOUT != BM_ROWS_INT2;
     typedef typename R::T E;
     const int lane = threadIdx.x & (BM_WAVE - 1);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / BM_WAVE;
@@ -249,9 +269,10 @@ __device__ __forceinline__ int4 load_entry(const int4* p) {
 #endif
 }
 
-template <bool PACKED, int U, bool WINDOWED>
+template <int LAYOUT, int U, bool WINDOWED>
 __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGain G, int32_t col0_arg, int32_t ncols) {
-    typedef Row<PACKED> R;
+    typedef Row<LAYOUT> R;
+    constexpr bool PACKED = LAYOUT != BM_ROWS_INT2;
     typedef typename R::T E;
     /* the row window sits at LDS address 0 (the edge counter after it), and
      * a single window starts at column 0, so a column's LDS address is just
@@ -332,24 +353,23 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
 #pragma unroll
             for (int k = 0; k < U; k++) r[k] = R::load(rs, lane + BM_WAVE * k);
         };
-        /* rowc[f] is column f of the window (the shift is folded into the
-         * LDS base); a full chunk needs no lane mask, the last chunk of an
-         * edge masks the lanes past its end (their zero entries must not
-         * add min(w, 0) for a negative w) */
-        int32_t* const rowc = row - col0;
+        /* column f of the window is at LDS byte 4 (f - col0); a full chunk
+         * needs no lane mask, the last chunk of an edge masks the lanes past
+         * its end (their zero entries must not add min(w, 0) for a negative w) */
+        const uint32_t shift = 4u * (uint32_t)col0;
         auto add_chunk = [&](const E (&r)[U], int32_t cnt, int32_t wt) {
             if (cnt == kStep) {
 #pragma unroll
                 for (int k = 0; k < U; k++) {
-                    const int32_t f = R::key(r[k]), fw = R::w(r[k]);
-                    atomicAdd(&rowc[f], wt < fw ? wt : fw);
+                    const int32_t fw = R::w(r[k]);
+                    lds_add(WINDOWED ? R::addr(r[k]) - shift : R::addr(r[k]), wt < fw ? wt : fw);
                 }
             } else {
 #pragma unroll
                 for (int k = 0; k < U; k++) {
                     if (lane + BM_WAVE * k < cnt) {
-                        const int32_t f = R::key(r[k]), fw = R::w(r[k]);
-                        atomicAdd(&rowc[f], wt < fw ? wt : fw);
+                        const int32_t fw = R::w(r[k]);
+                        lds_add(WINDOWED ? R::addr(r[k]) - shift : R::addr(r[k]), wt < fw ? wt : fw);
                     }
                 }
             }
@@ -369,6 +389,9 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
             load_chunk(a, pa, ca);
             add_chunk(b, cb, wb);
         }
+        /* the compiler does not count the inline-asm LDS adds: wait for them
+         * before the barrier that publishes the row */
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         __syncthreads();
         int32_t* out = G.G + (int64_t)u * G.stride + col0;
         for (int32_t c = tid; c < ncols; c += kBuildThreads) out[c] = row[c];
@@ -376,10 +399,11 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
     }
 }
 
-__global__ void k_pack_rows(const int2* __restrict__ in, int64_t m, uint32_t* __restrict__ out) {
+__global__ void k_pack_rows(const int2* __restrict__ in, int64_t m, int layout, uint32_t* __restrict__ out) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
-        out[i] = ((uint32_t)in[i].x << 15) | (uint32_t)in[i].y;
+        out[i] = layout == BM_ROWS_P19 ? ((uint32_t)in[i].y << 19) | ((uint32_t)in[i].x << 2)
+                                       : ((uint32_t)in[i].x << 15) | (uint32_t)in[i].y;
 }
 
 /* Patch the rows of the neighbours of each moved vertex x (pi(x): old -> new).
@@ -530,8 +554,19 @@ __global__ void k_gain_pis(BmDevGraph g, BmGain G, int4* __restrict__ pis) {
  * weight (w_M(u, v), w_M(v, u), w_F(f1, f2), w_F(f2, f1)) is one direct load;
  * self-loops add their exact terms. */
 constexpr int kUmThreads = 1024;
-constexpr int kUmStage = 4;    /* 16-byte row words per thread and row in flight while staging */
+constexpr int kUmStage = 5;    /* 16-byte row words per thread and row in flight while staging (5 x 1024 x 4
+                                  * covers a row of up to 20,480 entries in one round) */
 constexpr int kUmPairs = 4;    /* pairs per thread in flight */
+
+typedef unsigned V4u __attribute__((ext_vector_type(4)));
+
+/* Neighbour id of entry i of an {id, w} list of len entries, 0 past its end
+ * (a bounded buffer load: nothing is read there). */
+__device__ __forceinline__ int32_t list_id(const int2* base, int64_t len, int32_t i) {
+    const int32_t bytes = len < (int64_t)(INT32_MAX / 8) ? (int32_t)len * 8 : (INT32_MAX / 8) * 8;
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<int2*>(base), 0, bytes, kBufWord3);
+    return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, i * 8, 0, 0);
+}
 
 /* Rank of x in a bitmap (uint32 words) with prefix counts per 64-bit word. */
 __device__ __forceinline__ int32_t bit_rank(const uint32_t* bm, const uint16_t* pre, int32_t x) {
@@ -567,47 +602,59 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
         const int4 pu = pis[u];
         const int32_t f1 = pu.x;
         const int32_t F11 = pu.w;                           /* w_F(f1, f1) */
-        const int4* __restrict__ Gu = reinterpret_cast<const int4*>(G.G + (int64_t)u * stride);
-        const int4* __restrict__ Tu = reinterpret_cast<const int4*>(GT + (int64_t)f1 * stride);
-        int4* sG = reinterpret_cast<int4*>(rowG);
-        int4* sT = reinterpret_cast<int4*>(rowT);
-        for (int32_t c0 = tid; c0 < s4; c0 += kUmThreads * kUmStage) {
-            int4 a[kUmStage], t[kUmStage];
-#pragma unroll
-            for (int j = 0; j < kUmStage; j++) {
-                const int32_t c = c0 + kUmThreads * j;
-                a[j] = c < s4 ? Gu[c] : make_int4(0, 0, 0, 0);
-                t[j] = c < s4 ? Tu[c] : make_int4(0, 0, 0, 0);
-            }
-#pragma unroll
-            for (int j = 0; j < kUmStage; j++) {
-                const int32_t c = c0 + kUmThreads * j;
-                if (c < s4) {
-                    sG[c] = a[j];
-                    sT[c] = t[j];
-                }
-            }
-        }
+        /* Staging.  The first kUmThreads entries of the four adjacency lists and
+         * both rows are requested together as bounded buffer loads (zero past
+         * the end), lists first, so the bitmap marks wait only on the list
+         * loads while the rows are still arriving. */
         for (int32_t w = tid; w < 8 * w64; w += kUmThreads) bits[w] = 0;
         __syncthreads();
-        const int64_t mob = g.m_out_ptr[u], mib = g.m_in_ptr[u];
-        for (int64_t e = mob + tid; e < g.m_out_ptr[u + 1]; e += kUmThreads) {
+        const int64_t mob = g.m_out_ptr[u], moe = g.m_out_ptr[u + 1];
+        const int64_t mib = g.m_in_ptr[u], mie = g.m_in_ptr[u + 1];
+        const int64_t fob = f1 ? g.f_out_ptr[f1] : 0, foe = f1 ? g.f_out_ptr[f1 + 1] : 0;
+        const int64_t fib = f1 ? g.f_in_ptr[f1] : 0, fie = f1 ? g.f_in_ptr[f1 + 1] : 0;
+        const int32_t x_mo = list_id(g.m_out + mob, moe - mob, tid);
+        const int32_t x_mi = list_id(g.m_in + mib, mie - mib, tid);
+        const int32_t x_fo = list_id(g.f_out + fob, foe - fob, tid);
+        const int32_t x_fi = list_id(g.f_in + fib, fie - fib, tid);
+        const __amdgpu_buffer_rsrc_t rG =
+            __builtin_amdgcn_make_buffer_rsrc(G.G + (int64_t)u * stride, 0, (int32_t)(stride * 4), kBufWord3);
+        const __amdgpu_buffer_rsrc_t rT =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(GT) + (int64_t)f1 * stride, 0, (int32_t)(stride * 4),
+                                              kBufWord3);
+        V4u a[kUmStage], t[kUmStage];
+#pragma unroll
+        for (int j = 0; j < kUmStage; j++) {
+            a[j] = __builtin_amdgcn_raw_buffer_load_b128(rG, (tid + kUmThreads * j) * 16, 0, 0);
+            t[j] = __builtin_amdgcn_raw_buffer_load_b128(rT, (tid + kUmThreads * j) * 16, 0, 0);
+        }
+        if (x_mo) atomicOr(&mo[x_mo >> 5], 1u << (x_mo & 31));   /* ids are >= 1: 0 marks nothing */
+        if (x_mi) atomicOr(&mi[x_mi >> 5], 1u << (x_mi & 31));
+        if (x_fo) atomicOr(&fo[x_fo >> 5], 1u << (x_fo & 31));
+        if (x_fi) atomicOr(&fi[x_fi >> 5], 1u << (x_fi & 31));
+        for (int64_t e = mob + tid + kUmThreads; e < moe; e += kUmThreads) {   /* lists longer than the block */
             const int32_t x = g.m_out[e].x;
             atomicOr(&mo[x >> 5], 1u << (x & 31));
         }
-        for (int64_t e = mib + tid; e < g.m_in_ptr[u + 1]; e += kUmThreads) {
+        for (int64_t e = mib + tid + kUmThreads; e < mie; e += kUmThreads) {
             const int32_t x = g.m_in[e].x;
             atomicOr(&mi[x >> 5], 1u << (x & 31));
         }
-        const int64_t fob = f1 ? g.f_out_ptr[f1] : 0, fib = f1 ? g.f_in_ptr[f1] : 0;
-        if (f1) {
-            for (int64_t e = fob + tid; e < g.f_out_ptr[f1 + 1]; e += kUmThreads) {
-                const int32_t y = g.f_out[e].x;
-                atomicOr(&fo[y >> 5], 1u << (y & 31));
-            }
-            for (int64_t e = fib + tid; e < g.f_in_ptr[f1 + 1]; e += kUmThreads) {
-                const int32_t y = g.f_in[e].x;
-                atomicOr(&fi[y >> 5], 1u << (y & 31));
+        for (int64_t e = fob + tid + kUmThreads; e < foe; e += kUmThreads) {
+            const int32_t y = g.f_out[e].x;
+            atomicOr(&fo[y >> 5], 1u << (y & 31));
+        }
+        for (int64_t e = fib + tid + kUmThreads; e < fie; e += kUmThreads) {
+            const int32_t y = g.f_in[e].x;
+            atomicOr(&fi[y >> 5], 1u << (y & 31));
+        }
+        V4u* sG = reinterpret_cast<V4u*>(rowG);
+        V4u* sT = reinterpret_cast<V4u*>(rowT);
+#pragma unroll
+        for (int j = 0; j < kUmStage; j++) {
+            const int32_t c = tid + kUmThreads * j;
+            if (c < s4) {
+                sG[c] = a[j];
+                sT[c] = t[j];
             }
         }
         __syncthreads();
@@ -747,12 +794,13 @@ int32_t bm_gain_chunk_cols(int64_t stride) {
     return (int32_t)(w < stride ? w : stride);
 }
 
-template <bool PACKED, int U>
+template <int LAYOUT, int U>
 static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
     const int32_t chunk = bm_gain_chunk_cols(G.stride);
     const bool windowed = chunk < G.stride;
     const size_t lds = ((size_t)chunk + 1) * sizeof(int32_t);   /* row window + edge counter */
-    const void* kern = windowed ? (const void*)k_gain_build<PACKED, U, true> : (const void*)k_gain_build<PACKED, U, false>;
+    const void* kern =
+        windowed ? (const void*)k_gain_build<LAYOUT, U, true> : (const void*)k_gain_build<LAYOUT, U, false>;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int occ = 0;
@@ -768,27 +816,33 @@ static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_
     if (dblocks < 1) dblocks = 1;
     for (int64_t c0 = 0; c0 < G.stride; c0 += chunk) {
         const int32_t nc = (int32_t)(G.stride - c0 < chunk ? G.stride - c0 : chunk);
-        hipLaunchKernelGGL(k_gain_edge_desc<PACKED>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc);
+        hipLaunchKernelGGL(k_gain_edge_desc<LAYOUT>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc);
         if (windowed)
-            hipLaunchKernelGGL((k_gain_build<PACKED, U, true>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
+            hipLaunchKernelGGL((k_gain_build<LAYOUT, U, true>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
                                G, (int32_t)c0, nc);
         else
-            hipLaunchKernelGGL((k_gain_build<PACKED, U, false>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
+            hipLaunchKernelGGL((k_gain_build<LAYOUT, U, false>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
                                G, (int32_t)c0, nc);
     }
     return hipGetLastError();
 }
 
+template <int U>
+static hipError_t launch_build_layout(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
+    switch (G.row_layout) {
+    case BM_ROWS_P19: return launch_build_rows<BM_ROWS_P19, U>(g, G, num_cus, s);
+    case BM_ROWS_P15: return launch_build_rows<BM_ROWS_P15, U>(g, G, num_cus, s);
+    default: return launch_build_rows<BM_ROWS_INT2, U>(g, G, num_cus, s);
+    }
+}
+
 hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
     if (!G.edesc) return hipErrorInvalidValue;
-    const bool packed = G.f_out_p != nullptr;
+    if (G.row_layout != BM_ROWS_INT2 && !G.f_out_p) return hipErrorInvalidValue;
     switch (G.build_unroll) {
-    case 16:
-        return packed ? launch_build_rows<true, 16>(g, G, num_cus, s) : launch_build_rows<false, 16>(g, G, num_cus, s);
-    case 32:
-        return packed ? launch_build_rows<true, 32>(g, G, num_cus, s) : launch_build_rows<false, 32>(g, G, num_cus, s);
-    default:
-        return packed ? launch_build_rows<true, 8>(g, G, num_cus, s) : launch_build_rows<false, 8>(g, G, num_cus, s);
+    case 16: return launch_build_layout<16>(g, G, num_cus, s);
+    case 32: return launch_build_layout<32>(g, G, num_cus, s);
+    default: return launch_build_layout<8>(g, G, num_cus, s);
     }
 }
 
@@ -880,10 +934,11 @@ hipError_t bm_launch_moves_from_perm3(const int32_t* pi_before, int32_t a, int32
     return hipGetLastError();
 }
 
-hipError_t bm_launch_pack_rows(const int2* in, int64_t m, uint32_t* out, hipStream_t s) {
+hipError_t bm_launch_pack_rows(const int2* in, int64_t m, int layout, uint32_t* out, hipStream_t s) {
     if (m <= 0) return hipSuccess;
+    if (layout != BM_ROWS_P15 && layout != BM_ROWS_P19) return hipErrorInvalidValue;
     int64_t blocks = (m + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)blocks), dim3(256), 0, s, in, m, out);
+    hipLaunchKernelGGL(k_pack_rows, dim3((unsigned)blocks), dim3(256), 0, s, in, m, layout, out);
     return hipGetLastError();
 }
