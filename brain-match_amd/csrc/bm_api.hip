@@ -1249,8 +1249,8 @@ extern "C" int bm_set_tuning(bm_ctx* c, int32_t kernel, const bm_tuning* t) {
     if (t->group_lanes != 64) return fail(c, BM_ERR_ARG, "bm_set_tuning: group_lanes other than 64 not built");
     const int cus = c->num_cus > 0 ? c->num_cus : 256;
     if (kernel == BM_KERNEL_GAIN_BUILD) {
-        if (t->unroll != 8 && t->unroll != 16 && t->unroll != 32)
-            return fail(c, BM_ERR_ARG, "bm_set_tuning: table-build unroll must be 8, 16 or 32");
+        if (t->unroll != 4 && t->unroll != 8 && t->unroll != 16 && t->unroll != 32)
+            return fail(c, BM_ERR_ARG, "bm_set_tuning: table-build unroll must be 4, 8, 16 or 32");
     } else if (t->unroll != 2 && t->unroll != 4 && t->unroll != 8) {
         return fail(c, BM_ERR_ARG, "bm_set_tuning: unroll must be 2, 4 or 8");
     }

@@ -840,6 +840,7 @@ hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t nu
     if (!G.edesc) return hipErrorInvalidValue;
     if (G.row_layout != BM_ROWS_INT2 && !G.f_out_p) return hipErrorInvalidValue;
     switch (G.build_unroll) {
+    case 4: return launch_build_layout<4>(g, G, num_cus, s);
     case 16: return launch_build_layout<16>(g, G, num_cus, s);
     case 32: return launch_build_layout<32>(g, G, num_cus, s);
     default: return launch_build_layout<8>(g, G, num_cus, s);

@@ -343,7 +343,7 @@ typedef struct bm_tuning {
     int32_t blocks_per_cu;
     int32_t group_lanes;  /* lanes cooperating on one candidate (64) */
     int32_t unroll;       /* list entries per lane in flight: 2, 4 or 8 (swap kernel);
-                             female-row entries per lane, 8, 16 or 32 (table build,
+                             female-row entries per lane, 4, 8, 16 or 32 (table build,
                              which uses only this field) */
 } bm_tuning;
 int bm_get_tuning(const bm_ctx* ctx, int32_t kernel, bm_tuning* t);
