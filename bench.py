@@ -422,8 +422,8 @@ def bench_anneal(args, bm):
     ann = bm.Anneal(ctxs, chains_per_instance=args.chains_per_instance, seed=5, t0=50.0, decay=0.9999, p_rot3=0.1,
                     traj_cap=args.anneal)
     best, trials = ann.autotune(steps=200)
-    log(f"autotuner: {len(trials)} shapes, best {best} ({trials[:, 3].min():.2f} ms for 200 steps; "
-        f"slowest {trials[:, 3].max():.2f} ms)")
+    log(f"autotuner: {len(trials)} shapes, best {best} ({trials[:, 4].min():.2f} ms for 200 steps; "
+        f"slowest {trials[:, 4].max():.2f} ms)")
     init = ann.state()["score"].copy()
     ann.run(steps, 1)                      # capture + first launch (warm-up, part of the chains' history)
     n0, ms0 = ann.timing()

@@ -661,16 +661,16 @@ class Anneal:
         self._chk(_lib.bm_anneal_get_tuning(self.h, C.byref(t)))
         return {k: getattr(t, k) for k, _ in _Tuning._fields_}
 
-    def set_tuning(self, waves_per_block, lds_slots, blocks_per_cu):
-        t = _Tuning(waves_per_block, lds_slots, blocks_per_cu, 64, 4)
+    def set_tuning(self, waves_per_block, lds_slots, blocks_per_cu, unroll=4):
+        t = _Tuning(waves_per_block, lds_slots, blocks_per_cu, 64, unroll)
         self._chk(_lib.bm_anneal_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, steps=100):
         best = _Tuning()
-        trials = np.zeros((64, 4), np.float64)
+        trials = np.zeros((96, 5), np.float64)   # waves, slots, blocks_per_cu, unroll, ms
         nt = C.c_int32(0)
         self._chk(_lib.bm_anneal_autotune(self.h, steps, C.byref(best), trials.ctypes.data_as(C.POINTER(C.c_double)),
-                                          64, C.byref(nt)))
+                                          96, C.byref(nt)))
         return {k: getattr(best, k) for k, _ in _Tuning._fields_}, trials[:nt.value].copy()
 
     def schedule(self):

@@ -415,7 +415,7 @@ extern "C" int bm_anneal_get_tuning(const bm_anneal* a, bm_tuning* t) {
     t->lds_slots = a->L.lds_slots;
     t->blocks_per_cu = a->L.blocks_per_cu;
     t->group_lanes = 64;
-    t->unroll = 4;
+    t->unroll = a->L.unroll == 2 ? 2 : 4;
     return BM_OK;
 }
 
@@ -424,13 +424,15 @@ extern "C" int bm_anneal_set_tuning(bm_anneal* a, const bm_tuning* t) {
     const int w = t->waves_per_block;
     const size_t sb = a->compact ? 4 : 8;
     if ((w != 1 && w != 2 && w != 4 && w != 8) || t->lds_slots < 4 || (t->lds_slots & 3) ||
-        (size_t)w * (size_t)t->lds_slots * sb > 160 * 1024 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64)
+        (size_t)w * (size_t)t->lds_slots * sb > 160 * 1024 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
+        (t->unroll != 2 && t->unroll != 4))
         return afail(a, BM_ERR_ARG, "bm_anneal_set_tuning: waves 1/2/4/8, lds_slots a multiple of 4 within 160 KiB, "
-                                    "blocks_per_cu 1..64");
+                                    "blocks_per_cu 1..64, unroll 2 or 4");
     BmLaunch L = a->L;
     L.waves_per_block = w;
     L.lds_slots = t->lds_slots;
     L.blocks_per_cu = t->blocks_per_cu;
+    L.unroll = t->unroll;
     dim3 grid, block;
     size_t lds;
     AHIP(a, hipSetDevice(a->device));
@@ -447,15 +449,16 @@ extern "C" int bm_anneal_set_tuning(bm_anneal* a, const bm_tuning* t) {
 }
 
 /* Times one launch of `steps` steps for every launch shape in a small grid
- * (waves per block x table slots per wave x blocks per CU), keeps the fastest,
- * and resets the chains.  trials (may be NULL) gets up to max_trials records
- * {waves, slots, blocks_per_cu, ms}. */
+ * (list entries per lane x waves per block x table slots per wave x blocks
+ * per CU), keeps the fastest, and resets the chains.  trials (may be NULL)
+ * gets up to max_trials records {waves, slots, blocks_per_cu, unroll, ms}. */
 extern "C" int bm_anneal_autotune(bm_anneal* a, int64_t steps, bm_tuning* best, double* trials, int32_t max_trials,
                                   int32_t* n_trials) {
     if (!a || steps < 1) return afail(a, BM_ERR_ARG, "bm_anneal_autotune: invalid argument");
     const int waves[4] = {1, 2, 4, 8};
     const int slots_c[3] = {640, 1280, 2560}, slots_w[3] = {320, 640, 1280};
     const int bpc[2] = {4, 16};
+    const int unrolls[2] = {2, 4};
     bm_tuning bt;
     memset(&bt, 0, sizeof bt);
     double best_ms = 1e300;
@@ -465,6 +468,7 @@ extern "C" int bm_anneal_autotune(bm_anneal* a, int64_t steps, bm_tuning* best, 
     AHIP(a, hipEventCreate(&e0));
     AHIP(a, hipEventCreate(&e1));
     int rc = BM_OK;
+    for (int ui = 0; ui < 2 && rc == BM_OK; ui++)
     for (int wi = 0; wi < 4 && rc == BM_OK; wi++)
         for (int si = 0; si < 3 && rc == BM_OK; si++)
             for (int bi = 0; bi < 2 && rc == BM_OK; bi++) {
@@ -473,7 +477,7 @@ extern "C" int bm_anneal_autotune(bm_anneal* a, int64_t steps, bm_tuning* best, 
                 t.lds_slots = a->compact ? slots_c[si] : slots_w[si];
                 t.blocks_per_cu = bpc[bi];
                 t.group_lanes = 64;
-                t.unroll = 4;
+                t.unroll = unrolls[ui];
                 if ((size_t)t.waves_per_block * t.lds_slots * (a->compact ? 4 : 8) > 160 * 1024) continue;
                 if ((rc = bm_anneal_set_tuning(a, &t)) != BM_OK) break;
                 if ((rc = reset_chains(a)) != BM_OK) break;
@@ -490,10 +494,11 @@ extern "C" int bm_anneal_autotune(bm_anneal* a, int64_t steps, bm_tuning* best, 
                     break;
                 }
                 if (trials && nt < max_trials) {
-                    trials[4 * nt] = t.waves_per_block;
-                    trials[4 * nt + 1] = t.lds_slots;
-                    trials[4 * nt + 2] = t.blocks_per_cu;
-                    trials[4 * nt + 3] = ms;
+                    trials[5 * nt] = t.waves_per_block;
+                    trials[5 * nt + 1] = t.lds_slots;
+                    trials[5 * nt + 2] = t.blocks_per_cu;
+                    trials[5 * nt + 3] = t.unroll;
+                    trials[5 * nt + 4] = ms;
                 }
                 nt++;
                 if (ms < best_ms) {

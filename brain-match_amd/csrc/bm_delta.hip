@@ -420,16 +420,16 @@ struct Rot3Acc {
  * their contribution under each of the three female images; edges inside
  * the triple (self-loops included) are summed per permutation in the
  * out-phase and skipped in the in-phase. */
-template <bool COMPACT, bool OUTP>
+template <bool COMPACT, bool OUTP, int U3>
 __device__ __forceinline__ void rot3_scan(const int2* __restrict__ adj, const int32_t* __restrict__ pi,
                                           const int64_t* b, const int64_t* len, const typename Slot<COMPACT>::T* lds,
                                           const Tab* T, const int32_t* vtx, const int32_t* f, Rot3Acc& A, int lane) {
     const int64_t L = len[0] + len[1] + len[2];
-    for (int64_t base = 0; base < L; base += (int64_t)BM_WAVE * kU3) {
-        int2 ed[kU3];
-        int pos[kU3];
+    for (int64_t base = 0; base < L; base += (int64_t)BM_WAVE * U3) {
+        int2 ed[U3];
+        int pos[U3];
 #pragma unroll
-        for (int j = 0; j < kU3; j++) {
+        for (int j = 0; j < U3; j++) {
             int64_t i = base + lane + (int64_t)BM_WAVE * j;
             pos[j] = -1;
             ed[j] = make_int2(0, 0);
@@ -439,22 +439,22 @@ __device__ __forceinline__ void rot3_scan(const int2* __restrict__ adj, const in
                 else { pos[j] = 2; ed[j] = adj[b[2] + i - len[0] - len[1]]; }
             }
         }
-        int32_t gx[kU3];
+        int32_t gx[U3];
 #pragma unroll
-        for (int j = 0; j < kU3; j++) gx[j] = pi[ed[j].x];
-        int32_t key[kU3], r0[kU3], r1[kU3], r2[kU3];
-        int jxs[kU3];
+        for (int j = 0; j < U3; j++) gx[j] = pi[ed[j].x];
+        int32_t key[U3], r0[U3], r1[U3], r2[U3];
+        int jxs[U3];
 #pragma unroll
-        for (int j = 0; j < kU3; j++) {
+        for (int j = 0; j < U3; j++) {
             const int32_t x = ed[j].x;
             jxs[j] = x == 0 ? -2 : (x == vtx[0] ? 0 : (x == vtx[1] ? 1 : (x == vtx[2] ? 2 : -1)));
             key[j] = jxs[j] == -1 ? gx[j] : 0;   /* external edges only */
         }
-        look_many<COMPACT, kU3>(lds, T[0], key, r0);
-        look_many<COMPACT, kU3>(lds, T[1], key, r1);
-        look_many<COMPACT, kU3>(lds, T[2], key, r2);
+        look_many<COMPACT, U3>(lds, T[0], key, r0);
+        look_many<COMPACT, U3>(lds, T[1], key, r1);
+        look_many<COMPACT, U3>(lds, T[2], key, r2);
 #pragma unroll
-        for (int j = 0; j < kU3; j++) {
+        for (int j = 0; j < U3; j++) {
             const int32_t w = ed[j].y;
             const int i = pos[j];
             const int jx = jxs[j];
@@ -484,7 +484,7 @@ __device__ __forceinline__ void rot3_scan(const int2* __restrict__ adj, const in
 /* Exact deltas d[p], p = 1..5, of the five non-identity permutations of the
  * images f of the distinct vertices vtx in g.pi, computed by the whole wave;
  * every lane returns them (d[0] = 0). */
-template <bool COMPACT>
+template <bool COMPACT, int U3 = kU3>
 __device__ __forceinline__ void rot3_delta_wave(const BmDevGraph& g, const int32_t* vtx, const int32_t* f,
                                                 typename Slot<COMPACT>::T* lds, uint32_t cap, int lane,
                                                 int64_t* d) {
@@ -500,28 +500,28 @@ __device__ __forceinline__ void rot3_delta_wave(const BmDevGraph& g, const int32
                     make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f[1]),
                     make_tab(g.f_out_desc, g.f_out_tab, g.f_out_ptr, f[2])};
         place<3>(T, cap);
-        build<COMPACT, 3>(lds, T, g.f_out, lane);
+        build<COMPACT, 3, U3>(lds, T, g.f_out, lane);
         int64_t b[3], len[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             b[i] = uni64(g.m_out_ptr[vtx[i]]);
             len[i] = uni64(g.m_out_ptr[vtx[i] + 1]) - b[i];
         }
-        rot3_scan<COMPACT, true>(g.m_out, g.pi, b, len, lds, T, vtx, f, A, lane);
+        rot3_scan<COMPACT, true, U3>(g.m_out, g.pi, b, len, lds, T, vtx, f, A, lane);
     }
     {
         Tab T[3] = {make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[0]),
                     make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[1]),
                     make_tab(g.f_in_desc, g.f_in_tab, g.f_in_ptr, f[2])};
         place<3>(T, cap);
-        build<COMPACT, 3>(lds, T, g.f_in, lane);
+        build<COMPACT, 3, U3>(lds, T, g.f_in, lane);
         int64_t b[3], len[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             b[i] = uni64(g.m_in_ptr[vtx[i]]);
             len[i] = uni64(g.m_in_ptr[vtx[i] + 1]) - b[i];
         }
-        rot3_scan<COMPACT, false>(g.m_in, g.pi, b, len, lds, T, vtx, f, A, lane);
+        rot3_scan<COMPACT, false, U3>(g.m_in, g.pi, b, len, lds, T, vtx, f, A, lane);
     }
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -540,7 +540,7 @@ __device__ __forceinline__ void rot3_delta_wave(const BmDevGraph& g, const int32
     }
 }
 
-template <bool COMPACT>
+template <bool COMPACT, int U3>
 __global__ void __launch_bounds__(512) k_rot3(BmDevGraph g, int4* __restrict__ cand, int64_t k,
                                              int64_t* __restrict__ best, uint8_t* __restrict__ perm,
                                              int write_perm_into_cand, uint32_t cap) {
@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(512) k_rot3(BmDevGraph g, int4* __restrict__ c
         const int32_t vtx[3] = {uni(q.x), uni(q.y), uni(q.z)};
         const int32_t f[3] = {uni(g.pi[vtx[0]]), uni(g.pi[vtx[1]]), uni(g.pi[vtx[2]])};
         int64_t d5[6];
-        rot3_delta_wave<COMPACT>(g, vtx, f, lds, cap, lane, d5);
+        rot3_delta_wave<COMPACT, U3>(g, vtx, f, lds, cap, lane, d5);
         if (lane == 0) {
             int64_t bd = 0;
             int bp = 0;
@@ -671,7 +671,7 @@ __global__ void __launch_bounds__(512) k_anneal(const BmDevGraph* __restrict__ i
 #pragma unroll
                 for (int i = 0; i < 3; i++) f[i] = uni(pi[vtx[i]]);
                 int64_t d5[6];
-                rot3_delta_wave<COMPACT>(g, vtx, f, lds, cap, lane, d5);
+                rot3_delta_wave<COMPACT, U>(g, vtx, f, lds, cap, lane, d5);
                 p = 1 + (int)bm_range32((uint32_t)(rk >> 32), 5u);
                 d = d5[1];
 #pragma unroll
@@ -773,33 +773,40 @@ hipError_t bm_launch_swap(const BmDevGraph& g, const BmLaunch& L, int mode, cons
     return hipGetLastError();
 }
 
-hipError_t bm_launch_rot3(const BmDevGraph& g, const BmLaunch& L, const int4* cand, int64_t k, int64_t* best,
-                          uint8_t* perm, int4* cand_out_perm, hipStream_t s) {
-    if (k <= 0) return hipSuccess;
+template <bool C, int U3>
+static hipError_t launch_rot3_u(const BmDevGraph& g, const BmLaunch& L, const int4* cand, int64_t k, int64_t* best,
+                                uint8_t* perm, int4* cand_out_perm, hipStream_t s) {
     dim3 grid, block;
     size_t lds;
-    hipError_t e;
-    if (g.compact) {
-        e = launch_delta(k_rot3<true>, L, k, 4, s, &grid, &block, &lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rot3<true>, grid, block, lds, s, g, const_cast<int4*>(cand), k, best, perm,
-                           cand_out_perm ? 1 : 0, (uint32_t)L.lds_slots);
-    } else {
-        e = launch_delta(k_rot3<false>, L, k, 8, s, &grid, &block, &lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_rot3<false>, grid, block, lds, s, g, const_cast<int4*>(cand), k, best, perm,
-                           cand_out_perm ? 1 : 0, (uint32_t)L.lds_slots);
-    }
+    const hipError_t e = launch_delta(k_rot3<C, U3>, L, k, C ? 4 : 8, s, &grid, &block, &lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_rot3<C, U3>), grid, block, lds, s, g, const_cast<int4*>(cand), k, best, perm,
+                       cand_out_perm ? 1 : 0, (uint32_t)L.lds_slots);
     return hipGetLastError();
 }
 
+/* 3-vertex kernel: 2 or 4 list entries per lane in flight (L.unroll). */
+hipError_t bm_launch_rot3(const BmDevGraph& g, const BmLaunch& L, const int4* cand, int64_t k, int64_t* best,
+                          uint8_t* perm, int4* cand_out_perm, hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    if (g.compact)
+        return L.unroll == 2 ? launch_rot3_u<true, 2>(g, L, cand, k, best, perm, cand_out_perm, s)
+                             : launch_rot3_u<true, 4>(g, L, cand, k, best, perm, cand_out_perm, s);
+    return L.unroll == 2 ? launch_rot3_u<false, 2>(g, L, cand, k, best, perm, cand_out_perm, s)
+                         : launch_rot3_u<false, 4>(g, L, cand, k, best, perm, cand_out_perm, s);
+}
+
+/* Annealing chains: 2 or 4 list entries per lane in flight (L.unroll). */
 hipError_t bm_anneal_grid(int compact, int rot, const BmLaunch& L, int64_t n_chains, dim3* grid, dim3* block,
                           size_t* lds) {
-    if (compact)
-        return rot ? launch_delta(k_anneal<true, 4, true>, L, n_chains, 4, nullptr, grid, block, lds)
-                   : launch_delta(k_anneal<true, 4, false>, L, n_chains, 4, nullptr, grid, block, lds);
-    return rot ? launch_delta(k_anneal<false, 4, true>, L, n_chains, 8, nullptr, grid, block, lds)
-               : launch_delta(k_anneal<false, 4, false>, L, n_chains, 8, nullptr, grid, block, lds);
+#define BM_ANNEAL_GRID(C, U, R) launch_delta(k_anneal<C, U, R>, L, n_chains, C ? 4 : 8, nullptr, grid, block, lds)
+    if (L.unroll == 2) {
+        if (compact) return rot ? BM_ANNEAL_GRID(true, 2, true) : BM_ANNEAL_GRID(true, 2, false);
+        return rot ? BM_ANNEAL_GRID(false, 2, true) : BM_ANNEAL_GRID(false, 2, false);
+    }
+    if (compact) return rot ? BM_ANNEAL_GRID(true, 4, true) : BM_ANNEAL_GRID(true, 4, false);
+    return rot ? BM_ANNEAL_GRID(false, 4, true) : BM_ANNEAL_GRID(false, 4, false);
+#undef BM_ANNEAL_GRID
 }
 
 hipError_t bm_launch_anneal(const BmDevGraph* insts, int compact, BmChain* chains, int64_t n_chains,
@@ -807,12 +814,21 @@ hipError_t bm_launch_anneal(const BmDevGraph* insts, int compact, BmChain* chain
                             const BmLaunch& L, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
     if (n_chains <= 0 || steps <= 0) return hipSuccess;
     const uint32_t cap = (uint32_t)L.lds_slots;
-#define BM_ANNEAL_LAUNCH(C, R) \
-    hipLaunchKernelGGL((k_anneal<C, 4, R>), grid, block, lds, s, insts, chains, n_chains, pool, steps, sc, traj, cap)
-    if (compact) {
-        if (sc.p_rot3) BM_ANNEAL_LAUNCH(true, true); else BM_ANNEAL_LAUNCH(true, false);
+#define BM_ANNEAL_LAUNCH(C, U, R) \
+    hipLaunchKernelGGL((k_anneal<C, U, R>), grid, block, lds, s, insts, chains, n_chains, pool, steps, sc, traj, cap)
+    const bool rot = sc.p_rot3 != 0;
+    if (L.unroll == 2) {
+        if (compact) {
+            if (rot) BM_ANNEAL_LAUNCH(true, 2, true); else BM_ANNEAL_LAUNCH(true, 2, false);
+        } else {
+            if (rot) BM_ANNEAL_LAUNCH(false, 2, true); else BM_ANNEAL_LAUNCH(false, 2, false);
+        }
     } else {
-        if (sc.p_rot3) BM_ANNEAL_LAUNCH(false, true); else BM_ANNEAL_LAUNCH(false, false);
+        if (compact) {
+            if (rot) BM_ANNEAL_LAUNCH(true, 4, true); else BM_ANNEAL_LAUNCH(true, 4, false);
+        } else {
+            if (rot) BM_ANNEAL_LAUNCH(false, 4, true); else BM_ANNEAL_LAUNCH(false, 4, false);
+        }
     }
 #undef BM_ANNEAL_LAUNCH
     return hipGetLastError();

@@ -342,7 +342,8 @@ typedef struct bm_tuning {
     int32_t lds_slots;
     int32_t blocks_per_cu;
     int32_t group_lanes;  /* lanes cooperating on one candidate (64) */
-    int32_t unroll;       /* list entries per lane in flight: 2, 4 or 8 (swap kernel);
+    int32_t unroll;       /* list entries per lane in flight: 2, 4 or 8 (swap kernel),
+                             2 or 4 (3-vertex kernel, annealing chains);
                              female-row entries per lane, 4, 8, 16 or 32 (table build,
                              which uses only this field) */
 } bm_tuning;
@@ -524,9 +525,10 @@ int bm_anneal_reset(bm_anneal* a);
 int bm_anneal_get_tuning(const bm_anneal* a, bm_tuning* t);
 int bm_anneal_set_tuning(bm_anneal* a, const bm_tuning* t);
 /* Autotuner: times one launch of `steps` steps for each shape of a grid of
- * waves per block {1,2,4,8} x table slots per wave x blocks per CU {4,16},
- * keeps the fastest and resets the chains.  trials (4 doubles per trial:
- * waves, slots, blocks_per_cu, ms; up to max_trials) may be NULL. */
+ * list entries per lane {2,4} x waves per block {1,2,4,8} x table slots per
+ * wave x blocks per CU {4,16}, keeps the fastest and resets the chains.
+ * trials (5 doubles per trial: waves, slots, blocks_per_cu, unroll, ms; up to
+ * max_trials) may be NULL.  bm_anneal_set_tuning takes unroll 2 or 4. */
 int bm_anneal_autotune(bm_anneal* a, int64_t steps, bm_tuning* best, double* trials, int32_t max_trials,
                        int32_t* n_trials);
 /* Last error of an annealing object (or of the last failed create). */

@@ -162,8 +162,9 @@ def test_autotune_then_replay(bm, orc):
     try:
         with bm.Anneal([ctx], chains_per_instance=8, seed=17, t0=10.0, decay=0.999, p_rot3=0.2, traj_cap=2) as ann:
             best, trials = ann.autotune(steps=50)
-            assert len(trials) >= 12 and (trials[:, 3] > 0).all()
+            assert len(trials) >= 24 and (trials[:, 4] > 0).all() and set(trials[:, 3]) == {2.0, 4.0}
             assert ann.tuning()["waves_per_block"] == best["waves_per_block"]
+            assert ann.tuning()["unroll"] == best["unroll"]
             st = ann.state()
             assert (st["steps"] == 0).all() and (st["accepted"] == 0).all()
             ann.run(150, 2)
