@@ -80,6 +80,8 @@ struct bm_ctx {
     uint32_t* d_f_out_p;    /* packed female rows for the table build (compact graphs) */
     uint32_t* d_f_in_p;
     int4* d_edesc;          /* table-build work list, 2 x male edges entries */
+    int2* d_echunks;        /* {u, first slot} chunks of the work list (fixed by the male graph) */
+    int64_t n_echunks;
     int64_t gt_version;
     char err[512];
 };
@@ -245,6 +247,7 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_f_out_p);
     (void)hipFree(c->d_f_in_p);
     (void)hipFree(c->d_edesc);
+    (void)hipFree(c->d_echunks);
     bm_graph_free(&c->male);
     free(c->f_out_deg);
     free(c->f_in_deg);
@@ -265,6 +268,8 @@ static BmGain devgain(const bm_ctx* c) {
     G.f_in_p = c->d_f_in_p;
     G.row_layout = c->d_f_out_p ? c->row_layout : BM_ROWS_INT2;
     G.edesc = c->d_edesc;
+    G.echunks = c->d_echunks;
+    G.n_echunks = c->n_echunks;
     G.build_unroll = c->tune[BM_KERNEL_GAIN_BUILD].unroll;
     return G;
 }
@@ -1138,6 +1143,18 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
     /* one spare entry: the build's look-ahead reads entry base(u) of a row with no edges */
     int rc = dalloc(c, &c->d_edesc, (size_t)(2 * c->male.m + 1));
     if (rc != BM_OK) return rc;
+    if (!c->d_echunks) {
+        std::vector<int2> ch;
+        for (int32_t u = 1; u <= c->n; u++) {
+            const int64_t L = (mg.out.ptr[u + 1] - mg.out.ptr[u]) + (mg.in.ptr[u + 1] - mg.in.ptr[u]);
+            for (int64_t t = 0; t < L; t += BM_DESC_CHUNK) ch.push_back(make_int2(u, (int32_t)t));
+        }
+        c->n_echunks = (int64_t)ch.size();
+        rc = dalloc(c, &c->d_echunks, ch.empty() ? (size_t)1 : ch.size());
+        if (rc == BM_OK && !ch.empty()) rc = h2d(c, c->d_echunks, ch.data(), ch.size());
+        if (rc == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, BM_ERR_HIP, "sync failed");
+        if (rc != BM_OK) return rc;
+    }
     rc = dalloc(c, &c->d_G, bytes / sizeof(int32_t));
     if (rc != BM_OK) return rc;
     /* row 0 is never written by the build; clear it once */

@@ -43,6 +43,8 @@ struct BmGain {
     const uint32_t* f_in_p;
     int32_t row_layout;         /* BM_ROWS_*: layout of the rows the table build reads */
     int4* edesc;                /* build work list: one entry per male edge slot (2 x male edges + 1) */
+    const int2* echunks;        /* {u, first slot} chunks of at most BM_DESC_CHUNK slots covering every list */
+    int64_t n_echunks;
     int32_t build_unroll;       /* build: female-row entries per lane in flight (8, 16 or 32) */
 };
 
@@ -55,6 +57,7 @@ int64_t bm_gain_stride(int32_t n);
 /* Female-row layouts read by the table build: {f, w} int2 rows; packed
  * (f << 15) | w (ids < 2^17, weights 0..32767); packed (w << 19) | 4 f (ids
  * < 2^17, weights 0..8191). */
+#define BM_DESC_CHUNK 256   /* slots of one vertex's list per chunk of the work-list kernel */
 #define BM_ROWS_INT2 0
 #define BM_ROWS_P15 1
 #define BM_ROWS_P19 2

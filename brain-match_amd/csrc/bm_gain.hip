@@ -41,6 +41,7 @@ using namespace bmd;
 namespace {
 
 constexpr int kBuildThreads = 1024;   /* build: 16 waves per row */
+constexpr int kDescChunk = BM_DESC_CHUNK;   /* work-list slots per wave of k_gain_edge_desc */
 constexpr int kPatchThreads = 512;
 constexpr int kPatchWaves = kPatchThreads / BM_WAVE;
 
@@ -200,12 +201,16 @@ __device__ __forceinline__ void lds_add(uint32_t byte_addr, int32_t v) {
  * out-row for an in-edge) clipped to the column window, as {start lo,
  * start hi | in-edge flag << 31, length, w}; length 0 for self-loops and
  * unmatched neighbours.
- * One wave per male vertex, one lane per edge, so the dependent loads (edge,
- * pi of the neighbour, row bounds, window search) of all edges overlap. */
+ * One wave per chunk of at most kDescChunk slots of one vertex's list (the
+ * chunks are fixed by the male graph, so a hub's list is spread over many
+ * waves), kDescChunk / 64 slots per lane with each level of the dependent
+ * loads (edge, pi of the neighbour, row bounds) issued for all of them
+ * together. */
 template <int LAYOUT>
 __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, int32_t col0, int32_t ncols) {
     typedef Row<LAYOUT> R;
     constexpr bool PACKED = LAYOUT != BM_ROWS_INT2;
+    constexpr int kJ = kDescChunk / BM_WAVE;   /* edge slots per lane, their loads in flight together */
     typedef typename R::T E;
     const int lane = threadIdx.x & (BM_WAVE - 1);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / BM_WAVE;
@@ -213,39 +218,54 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
     const bool windowed = !(col0 == 0 && (int64_t)ncols >= G.stride);
     const E* __restrict__ f_in_rows = PACKED ? (const E*)G.f_in_p : (const E*)g.f_in;
     const E* __restrict__ f_out_rows = PACKED ? (const E*)G.f_out_p : (const E*)g.f_out;
-    for (int64_t uu = wave + 1; uu <= g.n; uu += nwaves) {
-        const int32_t u = (int32_t)uu;
+    for (int64_t ci = wave; ci < G.n_echunks; ci += nwaves) {
+        const int2 ch = G.echunks[ci];   /* {u, first slot}: up to kDescChunk slots of u's list */
+        const int32_t u = uni(ch.x);
         const int64_t ob = g.m_out_ptr[u], nout = g.m_out_ptr[u + 1] - ob;
         const int64_t ib = g.m_in_ptr[u], L = nout + (g.m_in_ptr[u + 1] - ib);
+        const int64_t t0 = uni(ch.y), t1 = t0 + kDescChunk < L ? t0 + kDescChunk : L;
         int4* out = G.edesc + ob + ib;
-        for (int64_t t = lane; t < L; t += BM_WAVE) {
+        int2 ed[kJ];
+        int32_t gx[kJ];
+        int64_t rb[kJ], re[kJ];
+#pragma unroll
+        for (int j = 0; j < kJ; j++) {
+            const int64_t t = t0 + lane + BM_WAVE * j;
+            ed[j] = t < t1 ? (t < nout ? g.m_out[ob + t] : g.m_in[ib + (t - nout)]) : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < kJ; j++)   /* self-loops and empty slots are not in the table */
+            gx[j] = (ed[j].x != 0 && ed[j].x != u) ? g.pi[ed[j].x] : 0;
+#pragma unroll
+        for (int j = 0; j < kJ; j++) {
+            const bool outp = t0 + lane + BM_WAVE * j < nout;
+            const int64_t* fptr = outp ? g.f_in_ptr : g.f_out_ptr;
+            rb[j] = gx[j] ? fptr[gx[j]] : 0;
+            re[j] = gx[j] ? fptr[gx[j] + 1] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kJ; j++) {
+            const int64_t t = t0 + lane + BM_WAVE * j;
+            if (t >= t1) continue;
             const bool outp = t < nout;
-            const int2 ed = outp ? g.m_out[ob + t] : g.m_in[ib + (t - nout)];
-            const int32_t gx = ed.x == u ? 0 : g.pi[ed.x];   /* self-loops are not in the table */
             const E* __restrict__ frow = outp ? f_in_rows : f_out_rows;
-            int64_t rb = 0, re = 0;
-            if (gx != 0) {
-                const int64_t* fptr = outp ? g.f_in_ptr : g.f_out_ptr;
-                rb = fptr[gx];
-                re = fptr[gx + 1];
-                if (windowed) {
-                    int64_t a = rb, b = re;
-                    while (a < b) {
-                        const int64_t m = (a + b) >> 1;
-                        if (R::key(frow[m]) < col0) a = m + 1; else b = m;
-                    }
-                    const int64_t lo = a;
-                    b = re;
-                    while (a < b) {
-                        const int64_t m = (a + b) >> 1;
-                        if (R::key(frow[m]) < col0 + ncols) a = m + 1; else b = m;
-                    }
-                    rb = lo;
-                    re = a;
+            if (windowed && gx[j]) {
+                int64_t a = rb[j], b = re[j];
+                while (a < b) {
+                    const int64_t m = (a + b) >> 1;
+                    if (R::key(frow[m]) < col0) a = m + 1; else b = m;
                 }
+                const int64_t lo = a;
+                b = re[j];
+                while (a < b) {
+                    const int64_t m = (a + b) >> 1;
+                    if (R::key(frow[m]) < col0 + ncols) a = m + 1; else b = m;
+                }
+                rb[j] = lo;
+                re[j] = a;
             }
-            const uint32_t hi = (uint32_t)((uint64_t)rb >> 32) | (outp ? 0u : 0x80000000u);
-            out[t] = make_int4((int32_t)(uint32_t)rb, (int32_t)hi, (int32_t)(re - rb), ed.y);
+            const uint32_t hi = (uint32_t)((uint64_t)rb[j] >> 32) | (outp ? 0u : 0x80000000u);
+            out[t] = make_int4((int32_t)(uint32_t)rb[j], (int32_t)hi, (int32_t)(re[j] - rb[j]), ed[j].y);
         }
     }
 }
@@ -811,7 +831,7 @@ static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_
     int64_t blocks = (int64_t)occ * cus;
     if (blocks > g.n) blocks = g.n;
     if (blocks < 1) blocks = 1;
-    int64_t dblocks = (g.n + 3) / 4;               /* 4 waves (vertices) per 256-thread block */
+    int64_t dblocks = (G.n_echunks + 3) / 4;       /* 4 waves (chunks) per 256-thread block */
     if (dblocks > cus * 64) dblocks = cus * 64;
     if (dblocks < 1) dblocks = 1;
     for (int64_t c0 = 0; c0 < G.stride; c0 += chunk) {
