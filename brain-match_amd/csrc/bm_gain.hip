@@ -42,6 +42,7 @@ namespace {
 
 constexpr int kBuildThreads = 1024;   /* build: 16 waves per row */
 constexpr int kDescChunk = BM_DESC_CHUNK;   /* work-list slots per wave of k_gain_edge_desc */
+constexpr int kDescLong = 1024;             /* rows longer than this lead their work-list chunk */
 constexpr int kPatchThreads = 512;
 constexpr int kPatchWaves = kPatchThreads / BM_WAVE;
 
@@ -264,8 +265,36 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
                 rb[j] = lo;
                 re[j] = a;
             }
-            const uint32_t hi = (uint32_t)((uint64_t)rb[j] >> 32) | (outp ? 0u : 0x80000000u);
-            out[t] = make_int4((int32_t)(uint32_t)rb[j], (int32_t)hi, (int32_t)(re[j] - rb[j]), ed[j].y);
+        }
+        /* Within the chunk, rows longer than kDescLong entries are written
+         * first (the build takes entries in list order, so long rows start
+         * early and a male row's last entries are short ones); the order
+         * within each class follows the slots. */
+        uint64_t mlong[kJ], mshort[kJ];
+        int32_t nlong = 0;
+#pragma unroll
+        for (int j = 0; j < kJ; j++) {
+            const bool valid = t0 + lane + BM_WAVE * j < t1;
+            const bool lng = valid && re[j] - rb[j] > kDescLong;
+            mlong[j] = __ballot(lng);
+            mshort[j] = __ballot(valid && !lng);
+            nlong += __popcll(mlong[j]);
+        }
+        const uint64_t below = (1ull << lane) - 1ull;
+        int32_t before_l = 0, before_s = 0;
+#pragma unroll
+        for (int j = 0; j < kJ; j++) {
+            const int64_t t = t0 + lane + BM_WAVE * j;
+            if (t < t1) {
+                const bool lng = (mlong[j] >> lane) & 1ull;
+                const int32_t pos = lng ? before_l + __popcll(mlong[j] & below)
+                                        : nlong + before_s + __popcll(mshort[j] & below);
+                const bool outp = t < nout;
+                const uint32_t hi = (uint32_t)((uint64_t)rb[j] >> 32) | (outp ? 0u : 0x80000000u);
+                out[t0 + pos] = make_int4((int32_t)(uint32_t)rb[j], (int32_t)hi, (int32_t)(re[j] - rb[j]), ed[j].y);
+            }
+            before_l += __popcll(mlong[j]);
+            before_s += __popcll(mshort[j]);
         }
     }
 }
