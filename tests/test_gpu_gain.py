@@ -207,3 +207,29 @@ def test_multi_chunk_build_equals_scan(bm, orc):
         for u, row in zip((1, 11, 12, 40000, 44999), patched):
             assert np.array_equal(ctx.gain_row(u), row), u
         b.close()
+
+
+@pytest.mark.parametrize("wmax", [8191, 8192, 32767, 32768])
+def test_row_layouts_at_weight_boundaries(bm, orc, wmax):
+    """The table build reads female rows packed (w << 19) | 4f when female
+    weights fit 0..8191, (f << 15) | w when they fit 0..32767, and as {f, w}
+    pairs otherwise: rows and u-major deltas match the oracle on both sides of
+    each boundary, with the largest weight present in both graphs."""
+    n = 400
+    male, female, truth, pi0 = bm.synth_pair(n, 10.0, bm.WEIGHTS_GEOMETRIC, 0.3, seed=21)
+    rng = np.random.default_rng(wmax)
+    female.w = rng.integers(1, 300, female.m).astype(np.int32)
+    female.w[rng.choice(female.m, 25, replace=False)] = wmax
+    male.w = rng.integers(1, 300, male.m).astype(np.int32)
+    male.w[rng.choice(male.m, 25, replace=False)] = wmax + 5
+    with bm.Context(n, male, female) as ctx:
+        ctx.set_mapping(pi0)
+        ctx.gain_enable(True)
+        o = orc.Oracle(n, male, female)
+        for u in range(1, n + 1):
+            assert np.array_equal(ctx.gain_row(u), orc.gain_row(o, pi0, u)), (wmax, u)
+        k = 1 << 14
+        b = ctx.batch(k)
+        b.eval_swaps_umajor(3, k)
+        assert np.array_equal(b.deltas(), o.swap_deltas(pi0, orc.umajor_pairs(3, k, n)))
+        b.close()
