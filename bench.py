@@ -134,6 +134,8 @@ def main():
                     help="instead of the swap-delta benchmark: train the learned proposal, then compare the exact "
                          "scores reached in SECONDS by uniform, 2-hop and learned proposals (one JSON line)")
     ap.add_argument("--train-samples", type=int, default=50_000_000)
+    ap.add_argument("--evaluator", choices=["gain", "scan"], default="scan",
+                    help="exact delta strategy of the --proposals / --scale climbs (same deltas either way)")
     ap.add_argument("--anneal", type=int, default=0, metavar="LAUNCHES",
                     help="instead of the swap-delta benchmark: the batched-annealing workload (64 instances, "
                          "n in 1k..20k, ~200 n edges), LAUNCHES replays of 1000 steps per chain (one JSON line)")
@@ -342,11 +344,17 @@ def bench_proposals(args, bm):
     own exact kernel; then three disjoint-swap climbs from the same greedy
     matching run for the same wall clock, each sending 4096 candidates per
     round to the exact delta kernel: uniform pairs, 2-hop pairs (64 anchors x
-    64 targets), or the 4096 best-ranked of 2^20 uniform pairs."""
+    64 targets), or the 4096 best-ranked of 2^20 uniform pairs.  With
+    --evaluator scan (default) the scan kernel evaluates the candidates;
+    with --evaluator gain the placement-gain table does (rebuilt when a climb
+    sets its start matching, patched after every accepted swap).  Both give
+    the same exact deltas."""
     male, female, truth, _ = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3, seed=1)
     pi0 = bm.greedy_match(N_VERTICES, male, female)
     ctx = bm.Context(N_VERTICES, male, female)
     ctx.set_mapping(pi0)
+    if args.evaluator == "gain":
+        ctx.gain_enable(True)
     score0 = ctx.score()
     batch = 1 << 16
     steps = max(1, args.train_samples // batch)
@@ -381,6 +389,7 @@ def bench_proposals(args, bm):
         "dtype": "int64 (exact) / bf16 MLP ranking",
         "data": "synthetic",
         "config": {"workload": "learned proposal on the n=18524 ~4M-edge pair", "n": N_VERTICES,
+                   "evaluator": args.evaluator,
                    "initial_matching": "greedy-match", "initial_score": score0, "per_round": 4096,
                    "learned_pool": 1 << 20, "train_samples": steps * batch, "train_seconds": train_s,
                    "train_mse_first": float(losses[:10].mean()), "train_mse_last": float(losses[-10:].mean())},
@@ -476,8 +485,10 @@ def bench_scale(args, bm):
     the true matching shuffled.  Exact swap-delta throughput of both
     strategies on 2^26 u-major pairs (the gain table is 40 GB here, built in
     column chunks, with per-pair lookups), then, for SECONDS each: a uniform
-    and a learned-proposal climb (4096 exact candidates per round), and
-    annealing chains with swaps and 3-vertex moves (uniform proposals)."""
+    and a learned-proposal climb (4096 exact candidates per round, evaluated
+    from the gain table with --evaluator gain, by the scan kernel with
+    --evaluator scan), and annealing chains with swaps and 3-vertex moves
+    (uniform proposals)."""
     n = 100_000
     t = time.perf_counter()
     male, female, truth, pi0 = bm.synth_pair(n, 200.0, bm.WEIGHTS_LOGNORMAL, 0.4, seed=5)
@@ -507,9 +518,9 @@ def bench_scale(args, bm):
                    "table_GB": (n + 1) ** 2 * 4 / 1e9}
     log(f"scale deltas: scan {out['scan']['deltas_per_s']:.4g}/s, gain {out['gain']['deltas_per_s']:.4g}/s "
         f"(build {out['gain']['build_ms']:.1f} ms, lookups {out['gain']['lookup_ms']:.1f} ms); identical: {same}")
-    ctx.gain_enable(False)
     b.close()
     ctx.timing_enable(False)
+    ctx.gain_enable(args.evaluator == "gain")   # the climbs' exact deltas: table or scan
     mlp = bm.Mlp(ctx, seed=2)
     mlp.refresh_features()
     t = time.perf_counter()
@@ -529,6 +540,7 @@ def bench_scale(args, bm):
                                 "score_at": marks}
         log(f"climb {name}: {score0} -> {res['final_score']} in {res['seconds']:.0f}s")
     mlp.close()
+    ctx.gain_enable(False)
     ctx.set_mapping(pi0)
     ann = bm.Anneal([ctx], chains_per_instance=2048, seed=9, t0=50.0, decay=0.9999, p_rot3=0.25)
     t = time.perf_counter()
@@ -546,7 +558,8 @@ def bench_scale(args, bm):
     ctx.close()
     result = {"metric": f"exact score at {args.scale:g} s on the n=100,000 pair, learned vs uniform proposals",
               "value": out["climb_learned"]["final_score"], "unit": "score", "n_gpus": 1, "higher_is_better": True,
-              "data": "synthetic", "config": {"workload": "n=100000, ~20M edges, 40% shuffled", "seconds": args.scale},
+              "data": "synthetic", "config": {"workload": "n=100000, ~20M edges, 40% shuffled", "seconds": args.scale,
+                                              "evaluator": args.evaluator},
               "results": out}
     print(json.dumps(result), flush=True)
 

@@ -52,6 +52,7 @@ struct bm_ctx {
     int64_t* d_small;       /* 16 int64 of scratch for scores / flags */
     int32_t* d_reorder;     /* n ints for the changed-first enumerations */
     BmLaunch tune[BM_KERNEL_COUNT];
+    int32_t user_tuned[BM_KERNEL_COUNT];   /* bm_set_tuning was called: no batch-size choice */
     int32_t group_lanes[BM_KERNEL_COUNT];
     int timing_on;
     std::vector<KTiming>* pending;
@@ -311,18 +312,40 @@ extern "C" void bm_destroy(bm_ctx* ctx) {
     free_ctx(ctx);
 }
 
+/* Launch shape of the scan kernels for a batch of k candidates.  Unless the
+ * caller set one (bm_set_tuning), a batch too small to fill the device's wave
+ * slots (k < 32,768: at most four candidates per slot) runs in 4-wave blocks
+ * with 8 list entries per lane -- there the slowest candidate's latency sets
+ * the time (a climb round of 4,096 pairs: 1.0 vs 1.15 ms) -- and a larger
+ * one in the default throughput shape. */
+static BmLaunch scan_launch(const bm_ctx* c, int kernel, int64_t k);
+
 static void default_tuning(bm_ctx* c) {
     const int cus = c->num_cus > 0 ? c->num_cus : 256;
     /* slots per wave; 32 waves per CU fit 160 KiB when slots * slot bytes <= 5 KiB */
     const int s = c->compact ? 1280 : 640;
-    /* swap kernel: 8-wave blocks, 2 list entries per lane (occupancy over
-     * per-lane depth: 7.1e7 vs 4.9e7 pairs/s with 4-wave blocks and 8 at
-     * n = 18,524) */
+    /* swap and 3-vertex kernels: the shape for large batches (see
+     * scan_launch for small ones) -- 8-wave blocks, 2 list entries per lane:
+     * occupancy over per-lane depth, 7.1e7 vs 4.9e7 pairs/s (4-wave blocks, 8
+     * entries) at n = 18,524; 148.8 vs 156.1 ms per 2^22 triples */
     c->tune[BM_KERNEL_SWAP] = BmLaunch{8, s, 8, cus, 2};
-    c->tune[BM_KERNEL_ROT3] = BmLaunch{8, 2 * s, 8, cus, 2};   /* 148.8 vs 156.1 ms per 2^22 triples */
+    c->tune[BM_KERNEL_ROT3] = BmLaunch{8, 2 * s, 8, cus, 2};
     c->tune[BM_KERNEL_SCORE] = BmLaunch{4, s, 8, cus, 8};
     c->tune[BM_KERNEL_GAIN_BUILD] = BmLaunch{4, s, 8, cus, 8};   /* only unroll is used */
-    for (int i = 0; i < BM_KERNEL_COUNT; i++) c->group_lanes[i] = 64;
+    for (int i = 0; i < BM_KERNEL_COUNT; i++) {
+        c->group_lanes[i] = 64;
+        c->user_tuned[i] = 0;
+    }
+}
+
+static BmLaunch scan_launch(const bm_ctx* c, int kernel, int64_t k) {
+    BmLaunch L = c->tune[kernel];
+    if (c->user_tuned[kernel] || k >= 32768) return L;
+    L.waves_per_block = 4;
+    L.lds_slots = c->compact ? 1280 : 640;
+    L.blocks_per_cu = 8;
+    L.unroll = kernel == BM_KERNEL_ROT3 ? 4 : 8;
+    return L;
 }
 
 extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm_edges* female, int32_t device) {
@@ -739,7 +762,7 @@ extern "C" int bm_batch_eval_swaps(bm_batch* b) {
                                       b->d_delta, c->stream));
     } else {
         timing_begin(c, BM_KERNEL_SWAP, &t);
-        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_LIST, (const int2*)b->d_cand, nullptr, 0, b->k,
+        HIPCHK(c, bm_launch_swap(devgraph(c), scan_launch(c, BM_KERNEL_SWAP, b->k), BM_CAND_LIST, (const int2*)b->d_cand, nullptr, 0, b->k,
                                  b->d_delta, c->stream));
     }
     timing_end(c, &t);
@@ -759,7 +782,7 @@ extern "C" int bm_batch_eval_swaps_uniform(bm_batch* b, uint64_t seed, int64_t k
                                       c->stream));
     } else {
         timing_begin(c, BM_KERNEL_SWAP, &t);
-        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_UNIFORM, nullptr, nullptr, seed, k, b->d_delta,
+        HIPCHK(c, bm_launch_swap(devgraph(c), scan_launch(c, BM_KERNEL_SWAP, k), BM_CAND_UNIFORM, nullptr, nullptr, seed, k, b->d_delta,
                                  c->stream));
     }
     timing_end(c, &t);
@@ -832,7 +855,7 @@ extern "C" int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k)
         }
     } else {
         timing_begin(c, BM_KERNEL_SWAP, &t);
-        HIPCHK(c, bm_launch_swap(devgraph(c), c->tune[BM_KERNEL_SWAP], BM_CAND_UMAJOR, nullptr, c->d_um_off, seed, k,
+        HIPCHK(c, bm_launch_swap(devgraph(c), scan_launch(c, BM_KERNEL_SWAP, k), BM_CAND_UMAJOR, nullptr, c->d_um_off, seed, k,
                                  b->d_delta, c->stream));
     }
     timing_end(c, &t);
@@ -855,7 +878,7 @@ extern "C" int bm_batch_eval_rot3(bm_batch* b) {
                                       (int4*)b->d_cand, c->stream));
     } else {
         timing_begin(c, BM_KERNEL_ROT3, &t);
-        HIPCHK(c, bm_launch_rot3(devgraph(c), c->tune[BM_KERNEL_ROT3], (const int4*)b->d_cand, b->k, b->d_delta,
+        HIPCHK(c, bm_launch_rot3(devgraph(c), scan_launch(c, BM_KERNEL_ROT3, b->k), (const int4*)b->d_cand, b->k, b->d_delta,
                                  b->d_perm, (int4*)b->d_cand, c->stream));
     }
     timing_end(c, &t);
@@ -1276,6 +1299,7 @@ extern "C" int bm_set_tuning(bm_ctx* c, int32_t kernel, const bm_tuning* t) {
     }
     c->tune[kernel] = BmLaunch{w, t->lds_slots, t->blocks_per_cu, cus, t->unroll};
     c->group_lanes[kernel] = t->group_lanes;
+    c->user_tuned[kernel] = 1;
     return BM_OK;
 }
 

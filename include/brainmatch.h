@@ -347,6 +347,10 @@ typedef struct bm_tuning {
                              female-row entries per lane, 4, 8, 16 or 32 (table build,
                              which uses only this field) */
 } bm_tuning;
+/* Until bm_set_tuning is called for it, the swap (3-vertex) kernel runs a
+ * batch of fewer than 32,768 candidates in 4-wave blocks with 8 (4) list
+ * entries per lane, where one candidate's latency sets the time, and a larger
+ * batch in the shape bm_get_tuning reports. */
 int bm_get_tuning(const bm_ctx* ctx, int32_t kernel, bm_tuning* t);
 int bm_set_tuning(bm_ctx* ctx, int32_t kernel, const bm_tuning* t);
 
