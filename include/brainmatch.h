@@ -260,9 +260,9 @@ int bm_batch_eval_rot3(bm_batch* b);
  * partners of one changed vertex together, src/graph-re-alignment-gpu-swap.cu:163-195).
  * gen_umajor writes the pairs into the batch; eval_swaps_umajor evaluates the
  * same stream without materialising it.  With the gain table enabled and
- * (n+1) <= 20,000, one block per first vertex stages the two table rows it
- * needs in LDS (the table's transpose is kept next to it, 2 x (n+1)^2 int32
- * in all); otherwise each pair is evaluated on its own. */
+ * n <= 18,975, one block per first vertex stages the two table rows it needs
+ * and its adjacency bitmaps in LDS (the table's transpose is kept next to it,
+ * doubling the table's memory); otherwise each pair is evaluated on its own. */
 int bm_batch_gen_umajor(bm_batch* b, uint64_t seed, int64_t k);
 int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k);
 
