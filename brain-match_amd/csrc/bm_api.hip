@@ -82,6 +82,8 @@ struct bm_ctx {
     uint32_t* d_f_in_p;
     int4* d_edesc;          /* table-build work list, 2 x male edges entries */
     int2* d_echunks;        /* {u, first slot} chunks of the work list (fixed by the male graph) */
+    uint32_t* d_m_adj;      /* dense male adjacency bitmap (NULL when it would exceed 2 GiB) */
+    int64_t m_adj_words;
     int64_t n_echunks;
     int64_t gt_version;
     char err[512];
@@ -249,6 +251,7 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_f_in_p);
     (void)hipFree(c->d_edesc);
     (void)hipFree(c->d_echunks);
+    (void)hipFree(c->d_m_adj);
     bm_graph_free(&c->male);
     free(c->f_out_deg);
     free(c->f_in_deg);
@@ -271,6 +274,8 @@ static BmGain devgain(const bm_ctx* c) {
     G.edesc = c->d_edesc;
     G.echunks = c->d_echunks;
     G.n_echunks = c->n_echunks;
+    G.m_adj = c->d_m_adj;
+    G.m_adj_words = c->m_adj_words;
     G.build_unroll = c->tune[BM_KERNEL_GAIN_BUILD].unroll;
     return G;
 }
@@ -1177,6 +1182,19 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         if (rc == BM_OK && !ch.empty()) rc = h2d(c, c->d_echunks, ch.data(), ch.size());
         if (rc == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, BM_ERR_HIP, "sync failed");
         if (rc != BM_OK) return rc;
+    }
+    /* dense male adjacency for the table lookups' edge tests, when it fits
+     * 2 GiB (43 MB at n = 18,524); otherwise they probe the hash tables */
+    if (!c->d_m_adj) {
+        const int64_t words = ((int64_t)c->n + 1 + 31) / 32;
+        const size_t abytes = (size_t)((int64_t)c->n + 1) * (size_t)words * sizeof(uint32_t);
+        if (abytes <= ((size_t)2 << 30) && bytes + abytes + ((size_t)1 << 30) <= free_b) {
+            rc = dalloc(c, &c->d_m_adj, abytes / sizeof(uint32_t));
+            if (rc != BM_OK) return rc;
+            c->m_adj_words = words;
+            HIPCHK(c, hipMemsetAsync(c->d_m_adj, 0, abytes, c->stream));
+            HIPCHK(c, bm_launch_adj_bits(devgraph(c), c->d_m_adj, words, c->num_cus, c->stream));
+        }
     }
     rc = dalloc(c, &c->d_G, bytes / sizeof(int32_t));
     if (rc != BM_OK) return rc;

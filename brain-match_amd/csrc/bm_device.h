@@ -37,6 +37,8 @@ struct BmGain {
     int64_t stride;             /* row stride in int32: n + 1 rounded up to 32 (rows start on 128-byte lines) */
     const uint64_t* m_out_desc; /* male out-row hash tables (adjacency tests) */
     const int2* m_out_tab;
+    const uint32_t* m_adj;      /* dense male adjacency, m_adj_words uint32 per row (n+1 rows), or NULL */
+    int64_t m_adj_words;
     const int32_t* m_self;      /* male self-loop weight per vertex or BM_ABSENT */
     const int32_t* f_self;      /* female self-loop weight per vertex or BM_ABSENT */
     const uint32_t* f_out_p;    /* female rows packed in row_layout when the graph is compact, else NULL */
@@ -61,6 +63,9 @@ int64_t bm_gain_stride(int32_t n);
 #define BM_ROWS_INT2 0
 #define BM_ROWS_P15 1
 #define BM_ROWS_P19 2
+/* Sets bit b of row a of the zeroed dense adjacency bitmap for every male
+ * edge a -> b (words uint32 per row, rows 0..n). */
+hipError_t bm_launch_adj_bits(const BmDevGraph& g, uint32_t* bits, int64_t words, int32_t num_cus, hipStream_t s);
 /* Packs {f, w} rows into one of the packed layouts. */
 hipError_t bm_launch_pack_rows(const int2* in, int64_t m, int layout, uint32_t* out, hipStream_t s);
 /* Column-window width of the table build for a row stride. */

@@ -66,7 +66,10 @@ __device__ __forceinline__ int32_t female_w(const BmDevGraph& g, int32_t a, int3
 }
 
 /* w_M(a, b) or BM_ABSENT. */
+/* w_M(a, b) or BM_ABSENT.  With the dense adjacency bitmap a missing edge
+ * (the common case) costs one load instead of a hash probe sequence. */
 __device__ __forceinline__ int32_t male_w(const BmGain& G, int32_t a, int32_t b) {
+    if (G.m_adj && !((G.m_adj[(int64_t)a * G.m_adj_words + (b >> 5)] >> (b & 31)) & 1u)) return BM_ABSENT;
     const uint64_t d = G.m_out_desc[a];
     return probe_g(G.m_out_tab + (d >> 6), (uint32_t)(d & 63u), b);
 }
@@ -445,6 +448,21 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
         int32_t* out = G.G + (int64_t)u * G.stride + col0;
         for (int32_t c = tid; c < ncols; c += kBuildThreads) out[c] = row[c];
         __syncthreads();
+    }
+}
+
+/* Dense male adjacency: bit b of row a set iff M has the edge a -> b.  One
+ * wave per vertex, lanes over its out-list. */
+__global__ void __launch_bounds__(256) k_adj_bits(BmDevGraph g, uint32_t* __restrict__ bits, int64_t words) {
+    const int lane = threadIdx.x & (BM_WAVE - 1);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / BM_WAVE;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / BM_WAVE;
+    for (int64_t a = wave + 1; a <= g.n; a += nwaves) {
+        uint32_t* row = bits + a * words;
+        for (int64_t e = g.m_out_ptr[a] + lane; e < g.m_out_ptr[a + 1]; e += BM_WAVE) {
+            const int32_t b = g.m_out[e].x;
+            atomicOr(&row[b >> 5], 1u << (b & 31));
+        }
     }
 }
 
@@ -981,6 +999,14 @@ hipError_t bm_launch_moves_from_swap(const int32_t* pi_before, int32_t u, int32_
 hipError_t bm_launch_moves_from_perm3(const int32_t* pi_before, int32_t a, int32_t b, int32_t c, int32_t p,
                                       int4* moves, int64_t* count, hipStream_t s) {
     hipLaunchKernelGGL(k_moves_from_perm3, dim3(1), dim3(1), 0, s, pi_before, a, b, c, p, moves, count);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_adj_bits(const BmDevGraph& g, uint32_t* bits, int64_t words, int32_t num_cus, hipStream_t s) {
+    int64_t blocks = ((int64_t)g.n + 4) / 4;
+    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 32;
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(k_adj_bits, dim3((unsigned)blocks), dim3(256), 0, s, g, bits, words);
     return hipGetLastError();
 }
 
