@@ -21,8 +21,10 @@ identical deltas:
            for adjacent pairs (default headline);
   scan  -- each pair scans its four male lists against LDS-staged female rows
            (one wavefront per pair).
-The 3-vertex permutation kernels are timed on 2^26 uniform triples in the
-same run and reported under "rot3".
+The 3-vertex permutation kernels are timed on 2^26 uniform triples per step
+in the same run (the a-major stream: triples grouped by first vertex, same law
+as independent triples, see bm_amajor_partners in bm_rng.h) and reported
+under "rot3".
 
 value = candidates / second over the K timed steps of the headline strategy
 (graph, matching and output buffers resident in HBM; the table build is inside
@@ -565,29 +567,47 @@ def bench_scale(args, bm):
 
 
 def bench_rot3(ctx, bm, args):
+    """3-vertex permutations on the a-major uniform triple stream (triples
+    grouped by first vertex, the law of independent uniform triples), T per
+    step.  gain: the staged table kernel (one block per first vertex, two
+    table reads from HBM per triple); scan: the stream materialised and
+    evaluated by the scan kernel.  value = triples per second of whole steps
+    (stream offsets included); the two strategies' results are compared on
+    one stream."""
     T = args.triples
     b3 = ctx.batch(T)
-    out = {"triples_per_step": T, "unit": "triples/s (5 permutations each)"}
+    n = ctx.n
+    mo, _ = ctx.degrees(female=False)
+    out = {"triples_per_step": T, "stream": "amajor", "unit": "triples/s (5 permutations each)"}
+    heads = {}
     for mode in ("gain", "scan"):
         ctx.gain_enable(mode == "gain")
-        b3.gen_triples(3000, T)
-        b3.eval_rot3()
+        b3.eval_rot3_amajor(3000, T)
         kid = bm.KERNEL_GAIN_ROT3 if mode == "gain" else bm.KERNEL_ROT3
         steps = args.steps if mode == "gain" else max(1, args.scan_steps)
-
-        def step(i):
-            b3.gen_triples(4000 + i, T)
-            b3.eval_rot3()
-
-        wall, kt = timed(ctx, (kid,), step, steps)
+        wall, kt = timed(ctx, (kid,), lambda i: b3.eval_rot3_amajor(4000 + i, T), steps)
         l3 = per_launch_ms(kt, kid) / 1000.0
-        ab3, d3 = b3.algo_bytes(generated=False)
         if mode == "gain":
-            ab3 = T * (12 + 36 + 3 * 8 + 6 * 8 + 12 + 9)   # triple, 9 table entries, 3 self words, 6 adjacency probes, out
-        out[mode] = {"value": T / l3, "launch_ms": 1000 * l3,
+            # compulsory: two staged rows per first vertex, the per-vertex words,
+            # the male lists (for a's bitmaps) and the stream offsets once; per
+            # triple two table entries, two dense adjacency words, the best
+            # delta (8 B) and permutation (1 B)
+            ab3 = (n * 8 * gain_stride(n) + (n + 1) * 16 + 16 * int(mo.sum()) + (n + 2) * 8
+                   + T * (2 * 4 + 2 * 4 + 9))
+        else:
+            ab3, _ = b3.algo_bytes(generated=False)
+        out[mode] = {"value": T * steps / wall, "ms_per_step": 1000 * wall / steps, "launch_ms": 1000 * l3,
                      "roofline": {"bound": "hbm", "achieved": ab3 / l3 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": ab3 / l3 / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": ab3}}
-        log(f"rot3 {mode}: {T} triples in {1000 * l3:.1f} ms/launch -> {T / l3:.4g} triples/s")
+        b3.eval_rot3_amajor(99, T)
+        heads[mode] = (b3.deltas(0, 1 << 16), b3.perms(0, 1 << 16))
+        log(f"rot3 {mode}: {T} a-major triples in {1000 * wall / steps:.1f} ms/step ({1000 * l3:.1f} ms kernel) "
+            f"-> {T * steps / wall:.4g} triples/s")
+    same = bool((heads["gain"][0] == heads["scan"][0]).all() and (heads["gain"][1] == heads["scan"][1]).all())
+    out["gain_matches_scan"] = same
+    log(f"rot3 gain and scan results identical on the first 65536 triples of one stream: {same}")
+    if not same:
+        raise SystemExit("the two exact 3-vertex strategies disagree")
     ctx.gain_enable(False)
     b3.close()
     return out

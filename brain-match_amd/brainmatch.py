@@ -142,6 +142,8 @@ _sig("bm_hillclimb", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int64, C.c_int32, 
 _sig("bm_gain_enable", C.c_int, [_c, C.c_int])
 _sig("bm_gain_enabled", C.c_int, [_c])
 _sig("bm_gain_rebuild", C.c_int, [_c])
+_sig("bm_batch_gen_amajor", C.c_int, [_c, C.c_uint64, C.c_int64])
+_sig("bm_batch_eval_rot3_amajor", C.c_int, [_c, C.c_uint64, C.c_int64])
 _sig("bm_gain_table_bytes", C.c_int64, [C.c_int32])
 _sig("bm_gain_row", C.c_int, [_c, C.c_int32, i32p])
 _sig("bm_twohop_set", C.c_int, [_c, C.c_int32, i32p, C.POINTER(C.c_int32)])
@@ -311,6 +313,12 @@ class Batch:
 
     def eval_swaps_umajor(self, seed, k):
         self._chk(_lib.bm_batch_eval_swaps_umajor(self.h, seed, k))
+
+    def gen_amajor(self, seed, k):
+        self._chk(_lib.bm_batch_gen_amajor(self.h, seed, k))
+
+    def eval_rot3_amajor(self, seed, k):
+        self._chk(_lib.bm_batch_eval_rot3_amajor(self.h, seed, k))
 
     def eval_rot3(self):
         self._chk(_lib.bm_batch_eval_rot3(self.h))

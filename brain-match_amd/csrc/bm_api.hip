@@ -812,6 +812,23 @@ static int umajor_offsets(bm_ctx* c, uint64_t seed, int64_t k) {
     return BM_OK;
 }
 
+/* The transposed table and the per-vertex words for the staged stream
+ * lookups: GT is rebuilt only when the table changed since. */
+static int gain_transposed(bm_ctx* c) {
+    if (c->gt_version == c->gain_version) return BM_OK;
+    if (!c->d_GT) {
+        int rc = dalloc(c, &c->d_GT, (size_t)bm_gain_table_bytes(c->n) / sizeof(int32_t));
+        if (rc == BM_OK && !c->d_pis) rc = dalloc(c, &c->d_pis, (size_t)c->n + 1);
+        if (rc != BM_OK) return rc;
+    }
+    KTiming t;
+    timing_begin(c, BM_KERNEL_AUX, &t);
+    HIPCHK(c, bm_launch_gain_transpose(devgain(c), c->n + 1, c->d_GT, c->num_cus, c->stream));
+    timing_end(c, &t);
+    c->gt_version = c->gain_version;
+    return BM_OK;
+}
+
 extern "C" int bm_batch_gen_umajor(bm_batch* b, uint64_t seed, int64_t k) {
     if (!b) return BM_ERR_ARG;
     bm_ctx* c = b->ctx;
@@ -837,17 +854,7 @@ extern "C" int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k)
     KTiming t;
     if (c->gain_on) {
         const bool staged = bm_gain_umajor_lds(c->n) <= 160 * 1024;
-        if (staged && c->gt_version != c->gain_version) {
-            if (!c->d_GT) {
-                rc = dalloc(c, &c->d_GT, (size_t)bm_gain_table_bytes(c->n) / sizeof(int32_t));
-                if (rc == BM_OK && !c->d_pis) rc = dalloc(c, &c->d_pis, (size_t)c->n + 1);
-                if (rc != BM_OK) return rc;
-            }
-            timing_begin(c, BM_KERNEL_AUX, &t);
-            HIPCHK(c, bm_launch_gain_transpose(devgain(c), c->n + 1, c->d_GT, c->num_cus, c->stream));
-            timing_end(c, &t);
-            c->gt_version = c->gain_version;
-        }
+        if (staged && (rc = gain_transposed(c)) != BM_OK) return rc;
         timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
         if (staged) {
             /* pi and the diagonal G(v, pi(v)) are cheap: refreshed every call */
@@ -891,6 +898,49 @@ extern "C" int bm_batch_eval_rot3(bm_batch* b) {
     return BM_OK;
 }
 
+extern "C" int bm_batch_gen_amajor(bm_batch* b, uint64_t seed, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (k < 0 || k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_gen_amajor: k exceeds capacity");
+    if (c->n < 3) return fail(c, BM_ERR_ARG, "bm_batch_gen_amajor: triples need n >= 3");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = umajor_offsets(c, bm_amajor_seed(seed), k);
+    if (rc != BM_OK) return rc;
+    HIPCHK(c, bm_launch_gen_amajor(c->n, seed, c->d_um_off, k, (int4*)b->d_cand, c->stream));
+    b->k = k;
+    b->kind = 2;
+    b->generated = 1;
+    b->evaluated = 0;
+    return BM_OK;
+}
+
+extern "C" int bm_batch_eval_rot3_amajor(bm_batch* b, uint64_t seed, int64_t k) {
+    if (!b) return BM_ERR_ARG;
+    bm_ctx* c = b->ctx;
+    if (k < 0 || k > b->cap) return fail(c, BM_ERR_ARG, "bm_batch_eval_rot3_amajor: k exceeds capacity");
+    if (c->n < 3) return fail(c, BM_ERR_ARG, "bm_batch_eval_rot3_amajor: triples need n >= 3");
+    if (!c->gain_on || bm_gain_rot3_amajor_lds(c->n) > 160 * 1024) {
+        /* no staged rows: materialise the stream and evaluate it as a list */
+        int rc = bm_batch_gen_amajor(b, seed, k);
+        return rc != BM_OK ? rc : bm_batch_eval_rot3(b);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = umajor_offsets(c, bm_amajor_seed(seed), k);
+    if (rc == BM_OK) rc = gain_transposed(c);
+    if (rc != BM_OK) return rc;
+    KTiming t;
+    timing_begin(c, BM_KERNEL_GAIN_ROT3, &t);
+    HIPCHK(c, bm_launch_gain_pis(devgraph(c), devgain(c), c->d_pis, c->stream));
+    HIPCHK(c, bm_launch_gain_rot3_amajor(devgraph(c), devgain(c), c->d_GT, c->d_pis, c->d_um_off, seed, b->d_delta,
+                                         b->d_perm, c->num_cus, c->stream));
+    timing_end(c, &t);
+    b->k = k;
+    b->kind = 4; /* best deltas and permutations only: triples implicit in (seed, index) */
+    b->generated = 1;
+    b->evaluated = 1;
+    return BM_OK;
+}
+
 extern "C" int bm_batch_get(bm_batch* b, int64_t first, int64_t count, int64_t* delta, uint8_t* perm, int32_t* cand) {
     if (!b) return BM_ERR_ARG;
     bm_ctx* c = b->ctx;
@@ -902,7 +952,7 @@ extern "C" int bm_batch_get(bm_batch* b, int64_t first, int64_t count, int64_t* 
         HIPCHK(c, hipMemcpyAsync(delta, b->d_delta + first, (size_t)count * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
     }
     if (perm) {
-        if (b->kind != 2) return fail(c, BM_ERR_STATE, "bm_batch_get: perm only for triple batches");
+        if (b->kind != 2 && b->kind != 4) return fail(c, BM_ERR_STATE, "bm_batch_get: perm only for triple batches");
         HIPCHK(c, hipMemcpyAsync(perm, b->d_perm + first, (size_t)count, hipMemcpyDeviceToHost, c->stream));
     }
     if (cand) {
@@ -910,8 +960,8 @@ extern "C" int bm_batch_get(bm_batch* b, int64_t first, int64_t count, int64_t* 
             HIPCHK(c, hipMemcpyAsync(cand, (int2*)b->d_cand + first, (size_t)count * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
         } else if (b->kind == 2) {
             HIPCHK(c, hipMemcpyAsync(cand, (int4*)b->d_cand + first, (size_t)count * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
-        } else if (b->kind == 3) {
-            return fail(c, BM_ERR_STATE, "bm_batch_get: uniform-evaluated batches do not store candidates");
+        } else if (b->kind == 3 || b->kind == 4) {
+            return fail(c, BM_ERR_STATE, "bm_batch_get: stream-evaluated batches do not store candidates");
         }
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -83,6 +83,12 @@ hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, c
 size_t bm_gain_umajor_lds(int32_t n);
 hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t nrows, int32_t* GT, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_pis(const BmDevGraph& g, const BmGain& G, int4* pis, hipStream_t s);
+/* a-major 3-vertex lookups (best delta and permutation per stream entry);
+ * needs bm_gain_rot3_amajor_lds(n) bytes of LDS. */
+size_t bm_gain_rot3_amajor_lds(int32_t n);
+hipError_t bm_launch_gain_rot3_amajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
+                                      const int64_t* off, uint64_t seed, int64_t* best, uint8_t* perm, int32_t num_cus,
+                                      hipStream_t s);
 hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
                                       const int64_t* off, uint64_t seed, int64_t* delta, int32_t num_cus,
                                       hipStream_t s);
@@ -112,6 +118,9 @@ hipError_t bm_launch_swap(const BmDevGraph& g, const BmLaunch& L, int mode, cons
 hipError_t bm_launch_umajor_offsets(int32_t n, uint64_t seed, int64_t k, int32_t* counts, int64_t* off,
                                     int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gen_umajor(int32_t n, uint64_t seed, const int64_t* off, int64_t k, int2* cand, hipStream_t s);
+/* a-major triple stream entries 0..k-1 as {a, b, c, 0} (off from the
+ * first-vertex draws of bm_amajor_seed(seed)). */
+hipError_t bm_launch_gen_amajor(int32_t n, uint64_t seed, const int64_t* off, int64_t k, int4* cand, hipStream_t s);
 hipError_t bm_launch_rot3(const BmDevGraph& g, const BmLaunch& L, const int4* cand, int64_t k,
                           int64_t* best, uint8_t* perm, int4* cand_out_perm, hipStream_t s);
 hipError_t bm_launch_score(const BmDevGraph& g, const BmLaunch& L, int64_t* acc, hipStream_t s);

@@ -783,6 +783,34 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
     }
 }
 
+/* Best of the five permutations of a triple from its nine gains Gv[i][j] =
+ * G(vtx_i, f_j), the male self-loops s and female self-loops fs of f_0..f_2
+ * (BM_ABSENT when missing), and the male weights w[i][j] of edges inside the
+ * triple (an edge correction probes F only where an edge exists).  bd = the
+ * largest positive delta (0 if none), bp its permutation (first p on ties). */
+__device__ __forceinline__ void rot3_from_gains(const BmDevGraph& g, const int32_t* f, const int64_t (*Gv)[3],
+                                                const int32_t (*w)[3], const int32_t* s, const int32_t* fs, int64_t& bd,
+                                                int& bp) {
+    bd = 0;
+    bp = 0;
+#pragma unroll
+    for (int p = 1; p <= 5; p++) {
+        int32_t ng[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) ng[i] = f[perm_of(p, i)];
+        int64_t d = 0;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            d += sel3(perm_of(p, i), Gv[i][0], Gv[i][1], Gv[i][2]) - Gv[i][i];
+            if (s[i] != BM_ABSENT) d += contrib(s[i], fs[perm_of(p, i)]) - (int64_t)contrib(s[i], fs[i]);
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                if (i != j) d += edge_corr(g, w[i][j], f[i], f[j], ng[i], ng[j]);
+        }
+        if (d > bd) { bd = d; bp = p; }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_gain_rot3(BmDevGraph g, BmGain G, int4* __restrict__ cand, int64_t k,
                                                    int64_t* __restrict__ best, uint8_t* __restrict__ perm,
                                                    int write_perm_into_cand) {
@@ -799,34 +827,133 @@ __global__ void __launch_bounds__(256) k_gain_rot3(BmDevGraph g, BmGain G, int4*
             for (int j = 0; j < 3; j++) Gv[i][j] = Gr[f[j]];
         }
         int32_t w[3][3];    /* w[i][j] = w_M(vtx[i], vtx[j]) for i != j */
-        int32_t s[3];
+        int32_t s[3], fs[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             s[i] = G.m_self[vtx[i]];
+            fs[i] = G.f_self[f[i]];
 #pragma unroll
             for (int j = 0; j < 3; j++) w[i][j] = i == j ? BM_ABSENT : male_w(G, vtx[i], vtx[j]);
         }
-        int64_t bd = 0;
-        int bp = 0;
-#pragma unroll
-        for (int p = 1; p <= 5; p++) {
-            int32_t ng[3];
-#pragma unroll
-            for (int i = 0; i < 3; i++) ng[i] = f[perm_of(p, i)];
-            int64_t d = 0;
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                d += sel3(perm_of(p, i), Gv[i][0], Gv[i][1], Gv[i][2]) - Gv[i][i];
-                if (s[i] != BM_ABSENT) d += contrib(s[i], G.f_self[ng[i]]) - (int64_t)contrib(s[i], G.f_self[f[i]]);
-#pragma unroll
-                for (int j = 0; j < 3; j++)
-                    if (i != j) d += edge_corr(g, w[i][j], f[i], f[j], ng[i], ng[j]);
-            }
-            if (d > bd) { bd = d; bp = p; }
-        }
+        int64_t bd;
+        int bp;
+        rot3_from_gains(g, f, Gv, w, s, fs, bd, bp);
         best[c] = bd;
         if (perm) perm[c] = (uint8_t)bp;
         if (write_perm_into_cand) cand[c].w = bp;
+    }
+}
+
+/* a-major 3-vertex lookups: one block per first vertex a of the stream.  The
+ * block stages rows G(a, .) and GT(pi(a), .) in LDS as the u-major pair
+ * kernel does, and a's male out- and in-neighbour bitmaps.  A triple (a, b, c)
+ * then reads G(a, f_b), G(a, f_c), G(b, f_a), G(c, f_a) from LDS, the diagonal
+ * terms and self-loops from the gathered per-vertex words of b and c, and
+ * only G(b, f_c) and G(c, f_b) from the table in HBM; male edges between a
+ * and b or c are tested in the bitmaps, between b and c in the dense
+ * adjacency bitmap, and a present edge's weight and the female weights its
+ * correction needs come from the hash tables (rare). */
+constexpr int kR3Triples = 2;   /* triples per thread in flight */
+
+__global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, BmGain G, const int32_t* __restrict__ GT,
+                                                                 const int4* __restrict__ pis,
+                                                                 const int64_t* __restrict__ off, uint64_t seed,
+                                                                 int64_t* __restrict__ best,
+                                                                 uint8_t* __restrict__ perm) {
+    extern __shared__ int4 sm4[];
+    const int32_t n = g.n;
+    const int64_t stride = G.stride;                   /* multiple of 32 */
+    const int32_t s4 = (int32_t)(stride >> 2);
+    const int32_t w32 = (n + 1 + 31) >> 5;
+    int32_t* rowG = reinterpret_cast<int32_t*>(sm4);
+    int32_t* rowT = rowG + stride;
+    uint32_t* mo = reinterpret_cast<uint32_t*>(rowT + stride);   /* a -> v */
+    uint32_t* mi = mo + w32;                                     /* v -> a */
+    const int tid = threadIdx.x;
+    for (int32_t a = blockIdx.x + 1; a <= n; a += gridDim.x) {
+        const int64_t i0 = off[a], i1 = off[a + 1];
+        if (i0 == i1) continue;   /* block-uniform */
+        const int4 pa = pis[a];
+        const int32_t fa = pa.x;
+        for (int32_t w = tid; w < 2 * w32; w += kUmThreads) mo[w] = 0;
+        __syncthreads();
+        const int64_t mob = g.m_out_ptr[a], moe = g.m_out_ptr[a + 1];
+        const int64_t mib = g.m_in_ptr[a], mie = g.m_in_ptr[a + 1];
+        const int32_t x_mo = list_id(g.m_out + mob, moe - mob, tid);
+        const int32_t x_mi = list_id(g.m_in + mib, mie - mib, tid);
+        const __amdgpu_buffer_rsrc_t rG =
+            __builtin_amdgcn_make_buffer_rsrc(G.G + (int64_t)a * stride, 0, (int32_t)(stride * 4), kBufWord3);
+        const __amdgpu_buffer_rsrc_t rT =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(GT) + (int64_t)fa * stride, 0, (int32_t)(stride * 4),
+                                              kBufWord3);
+        V4u ra[kUmStage], rt[kUmStage];
+#pragma unroll
+        for (int j = 0; j < kUmStage; j++) {
+            ra[j] = __builtin_amdgcn_raw_buffer_load_b128(rG, (tid + kUmThreads * j) * 16, 0, 0);
+            rt[j] = __builtin_amdgcn_raw_buffer_load_b128(rT, (tid + kUmThreads * j) * 16, 0, 0);
+        }
+        if (x_mo) atomicOr(&mo[x_mo >> 5], 1u << (x_mo & 31));
+        if (x_mi) atomicOr(&mi[x_mi >> 5], 1u << (x_mi & 31));
+        for (int64_t e = mob + tid + kUmThreads; e < moe; e += kUmThreads) {
+            const int32_t x = g.m_out[e].x;
+            atomicOr(&mo[x >> 5], 1u << (x & 31));
+        }
+        for (int64_t e = mib + tid + kUmThreads; e < mie; e += kUmThreads) {
+            const int32_t x = g.m_in[e].x;
+            atomicOr(&mi[x >> 5], 1u << (x & 31));
+        }
+        V4u* sG = reinterpret_cast<V4u*>(rowG);
+        V4u* sT = reinterpret_cast<V4u*>(rowT);
+#pragma unroll
+        for (int j = 0; j < kUmStage; j++) {
+            const int32_t c = tid + kUmThreads * j;
+            if (c < s4) {
+                sG[c] = ra[j];
+                sT[c] = rt[j];
+            }
+        }
+        __syncthreads();
+        for (int64_t ib = i0 + tid; ib < i1; ib += (int64_t)kUmThreads * kR3Triples) {
+            int32_t vb[kR3Triples], vc[kR3Triples];
+            int4 pb[kR3Triples], pc[kR3Triples];
+            int32_t gbc[kR3Triples], gcb[kR3Triples];
+#pragma unroll
+            for (int j = 0; j < kR3Triples; j++) {   /* all gathers in flight before any use */
+                const int64_t i = ib + (int64_t)kUmThreads * j;
+                vb[j] = vc[j] = 0;
+                if (i < i1) bm_amajor_partners(seed, (uint64_t)i, a, n, &vb[j], &vc[j]);
+                pb[j] = i < i1 ? pis[vb[j]] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
+                pc[j] = i < i1 ? pis[vc[j]] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
+            }
+#pragma unroll
+            for (int j = 0; j < kR3Triples; j++) {
+                gbc[j] = G.G[(int64_t)vb[j] * stride + pc[j].x];   /* G(b, f_c) */
+                gcb[j] = G.G[(int64_t)vc[j] * stride + pb[j].x];   /* G(c, f_b) */
+            }
+#pragma unroll
+            for (int j = 0; j < kR3Triples; j++) {
+                const int64_t i = ib + (int64_t)kUmThreads * j;
+                if (i >= i1) continue;
+                const int32_t b = vb[j], c = vc[j];
+                const int32_t f[3] = {fa, pb[j].x, pc[j].x};
+                const int64_t Gv[3][3] = {{pa.y, rowG[f[1]], rowG[f[2]]},
+                                          {rowT[b], pb[j].y, gbc[j]},
+                                          {rowT[c], gcb[j], pc[j].y}};
+                const bool ab = (mo[b >> 5] >> (b & 31)) & 1u, ba = (mi[b >> 5] >> (b & 31)) & 1u;
+                const bool ac = (mo[c >> 5] >> (c & 31)) & 1u, ca = (mi[c >> 5] >> (c & 31)) & 1u;
+                const int32_t w[3][3] = {{BM_ABSENT, ab ? male_w(G, a, b) : BM_ABSENT, ac ? male_w(G, a, c) : BM_ABSENT},
+                                         {ba ? male_w(G, b, a) : BM_ABSENT, BM_ABSENT, male_w(G, b, c)},
+                                         {ca ? male_w(G, c, a) : BM_ABSENT, male_w(G, c, b), BM_ABSENT}};
+                const int32_t sv[3] = {pa.z, pb[j].z, pc[j].z};
+                const int32_t fs[3] = {pa.w, pb[j].w, pc[j].w};
+                int64_t bd;
+                int bp;
+                rot3_from_gains(g, f, Gv, w, sv, fs, bd, bp);
+                best[i] = bd;
+                perm[i] = (uint8_t)bp;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -941,6 +1068,10 @@ hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, c
 
 int64_t bm_gain_stride(int32_t n) { return (((int64_t)n + 1) + 31) & ~(int64_t)31; }
 
+size_t bm_gain_rot3_amajor_lds(int32_t n) {
+    return 2 * (size_t)bm_gain_stride(n) * 4 + 2 * (size_t)((n + 1 + 31) >> 5) * 4;   /* rows, 2 bitmaps */
+}
+
 size_t bm_gain_umajor_lds(int32_t n) {
     const size_t w64 = ((size_t)n + 1 + 63) >> 6;
     return 2 * (size_t)bm_gain_stride(n) * 4 + 4 * w64 * 8 + 4 * w64 * 2;   /* rows, 4 bitmaps, 4 prefix arrays */
@@ -977,6 +1108,24 @@ hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, cons
     if (blocks > g.n) blocks = g.n;
     hipLaunchKernelGGL(k_gain_swap_umajor, dim3((unsigned)blocks), dim3(kUmThreads), lds, s, g, G, GT, pis, off, seed,
                        delta);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_gain_rot3_amajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
+                                      const int64_t* off, uint64_t seed, int64_t* best, uint8_t* perm, int32_t num_cus,
+                                      hipStream_t s) {
+    const size_t lds = bm_gain_rot3_amajor_lds(g.n);
+    const void* kern = (const void*)k_gain_rot3_amajor;
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int occ = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kUmThreads, lds);
+    if (e != hipSuccess) return e;
+    if (occ < 1) occ = 1;
+    int64_t blocks = (int64_t)occ * (num_cus > 0 ? num_cus : 256);
+    if (blocks > g.n) blocks = g.n;
+    hipLaunchKernelGGL(k_gain_rot3_amajor, dim3((unsigned)blocks), dim3(kUmThreads), lds, s, g, G, GT, pis, off, seed,
+                       best, perm);
     return hipGetLastError();
 }
 

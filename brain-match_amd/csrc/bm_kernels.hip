@@ -470,6 +470,19 @@ __global__ void k_gen_umajor(int32_t n, uint64_t seed, const int64_t* __restrict
     }
 }
 
+/* a-major triples: entry i of the stream (seed) is (a, b, c, 0), a the
+ * owner of i in the offsets of the first-vertex draws. */
+__global__ void k_gen_amajor(int32_t n, uint64_t seed, const int64_t* __restrict__ off, int64_t k,
+                             int4* __restrict__ cand) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride) {
+        const int32_t a = umajor_owner(off, n, i);
+        int32_t b, c;
+        bm_amajor_partners(seed, (uint64_t)i, a, n, &b, &c);
+        cand[i] = make_int4(a, b, c, 0);
+    }
+}
+
 int grid_for(int64_t work, int threads) {
     int64_t b = (work + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -632,6 +645,12 @@ hipError_t bm_launch_umajor_offsets(int32_t n, uint64_t seed, int64_t k, int32_t
         hipLaunchKernelGGL(k_umajor_hist, dim3((unsigned)blocks), dim3(1024), use_lds ? lds : 0, s, n, seed, k, counts,
                            use_lds);
     hipLaunchKernelGGL(k_umajor_offsets, dim3(1), dim3(1024), 0, s, n, counts, off);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_gen_amajor(int32_t n, uint64_t seed, const int64_t* off, int64_t k, int4* cand, hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_amajor, dim3(grid_for(k, 256)), dim3(256), 0, s, n, seed, off, k, cand);
     return hipGetLastError();
 }
 

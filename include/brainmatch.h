@@ -266,6 +266,19 @@ int bm_batch_eval_rot3(bm_batch* b);
 int bm_batch_gen_umajor(bm_batch* b, uint64_t seed, int64_t k);
 int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k);
 
+/* a-major uniform triples (bm_amajor_seed / bm_amajor_partners in
+ * brain-match_amd/csrc/bm_rng.h): k independent uniform ordered triples of
+ * distinct vertices (the law of bm_batch_gen_triples) listed grouped by their
+ * first vertex.  gen_amajor writes them into the batch; eval_rot3_amajor
+ * evaluates the best 3-vertex permutation of each (as bm_batch_eval_rot3) and,
+ * with the gain table enabled and n <= 20,000, without materialising them:
+ * one block per first vertex a stages the rows G(a, .) and GT(pi(a), .), so a
+ * triple reads two table entries from HBM instead of nine.  Deltas and
+ * permutations are read with bm_batch_get (cand is not available after the
+ * staged evaluation). */
+int bm_batch_gen_amajor(bm_batch* b, uint64_t seed, int64_t k);
+int bm_batch_eval_rot3_amajor(bm_batch* b, uint64_t seed, int64_t k);
+
 /* Read back a slice [first, first+count).  Any output pointer may be NULL.
  * cand gets 2 ints per pair batch / 4 ints (a,b,c,perm) per triple batch. */
 int bm_batch_get(bm_batch* b, int64_t first, int64_t count, int64_t* delta,

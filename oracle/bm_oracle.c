@@ -66,6 +66,13 @@
  *                               its law (first vertices = the uniform pair
  *                               stream's multiset, partners uniform) and by
  *                               equality with pairs downloaded from the device.
+ *  or_amajor_triples            the a-major uniform triple stream.  Pinned by
+ *                               its law (first vertices = the u-major first
+ *                               vertices of the derived seed, the ordered
+ *                               partner pairs uniform over the (n-1)(n-2)
+ *                               choices, checked exhaustively at small n) and
+ *                               by equality with triples downloaded from the
+ *                               device.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -1010,6 +1017,38 @@ int or_umajor_pairs(uint64_t seed, int64_t k, int32_t n, int32_t* uv) {
             int32_t b = (int32_t)range32((uint32_t)(r >> 32), (uint32_t)(n - 1)) + 1;
             uv[2 * i] = u;
             uv[2 * i + 1] = b >= u ? b + 1 : b;
+        }
+    free(cnt);
+    return 0;
+}
+
+/* Seed of the first-vertex draws of the a-major triple stream. */
+uint64_t or_amajor_seed(uint64_t seed) { return mix64(seed ^ 0x3C6EF372FE94F82BULL); }
+
+/* a-major uniform triples: the first vertices are those of the u-major
+ * stream of or_amajor_seed(seed), grouped ascending; entry i of vertex a
+ * takes r = rand(mix(seed ^ K'), i): b = 1 + range(lo32 r, n - 1) skipping a,
+ * c = 1 + range(hi32 r, n - 2) skipping a and b in ascending order.  Writes
+ * k triples (3 ints each); returns 0. */
+int or_amajor_triples(uint64_t seed, int64_t k, int32_t n, int32_t* abc) {
+    int64_t* cnt = (int64_t*)calloc((size_t)n + 2, sizeof(int64_t));
+    const uint64_t s1 = or_amajor_seed(seed);
+    for (int64_t c = 0; c < k; c++) cnt[1 + range32((uint32_t)rand_at(s1, (uint64_t)c), (uint32_t)n)]++;
+    const uint64_t q = mix64(seed ^ 0xA54FF53A5F1D36F1ULL);
+    int64_t i = 0;
+    for (int32_t a = 1; a <= n; a++)
+        for (int64_t j = 0; j < cnt[a]; j++, i++) {
+            const uint64_t r = rand_at(q, (uint64_t)i);
+            uint32_t b = range32((uint32_t)r, (uint32_t)(n - 1)) + 1u;
+            if (b >= (uint32_t)a) b++;
+            uint32_t c = range32((uint32_t)(r >> 32), (uint32_t)(n - 2)) + 1u;
+            const uint32_t lo = (uint32_t)a < b ? (uint32_t)a : b;
+            const uint32_t hi = (uint32_t)a < b ? b : (uint32_t)a;
+            if (c >= lo) c++;
+            if (c >= hi) c++;
+            abc[3 * i] = a;
+            abc[3 * i + 1] = (int32_t)b;
+            abc[3 * i + 2] = (int32_t)c;
         }
     free(cnt);
     return 0;

@@ -388,6 +388,22 @@ def umajor_pairs(seed, k, n):
 _sig(_o, "or_umajor_pairs_at", None, [C.c_uint64, C.c_int64, C.c_int32, i64p, C.c_int64, i32p])
 
 
+_sig(_o, "or_amajor_triples", C.c_int, [C.c_uint64, C.c_int64, C.c_int32, i32p])
+_sig(_o, "or_amajor_seed", C.c_uint64, [C.c_uint64])
+
+
+def amajor_triples(seed, k, n):
+    """The a-major uniform triple stream (k triples, grouped by first vertex)."""
+    abc = np.empty((k, 3), np.int32)
+    _o.or_amajor_triples(seed, k, n, _p32(abc))
+    return abc
+
+
+def amajor_seed(seed):
+    """Seed of the a-major stream's first-vertex draws (a u-major stream seed)."""
+    return int(_o.or_amajor_seed(seed))
+
+
 def umajor_pairs_at(seed, k, n, idx):
     """Pairs at entry indices idx of the u-major stream (seed, k)."""
     idx = np.ascontiguousarray(idx, dtype=np.int64)

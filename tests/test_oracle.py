@@ -216,6 +216,31 @@ def test_golden_greedy_match(bm):
     assert np.array_equal(bm.greedy_match(n, male, female), z["ref_matching"])
 
 
+def test_amajor_triple_stream_law(orc):
+    """Grouped by first vertex, three distinct vertices, first vertices the
+    u-major first vertices of the derived seed, and (b, c) uniform over the
+    (n-1)(n-2) ordered choices for a given a (exhaustively at n = 6)."""
+    n, k = 6, 240000
+    abc = orc.amajor_triples(5, k, n)
+    assert (np.diff(abc[:, 0]) >= 0).all()
+    assert ((abc[:, 0] != abc[:, 1]) & (abc[:, 0] != abc[:, 2]) & (abc[:, 1] != abc[:, 2])).all()
+    assert abc.min() >= 1 and abc.max() <= n
+    assert np.array_equal(abc[:, 0], orc.umajor_pairs(orc.amajor_seed(5), k, n)[:, 0])
+    for a in range(1, n + 1):
+        sub = abc[abc[:, 0] == a]
+        h = np.bincount((sub[:, 1] - 1) * n + (sub[:, 2] - 1), minlength=n * n).reshape(n, n)
+        cells = [(b, c) for b in range(n) for c in range(n) if len({a - 1, b, c}) == 3]
+        e = len(sub) / len(cells)
+        assert sum(h[b, c] for b, c in cells) == len(sub)
+        assert max(abs(h[b, c] - e) for b, c in cells) < 6 * np.sqrt(e)
+    # the same law as the uniform triple stream: marginals of a large n agree
+    n2, k2 = 40, 400000
+    t = orc.amajor_triples(6, k2, n2)
+    for col in range(3):
+        cnt = np.bincount(t[:, col], minlength=n2 + 1)[1:]
+        assert np.abs(cnt - k2 / n2).max() < 6 * np.sqrt(k2 / n2)
+
+
 def test_umajor_stream_law(orc):
     """Grouped by first vertex, u != v, first vertices distributed like k
     independent uniform draws, partners uniform over the other vertices."""
