@@ -74,17 +74,96 @@ __device__ __forceinline__ int32_t male_w(const BmGain& G, int32_t a, int32_t b)
     return probe_g(G.m_out_tab + (d >> 6), (uint32_t)(d & 63u), b);
 }
 
-/* c(w, a, b) for a male weight that may be absent. */
-__device__ __forceinline__ int64_t cm(const BmDevGraph& g, int32_t w, int32_t a, int32_t b) {
-    if (w == BM_ABSENT) return 0;
-    return contrib(w, female_w(g, a, b));
+/* False when the dense adjacency bitmap shows no male edge a -> b; true
+ * when it shows one or there is no bitmap. */
+__device__ __forceinline__ bool male_maybe(const BmGain& G, int32_t a, int32_t b) {
+    return !G.m_adj || ((G.m_adj[(int64_t)a * G.m_adj_words + (b >> 5)] >> (b & 31)) & 1u);
 }
 
-/* Exact correction of an edge i -> j (i != j) inside the moved set. */
-__device__ __forceinline__ int64_t edge_corr(const BmDevGraph& g, int32_t w, int32_t fi, int32_t fj, int32_t gi,
-                                             int32_t gj) {
-    if (w == BM_ABSENT) return 0;
-    return cm(g, w, gi, gj) - cm(g, w, gi, fj) - cm(g, w, fi, gj) + cm(g, w, fi, fj);
+/* K hash lookups at once: key[k] in table (T[k], bits[k]) where go[k], else
+ * BM_ABSENT without a load.  Every first slot is loaded before any collision
+ * chain is followed, so the K lookups cost one memory latency unless a key
+ * sits past its home slot. */
+template <int K>
+__device__ __forceinline__ void probe_many(const int2* const* T, const uint32_t* bits, const int32_t* key,
+                                           const bool* go, int32_t* out) {
+    int2 e[K];
+    uint32_t s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const bool on = go[k] && bits[k] != 0u && key[k] != 0;
+        s[k] = on ? slot_hash(key[k], bits[k]) : 0u;
+        e[k] = make_int2(0, 0);
+        if (on) e[k] = T[k][s[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint32_t mask = (1u << bits[k]) - 1u;
+        while (e[k].x != 0 && e[k].x != key[k]) {
+            s[k] = (s[k] + 1u) & mask;
+            e[k] = T[k][s[k]];
+        }
+        out[k] = e[k].x != 0 ? e[k].y : BM_ABSENT;
+    }
+}
+
+/* w_M(v[i], v[j]) for the six ordered pairs of a triple (BM_ABSENT on the
+ * diagonal and where no edge): a pair with have[i][j] false is known absent
+ * and costs nothing, the hash probes of the others go out together. */
+__device__ __forceinline__ void male_w_triple(const BmGain& G, const int32_t* v, const bool (*have)[3],
+                                              int32_t (*w)[3]) {
+    const int2* T[6];
+    uint32_t bits[6];
+    int32_t key[6], out[6];
+    bool go[6];
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const uint64_t d = have[i][(i + 1) % 3] || have[i][(i + 2) % 3] ? G.m_out_desc[v[i]] : 0ull;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if (i == j) continue;
+            T[k] = G.m_out_tab + (d >> 6);
+            bits[k] = (uint32_t)(d & 63u);
+            key[k] = v[j];
+            go[k] = have[i][j];
+            k++;
+        }
+    }
+    probe_many<6>(T, bits, key, go, out);
+    k = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        w[i][i] = BM_ABSENT;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            if (i != j) w[i][j] = out[k++];
+    }
+}
+
+/* w_F(f[x], f[y]) for all nine ordered pairs of three female vertices
+ * (diagonal = female self-loops), probes issued together. */
+__device__ __forceinline__ void female_w_triple(const BmDevGraph& g, const int32_t* f, int32_t (*F)[3]) {
+    const int2* T[9];
+    uint32_t bits[9];
+    int32_t key[9], out[9];
+    bool go[9];
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        const uint64_t d = f[x] != 0 ? g.f_out_desc[f[x]] : 0ull;
+#pragma unroll
+        for (int y = 0; y < 3; y++) {
+            T[3 * x + y] = g.f_out_tab + (d >> 6);
+            bits[3 * x + y] = (uint32_t)(d & 63u);
+            key[3 * x + y] = f[y];
+            go[3 * x + y] = true;
+        }
+    }
+    probe_many<9>(T, bits, key, go, out);
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 3; y++) F[x][y] = out[3 * x + y];
 }
 
 /* Per-wave scratch for the flattened (edge, female entry) loops. */
@@ -527,9 +606,24 @@ __device__ __forceinline__ int64_t gain_swap_delta(const BmDevGraph& g, const Bm
     const int32_t* Gu = G.G + (int64_t)u * G.stride;
     const int32_t* Gv = G.G + (int64_t)v * G.stride;
     int64_t d = (int64_t)Gu[f2] - Gu[f1] + Gv[f1] - Gv[f2];
-    const int32_t w1 = male_w(G, u, v), w2 = male_w(G, v, u);
-    d += edge_corr(g, w1, f1, f2, f2, f1);     /* u -> v */
-    d += edge_corr(g, w2, f2, f1, f1, f2);     /* v -> u */
+    /* male edges u -> v and v -> u; their corrections need the four female
+     * weights among f1 and f2, probed together (rare: only with an edge) */
+    const int32_t vtx[3] = {u, v, 0};
+    const bool have[3][3] = {{false, male_maybe(G, u, v), false}, {male_maybe(G, v, u), false, false},
+                             {false, false, false}};
+    int32_t w[3][3];
+    male_w_triple(G, vtx, have, w);
+    const int32_t w1 = w[0][1], w2 = w[1][0];
+    if (w1 != BM_ABSENT || w2 != BM_ABSENT) {
+        const int32_t f[3] = {f1, f2, 0};
+        int32_t F[3][3];
+        female_w_triple(g, f, F);
+        /* u -> v lands on f2 -> f1; v -> u on f1 -> f2 */
+        if (w1 != BM_ABSENT)
+            d += (int64_t)contrib(w1, F[1][0]) - contrib(w1, F[1][1]) - contrib(w1, F[0][0]) + contrib(w1, F[0][1]);
+        if (w2 != BM_ABSENT)
+            d += (int64_t)contrib(w2, F[0][1]) - contrib(w2, F[0][0]) - contrib(w2, F[1][1]) + contrib(w2, F[1][0]);
+    }
     const int32_t su = G.m_self[u], sv = G.m_self[v];
     if (su != BM_ABSENT) d += contrib(su, G.f_self[f2]) - (int64_t)contrib(su, G.f_self[f1]);
     if (sv != BM_ABSENT) d += contrib(sv, G.f_self[f1]) - (int64_t)contrib(sv, G.f_self[f2]);
@@ -786,26 +880,46 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
 /* Best of the five permutations of a triple from its nine gains Gv[i][j] =
  * G(vtx_i, f_j), the male self-loops s and female self-loops fs of f_0..f_2
  * (BM_ABSENT when missing), and the male weights w[i][j] of edges inside the
- * triple (an edge correction probes F only where an edge exists).  bd = the
- * largest positive delta (0 if none), bp its permutation (first p on ties). */
+ * triple.  An edge i -> j is corrected exactly: under permutation p it
+ * contributes c(w, F(f_pi, f_pj)) where the gains counted c(w, F(f_pi, f_j))
+ * and c(w, F(f_i, f_pj)) and the old score c(w, F(f_i, f_j)); all of these
+ * are among the nine female weights F(f_x, f_y), probed together once and
+ * only when the triple holds a male edge.  bd = the largest positive delta
+ * (0 if none), bp its permutation (first p on ties). */
 __device__ __forceinline__ void rot3_from_gains(const BmDevGraph& g, const int32_t* f, const int64_t (*Gv)[3],
                                                 const int32_t (*w)[3], const int32_t* s, const int32_t* fs, int64_t& bd,
                                                 int& bp) {
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+            if (i != j) any = any || w[i][j] != BM_ABSENT;
+    int32_t F[3][3];
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int y = 0; y < 3; y++) F[x][y] = BM_ABSENT;
+    if (any) female_w_triple(g, f, F);
     bd = 0;
     bp = 0;
 #pragma unroll
     for (int p = 1; p <= 5; p++) {
-        int32_t ng[3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) ng[i] = f[perm_of(p, i)];
         int64_t d = 0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            d += sel3(perm_of(p, i), Gv[i][0], Gv[i][1], Gv[i][2]) - Gv[i][i];
-            if (s[i] != BM_ABSENT) d += contrib(s[i], fs[perm_of(p, i)]) - (int64_t)contrib(s[i], fs[i]);
+            const int pi = perm_of(p, i);
+            d += sel3(pi, Gv[i][0], Gv[i][1], Gv[i][2]) - Gv[i][i];
+            if (s[i] != BM_ABSENT) d += contrib(s[i], fs[pi]) - (int64_t)contrib(s[i], fs[i]);
 #pragma unroll
-            for (int j = 0; j < 3; j++)
-                if (i != j) d += edge_corr(g, w[i][j], f[i], f[j], ng[i], ng[j]);
+            for (int j = 0; j < 3; j++) {
+                if (i == j) continue;
+                const int pj = perm_of(p, j);
+                const int32_t x = w[i][j];
+                if (x != BM_ABSENT)
+                    d += (int64_t)contrib(x, F[pi][pj]) - contrib(x, F[pi][j]) - contrib(x, F[i][pj]) +
+                         contrib(x, F[i][j]);
+            }
         }
         if (d > bd) { bd = d; bp = p; }
     }
@@ -832,9 +946,13 @@ __global__ void __launch_bounds__(256) k_gain_rot3(BmDevGraph g, BmGain G, int4*
         for (int i = 0; i < 3; i++) {
             s[i] = G.m_self[vtx[i]];
             fs[i] = G.f_self[f[i]];
-#pragma unroll
-            for (int j = 0; j < 3; j++) w[i][j] = i == j ? BM_ABSENT : male_w(G, vtx[i], vtx[j]);
         }
+        bool have[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) have[i][j] = i != j && male_maybe(G, vtx[i], vtx[j]);
+        male_w_triple(G, vtx, have, w);
         int64_t bd;
         int bp;
         rot3_from_gains(g, f, Gv, w, s, fs, bd, bp);
@@ -941,9 +1059,10 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
                                           {rowT[c], gcb[j], pc[j].y}};
                 const bool ab = (mo[b >> 5] >> (b & 31)) & 1u, ba = (mi[b >> 5] >> (b & 31)) & 1u;
                 const bool ac = (mo[c >> 5] >> (c & 31)) & 1u, ca = (mi[c >> 5] >> (c & 31)) & 1u;
-                const int32_t w[3][3] = {{BM_ABSENT, ab ? male_w(G, a, b) : BM_ABSENT, ac ? male_w(G, a, c) : BM_ABSENT},
-                                         {ba ? male_w(G, b, a) : BM_ABSENT, BM_ABSENT, male_w(G, b, c)},
-                                         {ca ? male_w(G, c, a) : BM_ABSENT, male_w(G, c, b), BM_ABSENT}};
+                const int32_t vtx[3] = {a, b, c};
+                const bool have[3][3] = {{false, ab, ac}, {ba, false, male_maybe(G, b, c)}, {ca, male_maybe(G, c, b), false}};
+                int32_t w[3][3];
+                male_w_triple(G, vtx, have, w);
                 const int32_t sv[3] = {pa.z, pb[j].z, pc[j].z};
                 const int32_t fs[3] = {pa.w, pb[j].w, pc[j].w};
                 int64_t bd;
