@@ -88,82 +88,80 @@ template <int K>
 __device__ __forceinline__ void probe_many(const int2* const* T, const uint32_t* bits, const int32_t* key,
                                            const bool* go, int32_t* out) {
     int2 e[K];
-    uint32_t s[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const bool on = go[k] && bits[k] != 0u && key[k] != 0;
-        s[k] = on ? slot_hash(key[k], bits[k]) : 0u;
         e[k] = make_int2(0, 0);
-        if (on) e[k] = T[k][s[k]];
+        if (go[k] && bits[k] != 0u && key[k] != 0) e[k] = T[k][slot_hash(key[k], bits[k])];
     }
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const uint32_t mask = (1u << bits[k]) - 1u;
-        while (e[k].x != 0 && e[k].x != key[k]) {
-            s[k] = (s[k] + 1u) & mask;
-            e[k] = T[k][s[k]];
+        if (e[k].x != 0 && e[k].x != key[k]) {   /* past its home slot: walk the chain */
+            const uint32_t mask = (1u << bits[k]) - 1u;
+            uint32_t s = slot_hash(key[k], bits[k]);
+            do {
+                s = (s + 1u) & mask;
+                e[k] = T[k][s];
+            } while (e[k].x != 0 && e[k].x != key[k]);
         }
         out[k] = e[k].x != 0 ? e[k].y : BM_ABSENT;
     }
 }
 
-/* w_M(v[i], v[j]) for the six ordered pairs of a triple (BM_ABSENT on the
- * diagonal and where no edge): a pair with have[i][j] false is known absent
- * and costs nothing, the hash probes of the others go out together. */
-__device__ __forceinline__ void male_w_triple(const BmGain& G, const int32_t* v, const bool (*have)[3],
-                                              int32_t (*w)[3]) {
-    const int2* T[6];
-    uint32_t bits[6];
-    int32_t key[6], out[6];
-    bool go[6];
+/* Weights inside a triple of male vertices v[0..2] with images f[0..2]:
+ * w[i][j] = w_M(v[i], v[j]) for the pairs with have[i][j] (the others are
+ * known absent and cost nothing) and, when any such pair exists, the nine
+ * female weights F[x][y] = w_F(f[x], f[y]) (the diagonal is the female
+ * self-loops fs[x]); BM_ABSENT elsewhere.  md / fd are the hash-table
+ * descriptors of the rows of v[i] in M and f[x] in F (loaded by the caller
+ * with its other gathers), so the up to 12 probes cost one memory latency.
+ * v[2] = f[2] = 0 serves a pair. */
+__device__ __forceinline__ void triple_weights(const BmDevGraph& g, const BmGain& G, const int32_t* v,
+                                               const uint64_t* md, const int32_t* f, const uint64_t* fd,
+                                               const int32_t* fs, const bool (*have)[3], int32_t (*w)[3],
+                                               int32_t (*F)[3]) {
+    bool any = false;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            w[i][j] = BM_ABSENT;
+            F[i][j] = BM_ABSENT;
+            if (i != j) any = any || have[i][j];
+        }
+    if (!any) return;
+    const int2* T[12];
+    uint32_t bits[12];
+    int32_t key[12], out[12];
+    bool go[12];
     int k = 0;
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-        const uint64_t d = have[i][(i + 1) % 3] || have[i][(i + 2) % 3] ? G.m_out_desc[v[i]] : 0ull;
+    for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             if (i == j) continue;
-            T[k] = G.m_out_tab + (d >> 6);
-            bits[k] = (uint32_t)(d & 63u);
+            T[k] = G.m_out_tab + (md[i] >> 6);
+            bits[k] = (uint32_t)(md[i] & 63u);
             key[k] = v[j];
             go[k] = have[i][j];
+            T[k + 6] = g.f_out_tab + (fd[i] >> 6);
+            bits[k + 6] = (uint32_t)(fd[i] & 63u);
+            key[k + 6] = f[j];
+            go[k + 6] = f[i] != 0;
             k++;
         }
-    }
-    probe_many<6>(T, bits, key, go, out);
+    probe_many<12>(T, bits, key, go, out);
     k = 0;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        w[i][i] = BM_ABSENT;
+        F[i][i] = fs[i];
 #pragma unroll
         for (int j = 0; j < 3; j++)
-            if (i != j) w[i][j] = out[k++];
+            if (i != j) {
+                w[i][j] = out[k];
+                F[i][j] = out[k + 6];
+                k++;
+            }
     }
-}
-
-/* w_F(f[x], f[y]) for all nine ordered pairs of three female vertices
- * (diagonal = female self-loops), probes issued together. */
-__device__ __forceinline__ void female_w_triple(const BmDevGraph& g, const int32_t* f, int32_t (*F)[3]) {
-    const int2* T[9];
-    uint32_t bits[9];
-    int32_t key[9], out[9];
-    bool go[9];
-#pragma unroll
-    for (int x = 0; x < 3; x++) {
-        const uint64_t d = f[x] != 0 ? g.f_out_desc[f[x]] : 0ull;
-#pragma unroll
-        for (int y = 0; y < 3; y++) {
-            T[3 * x + y] = g.f_out_tab + (d >> 6);
-            bits[3 * x + y] = (uint32_t)(d & 63u);
-            key[3 * x + y] = f[y];
-            go[3 * x + y] = true;
-        }
-    }
-    probe_many<9>(T, bits, key, go, out);
-#pragma unroll
-    for (int x = 0; x < 3; x++)
-#pragma unroll
-        for (int y = 0; y < 3; y++) F[x][y] = out[3 * x + y];
 }
 
 /* Per-wave scratch for the flattened (edge, female entry) loops. */
@@ -603,30 +601,28 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
 __device__ __forceinline__ int64_t gain_swap_delta(const BmDevGraph& g, const BmGain& G, int32_t u, int32_t v) {
     if (u == v) return 0;
     const int32_t f1 = g.pi[u], f2 = g.pi[v];
+    const bool have[3][3] = {{false, male_maybe(G, u, v), false}, {male_maybe(G, v, u), false, false},
+                             {false, false, false}};
+    const uint64_t md[3] = {G.m_out_desc[u], G.m_out_desc[v], 0ull};
     const int32_t* Gu = G.G + (int64_t)u * G.stride;
     const int32_t* Gv = G.G + (int64_t)v * G.stride;
     int64_t d = (int64_t)Gu[f2] - Gu[f1] + Gv[f1] - Gv[f2];
+    const uint64_t fd[3] = {g.f_out_desc[f1], g.f_out_desc[f2], 0ull};
     /* male edges u -> v and v -> u; their corrections need the four female
-     * weights among f1 and f2, probed together (rare: only with an edge) */
-    const int32_t vtx[3] = {u, v, 0};
-    const bool have[3][3] = {{false, male_maybe(G, u, v), false}, {male_maybe(G, v, u), false, false},
-                             {false, false, false}};
-    int32_t w[3][3];
-    male_w_triple(G, vtx, have, w);
+     * weights among f1 and f2 (rare: only with an edge) */
+    const int32_t vtx[3] = {u, v, 0}, f[3] = {f1, f2, 0};
+    const int32_t fs[3] = {G.f_self[f1], G.f_self[f2], BM_ABSENT};
+    int32_t w[3][3], F[3][3];
+    triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
     const int32_t w1 = w[0][1], w2 = w[1][0];
-    if (w1 != BM_ABSENT || w2 != BM_ABSENT) {
-        const int32_t f[3] = {f1, f2, 0};
-        int32_t F[3][3];
-        female_w_triple(g, f, F);
-        /* u -> v lands on f2 -> f1; v -> u on f1 -> f2 */
-        if (w1 != BM_ABSENT)
-            d += (int64_t)contrib(w1, F[1][0]) - contrib(w1, F[1][1]) - contrib(w1, F[0][0]) + contrib(w1, F[0][1]);
-        if (w2 != BM_ABSENT)
-            d += (int64_t)contrib(w2, F[0][1]) - contrib(w2, F[0][0]) - contrib(w2, F[1][1]) + contrib(w2, F[1][0]);
-    }
+    /* u -> v lands on f2 -> f1; v -> u on f1 -> f2 */
+    if (w1 != BM_ABSENT)
+        d += (int64_t)contrib(w1, F[1][0]) - contrib(w1, F[1][1]) - contrib(w1, F[0][0]) + contrib(w1, F[0][1]);
+    if (w2 != BM_ABSENT)
+        d += (int64_t)contrib(w2, F[0][1]) - contrib(w2, F[0][0]) - contrib(w2, F[1][1]) + contrib(w2, F[1][0]);
     const int32_t su = G.m_self[u], sv = G.m_self[v];
-    if (su != BM_ABSENT) d += contrib(su, G.f_self[f2]) - (int64_t)contrib(su, G.f_self[f1]);
-    if (sv != BM_ABSENT) d += contrib(sv, G.f_self[f1]) - (int64_t)contrib(sv, G.f_self[f2]);
+    if (su != BM_ABSENT) d += contrib(su, fs[1]) - (int64_t)contrib(su, fs[0]);
+    if (sv != BM_ABSENT) d += contrib(sv, fs[0]) - (int64_t)contrib(sv, fs[1]);
     return d;
 }
 
@@ -879,28 +875,15 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
 
 /* Best of the five permutations of a triple from its nine gains Gv[i][j] =
  * G(vtx_i, f_j), the male self-loops s and female self-loops fs of f_0..f_2
- * (BM_ABSENT when missing), and the male weights w[i][j] of edges inside the
- * triple.  An edge i -> j is corrected exactly: under permutation p it
- * contributes c(w, F(f_pi, f_pj)) where the gains counted c(w, F(f_pi, f_j))
- * and c(w, F(f_i, f_pj)) and the old score c(w, F(f_i, f_j)); all of these
- * are among the nine female weights F(f_x, f_y), probed together once and
- * only when the triple holds a male edge.  bd = the largest positive delta
- * (0 if none), bp its permutation (first p on ties). */
-__device__ __forceinline__ void rot3_from_gains(const BmDevGraph& g, const int32_t* f, const int64_t (*Gv)[3],
-                                                const int32_t (*w)[3], const int32_t* s, const int32_t* fs, int64_t& bd,
-                                                int& bp) {
-    bool any = false;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-            if (i != j) any = any || w[i][j] != BM_ABSENT;
-    int32_t F[3][3];
-#pragma unroll
-    for (int x = 0; x < 3; x++)
-#pragma unroll
-        for (int y = 0; y < 3; y++) F[x][y] = BM_ABSENT;
-    if (any) female_w_triple(g, f, F);
+ * (BM_ABSENT when missing), the male weights w[i][j] of edges inside the
+ * triple and the female weights F[x][y] = w_F(f_x, f_y) (needed only where a
+ * male edge exists).  An edge i -> j is corrected exactly: under permutation
+ * p it contributes c(w, F(f_pi, f_pj)) where the gains counted
+ * c(w, F(f_pi, f_j)) and c(w, F(f_i, f_pj)) and the old score
+ * c(w, F(f_i, f_j)); all of these are among the nine F[x][y].  bd = the
+ * largest positive delta (0 if none), bp its permutation (first p on ties). */
+__device__ __forceinline__ void rot3_from_gains(const int64_t (*Gv)[3], const int32_t (*w)[3], const int32_t (*F)[3],
+                                                const int32_t* s, const int32_t* fs, int64_t& bd, int& bp) {
     bd = 0;
     bp = 0;
 #pragma unroll
@@ -940,22 +923,23 @@ __global__ void __launch_bounds__(256) k_gain_rot3(BmDevGraph g, BmGain G, int4*
 #pragma unroll
             for (int j = 0; j < 3; j++) Gv[i][j] = Gr[f[j]];
         }
-        int32_t w[3][3];    /* w[i][j] = w_M(vtx[i], vtx[j]) for i != j */
         int32_t s[3], fs[3];
+        uint64_t md[3], fd[3];
+        bool have[3][3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             s[i] = G.m_self[vtx[i]];
             fs[i] = G.f_self[f[i]];
-        }
-        bool have[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++)
+            md[i] = G.m_out_desc[vtx[i]];
+            fd[i] = g.f_out_desc[f[i]];
 #pragma unroll
             for (int j = 0; j < 3; j++) have[i][j] = i != j && male_maybe(G, vtx[i], vtx[j]);
-        male_w_triple(G, vtx, have, w);
+        }
+        int32_t w[3][3], F[3][3];
+        triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
         int64_t bd;
         int bp;
-        rot3_from_gains(g, f, Gv, w, s, fs, bd, bp);
+        rot3_from_gains(Gv, w, F, s, fs, bd, bp);
         best[c] = bd;
         if (perm) perm[c] = (uint8_t)bp;
         if (write_perm_into_cand) cand[c].w = bp;
@@ -993,6 +977,7 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
         if (i0 == i1) continue;   /* block-uniform */
         const int4 pa = pis[a];
         const int32_t fa = pa.x;
+        const uint64_t mda = G.m_out_desc[a], fda = g.f_out_desc[fa];
         for (int32_t w = tid; w < 2 * w32; w += kUmThreads) mo[w] = 0;
         __syncthreads();
         const int64_t mob = g.m_out_ptr[a], moe = g.m_out_ptr[a + 1];
@@ -1034,19 +1019,27 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
         for (int64_t ib = i0 + tid; ib < i1; ib += (int64_t)kUmThreads * kR3Triples) {
             int32_t vb[kR3Triples], vc[kR3Triples];
             int4 pb[kR3Triples], pc[kR3Triples];
+            uint32_t abc[kR3Triples], acb[kR3Triples];   /* dense-bitmap words of b -> c and c -> b */
+            uint64_t mdb[kR3Triples], mdc[kR3Triples], fdb[kR3Triples], fdc[kR3Triples];
             int32_t gbc[kR3Triples], gcb[kR3Triples];
 #pragma unroll
-            for (int j = 0; j < kR3Triples; j++) {   /* all gathers in flight before any use */
+            for (int j = 0; j < kR3Triples; j++) {   /* every gather of a level in flight before any use */
                 const int64_t i = ib + (int64_t)kUmThreads * j;
                 vb[j] = vc[j] = 0;
                 if (i < i1) bm_amajor_partners(seed, (uint64_t)i, a, n, &vb[j], &vc[j]);
                 pb[j] = i < i1 ? pis[vb[j]] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
                 pc[j] = i < i1 ? pis[vc[j]] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
+                abc[j] = G.m_adj ? G.m_adj[(int64_t)vb[j] * G.m_adj_words + (vc[j] >> 5)] : ~0u;
+                acb[j] = G.m_adj ? G.m_adj[(int64_t)vc[j] * G.m_adj_words + (vb[j] >> 5)] : ~0u;
+                mdb[j] = G.m_out_desc[vb[j]];
+                mdc[j] = G.m_out_desc[vc[j]];
             }
 #pragma unroll
             for (int j = 0; j < kR3Triples; j++) {
                 gbc[j] = G.G[(int64_t)vb[j] * stride + pc[j].x];   /* G(b, f_c) */
                 gcb[j] = G.G[(int64_t)vc[j] * stride + pb[j].x];   /* G(c, f_b) */
+                fdb[j] = g.f_out_desc[pb[j].x];
+                fdc[j] = g.f_out_desc[pc[j].x];
             }
 #pragma unroll
             for (int j = 0; j < kR3Triples; j++) {
@@ -1059,15 +1052,17 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
                                           {rowT[c], gcb[j], pc[j].y}};
                 const bool ab = (mo[b >> 5] >> (b & 31)) & 1u, ba = (mi[b >> 5] >> (b & 31)) & 1u;
                 const bool ac = (mo[c >> 5] >> (c & 31)) & 1u, ca = (mi[c >> 5] >> (c & 31)) & 1u;
+                const bool bc = (abc[j] >> (c & 31)) & 1u, cb = (acb[j] >> (b & 31)) & 1u;
                 const int32_t vtx[3] = {a, b, c};
-                const bool have[3][3] = {{false, ab, ac}, {ba, false, male_maybe(G, b, c)}, {ca, male_maybe(G, c, b), false}};
-                int32_t w[3][3];
-                male_w_triple(G, vtx, have, w);
+                const bool have[3][3] = {{false, ab, ac}, {ba, false, bc}, {ca, cb, false}};
+                const uint64_t md[3] = {mda, mdb[j], mdc[j]}, fd[3] = {fda, fdb[j], fdc[j]};
                 const int32_t sv[3] = {pa.z, pb[j].z, pc[j].z};
                 const int32_t fs[3] = {pa.w, pb[j].w, pc[j].w};
+                int32_t w[3][3], F[3][3];
+                triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
                 int64_t bd;
                 int bp;
-                rot3_from_gains(g, f, Gv, w, sv, fs, bd, bp);
+                rot3_from_gains(Gv, w, F, sv, fs, bd, bp);
                 best[i] = bd;
                 perm[i] = (uint8_t)bp;
             }
