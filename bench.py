@@ -98,6 +98,21 @@ def gain_lookup_bytes(ctx, k):
     return n * 2 * gain_stride(n) * 4 + (n + 1) * 16 + 16 * int(mo.sum()) + 16 * int(fo.sum()) + (n + 2) * 8 + 8 * k
 
 
+def measured_traffic(args):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by what they read for a 1 GiB copy) for this workload,
+    by kernel (profiles/swap_n18524_traffic.md); {} for another workload."""
+    prof = os.path.join(ROOT, "profiles", "swap_n18524_traffic.json")
+    if not os.path.exists(prof):
+        return {}
+    with open(prof) as f:
+        p = json.load(f)
+    if (p.get("candidates") == args.candidates and p.get("triples") == args.triples and p.get("pi0") == args.pi0
+            and p.get("stream") == args.stream):
+        return p.get("hbm_bytes_per_launch", {})
+    return {}
+
+
 def timed(ctx, kernel_ids, fn, steps):
     ctx.sync()
     ctx.timing_reset()
@@ -232,8 +247,18 @@ def main():
             "lookup_roofline": {"bound": "hbm", "achieved": lb / look_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": lb / look_s / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": lb},
         }
+        # the per-candidate byte model of a list scan (4 male lists, 28 B per
+        # incident edge): what the same stream would cost a kernel that reads
+        # every candidate's lists, and the rate such a kernel reaches at HBM peak
+        pcb, pcd = stream_bytes(seeds[-1], K)
+        gain["per_candidate_model"] = {
+            "bytes_per_step": pcb, "mean_D": pcd / K,
+            "deltas_per_s_at_hbm_peak": K * HBM_PEAK_GBS * 1e9 / pcb,
+            "equivalent_GBps": pcb / (wall / args.steps) / 1e9,
+        }
         log(f"gain ({args.stream}): {args.steps} x {K} pairs in {wall:.3f}s -> {gain['value']:.4g} deltas/s "
-            f"(build {1000 * build_s:.2f} ms, lookups {1000 * look_s:.2f} ms, stream/transpose {1000 * aux_s:.2f} ms)")
+            f"(build {1000 * build_s:.2f} ms, lookups {1000 * look_s:.2f} ms, stream/transpose {1000 * aux_s:.2f} ms); "
+            f"a list scan at HBM peak: {gain['per_candidate_model']['deltas_per_s_at_hbm_peak']:.3g} deltas/s")
         gain_deltas_head = batch.deltas(0, 1 << 16)
         if umajor:
             # the same table against the draw-order stream (per-pair table reads)
@@ -276,15 +301,7 @@ def main():
     head = gain if args.headline == "gain" else scan
     stats = batch.stats()
 
-    # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE,
-    # scaled by a 1 GiB copy) for this workload; see profiles/swap_n18524_traffic.md
-    traffic = {}
-    prof = os.path.join(ROOT, "profiles", "swap_n18524_traffic.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            p = json.load(f)
-        if p.get("candidates") == K and p.get("pi0") == args.pi0 and p.get("stream") == args.stream:
-            traffic = p.get("hbm_bytes_per_launch", {})
+    traffic = measured_traffic(args)
 
     if args.headline == "gain":
         dom = "lookup" if gain["lookup_ms"] >= gain["build_ms"] else "build"
@@ -596,9 +613,11 @@ def bench_rot3(ctx, bm, args):
                    + T * (2 * 4 + 2 * 4 + 9))
         else:
             ab3, _ = b3.algo_bytes(generated=False)
+        kern = "k_gain_rot3_amajor" if mode == "gain" else "k_rot3"
         out[mode] = {"value": T * steps / wall, "ms_per_step": 1000 * wall / steps, "launch_ms": 1000 * l3,
                      "roofline": {"bound": "hbm", "achieved": ab3 / l3 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": ab3 / l3 / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": ab3}}
+                                  "frac": ab3 / l3 / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": ab3,
+                                  "kernel": kern, "traffic": measured_traffic(args).get(kern)}}
         b3.eval_rot3_amajor(99, T)
         heads[mode] = (b3.deltas(0, 1 << 16), b3.perms(0, 1 << 16))
         log(f"rot3 {mode}: {T} a-major triples in {1000 * wall / steps:.1f} ms/step ({1000 * l3:.1f} ms kernel) "

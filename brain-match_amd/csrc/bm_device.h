@@ -37,7 +37,8 @@ struct BmGain {
     int64_t stride;             /* row stride in int32: n + 1 rounded up to 32 (rows start on 128-byte lines) */
     const uint64_t* m_out_desc; /* male out-row hash tables (adjacency tests) */
     const int2* m_out_tab;
-    const uint32_t* m_adj;      /* dense male adjacency, m_adj_words uint32 per row (n+1 rows), or NULL */
+    const uint32_t* m_adj;      /* dense male adjacency, m_adj_words uint32 per row (n+1 rows), or NULL;
+                                 * symmetric: bit max(a,b) of row min(a,b) is set when a -> b or b -> a */
     int64_t m_adj_words;
     const int32_t* m_self;      /* male self-loop weight per vertex or BM_ABSENT */
     const int32_t* f_self;      /* female self-loop weight per vertex or BM_ABSENT */

@@ -66,18 +66,26 @@ __device__ __forceinline__ int32_t female_w(const BmDevGraph& g, int32_t a, int3
 }
 
 /* w_M(a, b) or BM_ABSENT. */
-/* w_M(a, b) or BM_ABSENT.  With the dense adjacency bitmap a missing edge
- * (the common case) costs one load instead of a hash probe sequence. */
-__device__ __forceinline__ int32_t male_w(const BmGain& G, int32_t a, int32_t b) {
-    if (G.m_adj && !((G.m_adj[(int64_t)a * G.m_adj_words + (b >> 5)] >> (b & 31)) & 1u)) return BM_ABSENT;
-    const uint64_t d = G.m_out_desc[a];
-    return probe_g(G.m_out_tab + (d >> 6), (uint32_t)(d & 63u), b);
+/* Word of the dense adjacency bitmap holding the pair {a, b}: bit (max & 31)
+ * of row min(a, b).  The bitmap is symmetric, set when a -> b or b -> a is a
+ * male edge, so one load answers both directions. */
+__device__ __forceinline__ uint32_t adj_word(const BmGain& G, int32_t a, int32_t b) {
+    const int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    return G.m_adj[(int64_t)lo * G.m_adj_words + (hi >> 5)];
 }
 
-/* False when the dense adjacency bitmap shows no male edge a -> b; true
- * when it shows one or there is no bitmap. */
+/* False when the dense adjacency bitmap shows no male edge between a and b;
+ * true when it shows one (either direction) or there is no bitmap. */
 __device__ __forceinline__ bool male_maybe(const BmGain& G, int32_t a, int32_t b) {
-    return !G.m_adj || ((G.m_adj[(int64_t)a * G.m_adj_words + (b >> 5)] >> (b & 31)) & 1u);
+    return !G.m_adj || ((adj_word(G, a, b) >> ((a < b ? b : a) & 31)) & 1u);
+}
+
+/* w_M(a, b) or BM_ABSENT.  With the dense adjacency bitmap a pair without
+ * an edge (the common case) costs one load instead of a hash probe sequence. */
+__device__ __forceinline__ int32_t male_w(const BmGain& G, int32_t a, int32_t b) {
+    if (!male_maybe(G, a, b)) return BM_ABSENT;
+    const uint64_t d = G.m_out_desc[a];
+    return probe_g(G.m_out_tab + (d >> 6), (uint32_t)(d & 63u), b);
 }
 
 /* K hash lookups at once: key[k] in table (T[k], bits[k]) where go[k], else
@@ -537,8 +545,9 @@ __global__ void __launch_bounds__(256) k_adj_bits(BmDevGraph g, uint32_t* __rest
     for (int64_t a = wave + 1; a <= g.n; a += nwaves) {
         uint32_t* row = bits + a * words;
         for (int64_t e = g.m_out_ptr[a] + lane; e < g.m_out_ptr[a + 1]; e += BM_WAVE) {
-            const int32_t b = g.m_out[e].x;
-            atomicOr(&row[b >> 5], 1u << (b & 31));
+            const int32_t b = g.m_out[e].x;   /* the pair's bit sits in the row of its smaller id */
+            if (b >= a) atomicOr(&row[b >> 5], 1u << (b & 31));
+            else atomicOr(&bits[(int64_t)b * words + (a >> 5)], 1u << (a & 31));
         }
     }
 }
@@ -601,8 +610,8 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
 __device__ __forceinline__ int64_t gain_swap_delta(const BmDevGraph& g, const BmGain& G, int32_t u, int32_t v) {
     if (u == v) return 0;
     const int32_t f1 = g.pi[u], f2 = g.pi[v];
-    const bool have[3][3] = {{false, male_maybe(G, u, v), false}, {male_maybe(G, v, u), false, false},
-                             {false, false, false}};
+    const bool e_uv = male_maybe(G, u, v);   /* either direction */
+    const bool have[3][3] = {{false, e_uv, false}, {e_uv, false, false}, {false, false, false}};
     const uint64_t md[3] = {G.m_out_desc[u], G.m_out_desc[v], 0ull};
     const int32_t* Gu = G.G + (int64_t)u * G.stride;
     const int32_t* Gv = G.G + (int64_t)v * G.stride;
@@ -932,9 +941,13 @@ __global__ void __launch_bounds__(256) k_gain_rot3(BmDevGraph g, BmGain G, int4*
             fs[i] = G.f_self[f[i]];
             md[i] = G.m_out_desc[vtx[i]];
             fd[i] = g.f_out_desc[f[i]];
-#pragma unroll
-            for (int j = 0; j < 3; j++) have[i][j] = i != j && male_maybe(G, vtx[i], vtx[j]);
         }
+        const bool e01 = male_maybe(G, vtx[0], vtx[1]), e02 = male_maybe(G, vtx[0], vtx[2]),
+                   e12 = male_maybe(G, vtx[1], vtx[2]);   /* either direction */
+        have[0][0] = have[1][1] = have[2][2] = false;
+        have[0][1] = have[1][0] = e01;
+        have[0][2] = have[2][0] = e02;
+        have[1][2] = have[2][1] = e12;
         int32_t w[3][3], F[3][3];
         triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
         int64_t bd;
@@ -1019,7 +1032,7 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
         for (int64_t ib = i0 + tid; ib < i1; ib += (int64_t)kUmThreads * kR3Triples) {
             int32_t vb[kR3Triples], vc[kR3Triples];
             int4 pb[kR3Triples], pc[kR3Triples];
-            uint32_t abc[kR3Triples], acb[kR3Triples];   /* dense-bitmap words of b -> c and c -> b */
+            uint32_t wbc[kR3Triples];   /* dense-bitmap word of the pair {b, c} */
             uint64_t mdb[kR3Triples], mdc[kR3Triples], fdb[kR3Triples], fdc[kR3Triples];
             int32_t gbc[kR3Triples], gcb[kR3Triples];
 #pragma unroll
@@ -1029,8 +1042,7 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
                 if (i < i1) bm_amajor_partners(seed, (uint64_t)i, a, n, &vb[j], &vc[j]);
                 pb[j] = i < i1 ? pis[vb[j]] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
                 pc[j] = i < i1 ? pis[vc[j]] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
-                abc[j] = G.m_adj ? G.m_adj[(int64_t)vb[j] * G.m_adj_words + (vc[j] >> 5)] : ~0u;
-                acb[j] = G.m_adj ? G.m_adj[(int64_t)vc[j] * G.m_adj_words + (vb[j] >> 5)] : ~0u;
+                wbc[j] = G.m_adj ? adj_word(G, vb[j], vc[j]) : ~0u;
                 mdb[j] = G.m_out_desc[vb[j]];
                 mdc[j] = G.m_out_desc[vc[j]];
             }
@@ -1052,9 +1064,9 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
                                           {rowT[c], gcb[j], pc[j].y}};
                 const bool ab = (mo[b >> 5] >> (b & 31)) & 1u, ba = (mi[b >> 5] >> (b & 31)) & 1u;
                 const bool ac = (mo[c >> 5] >> (c & 31)) & 1u, ca = (mi[c >> 5] >> (c & 31)) & 1u;
-                const bool bc = (abc[j] >> (c & 31)) & 1u, cb = (acb[j] >> (b & 31)) & 1u;
+                const bool bc = (wbc[j] >> ((b < c ? c : b) & 31)) & 1u;   /* either direction */
                 const int32_t vtx[3] = {a, b, c};
-                const bool have[3][3] = {{false, ab, ac}, {ba, false, bc}, {ca, cb, false}};
+                const bool have[3][3] = {{false, ab, ac}, {ba, false, bc}, {ca, bc, false}};
                 const uint64_t md[3] = {mda, mdb[j], mdc[j]}, fd[3] = {fda, fdb[j], fdc[j]};
                 const int32_t sv[3] = {pa.z, pb[j].z, pc[j].z};
                 const int32_t fs[3] = {pa.w, pb[j].w, pc[j].w};
