@@ -271,7 +271,7 @@ int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k);
  * distinct vertices (the law of bm_batch_gen_triples) listed grouped by their
  * first vertex.  gen_amajor writes them into the batch; eval_rot3_amajor
  * evaluates the best 3-vertex permutation of each (as bm_batch_eval_rot3) and,
- * with the gain table enabled and n <= 20,000, without materialising them:
+ * with the gain table enabled and n <= 19,839, without materialising them:
  * one block per first vertex a stages the rows G(a, .) and GT(pi(a), .), so a
  * triple reads two table entries from HBM instead of nine.  Deltas and
  * permutations are read with bm_batch_get (cand is not available after the
