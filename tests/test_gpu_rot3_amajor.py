@@ -82,3 +82,57 @@ def test_full_size_staged_equals_scan(bm, orc):
         rb, rp = o.rot3_many(pi0, orc.amajor_triples(21, k, n)[idx])
         assert np.array_equal(d_scan[idx], rb) and np.array_equal(p_scan[idx].astype(np.int32), rp)
         b.close()
+
+
+def test_large_n_falls_back_to_list_lookups(bm, orc):
+    """n = 20,000 is past the staged rows' LDS limit (n <= 19,839): the
+    stream is materialised and every triple reads its nine table entries."""
+    n, k = 20000, 1 << 15
+    male, female, truth, pi0 = bm.synth_pair(n, 6.0, bm.WEIGHTS_GEOMETRIC, 0.3, seed=6)
+    o = orc.Oracle(n, male, female)
+    with bm.Context(n, male, female) as ctx:
+        ctx.set_mapping(pi0)
+        ctx.gain_enable(True)
+        b = ctx.batch(k)
+        b.eval_rot3_amajor(13, k)
+        rb, rp = o.rot3_many(pi0, orc.amajor_triples(13, k, n))
+        assert np.array_equal(b.deltas(), rb) and np.array_equal(b.perms().astype(np.int32), rp)
+        assert np.array_equal(b.triples()[:, :3], orc.amajor_triples(13, k, n))
+        b.close()
+
+
+def tiny_pair(bm, n, seed):
+    """A dense pair on n vertices with self-loops and one unmatched vertex."""
+    rng = np.random.default_rng(seed)
+
+    def g():
+        src, dst = np.meshgrid(np.arange(1, n + 1), np.arange(1, n + 1))
+        keep = rng.random(n * n) < 0.6
+        w = rng.integers(1, 9, n * n)
+        return bm.Graph(src.ravel()[keep].astype(np.int32), dst.ravel()[keep].astype(np.int32),
+                        w[keep].astype(np.int32), n)
+
+    pi = np.concatenate([[0], rng.permutation(np.arange(1, n + 1))]).astype(np.int32)
+    pi[rng.integers(1, n + 1)] = 0
+    return g(), g(), pi
+
+
+@pytest.mark.parametrize("n", [3, 4, 7])
+def test_tiny_graphs_and_empty_batch(bm, orc, n):
+    """Smallest vertex counts (n = 3 leaves c a single choice), both paths,
+    dense tiny graphs with self-loops and an unmatched vertex; k = 0 is a
+    no-op."""
+    male, female, pi = tiny_pair(bm, n, seed=20 + n)
+    k = 4000
+    with bm.Context(n, male, female) as ctx:
+        ctx.set_mapping(pi)
+        o = orc.Oracle(n, male, female)
+        abc = orc.amajor_triples(7, k, n)
+        rb, rp = o.rot3_many(pi, abc)
+        b = ctx.batch(k)
+        for gain in (False, True):
+            ctx.gain_enable(gain)
+            b.eval_rot3_amajor(7, k)
+            assert np.array_equal(b.deltas(), rb) and np.array_equal(b.perms().astype(np.int32), rp)
+            b.eval_rot3_amajor(8, 0)
+        b.close()
