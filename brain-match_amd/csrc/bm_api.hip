@@ -84,6 +84,9 @@ struct bm_ctx {
     int2* d_echunks;        /* {u, first slot} chunks of the work list (fixed by the male graph) */
     uint32_t* d_m_adj;      /* dense male adjacency bitmap (NULL when it would exceed 2 GiB) */
     int64_t m_adj_words;
+    int32_t* d_f_in_bnd;    /* column-window boundaries of the female rows (windowed builds only) */
+    int32_t* d_f_out_bnd;
+    int32_t nwin;
     int64_t n_echunks;
     int64_t gt_version;
     char err[512];
@@ -251,6 +254,8 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_f_in_p);
     (void)hipFree(c->d_edesc);
     (void)hipFree(c->d_echunks);
+    (void)hipFree(c->d_f_in_bnd);
+    (void)hipFree(c->d_f_out_bnd);
     (void)hipFree(c->d_m_adj);
     bm_graph_free(&c->male);
     free(c->f_out_deg);
@@ -274,6 +279,9 @@ static BmGain devgain(const bm_ctx* c) {
     G.edesc = c->d_edesc;
     G.echunks = c->d_echunks;
     G.n_echunks = c->n_echunks;
+    G.f_in_bnd = c->d_f_in_bnd;
+    G.f_out_bnd = c->d_f_out_bnd;
+    G.nwin = c->nwin;
     G.m_adj = c->d_m_adj;
     G.m_adj_words = c->m_adj_words;
     G.build_unroll = c->tune[BM_KERNEL_GAIN_BUILD].unroll;
@@ -1232,6 +1240,23 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         if (rc == BM_OK && !ch.empty()) rc = h2d(c, c->d_echunks, ch.data(), ch.size());
         if (rc == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, BM_ERR_HIP, "sync failed");
         if (rc != BM_OK) return rc;
+    }
+    /* a table wider than one LDS window is built window by window; the
+     * work list takes each female row's part in a window from boundaries
+     * found once here (they depend only on the female graph) */
+    {
+        const int64_t stride = bm_gain_stride(c->n);
+        const int32_t cols = bm_gain_chunk_cols(stride);
+        const int32_t nwin = (int32_t)((stride + cols - 1) / cols);
+        if (nwin > 1 && !c->d_f_in_bnd) {
+            const size_t nb = ((size_t)c->n + 1) * (size_t)(nwin + 1);
+            rc = dalloc(c, &c->d_f_in_bnd, nb);
+            if (rc == BM_OK) rc = dalloc(c, &c->d_f_out_bnd, nb);
+            if (rc != BM_OK) return rc;
+            HIPCHK(c, bm_launch_row_bounds(c->d_f_in_ptr, c->d_f_in, c->n, cols, nwin, c->d_f_in_bnd, c->stream));
+            HIPCHK(c, bm_launch_row_bounds(c->d_f_out_ptr, c->d_f_out, c->n, cols, nwin, c->d_f_out_bnd, c->stream));
+        }
+        c->nwin = nwin;
     }
     /* dense male adjacency for the table lookups' edge tests, when it fits
      * 2 GiB (43 MB at n = 18,524); otherwise they probe the hash tables */

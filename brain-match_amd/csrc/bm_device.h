@@ -49,6 +49,13 @@ struct BmGain {
     const int2* echunks;        /* {u, first slot} chunks of at most BM_DESC_CHUNK slots covering every list */
     int64_t n_echunks;
     int32_t build_unroll;       /* build: female-row entries per lane in flight (8, 16 or 32) */
+    /* builds in column windows (nwin > 1): for female row r, entry k of
+     * r * (nwin + 1) + [0, nwin] is the offset in the row of its first entry
+     * with a column >= k * window width (nwin: the row's length); NULL when
+     * the table is built in one window */
+    const int32_t* f_in_bnd;
+    const int32_t* f_out_bnd;
+    int32_t nwin;
 };
 
 /* Row stride of the gain table and its transpose: n + 1 rounded up to a
@@ -71,6 +78,10 @@ hipError_t bm_launch_adj_bits(const BmDevGraph& g, uint32_t* bits, int64_t words
 hipError_t bm_launch_pack_rows(const int2* in, int64_t m, int layout, uint32_t* out, hipStream_t s);
 /* Column-window width of the table build for a row stride. */
 int32_t bm_gain_chunk_cols(int64_t stride);
+/* Window boundaries of every female row for a build in nwin windows of
+ * width cols (see BmGain::f_in_bnd): bnd[r * (nwin + 1) + k], rows 0..n. */
+hipError_t bm_launch_row_bounds(const int64_t* ptr, const int2* rows, int32_t n, int32_t cols, int32_t nwin,
+                                int32_t* bnd, hipStream_t s);
 
 /* Table build: per column chunk, the work list (k_gain_edge_desc) then the
  * rows.  G.edesc must hold 2 x male edges + 1 entries. */

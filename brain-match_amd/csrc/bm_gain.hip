@@ -284,29 +284,50 @@ __device__ __forceinline__ void lds_add(uint32_t byte_addr, int32_t v) {
     asm volatile("ds_add_u32 %0, %1" ::"v"(byte_addr), "v"(v) : "memory");
 }
 
+/* Column-window boundaries of the female rows: thread (r, k) finds the
+ * first entry of row r whose column is >= k * cols by binary search (k =
+ * nwin gives the row's length). */
+__global__ void k_row_bounds(const int64_t* __restrict__ ptr, const int2* __restrict__ rows, int32_t n, int32_t cols,
+                             int32_t nwin, int32_t* __restrict__ bnd) {
+    const int64_t total = ((int64_t)n + 1) * (nwin + 1);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int32_t r = (int32_t)(i / (nwin + 1)), k = (int32_t)(i % (nwin + 1));
+        const int64_t rb = ptr[r], re = ptr[r + 1];
+        int64_t a = rb, b = re;
+        if (k < nwin) {
+            const int64_t c0 = (int64_t)k * cols;
+            while (a < b) {
+                const int64_t m = (a + b) >> 1;
+                if (rows[m].x < c0) a = m + 1; else b = m;
+            }
+        } else {
+            a = re;
+        }
+        bnd[i] = (int32_t)(a - rb);
+    }
+}
+
 /* Work list of the build, one entry per male edge slot of row u (out-edges,
  * then in-edges, at base(u) = m_out_ptr[u] + m_in_ptr[u]): the female row the
  * edge expands (the in-row of the neighbour's image for an out-edge, its
- * out-row for an in-edge) clipped to the column window, as {start lo,
- * start hi | in-edge flag << 31, length, w}; length 0 for self-loops and
- * unmatched neighbours.
+ * out-row for an in-edge) clipped to the column window (the row's window
+ * boundaries are precomputed, k_row_bounds), as {start lo, start hi |
+ * in-edge flag << 31, length, w}; length 0 for self-loops and unmatched
+ * neighbours.
  * One wave per chunk of at most kDescChunk slots of one vertex's list (the
  * chunks are fixed by the male graph, so a hub's list is spread over many
  * waves), kDescChunk / 64 slots per lane with each level of the dependent
  * loads (edge, pi of the neighbour, row bounds) issued for all of them
  * together. */
 template <int LAYOUT>
-__global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, int32_t col0, int32_t ncols) {
-    typedef Row<LAYOUT> R;
-    constexpr bool PACKED = LAYOUT != BM_ROWS_INT2;
+__global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, int32_t col0, int32_t ncols,
+                                                        int32_t win) {
     constexpr int kJ = kDescChunk / BM_WAVE;   /* edge slots per lane, their loads in flight together */
-    typedef typename R::T E;
     const int lane = threadIdx.x & (BM_WAVE - 1);
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / BM_WAVE;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / BM_WAVE;
     const bool windowed = !(col0 == 0 && (int64_t)ncols >= G.stride);
-    const E* __restrict__ f_in_rows = PACKED ? (const E*)G.f_in_p : (const E*)g.f_in;
-    const E* __restrict__ f_out_rows = PACKED ? (const E*)G.f_out_p : (const E*)g.f_out;
     for (int64_t ci = wave; ci < G.n_echunks; ci += nwaves) {
         const int2 ch = G.echunks[ci];   /* {u, first slot}: up to kDescChunk slots of u's list */
         const int32_t u = uni(ch.x);
@@ -315,7 +336,7 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
         const int64_t t0 = uni(ch.y), t1 = t0 + kDescChunk < L ? t0 + kDescChunk : L;
         int4* out = G.edesc + ob + ib;
         int2 ed[kJ];
-        int32_t gx[kJ];
+        int32_t gx[kJ], lo32[kJ], hi32[kJ];
         int64_t rb[kJ], re[kJ];
 #pragma unroll
         for (int j = 0; j < kJ; j++) {
@@ -331,27 +352,17 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
             const int64_t* fptr = outp ? g.f_in_ptr : g.f_out_ptr;
             rb[j] = gx[j] ? fptr[gx[j]] : 0;
             re[j] = gx[j] ? fptr[gx[j] + 1] : 0;
+            if (windowed) {   /* the row's part inside the window, from its precomputed boundaries */
+                const int32_t* bnd = (outp ? G.f_in_bnd : G.f_out_bnd) + (int64_t)gx[j] * (G.nwin + 1) + win;
+                lo32[j] = gx[j] ? bnd[0] : 0;
+                hi32[j] = gx[j] ? bnd[1] : 0;
+            }
         }
+        if (windowed) {
 #pragma unroll
-        for (int j = 0; j < kJ; j++) {
-            const int64_t t = t0 + lane + BM_WAVE * j;
-            if (t >= t1) continue;
-            const bool outp = t < nout;
-            const E* __restrict__ frow = outp ? f_in_rows : f_out_rows;
-            if (windowed && gx[j]) {
-                int64_t a = rb[j], b = re[j];
-                while (a < b) {
-                    const int64_t m = (a + b) >> 1;
-                    if (R::key(frow[m]) < col0) a = m + 1; else b = m;
-                }
-                const int64_t lo = a;
-                b = re[j];
-                while (a < b) {
-                    const int64_t m = (a + b) >> 1;
-                    if (R::key(frow[m]) < col0 + ncols) a = m + 1; else b = m;
-                }
-                rb[j] = lo;
-                re[j] = a;
+            for (int j = 0; j < kJ; j++) {
+                re[j] = rb[j] + hi32[j];
+                rb[j] += lo32[j];
             }
         }
         /* Within the chunk, rows longer than kDescLong entries are written
@@ -1114,6 +1125,16 @@ int32_t bm_gain_chunk_cols(int64_t stride) {
     return (int32_t)(w < stride ? w : stride);
 }
 
+hipError_t bm_launch_row_bounds(const int64_t* ptr, const int2* rows, int32_t n, int32_t cols, int32_t nwin,
+                                int32_t* bnd, hipStream_t s) {
+    if (n < 0 || cols <= 0 || nwin < 1) return hipErrorInvalidValue;
+    const int64_t total = ((int64_t)n + 1) * (nwin + 1);
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_row_bounds, dim3((unsigned)blocks), dim3(256), 0, s, ptr, rows, n, cols, nwin, bnd);
+    return hipGetLastError();
+}
+
 template <int LAYOUT, int U>
 static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
     const int32_t chunk = bm_gain_chunk_cols(G.stride);
@@ -1134,9 +1155,12 @@ static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_
     int64_t dblocks = (G.n_echunks + 3) / 4;       /* 4 waves (chunks) per 256-thread block */
     if (dblocks > cus * 64) dblocks = cus * 64;
     if (dblocks < 1) dblocks = 1;
-    for (int64_t c0 = 0; c0 < G.stride; c0 += chunk) {
+    if (windowed && (!G.f_in_bnd || !G.f_out_bnd || G.nwin != (int32_t)((G.stride + chunk - 1) / chunk)))
+        return hipErrorInvalidValue;
+    for (int64_t c0 = 0, win = 0; c0 < G.stride; c0 += chunk, win++) {
         const int32_t nc = (int32_t)(G.stride - c0 < chunk ? G.stride - c0 : chunk);
-        hipLaunchKernelGGL(k_gain_edge_desc<LAYOUT>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc);
+        hipLaunchKernelGGL(k_gain_edge_desc<LAYOUT>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc,
+                           (int32_t)win);
         if (windowed)
             hipLaunchKernelGGL((k_gain_build<LAYOUT, U, true>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
                                G, (int32_t)c0, nc);
