@@ -186,3 +186,74 @@ int bm_write_mapping_csv(const char* path, int32_t n, const int32_t* pi) {
     if (fclose(f) != 0) { bm_set_global_error("write of %s failed", path); return BM_ERR_IO; }
     return BM_OK;
 }
+
+/* A field of a matching line, as the reference converters scan it with
+ * sscanf's %[...]: the longest non-empty run of characters from s that are
+ * not in stop.  Returns its end, or NULL when the run is empty. */
+static const char* scan_field(const char* s, const char* stop) {
+    const char* p = s;
+    while (*p && !strchr(stop, *p)) p++;
+    return p > s ? p : NULL;
+}
+
+static int all_digits(const char* s, const char* e) {
+    for (; s < e; s++)
+        if (!isdigit((unsigned char)*s)) return 0;
+    return 1;
+}
+
+int bm_convert_mapping_csv(const char* in_path, const char* out_path, int to_prefixed, int64_t* n_lines) {
+    FILE* in = fopen(in_path, "r");
+    if (!in) { bm_set_global_error("Error: Cannot open input file '%s'", in_path); return BM_ERR_IO; }
+    FILE* out = fopen(out_path, "w");
+    if (!out) {
+        fclose(in);
+        bm_set_global_error("Error: Cannot create output file '%s'", out_path);
+        return BM_ERR_IO;
+    }
+    fputs(to_prefixed ? "Male Node ID,Female Node ID\n" : "Male ID,Female ID\n", out);
+    /* lines are read as the reference reads them: fgets into 1,024 bytes,
+     * the first one (the header) skipped */
+    char line[1024];
+    int64_t line_number = 0;
+    int rc = BM_OK;
+    if (fgets(line, sizeof line, in)) line_number++;
+    while (fgets(line, sizeof line, in)) {
+        line_number++;
+        const char* m = line;
+        if (!to_prefixed) {   /* "m<male>,f<female>" (src/convert-VNC-matching.c:64-69) */
+            if (*m != 'm') { rc = BM_ERR_ARG; break; }
+            m++;
+        }
+        const char* me = scan_field(m, ",");
+        if (!me || *me != ',') { rc = BM_ERR_ARG; break; }
+        const char* f = me + 1;
+        if (!to_prefixed) {
+            if (*f != 'f') { rc = BM_ERR_ARG; break; }
+            f++;
+        }
+        const char* fe = scan_field(f, "\n\r");
+        if (!fe) { rc = BM_ERR_ARG; break; }
+        if (to_prefixed) {   /* "<male>,<female>", digits only (src/transform-solution.c:70-92) */
+            if (!all_digits(m, me) || !all_digits(f, fe)) {
+                bm_set_global_error("Error on line %lld: IDs must contain only numbers", (long long)line_number);
+                rc = BM_ERR_IO;
+                break;
+            }
+            fprintf(out, "m%.*s,f%.*s\n", (int)(me - m), m, (int)(fe - f), f);
+        } else {
+            fprintf(out, "%.*s,%.*s\n", (int)(me - m), m, (int)(fe - f), f);
+        }
+    }
+    if (rc == BM_ERR_ARG) {   /* a line the reference's sscanf would not match */
+        bm_set_global_error("Error on line %lld: Malformed input line", (long long)line_number);
+        rc = BM_ERR_IO;
+    }
+    fclose(in);
+    if (fclose(out) != 0 && rc == BM_OK) {
+        bm_set_global_error("write of %s failed", out_path);
+        rc = BM_ERR_IO;
+    }
+    if (n_lines) *n_lines = line_number;
+    return rc;
+}

@@ -94,6 +94,18 @@ int bm_load_mapping_csv(const char* path, int32_t n, int32_t* pi, int64_t* n_lin
 /* Writes "Male Node ID,Female Node ID" then "m<u>,f<pi[u]>" for pi[u] != 0.
  * Replaces write_mapping (src/graph-re-alignment-gpu-swap.cu:681-696). */
 int bm_write_mapping_csv(const char* path, int32_t n, const int32_t* pi);
+/* Converts a matching file between the two text forms, keeping line order
+ * and the id text: to_prefixed = 1 reads "male,female" lines (digits only)
+ * and writes "Male Node ID,Female Node ID" then "m<male>,f<female>", as
+ * transform-solution (src/transform-solution.c:63-92); to_prefixed = 0
+ * reads "m<male>,f<female>" lines and writes "Male ID,Female ID" then
+ * "<male>,<female>", as convert-VNC-matching (src/convert-VNC-matching.c:53-73).
+ * The first input line is a header and is skipped; lines are read 1,023
+ * bytes at a time, as the reference reads them.  A line that does not match
+ * stops the conversion with BM_ERR_IO ("Error on line <k>: ..." in
+ * bm_last_error(NULL)), the lines before it written.  *n_lines (may be NULL)
+ * = lines read, header included. */
+int bm_convert_mapping_csv(const char* in_path, const char* out_path, int to_prefixed, int64_t* n_lines);
 
 /* ------------------------------------------------------------------ */
 /* Synthetic workloads (host only)                                     */

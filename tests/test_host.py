@@ -125,3 +125,45 @@ def test_tool_reports_missing_input(bm, tmp_path):
                         str(tmp_path / "nope2.csv"), str(tmp_path / "pi.csv"), str(tmp_path / "out.csv")],
                        capture_output=True, text=True)
     assert r.returncode == 1 and "Failed to load male graph" in r.stderr
+
+
+CONVERT_CASES = sorted(os.listdir(os.path.join(ROOT, "tests", "golden", "convert")))
+
+
+@pytest.mark.parametrize("case", CONVERT_CASES)
+def test_matching_converters_match_reference(bm, tmp_path, case):
+    """transform-solution / convert-VNC-matching write the same bytes, print
+    the same lines and exit with the same status as the reference tools
+    (fixtures from the compiled reference, tests/golden/make_golden_convert.py):
+    CRLF lines, a last line without newline, a path without extension or with
+    two dots, a header-only file, a non-digit id, a blank line, a line
+    without prefix."""
+    d = os.path.join(ROOT, "tests", "golden", "convert", case)
+    tool, rc = open(os.path.join(d, "rc.txt")).read().split()
+    inp = [f for f in os.listdir(d) if f.startswith("input_")][0]
+    name = inp[len("input_"):]
+    with open(os.path.join(d, inp), "rb") as f:
+        (tmp_path / name).write_bytes(f.read())
+    r = subprocess.run([os.path.join(bm.BIN_DIR, tool), name], cwd=tmp_path, capture_output=True)
+    assert r.returncode == int(rc), r.stderr
+    assert r.stdout == open(os.path.join(d, "stdout.txt"), "rb").read()
+    dot = name.rfind(".")
+    out = (name[:dot] if dot >= 0 else name) + ("_matching.csv" if tool == "transform-solution" else "_generic.csv")
+    expected = os.path.join(d, "expected_output")
+    if os.path.exists(expected):
+        assert (tmp_path / out).read_bytes() == open(expected, "rb").read()
+    else:
+        assert not (tmp_path / out).exists()
+
+
+def test_converters_round_trip(bm, tmp_path):
+    """generic -> prefixed -> generic returns the same lines; the loader
+    reads all three forms as the same matching."""
+    pi = np.concatenate([[0], np.random.default_rng(4).permutation(np.arange(1, 51))]).astype(np.int32)
+    (tmp_path / "sol.csv").write_text("Male ID,Female ID\n" + "".join(f"{i},{pi[i]}\n" for i in range(1, 51)))
+    for tool, arg in (("transform-solution", "sol.csv"), ("convert-VNC-matching", "sol_matching.csv")):
+        r = subprocess.run([os.path.join(bm.BIN_DIR, tool), arg], cwd=tmp_path, capture_output=True)
+        assert r.returncode == 0, r.stderr
+    assert (tmp_path / "sol_matching_generic.csv").read_text() == (tmp_path / "sol.csv").read_text()
+    for f in ("sol.csv", "sol_matching.csv", "sol_matching_generic.csv"):
+        assert np.array_equal(bm.load_mapping_csv(str(tmp_path / f), 50), pi)
