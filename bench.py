@@ -358,7 +358,7 @@ def main():
 
 
 def bench_proposals(args, bm):
-    """Config C4: the learned proposal on the n = 18,524 pair.  The network is
+    """The learned-proposal workload on the n = 18,524 pair.  The network is
     trained on train_samples (candidate, exact delta) pairs from the context's
     own exact kernel; then three disjoint-swap climbs from the same greedy
     matching run for the same wall clock, each sending 4096 candidates per
@@ -421,9 +421,9 @@ def bench_proposals(args, bm):
 
 
 def bench_anneal(args, bm):
-    """Config C3: 64 independent synthetic problems of uneven size (n uniform in
-    1k..20k, ~200 n edges, lognormal weights, 30% of the true matching
-    shuffled), chains_per_instance annealing chains each (T_s = 50 * 0.9999^s,
+    """The batched-annealing workload: 64 independent synthetic problems of
+    uneven size (n uniform in 1k..20k, ~200 n edges, lognormal weights, 30% of
+    the true matching shuffled), chains_per_instance annealing chains each (T_s = 50 * 0.9999^s,
     swaps and 3-vertex moves), all advanced by one kernel launch replayed
     from a HIP graph; the launch shape comes from the autotuner.  Reports
     chain steps/s (one exact delta per step), the algorithmic GB/s of the
@@ -500,7 +500,7 @@ def bench_anneal(args, bm):
 
 
 def bench_scale(args, bm):
-    """Config C5: n = 100,000, ~20M directed edges, lognormal weights, 40% of
+    """The scale workload: n = 100,000, ~20M directed edges, lognormal weights, 40% of
     the true matching shuffled.  Exact swap-delta throughput of both
     strategies on 2^26 u-major pairs (the gain table is 40 GB here, built in
     column chunks, with per-pair lookups), then, for SECONDS each: a uniform
