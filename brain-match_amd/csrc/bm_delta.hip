@@ -163,6 +163,30 @@ __device__ __forceinline__ void look_many(const typename Slot<COMPACT>::T* lds, 
     }
     if (t.in_lds) {
         const S* T = lds + t.off;
+        /* first round for every key at once, with no per-key branch (a key 0
+         * reads slot 0 and is ignored); at load <= 3/4 most keys end here */
+        {
+            S e[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) e[j] = T[s[j]];
+            pend = 0;
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                bool hit;
+                int32_t w;
+                if (COMPACT) {
+                    hit = (uint32_t)(e[j] >> 15) == (uint32_t)key[j];
+                    w = (int32_t)(e[j] & 0x7fffu);
+                } else {
+                    hit = (int32_t)(uint32_t)e[j] == key[j];
+                    w = (int32_t)(uint32_t)(e[j] >> 32);
+                }
+                if (key[j] != 0 && hit) out[j] = w;
+                pend |= (key[j] != 0 && !hit && e[j] != (S)0 ? 1u : 0u) << j;
+                s[j] = (s[j] + 1u) & mask;
+            }
+            if (!__ballot(pend != 0)) return;
+        }
         while (pend) {
             S e[U];
 #pragma unroll
@@ -189,6 +213,20 @@ __device__ __forceinline__ void look_many(const typename Slot<COMPACT>::T* lds, 
         }
     } else {
         const int2* __restrict__ T = t.gt;
+        {   /* first round for every key at once (a key 0 reads slot 0 and is ignored) */
+            int2 e[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) e[j] = T[s[j]];
+            pend = 0;
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const bool hit = e[j].x == key[j];
+                if (key[j] != 0 && hit) out[j] = e[j].y;
+                pend |= (key[j] != 0 && !hit && e[j].x != 0 ? 1u : 0u) << j;
+                s[j] = (s[j] + 1u) & mask;
+            }
+            if (!__ballot(pend != 0)) return;
+        }
         while (pend) {
             int2 e[U];
 #pragma unroll
@@ -223,18 +261,19 @@ __device__ __forceinline__ void build(typename Slot<COMPACT>::T* lds, const Tab*
         for (uint32_t i = lane; i < n16; i += BM_WAVE) z[i] = make_uint4(0, 0, 0, 0);
     }
     wave_sync();
-    int64_t len[N], tot = 0;
+    /* positions in 32 bits: the tables fit the LDS slice, so the rows are short */
+    int32_t len[N], tot = 0;
 #pragma unroll
     for (int q = 0; q < N; q++) {
-        len[q] = t[q].in_lds ? t[q].re - t[q].rb : 0;
+        len[q] = t[q].in_lds ? (int32_t)(t[q].re - t[q].rb) : 0;
         tot += len[q];
     }
-    for (int64_t base = 0; base < tot; base += (int64_t)BM_WAVE * U) {
+    for (int32_t base = 0; base < tot; base += BM_WAVE * U) {
         int2 kv[U];
         int which[U];
 #pragma unroll
         for (int j = 0; j < U; j++) {
-            int64_t i = base + lane + (int64_t)BM_WAVE * j;
+            int32_t i = base + lane + BM_WAVE * j;
             which[j] = -1;
             kv[j] = make_int2(0, 0);
             if (i < tot) {
@@ -266,7 +305,20 @@ __device__ __forceinline__ void build(typename Slot<COMPACT>::T* lds, const Tab*
                 packed[j] = ((unsigned long long)(uint32_t)kv[j].y << 32) | (uint32_t)kv[j].x;
             pend |= (which[j] >= 0 ? 1u : 0u) << j;
         }
-        /* interleaved inserts: all pending compare-and-swaps issued per round */
+        /* interleaved inserts: all pending compare-and-swaps issued per
+         * round; after the first, only entries whose slot was taken go on */
+        {
+            S old[U];
+#pragma unroll
+            for (int j = 0; j < U; j++)
+                old[j] = which[j] >= 0 ? atomicCAS(&lds[toff[j] + slot[j]], (S)0, packed[j]) : (S)0;
+            pend = 0;
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                pend |= (which[j] >= 0 && old[j] != (S)0 ? 1u : 0u) << j;
+                slot[j] = (slot[j] + 1u) & msk[j];
+            }
+        }
         while (pend) {
             S old[U];
 #pragma unroll
@@ -301,13 +353,14 @@ __device__ __forceinline__ int64_t swap_scan(const int2* __restrict__ adj, const
                                              const typename Slot<COMPACT>::T* lds, const Tab& A, const Tab& B,
                                              int32_t u, int32_t v, int32_t f1, int32_t f2, int lane) {
     int64_t acc = 0;
-    const int64_t L = L0 + L1;
-    for (int64_t base = 0; base < L; base += (int64_t)BM_WAVE * U) {
+    /* list positions in 32 bits (a list is shorter than 2^31) */
+    const int32_t M0 = (int32_t)L0, M = (int32_t)(L0 + L1);
+    for (int32_t base = 0; base < M; base += BM_WAVE * U) {
         int2 ed[U];
 #pragma unroll
         for (int j = 0; j < U; j++) {
-            const int64_t i = base + lane + (int64_t)BM_WAVE * j;
-            ed[j] = i < L ? adj[i < L0 ? b0 + i : b1 + (i - L0)] : make_int2(0, 0);
+            const int32_t i = base + lane + BM_WAVE * j;
+            ed[j] = i < M ? adj[i < M0 ? b0 + i : b1 + (i - M0)] : make_int2(0, 0);
         }
         int32_t gx[U];
 #pragma unroll
@@ -316,7 +369,7 @@ __device__ __forceinline__ int64_t swap_scan(const int2* __restrict__ adj, const
 #pragma unroll
         for (int j = 0; j < U; j++) {
             const int32_t x = ed[j].x;
-            const bool side = base + lane + (int64_t)BM_WAVE * j >= L0;  /* false: list of u */
+            const bool side = base + lane + BM_WAVE * j >= M0;  /* false: list of u */
             int32_t go = gx[j], gn = gx[j];
             if (OUTP) {
                 if (x == u) { go = f1; gn = f2; }
@@ -332,7 +385,7 @@ __device__ __forceinline__ int64_t swap_scan(const int2* __restrict__ adj, const
         look_many<COMPACT, U>(lds, B, kB, rB);
 #pragma unroll
         for (int j = 0; j < U; j++) {
-            const bool side = base + lane + (int64_t)BM_WAVE * j >= L0;
+            const bool side = base + lane + BM_WAVE * j >= M0;
             const int32_t w = ed[j].y;
             const int32_t cA = contrib(w, rA[j]);
             const int32_t cB = contrib(w, rB[j]);
