@@ -107,19 +107,53 @@ struct Slot<false> {
     typedef unsigned long long T;
 };
 
+/* LDS tables are linear-probing tables whose home slots are the first slots
+ * of 16-byte buckets (4 compact or 2 wide slots): a key's probe sequence
+ * starts at its home bucket, so a lookup reads a whole bucket per round and
+ * most lookups, present or absent, end in the first round. */
+template <bool COMPACT>
+struct Bucket {
+    static constexpr uint32_t kLog = COMPACT ? 2u : 1u;   /* log2 slots per 16-byte bucket */
+    static constexpr uint32_t kSlots = 1u << kLog;
+};
+
+/* First slot of key's home bucket in a table of 2^bits slots (bits >= 2). */
+template <bool COMPACT>
+__device__ __forceinline__ uint32_t lds_home(int32_t key, uint32_t bits) {
+    const uint32_t nb = bits - Bucket<COMPACT>::kLog;
+    return nb ? slot_hash(key, nb) << Bucket<COMPACT>::kLog : 0u;
+}
+
+/* One bucket of a lookup: w = key's weight if the bucket holds it; stop =
+ * the lookup ends here (key found, or an empty slot shows it is absent). */
+template <bool COMPACT>
+__device__ __forceinline__ void bucket_find(const typename Slot<COMPACT>::T* T, uint32_t s, int32_t key, int32_t& w,
+                                            bool& stop) {
+    const uint4 v = *reinterpret_cast<const uint4*>(T + s);
+    if (COMPACT) {
+        const uint32_t k = (uint32_t)key;
+        const bool h0 = (v.x >> 15) == k, h1 = (v.y >> 15) == k, h2 = (v.z >> 15) == k, h3 = (v.w >> 15) == k;
+        const uint32_t e = h0 ? v.x : (h1 ? v.y : (h2 ? v.z : v.w));
+        const bool hit = h0 || h1 || h2 || h3;
+        w = hit ? (int32_t)(e & 0x7fffu) : BM_ABSENT;
+        stop = hit || v.x == 0u || v.y == 0u || v.z == 0u || v.w == 0u;
+    } else {   /* slots {key, w} in (x, y) and (z, w) */
+        const bool h0 = (int32_t)v.x == key, h1 = (int32_t)v.z == key;
+        w = h0 ? (int32_t)v.y : (h1 ? (int32_t)v.w : BM_ABSENT);
+        stop = h0 || h1 || v.x == 0u || v.z == 0u;
+    }
+}
+
 template <bool COMPACT>
 __device__ __forceinline__ int32_t probe_lds(const typename Slot<COMPACT>::T* T, uint32_t bits, int32_t key) {
     const uint32_t mask = (1u << bits) - 1u;
-    uint32_t s = slot_hash(key, bits);
+    uint32_t s = lds_home<COMPACT>(key, bits);
     for (;;) {
-        const typename Slot<COMPACT>::T e = T[s];
-        if (e == 0) return BM_ABSENT;
-        if (COMPACT) {
-            if ((uint32_t)(e >> 15) == (uint32_t)key) return (int32_t)(e & 0x7fffu);
-        } else {
-            if ((int32_t)(uint32_t)e == key) return (int32_t)(uint32_t)(e >> 32);
-        }
-        s = (s + 1u) & mask;
+        int32_t w;
+        bool stop;
+        bucket_find<COMPACT>(T, s, key, w, stop);
+        if (stop) return w;
+        s = (s + Bucket<COMPACT>::kSlots) & mask;
     }
 }
 
@@ -158,57 +192,33 @@ __device__ __forceinline__ void look_many(const typename Slot<COMPACT>::T* lds, 
     uint32_t pend = 0;
 #pragma unroll
     for (int j = 0; j < U; j++) {
-        s[j] = key[j] != 0 ? slot_hash(key[j], bits) : 0u;
+        s[j] = key[j] != 0 ? (t.in_lds ? lds_home<COMPACT>(key[j], bits) : slot_hash(key[j], bits)) : 0u;
         pend |= (key[j] != 0 ? 1u : 0u) << j;
     }
     if (t.in_lds) {
         const S* T = lds + t.off;
-        /* first round for every key at once, with no per-key branch (a key 0
-         * reads slot 0 and is ignored); at load <= 3/4 most keys end here */
-        {
-            S e[U];
+        /* first round (the home bucket) for every key at once, with no
+         * per-key branch (a key 0 reads bucket 0 and is ignored) */
+        pend = 0;
 #pragma unroll
-            for (int j = 0; j < U; j++) e[j] = T[s[j]];
-            pend = 0;
-#pragma unroll
-            for (int j = 0; j < U; j++) {
-                bool hit;
-                int32_t w;
-                if (COMPACT) {
-                    hit = (uint32_t)(e[j] >> 15) == (uint32_t)key[j];
-                    w = (int32_t)(e[j] & 0x7fffu);
-                } else {
-                    hit = (int32_t)(uint32_t)e[j] == key[j];
-                    w = (int32_t)(uint32_t)(e[j] >> 32);
-                }
-                if (key[j] != 0 && hit) out[j] = w;
-                pend |= (key[j] != 0 && !hit && e[j] != (S)0 ? 1u : 0u) << j;
-                s[j] = (s[j] + 1u) & mask;
-            }
-            if (!__ballot(pend != 0)) return;
+        for (int j = 0; j < U; j++) {
+            int32_t w;
+            bool stop;
+            bucket_find<COMPACT>(T, s[j], key[j], w, stop);
+            if (key[j] != 0) out[j] = w;
+            pend |= (key[j] != 0 && !stop ? 1u : 0u) << j;
+            s[j] = (s[j] + Bucket<COMPACT>::kSlots) & mask;
         }
         while (pend) {
-            S e[U];
-#pragma unroll
-            for (int j = 0; j < U; j++) e[j] = ((pend >> j) & 1u) ? T[s[j]] : (S)0;
 #pragma unroll
             for (int j = 0; j < U; j++) {
                 if (!((pend >> j) & 1u)) continue;
-                bool hit;
                 int32_t w;
-                if (COMPACT) {
-                    hit = (uint32_t)(e[j] >> 15) == (uint32_t)key[j];
-                    w = (int32_t)(e[j] & 0x7fffu);
-                } else {
-                    hit = (int32_t)(uint32_t)e[j] == key[j];
-                    w = (int32_t)(uint32_t)(e[j] >> 32);
-                }
-                if (e[j] == 0 || hit) {
-                    if (hit) out[j] = w;
-                    pend &= ~(1u << j);
-                } else {
-                    s[j] = (s[j] + 1u) & mask;
-                }
+                bool stop;
+                bucket_find<COMPACT>(T, s[j], key[j], w, stop);
+                out[j] = w;
+                if (stop) pend &= ~(1u << j);
+                else s[j] = (s[j] + Bucket<COMPACT>::kSlots) & mask;
             }
         }
     } else {
@@ -298,7 +308,7 @@ __device__ __forceinline__ void build(typename Slot<COMPACT>::T* lds, const Tab*
                 if (which[j] == q) { off = t[q].off; bits = t[q].lbits; }
             msk[j] = (1u << bits) - 1u;
             toff[j] = off;
-            slot[j] = slot_hash(kv[j].x, bits);
+            slot[j] = lds_home<COMPACT>(kv[j].x, bits);
             if (COMPACT)
                 packed[j] = ((uint32_t)kv[j].x << 15) | (uint32_t)kv[j].y;
             else
