@@ -316,12 +316,27 @@ __device__ __forceinline__ void build(typename Slot<COMPACT>::T* lds, const Tab*
             pend |= (which[j] >= 0 ? 1u : 0u) << j;
         }
         /* interleaved inserts: all pending compare-and-swaps issued per
-         * round; after the first, only entries whose slot was taken go on */
+         * round.  The first round reads each entry's home bucket and aims at
+         * its first empty slot (slots before it stay taken, so the probe
+         * sequence stays unbroken); after it, only entries that found the
+         * bucket full or lost a race go on, slot by slot. */
         {
+            uint32_t first[U];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const uint4 v = *reinterpret_cast<const uint4*>(lds + toff[j] + slot[j]);
+                if (COMPACT)
+                    first[j] = v.x == 0u ? 0u : (v.y == 0u ? 1u : (v.z == 0u ? 2u : (v.w == 0u ? 3u : 4u)));
+                else
+                    first[j] = v.x == 0u ? 0u : (v.z == 0u ? 1u : 2u);
+            }
             S old[U];
 #pragma unroll
-            for (int j = 0; j < U; j++)
-                old[j] = which[j] >= 0 ? atomicCAS(&lds[toff[j] + slot[j]], (S)0, packed[j]) : (S)0;
+            for (int j = 0; j < U; j++) {
+                const bool go = which[j] >= 0 && first[j] < Bucket<COMPACT>::kSlots;
+                slot[j] = (slot[j] + (go ? first[j] : Bucket<COMPACT>::kSlots - 1u)) & msk[j];
+                old[j] = go ? atomicCAS(&lds[toff[j] + slot[j]], (S)0, packed[j]) : (S)1;
+            }
             pend = 0;
 #pragma unroll
             for (int j = 0; j < U; j++) {
