@@ -1117,9 +1117,14 @@ __global__ void k_moves_from_perm3(const int32_t* __restrict__ pi_before, int32_
 /* ------------------------------------------------------------------ */
 
 int32_t bm_gain_chunk_cols(int64_t stride) {
-    /* at most 20,384 columns (81,540 bytes of int32 with the edge counter:
-     * two blocks per CU in 160 KiB of LDS), split evenly into multiples of 32 */
-    const int64_t cap = 20384;
+    /* at most 40,768 columns (163,076 bytes of int32 with the edge counter:
+     * one block per CU in 160 KiB of LDS), split evenly into multiples of 32.
+     * A row of up to 20,384 columns still gets two blocks per CU; a wider
+     * table is built in fewer, wider windows at one block per CU, which at
+     * n = 100,000 beats more windows at two blocks per CU (three windows of
+     * 33,344 columns: 81.8 ms; five of 20,032: 93.5 ms), since every window
+     * walks the whole work list again. */
+    const int64_t cap = 40768;
     const int64_t chunks = (stride + cap - 1) / cap;
     const int64_t w = ((stride + chunks - 1) / chunks + 31) & ~(int64_t)31;
     return (int32_t)(w < stride ? w : stride);

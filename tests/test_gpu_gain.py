@@ -176,8 +176,8 @@ def test_full_size_gain_equals_scan(bm, orc):
 
 
 def test_multi_chunk_build_equals_scan(bm, orc):
-    """n=45,000: the table is built in three column windows (each row's
-    female entries split by binary search); rows match their definition,
+    """n=45,000: the table is built in two column windows (each row's
+    female entries split at precomputed window boundaries); rows match their definition,
     table deltas match the scan kernels on uniform and u-major streams (the
     u-major stream falls back to per-pair lookups at this n), and a
     rebuild after moves matches the patched table."""
