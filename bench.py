@@ -502,8 +502,9 @@ def bench_anneal(args, bm):
 def bench_scale(args, bm):
     """The scale workload: n = 100,000, ~20M directed edges, lognormal weights, 40% of
     the true matching shuffled.  Exact swap-delta throughput of both
-    strategies on 2^26 u-major pairs (the gain table is 40 GB here, built in
-    column chunks, with per-pair lookups), then, for SECONDS each: a uniform
+    strategies on 2^28 u-major pairs per step, the step of the n = 18,524
+    benchmark (the gain table is 40 GB here, rebuilt every step in column
+    windows, with per-pair lookups), then, for SECONDS each: a uniform
     and a learned-proposal climb (4096 exact candidates per round, evaluated
     from the gain table with --evaluator gain, by the scan kernel with
     --evaluator scan), and annealing chains with swaps and 3-vertex moves
@@ -515,7 +516,7 @@ def bench_scale(args, bm):
     ctx.set_mapping(pi0)
     score0 = ctx.score()
     log(f"scale pair: n={n}, {male.m} male edges, initial score {score0} ({time.perf_counter() - t:.1f}s)")
-    K = 1 << 26
+    K = 1 << 28
     b = ctx.batch(K)
     ctx.timing_enable(True)
     out = {"initial_score": score0, "male_edges": male.m}
