@@ -650,6 +650,24 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const int2* __restrict__ cand,
                                                    const int64_t* __restrict__ off, uint64_t seed, int64_t k,
                                                    int64_t* __restrict__ delta) {
+    if (MODE == BM_CAND_UMAJOR) {
+        /* a u-major stream is read in one contiguous range per block, so a
+         * thread's next entry (blockDim later) almost always has the same
+         * first vertex or the next one: one offset load confirms it, and a
+         * binary search is needed only after a long jump */
+        const int64_t per = (k + gridDim.x - 1) / gridDim.x;
+        const int64_t c0 = (int64_t)blockIdx.x * per, c1 = c0 + per < k ? c0 + per : k;
+        int32_t u = 0;
+        for (int64_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+            if (u == 0 || off[u + 1] <= c) {
+                if (u != 0 && u < g.n && off[u + 2] > c) u++;
+                else u = umajor_owner(off, g.n, c);
+            }
+            const int32_t v = bm_umajor_partner(seed, (uint64_t)c, u, g.n);
+            delta[c] = gain_swap_delta(g, G, u, v);
+        }
+        return;
+    }
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < k; c += stride) {
         int32_t u, v;
@@ -657,9 +675,6 @@ __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const
             const int2 p = cand[c];
             u = p.x;
             v = p.y;
-        } else if (MODE == BM_CAND_UMAJOR) {
-            u = umajor_owner(off, g.n, c);
-            v = bm_umajor_partner(seed, (uint64_t)c, u, g.n);
         } else {
             bm_uniform_pair(seed, (uint64_t)c, g.n, &u, &v);
         }
