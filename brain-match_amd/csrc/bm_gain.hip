@@ -46,26 +46,6 @@ constexpr int kDescLong = 1024;             /* rows longer than this lead their 
 constexpr int kPatchThreads = 512;
 constexpr int kPatchWaves = kPatchThreads / BM_WAVE;
 
-__device__ __forceinline__ int32_t probe_g(const int2* __restrict__ T, uint32_t bits, int32_t key) {
-    if (!bits || key == 0) return BM_ABSENT;
-    const uint32_t mask = (1u << bits) - 1u;
-    uint32_t s = slot_hash(key, bits);
-    for (;;) {
-        const int2 e = T[s];
-        if (e.x == 0) return BM_ABSENT;
-        if (e.x == key) return e.y;
-        s = (s + 1u) & mask;
-    }
-}
-
-/* w_F(a, b) or BM_ABSENT. */
-__device__ __forceinline__ int32_t female_w(const BmDevGraph& g, int32_t a, int32_t b) {
-    if (a == 0 || b == 0) return BM_ABSENT;
-    const uint64_t d = g.f_out_desc[a];
-    return probe_g(g.f_out_tab + (d >> 6), (uint32_t)(d & 63u), b);
-}
-
-/* w_M(a, b) or BM_ABSENT. */
 /* Word of the dense adjacency bitmap holding the pair {a, b}: bit (max & 31)
  * of row min(a, b).  The bitmap is symmetric, set when a -> b or b -> a is a
  * male edge, so one load answers both directions. */
@@ -78,14 +58,6 @@ __device__ __forceinline__ uint32_t adj_word(const BmGain& G, int32_t a, int32_t
  * true when it shows one (either direction) or there is no bitmap. */
 __device__ __forceinline__ bool male_maybe(const BmGain& G, int32_t a, int32_t b) {
     return !G.m_adj || ((adj_word(G, a, b) >> ((a < b ? b : a) & 31)) & 1u);
-}
-
-/* w_M(a, b) or BM_ABSENT.  With the dense adjacency bitmap a pair without
- * an edge (the common case) costs one load instead of a hash probe sequence. */
-__device__ __forceinline__ int32_t male_w(const BmGain& G, int32_t a, int32_t b) {
-    if (!male_maybe(G, a, b)) return BM_ABSENT;
-    const uint64_t d = G.m_out_desc[a];
-    return probe_g(G.m_out_tab + (d >> 6), (uint32_t)(d & 63u), b);
 }
 
 /* K hash lookups at once: key[k] in table (T[k], bits[k]) where go[k], else
