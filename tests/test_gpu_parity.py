@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def c1(bm, orc):
-    """Config-1 shape: n=2000, ~40k edges, geometric weights, 30% shuffled."""
+    """The CPU-runnable pair: n=2000, ~40k edges, geometric weights, 30% shuffled."""
     male, female, truth, pi0 = bm.synth_pair(2000, 20.0, bm.WEIGHTS_GEOMETRIC, 0.3, seed=1)
     ctx = bm.Context(2000, male, female)
     ctx.set_mapping(pi0)
@@ -261,3 +261,47 @@ def test_full_size_sampled(bm, orc):
         assert np.array_equal(best, rb) and np.array_equal(perm.astype(np.int32), rp)
         b.close()
         b2.close()
+
+
+@pytest.mark.parametrize("case", ["male_empty", "female_empty", "both_empty", "n2", "unmatched"])
+def test_degenerate_inputs(bm, orc, case):
+    """Graphs without edges, the smallest vertex count for a swap (n = 2)
+    and a matching with every vertex unmatched: score, every pair's swap
+    delta and (n >= 3) 3-vertex deltas equal the oracle's on both the scan
+    and the table path, and the u-major stream's table deltas too."""
+    rng = np.random.default_rng(17)
+    n = 2 if case == "n2" else 40
+
+    def graph(m):
+        return bm.Graph(rng.integers(1, n + 1, m).astype(np.int32), rng.integers(1, n + 1, m).astype(np.int32),
+                        rng.integers(1, 9, m).astype(np.int32), n)
+
+    empty = bm.Graph(np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32), n)
+    male = empty if case in ("male_empty", "both_empty") else graph(3 * n)
+    female = empty if case in ("female_empty", "both_empty") else graph(3 * n)
+    pi = np.concatenate([[0], rng.permutation(np.arange(1, n + 1))]).astype(np.int32)
+    if case == "unmatched":
+        pi[:] = 0
+    uv = np.array([(u, v) for u in range(1, n + 1) for v in range(1, n + 1) if u != v], np.int32)
+    abc = np.stack([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(300)]).astype(np.int32) \
+        if n >= 3 else None
+    with bm.Context(n, male, female) as ctx:
+        ctx.set_mapping(pi)
+        o = orc.Oracle(n, male, female)
+        assert ctx.score() == o.score(pi)
+        ref = o.swap_deltas(pi, uv)
+        if abc is not None:
+            rb, rp = o.rot3_many(pi, abc)
+        for gain in (False, True):
+            ctx.gain_enable(gain)
+            assert np.array_equal(ctx.swap_deltas(uv), ref), (case, gain)
+            if abc is not None:
+                best, perm = ctx.rot3_deltas(abc)
+                assert np.array_equal(best, rb) and np.array_equal(perm.astype(np.int32), rp), (case, gain)
+            k = 4096
+            b = ctx.batch(k)
+            b.eval_swaps_umajor(3, k)
+            assert np.array_equal(b.deltas(), o.swap_deltas(pi, orc.umajor_pairs(3, k, n))), (case, gain)
+            b.close()
+        if case in ("male_empty", "both_empty", "female_empty", "unmatched"):
+            assert not ref.any()
