@@ -305,3 +305,34 @@ def test_degenerate_inputs(bm, orc, case):
             b.close()
         if case in ("male_empty", "both_empty", "female_empty", "unmatched"):
             assert not ref.any()
+
+
+def test_extreme_weights(bm, orc):
+    """Weights at the ends of int32 (male and female, negative too): the scan
+    kernels' int64 sums stay exact, and the table refuses a graph whose
+    entries could overflow int32."""
+    rng = np.random.default_rng(23)
+    n, m = 30, 200
+    big = np.array([2**31 - 1, -(2**31) + 1, 2**30, -(2**30), 1, -1, 0, 32768], np.int64)
+
+    def graph():
+        w = rng.choice(big, m).astype(np.int32)
+        return bm.Graph(rng.integers(1, n + 1, m).astype(np.int32), rng.integers(1, n + 1, m).astype(np.int32), w, n)
+
+    male, female = graph(), graph()
+    pi = np.concatenate([[0], rng.permutation(np.arange(1, n + 1))]).astype(np.int32)
+    uv = np.array([(u, v) for u in range(1, n + 1) for v in range(1, n + 1) if u != v], np.int32)
+    abc = np.stack([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(500)]).astype(np.int32)
+    with bm.Context(n, male, female) as ctx:
+        ctx.set_mapping(pi)
+        o = orc.Oracle(n, male, female)
+        assert ctx.score() == o.score(pi)
+        d = ctx.swap_deltas(uv)
+        assert np.array_equal(d, o.swap_deltas(pi, uv))
+        assert np.abs(d).max() > 2**31   # the sums leave int32
+        best, perm = ctx.rot3_deltas(abc)
+        rb, rp = o.rot3_many(pi, abc)
+        assert np.array_equal(best, rb) and np.array_equal(perm.astype(np.int32), rp)
+        with pytest.raises(bm.BMError) as e:
+            ctx.gain_enable(True)
+        assert e.value.code == -1
