@@ -188,7 +188,7 @@ def test_global_table_path_matches(c1, bm, orc):
         assert np.array_equal(best, rb) and np.array_equal(perm.astype(np.int32), rp)
     finally:
         for k, t in saved.items():
-            ctx.set_tuning(k, t["waves_per_block"], t["lds_slots"], t["blocks_per_cu"])
+            ctx.set_tuning(k, t["waves_per_block"], t["lds_slots"], t["blocks_per_cu"], t["group_lanes"], t["unroll"])
 
 
 def test_accept_disjoint_matches_oracle(c1, orc):
@@ -336,3 +336,19 @@ def test_extreme_weights(bm, orc):
         with pytest.raises(bm.BMError) as e:
             ctx.gain_enable(True)
         assert e.value.code == -1
+
+
+def test_hillclimb_ten_thousand_rounds(c1, orc):
+    """The CPU-runnable pair's climbing workload: 10,000 rounds of greedy
+    disjoint-swap climbing; the running score is checked against a device
+    recount after every round, and the final matching against the oracle's
+    score-matching sum."""
+    ctx, o, pi = c1["ctx"], c1["o"], c1["pi0"]
+    ctx.set_mapping(pi)
+    trace, res = ctx.hillclimb(seed=77, rounds=10000, batch=4096, verify=True)
+    assert res["verify_failures"] == 0
+    assert len(trace) == 10001 and (np.diff(trace) >= 0).all()
+    final = ctx.get_mapping()
+    assert o.score(final) == trace[-1] == ctx.score()
+    assert trace[-1] > trace[0]
+    ctx.set_mapping(pi)
