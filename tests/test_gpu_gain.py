@@ -209,6 +209,69 @@ def test_multi_chunk_build_equals_scan(bm, orc):
         b.close()
 
 
+def test_scale_pair_windowed_build(bm, orc):
+    """n=100,000, ~20M edges per graph, 40% of the matching shuffled (the
+    scale stress pair): the 40 GB table is built in three column windows at
+    one block per CU.  Whole rows (every column, so every window) of five
+    vertices up to degree 1,500 match their definition; the two largest hubs
+    (adjacent to nearly every vertex, rows too costly for the oracle's
+    O(n * degree) row) are checked through 2048 swap deltas each against the
+    oracle; table deltas equal the scan kernels on 2^16 uniform pairs and the
+    oracle on 4096 of them; 3-vertex deltas match the oracle; ten rounds of
+    disjoint-swap climbing with the patched table replay exactly on the CPU;
+    and a rebuild after the climb equals the patched table."""
+    n = 100_000
+    male, female, truth, pi0 = bm.synth_pair(n, 200.0, bm.WEIGHTS_LOGNORMAL, 0.4, seed=5)
+    print(f"scale pair n={n}, {male.m} male edges", flush=True)
+    o = orc.Oracle(n, male, female)
+    with bm.Context(n, male, female) as ctx:
+        ctx.set_mapping(pi0)
+        k = 1 << 16
+        b = ctx.batch(k)
+        b.eval_swaps_uniform(31, k)
+        d_scan = b.deltas()
+        ctx.gain_enable(True)
+        assert ctx.gain_enabled()
+        b.eval_swaps_uniform(31, k)
+        d_gain = b.deltas()
+        assert np.array_equal(d_gain, d_scan)
+        uv = orc.uniform_pairs(31, k, n)
+        rng = np.random.default_rng(2)
+        idx = rng.choice(k, 4096, replace=False)
+        assert np.array_equal(d_gain[idx], o.swap_deltas(pi0, uv[idx]))
+        print("table equals scan and oracle on uniform pairs", flush=True)
+        deg = np.bincount(male.src, minlength=n + 1) + np.bincount(male.dst, minlength=n + 1)
+        hubs = [int(x) for x in np.argsort(-deg)[:2] if x > 0]
+        bounded = np.nonzero(deg <= 1500)[0]
+        heavy = int(bounded[np.argmax(deg[bounded])])
+        for u in [1, 2, 50000, n, heavy]:
+            assert np.array_equal(ctx.gain_row(u), orc.gain_row(o, pi0, u)), u
+        print(f"rows match the definition (vertex {heavy} of degree {deg[heavy]})", flush=True)
+        for h in hubs:
+            uvh = np.stack([np.full(2048, h), rng.integers(1, n + 1, 2048)], 1).astype(np.int32)
+            assert np.array_equal(ctx.swap_deltas(uvh), o.swap_deltas(pi0, uvh)), h
+        print(f"hub deltas match (degrees {[int(deg[h]) for h in hubs]})", flush=True)
+        abc = np.stack([rng.choice(np.arange(1, n + 1), 3, replace=False) for _ in range(2000)]).astype(np.int32)
+        best, perm = ctx.rot3_deltas(abc)
+        rb, rp = o.rot3_many(pi0, abc)
+        assert np.array_equal(best, rb) and np.array_equal(perm.astype(np.int32), rp)
+        print("3-vertex deltas match", flush=True)
+        trace, res = ctx.hillclimb(seed=77, rounds=10, batch=4096, verify=True)
+        pi_o, trace_o = o.hillclimb(pi0, 77, 10, 4096)
+        assert res["verify_failures"] == 0
+        assert np.array_equal(trace, trace_o)
+        pi1 = ctx.get_mapping()
+        assert np.array_equal(pi1, pi_o)
+        print(f"climb replays on the CPU ({trace[0]} -> {trace[-1]})", flush=True)
+        moved = [int(x) for x in np.nonzero(pi1 != pi0)[0][:4]]
+        rows = moved + hubs
+        patched = [ctx.gain_row(u) for u in rows]
+        ctx.gain_rebuild()
+        for u, row in zip(rows, patched):
+            assert np.array_equal(ctx.gain_row(u), row), u
+        b.close()
+
+
 @pytest.mark.parametrize("wmax", [8191, 8192, 32767, 32768])
 def test_row_layouts_at_weight_boundaries(bm, orc, wmax):
     """The table build reads female rows packed (w << 19) | 4f when female
