@@ -146,6 +146,8 @@ _sig("bm_batch_gen_amajor", C.c_int, [_c, C.c_uint64, C.c_int64])
 _sig("bm_batch_eval_rot3_amajor", C.c_int, [_c, C.c_uint64, C.c_int64])
 _sig("bm_gain_table_bytes", C.c_int64, [C.c_int32])
 _sig("bm_gain_row", C.c_int, [_c, C.c_int32, i32p])
+_sig("bm_best_swap", C.c_int, [_c, i64p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)])
+_sig("bm_best_swap_climb", C.c_int, [_c, C.c_int64, C.c_int64, i64p, C.POINTER(_SearchResult)])
 _sig("bm_twohop_set", C.c_int, [_c, C.c_int32, i32p, C.POINTER(C.c_int32)])
 _sig("bm_batch_gen_twohop", C.c_int, [_c, i32p, C.c_int32, C.c_int32, C.c_uint64])
 _sig("bm_hillclimb_twohop", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32, i64p,
@@ -541,6 +543,20 @@ class Context:
         row = np.zeros(self.n + 1, np.int32)
         self._chk(_lib.bm_gain_row(self.h, u, _p32(row)))
         return row
+
+    def best_swap(self):
+        """(delta, u, v) of the best swap over all pairs u < v (needs the table)."""
+        d = np.zeros(1, np.int64)
+        u, v = C.c_int32(0), C.c_int32(0)
+        self._chk(_lib.bm_best_swap(self.h, _p64(d), C.byref(u), C.byref(v)))
+        return int(d[0]), u.value, v.value
+
+    def best_swap_climb(self, max_iters, verify_every=1):
+        trace = np.zeros(max_iters + 1, np.int64)
+        r = _SearchResult()
+        self._chk(_lib.bm_best_swap_climb(self.h, max_iters, verify_every, _p64(trace), C.byref(r)))
+        res = {k: getattr(r, k) for k, _ in _SearchResult._fields_}
+        return trace[:res["iterations"] + 1], res
 
     def hillclimb_twohop(self, seed, rounds, anchors, per_anchor, verify=True):
         trace = np.zeros(rounds + 1, np.int64)

@@ -89,6 +89,7 @@ struct bm_ctx {
     int32_t nwin;
     int64_t n_echunks;
     int64_t gt_version;
+    int4* d_best;           /* best-swap result, then kBestSwapParts per-block results */
     char err[512];
 };
 
@@ -257,6 +258,7 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_f_in_bnd);
     (void)hipFree(c->d_f_out_bnd);
     (void)hipFree(c->d_m_adj);
+    (void)hipFree(c->d_best);
     bm_graph_free(&c->male);
     free(c->f_out_deg);
     free(c->f_in_deg);
@@ -1134,6 +1136,38 @@ extern "C" int bm_apply_swap(bm_ctx* c, int32_t u, int32_t v) {
     }
     HIPCHK(c, bm_launch_apply_swap(c->d_pi, u, v, c->stream));
     if (c->gain_on) return gain_patch(c, c->d_nmoves, 2);
+    return BM_OK;
+}
+
+/* Exact best swap over all n(n-1)/2 pairs from the placement-gain table and
+ * its transpose (k_gain_best_swap): the maximum that the score-lookup
+ * driver's heap holds (src/best-swap/score-lookup.c:246-293 fills the n x n
+ * delta table, :612-620 extracts the maximum), computed afresh on every call,
+ * so no entry is stale. */
+extern "C" int bm_best_swap(bm_ctx* c, int64_t* delta, int32_t* u, int32_t* v) {
+    if (!c || !delta || !u || !v) return fail(c, BM_ERR_ARG, "bm_best_swap: null argument");
+    if (!c->gain_on) return fail(c, BM_ERR_ARG, "bm_best_swap: needs the placement-gain table (bm_gain_enable)");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = gain_transposed(c);
+    if (rc != BM_OK) return rc;
+    if (!c->d_best && (rc = dalloc(c, &c->d_best, (size_t)kBestSwapParts + 1)) != BM_OK) return rc;
+    KTiming t;
+    timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
+    HIPCHK(c, bm_launch_gain_pis(devgraph(c), devgain(c), c->d_pis, c->stream));
+    HIPCHK(c, bm_launch_gain_best_swap(devgraph(c), devgain(c), c->d_GT, c->d_pis, c->d_best + 1, kBestSwapParts,
+                                       c->d_best, c->num_cus, c->stream));
+    timing_end(c, &t);
+    int4 h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_best, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (h.z == INT32_MAX) {   /* n < 2: no pair */
+        *delta = 0;
+        *u = *v = 0;
+        return BM_OK;
+    }
+    *delta = (int64_t)(((uint64_t)(uint32_t)h.y << 32) | (uint64_t)(uint32_t)h.x);
+    *u = h.z;
+    *v = h.w;
     return BM_OK;
 }
 

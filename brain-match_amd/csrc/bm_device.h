@@ -104,6 +104,12 @@ hipError_t bm_launch_gain_rot3_amajor(const BmDevGraph& g, const BmGain& G, cons
 hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
                                       const int64_t* off, uint64_t seed, int64_t* delta, int32_t num_cus,
                                       hipStream_t s);
+/* Exact best swap over all pairs u < v (larger delta, then smaller u, then
+ * smaller v) into out[0] = {delta lo, delta hi, u, v}; u = INT32_MAX when
+ * n < 2.  part holds max_parts per-block results. */
+constexpr int32_t kBestSwapParts = 2048;
+hipError_t bm_launch_gain_best_swap(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
+                                    int4* part, int32_t max_parts, int4* out, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_rot3(const BmDevGraph& g, const BmGain& G, const int4* cand, int64_t k, int64_t* best,
                                uint8_t* perm, int4* cand_out_perm, hipStream_t s);
 hipError_t bm_launch_moves_from_swap(const int32_t* pi_before, int32_t u, int32_t v, int4* moves, int64_t* count,

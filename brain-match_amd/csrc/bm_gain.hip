@@ -590,15 +590,16 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
 /* Lookups                                                             */
 /* ------------------------------------------------------------------ */
 
-__device__ __forceinline__ int64_t gain_swap_delta(const BmDevGraph& g, const BmGain& G, int32_t u, int32_t v) {
-    if (u == v) return 0;
-    const int32_t f1 = g.pi[u], f2 = g.pi[v];
-    const bool e_uv = male_maybe(G, u, v);   /* either direction */
+/* Swap delta of u != v from its four table entries guf2 = G(u, f2),
+ * guf1 = G(u, f1), gvf1 = G(v, f1), gvf2 = G(v, f2), with e_uv = "a male
+ * edge between u and v may exist" (either direction): an edge's weight and
+ * the female weights its correction needs are hash-probed only then. */
+__device__ __forceinline__ int64_t gain_swap_delta_from(const BmDevGraph& g, const BmGain& G, int32_t u, int32_t v,
+                                                        int32_t f1, int32_t f2, bool e_uv, int64_t guf2, int64_t guf1,
+                                                        int64_t gvf1, int64_t gvf2) {
     const bool have[3][3] = {{false, e_uv, false}, {e_uv, false, false}, {false, false, false}};
     const uint64_t md[3] = {G.m_out_desc[u], G.m_out_desc[v], 0ull};
-    const int32_t* Gu = G.G + (int64_t)u * G.stride;
-    const int32_t* Gv = G.G + (int64_t)v * G.stride;
-    int64_t d = (int64_t)Gu[f2] - Gu[f1] + Gv[f1] - Gv[f2];
+    int64_t d = guf2 - guf1 + gvf1 - gvf2;
     const uint64_t fd[3] = {g.f_out_desc[f1], g.f_out_desc[f2], 0ull};
     /* male edges u -> v and v -> u; their corrections need the four female
      * weights among f1 and f2 (rare: only with an edge) */
@@ -616,6 +617,14 @@ __device__ __forceinline__ int64_t gain_swap_delta(const BmDevGraph& g, const Bm
     if (su != BM_ABSENT) d += contrib(su, fs[1]) - (int64_t)contrib(su, fs[0]);
     if (sv != BM_ABSENT) d += contrib(sv, fs[0]) - (int64_t)contrib(sv, fs[1]);
     return d;
+}
+
+__device__ __forceinline__ int64_t gain_swap_delta(const BmDevGraph& g, const BmGain& G, int32_t u, int32_t v) {
+    if (u == v) return 0;
+    const int32_t f1 = g.pi[u], f2 = g.pi[v];
+    const int32_t* Gu = G.G + (int64_t)u * G.stride;
+    const int32_t* Gv = G.G + (int64_t)v * G.stride;
+    return gain_swap_delta_from(g, G, u, v, f1, f2, male_maybe(G, u, v), Gu[f2], Gu[f1], Gv[f1], Gv[f2]);
 }
 
 template <int MODE>
@@ -652,6 +661,78 @@ __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const
         }
         delta[c] = gain_swap_delta(g, G, u, v);
     }
+}
+
+/* Exact best swap over all n(n-1)/2 pairs u < v.  One block per first vertex
+ * u (grid-stride over u); the block's threads take v = u+1..n in order, so
+ * every per-partner read is coalesced: GT(f1, v) = G(v, f1) from the
+ * transposed table, the packed per-vertex words {pi(v), G(v, pi(v)),
+ * self-loops}, and the dense adjacency bitmap's row u (bit v); only G(u, f2)
+ * is a gather, inside row u, which the block shares through the caches.  The
+ * order is: larger delta first, then smaller u, then smaller v, a total
+ * order, so the result does not depend on the launch shape.  Each block
+ * writes its best as {delta lo, delta hi, u, v}; k_best_swap_final reduces
+ * the blocks' bests. */
+constexpr int kBestThreads = 256;
+
+__device__ __forceinline__ bool best_before(int64_t d, int32_t u, int32_t v, int64_t bd, int32_t bu, int32_t bv) {
+    return d > bd || (d == bd && (u < bu || (u == bu && v < bv)));
+}
+
+/* Block-wide reduction of (d, u, v) under best_before; thread 0 gets the result. */
+__device__ __forceinline__ void best_block_reduce(int64_t& d, int32_t& u, int32_t& v) {
+    __shared__ int64_t sd[kBestThreads / BM_WAVE];
+    __shared__ int32_t su[kBestThreads / BM_WAVE], sv[kBestThreads / BM_WAVE];
+    const int lane = threadIdx.x & (BM_WAVE - 1), wave = threadIdx.x / BM_WAVE;
+#pragma unroll
+    for (int o = BM_WAVE / 2; o > 0; o >>= 1) {
+        const int64_t od = __shfl_down(d, o, BM_WAVE);
+        const int32_t ou = __shfl_down(u, o, BM_WAVE), ov = __shfl_down(v, o, BM_WAVE);
+        if (best_before(od, ou, ov, d, u, v)) { d = od; u = ou; v = ov; }
+    }
+    if (lane == 0) { sd[wave] = d; su[wave] = u; sv[wave] = v; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBestThreads / BM_WAVE; w++)
+            if (best_before(sd[w], su[w], sv[w], d, u, v)) { d = sd[w]; u = su[w]; v = sv[w]; }
+    }
+}
+
+__global__ void __launch_bounds__(kBestThreads) k_gain_best_swap(BmDevGraph g, BmGain G, const int32_t* __restrict__ GT,
+                                                                 const int4* __restrict__ pis, int4* __restrict__ part) {
+    const int32_t n = g.n;
+    int64_t bd = INT64_MIN;
+    int32_t bu = INT32_MAX, bv = INT32_MAX;
+    for (int32_t u = blockIdx.x + 1; u < n; u += gridDim.x) {
+        const int4 pu = pis[u];
+        const int32_t f1 = pu.x;
+        const int32_t* Gu = G.G + (int64_t)u * G.stride;
+        const int32_t* Tf1 = GT + (int64_t)f1 * G.stride;
+        for (int32_t v = u + 1 + (int32_t)threadIdx.x; v <= n; v += kBestThreads) {
+            const int4 pv = pis[v];
+            const int32_t f2 = pv.x;
+            const int64_t d = gain_swap_delta_from(g, G, u, v, f1, f2, male_maybe(G, u, v), Gu[f2], pu.y, Tf1[v], pv.y);
+            if (best_before(d, u, v, bd, bu, bv)) { bd = d; bu = u; bv = v; }
+        }
+    }
+    best_block_reduce(bd, bu, bv);
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = make_int4((int32_t)(uint32_t)(uint64_t)bd, (int32_t)(uint32_t)((uint64_t)bd >> 32), bu, bv);
+}
+
+/* One block: the best of nparts block results (nparts >= 1). */
+__global__ void __launch_bounds__(kBestThreads) k_best_swap_final(const int4* __restrict__ part, int32_t nparts,
+                                                                  int4* __restrict__ out) {
+    int64_t bd = INT64_MIN;
+    int32_t bu = INT32_MAX, bv = INT32_MAX;
+    for (int32_t i = threadIdx.x; i < nparts; i += kBestThreads) {
+        const int4 q = part[i];
+        const int64_t d = (int64_t)(((uint64_t)(uint32_t)q.y << 32) | (uint64_t)(uint32_t)q.x);
+        if (best_before(d, q.z, q.w, bd, bu, bv)) { bd = d; bu = q.z; bv = q.w; }
+    }
+    best_block_reduce(bd, bu, bv);
+    if (threadIdx.x == 0)
+        out[0] = make_int4((int32_t)(uint32_t)(uint64_t)bd, (int32_t)(uint32_t)((uint64_t)bd >> 32), bu, bv);
 }
 
 /* Transposed table: GT(f, u) = G(u, f) for f < nrows, with GT(f, u) = 0 in
@@ -1250,6 +1331,19 @@ hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, cons
     if (blocks > g.n) blocks = g.n;
     hipLaunchKernelGGL(k_gain_swap_umajor, dim3((unsigned)blocks), dim3(kUmThreads), lds, s, g, G, GT, pis, off, seed,
                        delta);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_gain_best_swap(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
+                                    int4* part, int32_t max_parts, int4* out, int32_t num_cus, hipStream_t s) {
+    int64_t blocks = (int64_t)8 * (num_cus > 0 ? num_cus : 256);
+    if (blocks > max_parts) blocks = max_parts;
+    if (blocks > (int64_t)g.n - 1) blocks = (int64_t)g.n - 1;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_gain_best_swap, dim3((unsigned)blocks), dim3(kBestThreads), 0, s, g, G, GT, pis, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_best_swap_final, dim3(1), dim3(kBestThreads), 0, s, part, (int32_t)blocks, out);
     return hipGetLastError();
 }
 

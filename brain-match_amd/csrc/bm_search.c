@@ -376,6 +376,55 @@ int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int3
     return climb(ctx, seed, rounds, batch, src, verify, score_trace, res);
 }
 
+/* Steepest ascent (src/best-swap/score-lookup.c:519-716): the best pair
+ * over all pairs, applied while its delta is positive. */
+int bm_best_swap_climb(bm_ctx* ctx, int64_t max_iters, int64_t verify_every, int64_t* score_trace,
+                       bm_search_result* res) {
+    if (!ctx || !res) return BM_ERR_ARG;
+    memset(res, 0, sizeof *res);
+    if (max_iters < 0 || verify_every < 0)
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_best_swap_climb: max_iters and verify_every must be >= 0");
+    if (!bm_gain_enabled(ctx))
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_best_swap_climb: needs the placement-gain table (bm_gain_enable)");
+    const double t0 = now_s();
+    int64_t score = 0;
+    int rc = bm_score(ctx, &score);
+    if (rc != BM_OK) return rc;
+    res->initial_score = score;
+    if (score_trace) score_trace[0] = score;
+    for (int64_t it = 0; it < max_iters; it++) {
+        int64_t d = 0;
+        int32_t u = 0, v = 0;
+        if ((rc = bm_best_swap(ctx, &d, &u, &v)) != BM_OK) return rc;
+        res->evaluations += (int64_t)bm_num_vertices(ctx) * (bm_num_vertices(ctx) - 1) / 2;
+        if (d <= 0) break;
+        if ((rc = bm_apply_swap(ctx, u, v)) != BM_OK) return rc;
+        score += d;
+        res->iterations++;
+        res->accepted++;
+        if (score_trace) score_trace[it + 1] = score;
+        if (verify_every > 0 && res->iterations % verify_every == 0) {
+            int64_t s = 0;
+            if ((rc = bm_score(ctx, &s)) != BM_OK) return rc;
+            if (s != score) {
+                res->verify_failures++;
+                score = s;
+            }
+        }
+    }
+    if (verify_every > 0) {
+        int64_t s = 0;
+        if ((rc = bm_score(ctx, &s)) != BM_OK) return rc;
+        if (s != score) {
+            res->verify_failures++;
+            score = s;
+        }
+    }
+    res->best_score = res->final_score = score;
+    res->seconds = now_s() - t0;
+    return BM_OK;
+}
+
 int bm_hillclimb_twohop(bm_ctx* ctx, uint64_t seed, int64_t rounds, int32_t anchors, int32_t per_anchor,
                         int32_t verify, int64_t* score_trace, bm_search_result* res) {
     if (anchors < 1 || per_anchor < 1) return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_hillclimb_twohop: anchors and per_anchor must be >= 1");

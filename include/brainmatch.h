@@ -327,6 +327,13 @@ int bm_gain_rebuild(bm_ctx* ctx);
 int64_t bm_gain_table_bytes(int32_t n);
 /* Copy row u (n+1 entries) of the table to the host. */
 int bm_gain_row(bm_ctx* ctx, int32_t u, int32_t* row);
+/* Exact best swap over all n(n-1)/2 pairs u < v of the context matching:
+ * the largest delta, ties to the smaller u, then the smaller v.  Replaces the n x n int16 delta table and max-heap of
+ * src/best-swap/score-lookup.c:246-293 (create_score_lookup) and :612-620
+ * (extract_max): recomputed from the table on every call, so no entry is
+ * stale and no delta is truncated to int16.  Needs the table (BM_ERR_ARG
+ * otherwise) and, once, a transposed copy of it (bm_gain_table_bytes more). */
+int bm_best_swap(bm_ctx* ctx, int64_t* delta, int32_t* u, int32_t* v);
 
 /* ------------------------------------------------------------------ */
 /* Measurement                                                         */
@@ -430,6 +437,16 @@ int bm_search(bm_ctx* ctx, const bm_search_opts* opts, int32_t* best_pi, bm_sear
  * verify != 0 every round is checked against a device recount. */
 int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int32_t verify,
                  int64_t* score_trace, bm_search_result* res);
+/* Steepest-ascent swaps, the main loop of the score-lookup driver's
+ * optimize_mapping (src/best-swap/score-lookup.c:519-716) without its heap:
+ * each iteration applies bm_best_swap's pair while its delta is > 0, at most
+ * max_iters times.  score_trace (max_iters+1 entries, may be NULL) gets the
+ * running score before the first swap and after each; verify_every > 0
+ * checks it against a device recount every that many swaps and at the end.
+ * res->iterations = swaps applied; a stop with a positive pair left is a
+ * max_iters stop.  Needs the placement-gain table. */
+int bm_best_swap_climb(bm_ctx* ctx, int64_t max_iters, int64_t verify_every, int64_t* score_trace,
+                       bm_search_result* res);
 /* The same climb over 2-hop proposals: round r evaluates
  * bm_batch_gen_twohop(NULL, anchors, per_anchor, seed + r) (refine-twohop-random's
  * proposal, src/refine-twohop-random.c:676-693, with 10 anchors x 1000

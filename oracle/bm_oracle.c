@@ -39,6 +39,16 @@
  *                               (delta desc, index asc; closed 1-hop
  *                               neighbourhood lock).  Pinned by the additivity
  *                               check: gain == brute-force score difference.
+ *  or_best_swap / _climb        the exact maximum over all pairs u < v of
+ *                               or_swap_delta (ties: smaller u, then smaller
+ *                               v) and the steepest-ascent loop of the
+ *                               score-lookup driver (src/best-swap/score-lookup.c:519-716)
+ *                               with that maximum in place of its heap.
+ *                               Pinned: the maximum equals the argmax of
+ *                               brute-force recounts over every pair of small
+ *                               graphs; the climb's score trace equals
+ *                               or_score after every swap and ends with no
+ *                               positive pair left.
  *  or_uniform_pair              the candidate stream definition; pinned by
  *                               comparing with pairs downloaded from the device.
  *  or_cpu_baseline_*            the reference CPU arithmetic (a4) driven by an
@@ -1079,4 +1089,59 @@ void or_umajor_pairs_at(uint64_t seed, int64_t k, int32_t n, const int64_t* idx,
         uv[2 * j + 1] = b >= lo ? b + 1 : b;
     }
     free(off);
+}
+
+/* Best swap over all pairs u < v: largest delta, then smaller u, then smaller
+ * v; (0, 0, 0) when n < 2. */
+void or_best_swap(const or_graph* g, const int32_t* pi, int64_t* delta, int32_t* bu, int32_t* bv) {
+    const int32_t n = g->n;
+    int64_t best = 0;
+    int32_t ru = 0, rv = 0;
+    int have = 0;
+#pragma omp parallel
+    {
+        int64_t tb = 0;
+        int32_t tu = 0, tv = 0;
+        int th = 0;
+#pragma omp for schedule(dynamic, 16)
+        for (int32_t u = 1; u < n; u++)
+            for (int32_t v = u + 1; v <= n; v++) {
+                const int64_t d = swap_delta_fast(g, pi, u, v);
+                if (!th || d > tb || (d == tb && (u < tu || (u == tu && v < tv)))) {
+                    tb = d;
+                    tu = u;
+                    tv = v;
+                    th = 1;
+                }
+            }
+#pragma omp critical
+        if (th && (!have || tb > best || (tb == best && (tu < ru || (tu == ru && tv < rv))))) {
+            best = tb;
+            ru = tu;
+            rv = tv;
+            have = 1;
+        }
+    }
+    *delta = best;
+    *bu = ru;
+    *bv = rv;
+}
+
+/* Steepest ascent: apply the best pair while its delta is > 0, at most
+ * max_iters times.  trace[0..k] = running score; returns k (swaps applied). */
+int64_t or_best_swap_climb(const or_graph* g, int32_t* pi, int64_t max_iters, int64_t* trace) {
+    int64_t s = or_score(g, pi), k = 0;
+    trace[0] = s;
+    while (k < max_iters) {
+        int64_t d;
+        int32_t u, v;
+        or_best_swap(g, pi, &d, &u, &v);
+        if (d <= 0) break;
+        const int32_t t = pi[u];
+        pi[u] = pi[v];
+        pi[v] = t;
+        s += d;
+        trace[++k] = s;
+    }
+    return k;
 }

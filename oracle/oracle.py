@@ -59,6 +59,8 @@ _sig(_o, "or_max_threads", C.c_int, [])
 _sig(_o, "or_uniform_pairs", None, [C.c_uint64, C.c_int64, C.c_int32, i32p])
 _sig(_o, "or_accept_disjoint", C.c_int64, [vp, i32p, i32p, i64p, C.c_int64, C.c_int64, i64p, i64p])
 _sig(_o, "or_hillclimb", None, [vp, i32p, C.c_uint64, C.c_int64, C.c_int64, i64p])
+_sig(_o, "or_best_swap", None, [vp, i32p, i64p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)])
+_sig(_o, "or_best_swap_climb", C.c_int64, [vp, i32p, C.c_int64, i64p])
 
 
 def _a32(x):
@@ -138,6 +140,19 @@ class Oracle:
         na = _o.or_accept_disjoint(self.h, _p32(pi), _p32(uv), _p64(d), uv.shape[0], max_accept, _p64(acc),
                                    C.byref(gain))
         return pi, acc[:na].copy(), gain.value
+
+    def best_swap(self, pi):
+        pi = _a32(pi)
+        d = np.zeros(1, np.int64)
+        u, v = C.c_int32(0), C.c_int32(0)
+        _o.or_best_swap(self.h, _p32(pi), _p64(d), C.byref(u), C.byref(v))
+        return int(d[0]), u.value, v.value
+
+    def best_swap_climb(self, pi, max_iters):
+        pi = _a32(pi).copy()
+        trace = np.zeros(max_iters + 1, np.int64)
+        k = _o.or_best_swap_climb(self.h, _p32(pi), max_iters, _p64(trace))
+        return pi, trace[:k + 1]
 
     def hillclimb(self, pi, seed, rounds, batch):
         pi = _a32(pi).copy()

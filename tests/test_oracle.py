@@ -258,3 +258,37 @@ def test_umajor_stream_law(orc):
     sub = uv[uv[:, 0] == 7][:, 1]
     h = np.bincount(sub, minlength=n + 1)[1:]
     assert h[6] == 0 and np.abs(np.delete(h, 6) - len(sub) / (n - 1)).max() < 6 * np.sqrt(len(sub) / (n - 1))
+
+
+@pytest.mark.parametrize("seed", [3, 8])
+def test_best_swap_is_brute_force_argmax(bm, orc, seed):
+    """or_best_swap = the argmax, over every pair u < v, of full recounts
+    (ties to the smaller u, then v), on the quirky graph; with no matching
+    every delta is 0 and the first pair (1, 2) is chosen."""
+    male, female, pi = quirky_pair(bm, seed=seed)
+    o = orc.Oracle(60, male, female)
+    best = None
+    for u in range(1, 61):
+        for v in range(u + 1, 61):
+            d = o.swap_delta_brute(pi, u, v)
+            if best is None or d > best[0]:
+                best = (d, u, v)
+    assert o.best_swap(pi) == best
+    assert o.best_swap(np.zeros(61, np.int32)) == (0, 1, 2)
+
+
+def test_best_swap_climb_ends_at_local_optimum(bm, orc):
+    """The steepest-ascent trace equals a recount after every swap, rises
+    strictly, and stops where no pair has a positive delta."""
+    male, female, pi = quirky_pair(bm, seed=5)
+    o = orc.Oracle(60, male, female)
+    pi_end, trace = o.best_swap_climb(pi, 10_000)
+    assert len(trace) > 2 and np.all(np.diff(trace) > 0)
+    assert trace[-1] == o.score(pi_end)
+    assert o.best_swap(pi_end)[0] <= 0
+    q = pi.copy()
+    for k in range(1, len(trace)):
+        d, u, v = o.best_swap(q)
+        q[u], q[v] = q[v], q[u]
+        assert o.score(q) == trace[k]
+    assert np.array_equal(q, pi_end)
