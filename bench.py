@@ -163,6 +163,9 @@ def main():
                     help="instead of the swap-delta benchmark: the n=100,000 ~20M-edge pair with 40%% of the matching "
                          "shuffled: delta throughput of both strategies, then learned vs uniform climbs and "
                          "swap + 3-vertex annealing for SECONDS each (one JSON line)")
+    ap.add_argument("--best-swap", type=int, default=0, metavar="SWAPS",
+                    help="instead of the swap-delta benchmark: steepest ascent on the n=18,524 pair from the greedy "
+                         "matching, SWAPS best swaps over all 171.6M pairs each (one JSON line)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (the known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -193,6 +196,8 @@ def main():
         return bench_anneal(args, bm)
     if args.scale > 0:
         return bench_scale(args, bm)
+    if args.best_swap > 0:
+        return bench_best_swap(args, bm)
 
     t = time.time()
     male, female, truth, pi_shuffled = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3,
@@ -583,6 +588,58 @@ def bench_scale(args, bm):
               "data": "synthetic", "config": {"workload": "n=100000, ~20M edges, 40% shuffled", "seconds": args.scale,
                                               "evaluator": args.evaluator},
               "results": out}
+    print(json.dumps(result), flush=True)
+
+
+def bench_best_swap(args, bm):
+    """Steepest ascent (the score-lookup driver's loop): every iteration
+    finds the exact best swap over all n(n-1)/2 pairs from the placement-gain
+    table and applies it (table patch + transposed-table refresh).  Reports
+    exact pair deltas/s of the all-pairs kernel (HIP events on the context
+    stream), swaps/s of the whole loop, and the recount check."""
+    n = N_VERTICES
+    male, female, truth, _ = bm.synth_pair(n, N_EDGES / n, bm.WEIGHTS_LOGNORMAL, 0.3, seed=1)
+    pi0 = bm.greedy_match(n, male, female)
+    ctx = bm.Context(n, male, female)
+    ctx.set_mapping(pi0)
+    ctx.gain_enable(True)
+    ctx.best_swap()                                   # transposed table + warm-up
+    pairs = n * (n - 1) // 2
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    k = 20
+    t = time.perf_counter()
+    for _ in range(k):
+        ctx.best_swap()
+    wall_call = (time.perf_counter() - t) / k
+    calls, ms = ctx.timing(bm.KERNEL_GAIN_SWAP)
+    launch_ms = ms / max(1, calls)
+    ctx.timing_reset()
+    t = time.perf_counter()
+    trace, res = ctx.best_swap_climb(args.best_swap, verify_every=100)
+    wall = time.perf_counter() - t
+    swaps = res["iterations"]
+    log(f"best swap over {pairs} pairs: {launch_ms:.2f} ms in the kernels per call ({pairs / launch_ms * 1e3:.4g} deltas/s), "
+        f"{wall_call * 1e3:.2f} ms per call; climb {trace[0]} -> {trace[-1]} in {swaps} swaps, "
+        f"{swaps / wall:.1f} swaps/s, recount failures {res['verify_failures']}")
+    result = {
+        "metric": "exact best swap over all pairs: pair deltas/s (steepest ascent)",
+        "value": pairs / launch_ms * 1e3,
+        "unit": "swap-deltas/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": "steepest ascent, n=18524 ~4M edges/graph, greedy initial matching",
+                   "pairs_per_call": pairs, "swaps": args.best_swap},
+        "ms_per_best_swap_kernels": launch_ms,
+        "ms_per_best_swap_call": wall_call * 1e3,
+        "swaps_per_s": swaps / wall,
+        "swaps_applied": swaps,
+        "score": {"initial": int(trace[0]), "final": int(trace[-1])},
+        "recount_failures": int(res["verify_failures"]),
+    }
+    ctx.close()
     print(json.dumps(result), flush=True)
 
 

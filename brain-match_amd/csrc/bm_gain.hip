@@ -543,20 +543,26 @@ __global__ void k_pack_rows(const int2* __restrict__ in, int64_t m, int layout, 
 }
 
 /* Patch the rows of the neighbours of each moved vertex x (pi(x): old -> new).
- * moves[i] = {x, old, new, 0}, *count moves, one block per move.  An in-edge
+ * moves[i] = {x, old, new, 0}, *count moves; each move's male edges are
+ * split over `slices` consecutive blocks (a block's waves take every
+ * slices-th group of 64 edges), so a few moves of high-degree vertices still
+ * fill the device.  An in-edge
  * y -> x contributes c(w, f, pi(x)) to G(y, f) over the female in-row of pi(x);
  * an out-edge x -> y contributes c(w, pi(x), f) over the female out-row.  The
  * old female's terms are subtracted and the new one's added; self-loops are
  * not in the table.  Updates of different moves commute (integer atomics). */
 __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmGain G, const int4* __restrict__ moves,
-                                                               const int64_t* __restrict__ count) {
+                                                               const int64_t* __restrict__ count, int32_t slices) {
     __shared__ WaveScratch scratch[kPatchWaves];
     const int tid = threadIdx.x;
     const int lane = tid & (BM_WAVE - 1);
     const int wave = tid / BM_WAVE;
     WaveScratch* S = &scratch[wave];
     const int64_t nm = *count;
-    for (int64_t mi = blockIdx.x; mi < nm; mi += gridDim.x) {
+    const int32_t slice = (int32_t)(blockIdx.x % (uint32_t)slices);
+    const int64_t first = (int64_t)((slice * kPatchWaves + wave) * BM_WAVE);
+    const int64_t step = (int64_t)slices * kPatchWaves * BM_WAVE;
+    for (int64_t mi = blockIdx.x / (uint32_t)slices; mi < nm; mi += gridDim.x / (uint32_t)slices) {
         const int4 mv = moves[mi];
         const int32_t x = mv.x;
         if (mv.y == mv.z) continue;
@@ -574,7 +580,7 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
                 const int2* fadj = dir ? g.f_out : g.f_in;
                 const int64_t b = mptr[x], e = mptr[x + 1];
                 const int64_t rb = fptr[gx], re = fptr[gx + 1];
-                for (int64_t e0 = b + (int64_t)wave * BM_WAVE; e0 < e; e0 += (int64_t)kPatchWaves * BM_WAVE) {
+                for (int64_t e0 = b + first; e0 < e; e0 += step) {
                     const int64_t i = e0 + lane;
                     bool have = i < e;
                     const int2 ed = have ? madj[i] : make_int2(0, 0);
@@ -674,6 +680,7 @@ __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const
  * writes its best as {delta lo, delta hi, u, v}; k_best_swap_final reduces
  * the blocks' bests. */
 constexpr int kBestThreads = 256;
+constexpr int kBestPairs = 4;   /* partners per thread in flight */
 
 __device__ __forceinline__ bool best_before(int64_t d, int32_t u, int32_t v, int64_t bd, int32_t bu, int32_t bv) {
     return d > bd || (d == bd && (u < bu || (u == bu && v < bv)));
@@ -708,11 +715,28 @@ __global__ void __launch_bounds__(kBestThreads) k_gain_best_swap(BmDevGraph g, B
         const int32_t f1 = pu.x;
         const int32_t* Gu = G.G + (int64_t)u * G.stride;
         const int32_t* Tf1 = GT + (int64_t)f1 * G.stride;
-        for (int32_t v = u + 1 + (int32_t)threadIdx.x; v <= n; v += kBestThreads) {
-            const int4 pv = pis[v];
-            const int32_t f2 = pv.x;
-            const int64_t d = gain_swap_delta_from(g, G, u, v, f1, f2, male_maybe(G, u, v), Gu[f2], pu.y, Tf1[v], pv.y);
-            if (best_before(d, u, v, bd, bu, bv)) { bd = d; bu = u; bv = v; }
+        for (int32_t v0 = u + 1 + (int32_t)threadIdx.x; v0 <= n; v0 += kBestThreads * kBestPairs) {
+            /* every load of kBestPairs partners in flight before any use */
+            int4 pv[kBestPairs];
+            int32_t tv[kBestPairs], gu[kBestPairs];
+            bool mm[kBestPairs];
+#pragma unroll
+            for (int j = 0; j < kBestPairs; j++) {
+                const int32_t v = v0 + kBestThreads * j;
+                const bool in = v <= n;
+                pv[j] = in ? pis[v] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
+                tv[j] = in ? Tf1[v] : 0;
+                mm[j] = in && male_maybe(G, u, v);
+            }
+#pragma unroll
+            for (int j = 0; j < kBestPairs; j++) gu[j] = Gu[pv[j].x];
+#pragma unroll
+            for (int j = 0; j < kBestPairs; j++) {
+                const int32_t v = v0 + kBestThreads * j;
+                if (v > n) continue;
+                const int64_t d = gain_swap_delta_from(g, G, u, v, f1, pv[j].x, mm[j], gu[j], pu.y, tv[j], pv[j].y);
+                if (best_before(d, u, v, bd, bu, bv)) { bd = d; bu = u; bv = v; }
+            }
         }
     }
     best_block_reduce(bd, bu, bv);
@@ -1267,8 +1291,13 @@ hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t nu
 hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int4* moves, const int64_t* count,
                                  int64_t max_moves, hipStream_t s) {
     if (max_moves <= 0) return hipSuccess;
-    int64_t blocks = max_moves < 4096 ? max_moves : 4096;
-    hipLaunchKernelGGL(k_gain_update, dim3((unsigned)blocks), dim3(kPatchThreads), 0, s, g, G, moves, count);
+    const int64_t per_move = max_moves < 4096 ? max_moves : 4096;
+    /* about 2048 blocks in all, at least one per move slot */
+    int32_t slices = (int32_t)(2048 / per_move);
+    if (slices < 1) slices = 1;
+    if (slices > 256) slices = 256;
+    hipLaunchKernelGGL(k_gain_update, dim3((unsigned)(per_move * slices)), dim3(kPatchThreads), 0, s, g, G, moves,
+                       count, slices);
     return hipGetLastError();
 }
 
