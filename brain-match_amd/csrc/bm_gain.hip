@@ -144,67 +144,6 @@ __device__ __forceinline__ void triple_weights(const BmDevGraph& g, const BmGain
     }
 }
 
-/* Per-wave scratch for the flattened (edge, female entry) loops. */
-struct WaveScratch {
-    int32_t incl[BM_WAVE];
-    int32_t w[BM_WAVE];
-    int32_t y[BM_WAVE];   /* per-edge target row (patch kernel) */
-    int64_t lo[BM_WAVE];
-};
-
-/* One wave's batch: lane l holds a male edge of weight w (have = false for no
- * edge) whose female row is frow[rb, re).  Every entry {f, fw} with f in
- * [col0, col0 + ncols) contributes min(w, fw) at column f - col0, through
- * add(owner lane, column, value).  Rows are sorted by key, so the column
- * window is located by binary search.  y is stored for the callback. */
-template <typename Add>
-__device__ __forceinline__ void wave_flat(bool have, int32_t w, int32_t y, const int2* __restrict__ frow, int64_t rb,
-                                          int64_t re, int32_t col0, int32_t ncols, bool windowed, WaveScratch* S,
-                                          int lane, Add add) {
-    int64_t lo = rb, hi = re;
-    if (have && windowed) {
-        int64_t a = rb, b = re;
-        while (a < b) {
-            const int64_t m = (a + b) >> 1;
-            if (frow[m].x < col0) a = m + 1; else b = m;
-        }
-        lo = a;
-        b = re;
-        while (a < b) {
-            const int64_t m = (a + b) >> 1;
-            if (frow[m].x < col0 + ncols) a = m + 1; else b = m;
-        }
-        hi = a;
-    }
-    int32_t incl = have ? (int32_t)(hi - lo) : 0;
-#pragma unroll
-    for (int o = 1; o < BM_WAVE; o <<= 1) {
-        const int32_t t = __shfl_up(incl, o, BM_WAVE);
-        if (lane >= o) incl += t;
-    }
-    const int32_t total = __shfl(incl, BM_WAVE - 1, BM_WAVE);
-    S->incl[lane] = incl;
-    S->w[lane] = w;
-    S->y[lane] = y;
-    S->lo[lane] = lo;
-    wave_sync();
-    for (int32_t base = 0; base < total; base += BM_WAVE) {
-        const int32_t t = base + lane;
-        if (t < total) {
-            int32_t a = 0, b = BM_WAVE - 1;
-            while (a < b) {
-                const int32_t m = (a + b) >> 1;
-                if (S->incl[m] > t) b = m; else a = m + 1;
-            }
-            const int32_t before = a ? S->incl[a - 1] : 0;
-            const int2 ent = frow[S->lo[a] + (t - before)];
-            const int32_t wv = S->w[a];
-            add(S->y[a], ent.x - col0, wv < ent.y ? wv : ent.y);
-        }
-    }
-    wave_sync();
-}
-
 /* Female-row entry {f, w} in one of three layouts: int2; the packed word
  * (f << 15) | w (ids < 2^17, weights in 0..32767); or (w << 19) | 4 f (ids
  * < 2^17, weights in 0..8191), whose low bits are already the byte address
@@ -553,11 +492,9 @@ __global__ void k_pack_rows(const int2* __restrict__ in, int64_t m, int layout, 
  * not in the table.  Updates of different moves commute (integer atomics). */
 __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmGain G, const int4* __restrict__ moves,
                                                                const int64_t* __restrict__ count, int32_t slices) {
-    __shared__ WaveScratch scratch[kPatchWaves];
     const int tid = threadIdx.x;
     const int lane = tid & (BM_WAVE - 1);
     const int wave = tid / BM_WAVE;
-    WaveScratch* S = &scratch[wave];
     const int64_t nm = *count;
     const int32_t slice = (int32_t)(blockIdx.x % (uint32_t)slices);
     const int64_t first = (int64_t)((slice * kPatchWaves + wave) * BM_WAVE);
@@ -570,9 +507,6 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
             const int32_t gx = side ? mv.z : mv.y;
             if (gx == 0) continue;
             const int32_t sign = side ? 1 : -1;
-            auto add = [&](int32_t y, int32_t c, int32_t v) {
-                atomicAdd(&G.G[(int64_t)y * G.stride + c], sign * v);
-            };
             for (int dir = 0; dir < 2; dir++) {
                 const int64_t* mptr = dir ? g.m_out_ptr : g.m_in_ptr;
                 const int2* madj = dir ? g.m_out : g.m_in;
@@ -580,12 +514,26 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
                 const int2* fadj = dir ? g.f_out : g.f_in;
                 const int64_t b = mptr[x], e = mptr[x + 1];
                 const int64_t rb = fptr[gx], re = fptr[gx + 1];
+                /* every edge of x expands the same female row (that of gx):
+                 * lanes hold 64 of x's edges, then 64 row entries at a time,
+                 * and each entry is applied for every edge in turn (the edge's
+                 * row y and weight read out of its lane as scalars) */
                 for (int64_t e0 = b + first; e0 < e; e0 += step) {
                     const int64_t i = e0 + lane;
-                    bool have = i < e;
-                    const int2 ed = have ? madj[i] : make_int2(0, 0);
-                    have = have && ed.x != x;
-                    wave_flat(have, ed.y, ed.x, fadj, rb, re, 0, 0, false, S, lane, add);
+                    const int2 ed = i < e ? madj[i] : make_int2(0, 0);
+                    const int32_t yl = ed.x != x ? ed.x : 0;   /* 0: past the list or a self-loop */
+                    const int32_t nv = e - e0 < BM_WAVE ? (int32_t)(e - e0) : BM_WAVE;
+                    for (int64_t r0 = rb; r0 < re; r0 += BM_WAVE) {
+                        const int64_t ri = r0 + lane;
+                        const bool rin = ri < re;
+                        const int2 ent = rin ? fadj[ri] : make_int2(0, 0);
+                        for (int32_t k = 0; k < nv; k++) {
+                            const int32_t y = __builtin_amdgcn_readlane(yl, k);
+                            if (y == 0) continue;
+                            const int32_t w = __builtin_amdgcn_readlane(ed.y, k);
+                            if (rin) atomicAdd(&G.G[(int64_t)y * G.stride + ent.x], sign * (w < ent.y ? w : ent.y));
+                        }
+                    }
                 }
             }
         }
