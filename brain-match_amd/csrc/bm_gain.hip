@@ -682,7 +682,14 @@ __global__ void __launch_bounds__(kBestThreads) k_gain_best_swap(BmDevGraph g, B
             for (int j = 0; j < kBestPairs; j++) {
                 const int32_t v = v0 + kBestThreads * j;
                 if (v > n) continue;
-                const int64_t d = gain_swap_delta_from(g, G, u, v, f1, pv[j].x, mm[j], gu[j], pu.y, tv[j], pv[j].y);
+                int64_t d;
+                if (mm[j]) {   /* a male edge between u and v may exist (rare): the full correction */
+                    d = gain_swap_delta_from(g, G, u, v, f1, pv[j].x, true, gu[j], pu.y, tv[j], pv[j].y);
+                } else {       /* only self-loops, whose weights the per-vertex words carry */
+                    d = (int64_t)gu[j] - pu.y + tv[j] - pv[j].y;
+                    if (pu.z != BM_ABSENT) d += contrib(pu.z, pv[j].w) - (int64_t)contrib(pu.z, pu.w);
+                    if (pv[j].z != BM_ABSENT) d += contrib(pv[j].z, pu.w) - (int64_t)contrib(pv[j].z, pv[j].w);
+                }
                 if (best_before(d, u, v, bd, bu, bv)) { bd = d; bu = u; bv = v; }
             }
         }
