@@ -153,6 +153,13 @@ def main():
                     help="instead of the swap-delta benchmark: train the learned proposal, then compare the exact "
                          "scores reached in SECONDS by uniform, 2-hop and learned proposals (one JSON line)")
     ap.add_argument("--train-samples", type=int, default=50_000_000)
+    ap.add_argument("--proposal-seconds", type=float, default=60.0,
+                    help="wall clock of each proposal climb in the default run (0: leave the comparison out)")
+    ap.add_argument("--sources", default=",".join(PROPOSAL_SOURCES),
+                    help="proposal sources compared (comma-separated subset of uniform,twohop,learned)")
+    ap.add_argument("--learned-plain", action="store_true",
+                    help="learned climb without online retraining and uniform fallback")
+    ap.add_argument("--retrain-lr", type=float, default=2e-4, help="Adam step size of the learned climb's retraining")
     ap.add_argument("--evaluator", choices=["gain", "scan"], default="scan",
                     help="exact delta strategy of the --proposals / --scale climbs (same deltas either way)")
     ap.add_argument("--anneal", type=int, default=0, metavar="LAUNCHES",
@@ -360,27 +367,35 @@ def main():
 
     ctx.timing_enable(False)
     batch.close()
+    if args.proposal_seconds > 0:
+        # the second half of the metric: exact score reached in the same wall
+        # clock by each proposal source, from the same greedy matching
+        sources = tuple(x for x in args.sources.split(",") if x)
+        pr = compare_proposals(bm, ctx, pi0, args.proposal_seconds, args, sources)
+        result["proposals"] = dict(metric=f"exact score at {args.proposal_seconds:g} s, learned vs uniform proposals",
+                                   **pr)
     ctx.close()
     print(json.dumps(result), flush=True)
 
 
-def bench_proposals(args, bm):
-    """The learned-proposal workload on the n = 18,524 pair.  The network is
-    trained on train_samples (candidate, exact delta) pairs from the context's
-    own exact kernel; then three disjoint-swap climbs from the same greedy
-    matching run for the same wall clock, each sending 4096 candidates per
-    round to the exact delta kernel: uniform pairs, 2-hop pairs (64 anchors x
-    64 targets), or the 4096 best-ranked of 2^20 uniform pairs.  With
-    --evaluator scan (default) the scan kernel evaluates the candidates;
-    with --evaluator gain the placement-gain table does (rebuilt when a climb
-    sets its start matching, patched after every accepted swap).  Both give
-    the same exact deltas."""
-    male, female, truth, _ = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3, seed=1)
-    pi0 = bm.greedy_match(N_VERTICES, male, female)
-    ctx = bm.Context(N_VERTICES, male, female)
-    ctx.set_mapping(pi0)
+PROPOSAL_SOURCES = ("uniform", "twohop", "learned")
+
+
+def compare_proposals(bm, ctx, pi0, seconds, args, sources=PROPOSAL_SOURCES):
+    """The learned-proposal workload on the n = 18,524 pair, from the greedy
+    matching.  The network is trained on train_samples (candidate, exact
+    delta) pairs from the context's own exact kernel; then disjoint-swap
+    climbs from the same matching run for the same wall clock, each sending
+    4096 candidates per round to the exact delta kernel: uniform pairs, 2-hop
+    pairs (64 anchors x 64 targets), or the 4096 best-ranked of 2^20 uniform
+    pairs.  The learned climb (unless --learned-plain) retrains the network
+    on every learned round's 4096 evaluated candidates and their exact deltas
+    and falls back to uniform rounds while those gain more per second.  The
+    climbs' candidates are evaluated by the scan kernel (--evaluator scan) or
+    the placement-gain table (--evaluator gain); the deltas are the same."""
     if args.evaluator == "gain":
         ctx.gain_enable(True)
+    ctx.set_mapping(pi0)
     score0 = ctx.score()
     batch = 1 << 16
     steps = max(1, args.train_samples // batch)
@@ -391,38 +406,72 @@ def bench_proposals(args, bm):
     losses = mlp.train(seed=11, steps=steps, batch=batch, lr=1e-3)
     train_s = time.perf_counter() - t
     log(f"trained on {steps * batch} samples in {train_s:.1f}s; mse {losses[:10].mean():.3f} -> {losses[-10:].mean():.3f}")
+    trained = mlp.params()
+    plain = args.learned_plain
+    learned_kw = {"pool": 1 << 20, "mlp": mlp, "refresh_every": 1}
+    if not plain:
+        learned_kw.update(retrain_every=1, retrain_lr=args.retrain_lr, fallback=True)
+    kinds = {"uniform": (bm.PROPOSAL_UNIFORM, {}),
+             "twohop": (bm.PROPOSAL_TWOHOP, {"per_anchor": 64}),
+             "learned": (bm.PROPOSAL_LEARNED, learned_kw)}
     out = {}
-    for name, prop, kw in (("uniform", bm.PROPOSAL_UNIFORM, {}),
-                           ("twohop", bm.PROPOSAL_TWOHOP, {"per_anchor": 64}),
-                           ("learned", bm.PROPOSAL_LEARNED, {"pool": 1 << 20, "mlp": mlp, "refresh_every": 1})):
+    for name in sources:
+        prop, kw = kinds[name]
+        if name == "learned":
+            mlp.set_params(trained)
         ctx.set_mapping(pi0)
-        tr, tt, res = ctx.climb(prop, seed=21, time_limit_s=args.proposals, per_round=4096, **kw)
+        tr, tt, res = ctx.climb(prop, seed=21, time_limit_s=seconds, per_round=4096, **kw)
         marks = {}
-        for frac in (0.1, 0.25, 0.5, 1.0):
-            i = int(np.searchsorted(tt, frac * args.proposals))
-            marks[f"{frac * args.proposals:g}s"] = int(tr[min(i, len(tr) - 1)]) if len(tr) else score0
+        for frac in (0.1, 0.25, 0.5, 0.75, 1.0):
+            i = int(np.searchsorted(tt, frac * seconds))
+            marks[f"{frac * seconds:g}s"] = int(tr[min(i, len(tr) - 1)]) if len(tr) else score0
+        last = tt >= 0.9 * seconds
         out[name] = {"final_score": res["final_score"], "gain": res["final_score"] - score0,
                      "rounds": res["iterations"], "accepted": res["accepted"], "evaluated": res["evaluations"],
-                     "verify_failures": res["verify_failures"], "seconds": res["seconds"], "score_at": marks}
+                     "verify_failures": res["verify_failures"], "seconds": res["seconds"], "score_at": marks,
+                     "gain_last_10pct": int(tr[-1] - tr[last][0]) if last.any() else 0}
+        if name == "learned":
+            out[name]["uniform_fallback_rounds"] = res["fallback_rounds"]
         log(f"{name}: score {score0} -> {res['final_score']} in {res['seconds']:.1f}s "
-            f"({res['iterations']} rounds, {res['accepted']} swaps)")
+            f"({res['iterations']} rounds, {res['accepted']} swaps"
+            + (f", {res['fallback_rounds']} uniform fallback rounds" if name == "learned" else "") + ")")
+    mlp.close()
+    ctx.set_mapping(pi0)
+    if args.evaluator == "gain":
+        ctx.gain_enable(False)
+    cfg = {"seconds": seconds, "evaluator": args.evaluator, "initial_matching": "greedy-match",
+           "initial_score": score0, "per_round": 4096, "learned_pool": 1 << 20, "learned_top": 4096,
+           "learned_online_retraining": not plain, "learned_uniform_fallback": not plain,
+           "retrain_lr": None if plain else args.retrain_lr,
+           "train_samples": steps * batch, "train_seconds": train_s,
+           "train_mse_first": float(losses[:10].mean()), "train_mse_last": float(losses[-10:].mean())}
+    res = {"config": cfg, "proposals": out}
+    if "learned" in out and "uniform" in out:
+        res["learned_minus_uniform"] = out["learned"]["final_score"] - out["uniform"]["final_score"]
+    return res
+
+
+def bench_proposals(args, bm):
+    """--proposals SECONDS: only the proposal comparison (one JSON line)."""
+    male, female, truth, _ = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3, seed=1)
+    pi0 = bm.greedy_match(N_VERTICES, male, female)
+    ctx = bm.Context(N_VERTICES, male, female)
+    sources = tuple(x for x in args.sources.split(",") if x)
+    r = compare_proposals(bm, ctx, pi0, args.proposals, args, sources)
+    out = r["proposals"]
     result = {
         "metric": f"exact score at {args.proposals:g} s, learned vs uniform proposals",
-        "value": out["learned"]["final_score"],
+        "value": out["learned"]["final_score"] if "learned" in out else None,
         "unit": "score",
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "int64 (exact) / bf16 MLP ranking",
         "data": "synthetic",
-        "config": {"workload": "learned proposal on the n=18524 ~4M-edge pair", "n": N_VERTICES,
-                   "evaluator": args.evaluator,
-                   "initial_matching": "greedy-match", "initial_score": score0, "per_round": 4096,
-                   "learned_pool": 1 << 20, "train_samples": steps * batch, "train_seconds": train_s,
-                   "train_mse_first": float(losses[:10].mean()), "train_mse_last": float(losses[-10:].mean())},
+        "config": dict(workload="learned proposal on the n=18524 ~4M-edge pair", n=N_VERTICES, **r["config"]),
         "proposals": out,
-        "learned_minus_uniform": out["learned"]["final_score"] - out["uniform"]["final_score"],
     }
-    mlp.close()
+    if "learned_minus_uniform" in r:
+        result["learned_minus_uniform"] = r["learned_minus_uniform"]
     ctx.close()
     print(json.dumps(result), flush=True)
 
@@ -692,6 +741,27 @@ def bench_rot3(ctx, bm, args):
     return out
 
 
+def host_cpu():
+    """The host the CPU baseline runs on: CPU model, logical CPUs of the
+    machine (nproc), CPUs this process may run on, and OMP_NUM_THREADS."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "affinity_cpus": aff,
+            "omp_num_threads": int(omp) if omp.isdigit() and int(omp) > 0 else None}
+
+
 def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
     """The reference CPU arithmetic on a bounded, evenly strided sample of the
     last timed stream."""
@@ -700,7 +770,7 @@ def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
     if orc.ref_available() and n <= 18524:
         kind = "reference"
         runner = orc.RefCode(male, female)
@@ -709,7 +779,6 @@ def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
         kind = "port"
         runner = orc.ListsRef(n, male, female)
         what = "oracle restatement of calculate_swap_delta (edge lists + dense int16 female matrix)"
-    run = lambda pi, uv: runner.deltas(pi, uv, threads)  # noqa: E731
     umajor = args.stream == "umajor"
 
     def pairs_at(idx):
@@ -720,9 +789,17 @@ def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
     calib = 1 << 16
     idx = np.linspace(0, K - 1, calib).astype(np.int64)
     uv = pairs_at(idx)
-    t = time.perf_counter()
-    run(pi0, uv)
-    rate = calib / max(1e-6, time.perf_counter() - t)
+    # calibrate with every logical CPU the process may run on and with the
+    # box's configured OpenMP share; the timed sample uses the faster count
+    counts = sorted({host["affinity_cpus"], host["nproc"], host["omp_num_threads"] or host["affinity_cpus"]})
+    calib_rates = {}
+    for th in counts:
+        t = time.perf_counter()
+        runner.deltas(pi0, uv, th)
+        calib_rates[th] = calib / max(1e-6, time.perf_counter() - t)
+    threads = max(calib_rates, key=calib_rates.get)
+    rate = calib_rates[threads]
+    run = lambda pi, uv: runner.deltas(pi, uv, threads)  # noqa: E731
     S = int(min(K, max(calib, rate * args.cpu_seconds)))
     S -= S % 1024
     idx = (np.arange(S, dtype=np.int64) * (K // S)).astype(np.int64)
@@ -740,7 +817,9 @@ def cpu_baseline(args, male, female, pi0, batch, seed, K, n):
         "cores": threads,
         "kind": kind,
         "sample": f"{S} pairs evenly strided over the last timed step's {args.stream} stream; {what}; "
-                  f"OpenMP schedule(dynamic,1024)",
+                  f"OpenMP schedule(dynamic,1024) on {threads} threads",
+        "host": host,
+        "calibration_rates_by_threads": {str(k): v for k, v in calib_rates.items()},
         "matches_device": match,
     }
 

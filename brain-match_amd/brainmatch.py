@@ -74,7 +74,7 @@ class _SearchOpts(C.Structure):
 class _SearchResult(C.Structure):
     _fields_ = [("initial_score", C.c_int64), ("best_score", C.c_int64), ("final_score", C.c_int64),
                 ("iterations", C.c_int64), ("accepted", C.c_int64), ("evaluations", C.c_int64),
-                ("verify_failures", C.c_int64), ("seconds", C.c_double)]
+                ("verify_failures", C.c_int64), ("seconds", C.c_double), ("fallback_rounds", C.c_int64)]
 
 
 def _sig(name, res, args):
@@ -515,9 +515,10 @@ class Context:
         return buf.value.decode()
 
     def search(self, mode, seed=1, time_limit_s=0.0, max_iters=0, out_path=None, verify_every=64, verbose=0,
-               restart_gap=0.002):
-        o = _SearchOpts(mode, seed, time_limit_s, max_iters, os.fsencode(out_path) if out_path else None, None,
-                        0.0, verify_every, verbose, None, restart_gap)
+               restart_gap=0.002, sync_path=None, sync_interval_s=0.0):
+        o = _SearchOpts(mode, seed, time_limit_s, max_iters, os.fsencode(out_path) if out_path else None,
+                        os.fsencode(sync_path) if sync_path else None, sync_interval_s, verify_every, verbose, None,
+                        restart_gap)
         best = np.zeros(self.n + 1, np.int32)
         r = _SearchResult()
         self._chk(_lib.bm_search(self.h, C.byref(o), _p32(best), C.byref(r)))
@@ -709,7 +710,8 @@ PROPOSAL_UNIFORM, PROPOSAL_TWOHOP, PROPOSAL_LEARNED = 0, 1, 2
 class _ClimbOpts(C.Structure):
     _fields_ = [("proposal", C.c_int32), ("seed", C.c_uint64), ("time_limit_s", C.c_double),
                 ("max_rounds", C.c_int64), ("per_round", C.c_int64), ("pool", C.c_int64),
-                ("per_anchor", C.c_int32), ("mlp", _c), ("refresh_every", C.c_int32), ("verify", C.c_int32)]
+                ("per_anchor", C.c_int32), ("mlp", _c), ("refresh_every", C.c_int32), ("verify", C.c_int32),
+                ("retrain_every", C.c_int32), ("retrain_lr", C.c_float), ("fallback", C.c_int32)]
 
 
 _fp = C.POINTER(C.c_float)
@@ -724,6 +726,7 @@ _sig("bm_mlp_vertex_features", C.c_int, [_c, _fp])
 _sig("bm_mlp_score_pairs", C.c_int, [_c, i32p, C.c_int64, _fp])
 _sig("bm_mlp_gradient", C.c_int, [_c, i32p, _fp, C.c_int64, _fp, _fp])
 _sig("bm_mlp_train", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int32, C.c_float, _fp])
+_sig("bm_mlp_train_batch", C.c_int, [_c, _c, C.c_float, _fp])
 _sig("bm_batch_gen_learned", C.c_int, [_c, _c, C.c_uint64, C.c_int64, C.c_int64])
 _sig("bm_climb", C.c_int, [_c, C.POINTER(_ClimbOpts), i64p, _dp, C.c_int64, C.POINTER(_SearchResult)])
 
@@ -810,6 +813,13 @@ class Mlp:
         self._chk(_lib.bm_mlp_train(self.h, seed, steps, batch, lr, _pf(tr)))
         return tr
 
+    def train_batch(self, batch: "Batch", lr=1e-3):
+        """One Adam step on an evaluated pair batch's pairs and exact deltas;
+        returns the batch loss before the step."""
+        mse = C.c_float(0)
+        self._chk(_lib.bm_mlp_train_batch(self.h, batch.h, lr, C.byref(mse)))
+        return mse.value
+
 
 def _batch_gen_learned(self, mlp, seed, pool, top):
     self._chk(_lib.bm_batch_gen_learned(self.h, mlp.h, seed, pool, top))
@@ -819,9 +829,11 @@ Batch.gen_learned = _batch_gen_learned
 
 
 def _ctx_climb(self, proposal, seed=1, time_limit_s=0.0, max_rounds=0, per_round=4096, pool=1 << 20,
-               per_anchor=64, mlp=None, refresh_every=1, verify=False, trace_cap=100000):
+               per_anchor=64, mlp=None, refresh_every=1, verify=False, trace_cap=100000, retrain_every=0,
+               retrain_lr=1e-3, fallback=False):
     o = _ClimbOpts(proposal, seed, time_limit_s, max_rounds, per_round, pool, per_anchor,
-                   mlp.h if mlp is not None else None, refresh_every, 1 if verify else 0)
+                   mlp.h if mlp is not None else None, refresh_every, 1 if verify else 0, retrain_every, retrain_lr,
+                   1 if fallback else 0)
     ts = np.zeros(trace_cap, np.int64)
     tt = np.zeros(trace_cap, np.float64)
     r = _SearchResult()

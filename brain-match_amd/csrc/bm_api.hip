@@ -77,7 +77,9 @@ struct bm_ctx {
     int32_t* d_um_counts;   /* n+1 */
     int64_t* d_um_off;      /* n+2 */
     int32_t* d_GT;          /* transposed table, valid when gt_version == gain_version */
-    int4* d_pis;            /* {pi, G(v, pi(v)), male self-loop, 0} per vertex, same validity */
+    int4* d_pis;            /* per vertex v: {pi(v), G(v, pi(v)), male self-loop weight of v, female
+                               self-loop weight of pi(v)}, BM_ABSENT for a missing self-loop;
+                               rewritten before every staged lookup */
     uint32_t* d_f_out_p;    /* packed female rows for the table build (compact graphs) */
     uint32_t* d_f_in_p;
     int4* d_edesc;          /* table-build work list, 2 x male edges entries */
@@ -826,9 +828,12 @@ static int umajor_offsets(bm_ctx* c, uint64_t seed, int64_t k) {
  * lookups: GT is rebuilt only when the table changed since. */
 static int gain_transposed(bm_ctx* c) {
     if (c->gt_version == c->gain_version) return BM_OK;
+    if (!c->d_pis) {
+        int rc = dalloc(c, &c->d_pis, (size_t)c->n + 1);
+        if (rc != BM_OK) return rc;
+    }
     if (!c->d_GT) {
         int rc = dalloc(c, &c->d_GT, (size_t)bm_gain_table_bytes(c->n) / sizeof(int32_t));
-        if (rc == BM_OK && !c->d_pis) rc = dalloc(c, &c->d_pis, (size_t)c->n + 1);
         if (rc != BM_OK) return rc;
     }
     KTiming t;
@@ -1443,6 +1448,7 @@ int bm_ctx_set_error(bm_ctx* c, int code, const char* msg) { return fail(c, code
 int2* bm_batch_pairs_dev(bm_batch* b) { return (int2*)b->d_cand; }
 int64_t* bm_batch_deltas_dev(bm_batch* b) { return b->d_delta; }
 int64_t bm_batch_capacity(const bm_batch* b) { return b->cap; }
+int bm_batch_has_pair_deltas(const bm_batch* b) { return b->kind == 1 && b->evaluated; }
 void bm_batch_mark_pairs(bm_batch* b, int64_t k) {
     b->k = k;
     b->kind = 1;

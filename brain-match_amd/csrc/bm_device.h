@@ -252,6 +252,8 @@ struct bm_batch;
 int2* bm_batch_pairs_dev(bm_batch* b);
 int64_t* bm_batch_deltas_dev(bm_batch* b);
 int64_t bm_batch_capacity(const bm_batch* b);
+/* Nonzero when the batch holds pairs whose exact deltas have been evaluated. */
+int bm_batch_has_pair_deltas(const bm_batch* b);
 /* The batch now holds k device-generated pairs (not yet evaluated). */
 void bm_batch_mark_pairs(bm_batch* b, int64_t k);
 

@@ -308,3 +308,27 @@ extern "C" int bm_batch_gen_learned(bm_batch* b, bm_mlp* m, uint64_t seed, int64
     bm_batch_mark_pairs(b, top);
     return BM_OK;
 }
+
+extern "C" int bm_mlp_train_batch(bm_mlp* m, bm_batch* b, float lr, float* mse) {
+    if (!m || !b || !(lr > 0.f)) return mfail(m, BM_ERR_ARG, "bm_mlp_train_batch: invalid argument");
+    if (!bm_batch_has_pair_deltas(b))
+        return mfail(m, BM_ERR_STATE, "bm_mlp_train_batch: the batch holds no evaluated pairs");
+    if (!m->features_ready) return mfail(m, BM_ERR_STATE, "bm_mlp_train_batch: call bm_mlp_refresh_features first");
+    const int64_t k = bm_batch_size(b);
+    if (k < 1) return BM_OK;
+    MHIP(m, hipSetDevice(m->device));
+    int rc = grow(m, &m->d_target, &m->target_cap, k);
+    if (rc != BM_OK) return rc;
+    MHIP(m, bm_launch_mlp_targets(bm_batch_deltas_dev(b), k, m->d_target, m->stream));
+    rc = batch_gradient(m, bm_batch_pairs_dev(b), 0, k, m->d_target);
+    if (rc != BM_OK) return rc;
+    m->adam_step++;
+    MHIP(m, bm_launch_mlp_adam(m->d, lr, m->adam_step, m->stream));
+    if (mse) {
+        float se = 0.f;
+        MHIP(m, hipMemcpyAsync(&se, m->d_loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+        MHIP(m, hipStreamSynchronize(m->stream));
+        *mse = se / (float)k;
+    }
+    return BM_OK;
+}

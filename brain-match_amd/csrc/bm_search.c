@@ -436,9 +436,12 @@ int bm_hillclimb_twohop(bm_ctx* ctx, uint64_t seed, int64_t rounds, int32_t anch
  * from stream bm_rand_at(seed, r) -- per_round uniform pairs, or 2-hop pairs
  * (per_round / per_anchor anchors), or the per_round best-scored of `pool`
  * uniform pairs under the learned proposal -- are evaluated exactly and a
- * disjoint set of positive swaps is applied.  trace_score / trace_time
- * (trace_cap entries each, may be NULL) get the score and elapsed seconds
- * after each round. */
+ * disjoint set of positive swaps is applied.  A learned climb may retrain the
+ * network on the candidates it has just evaluated (their exact deltas are
+ * the targets) and, with fallback, take uniform rounds while those gain more
+ * per second (the uniform and 2-hop proposals stay the fallback of the
+ * learned one).  trace_score / trace_time (trace_cap entries each, may be
+ * NULL) get the score and elapsed seconds after each round. */
 int bm_climb(bm_ctx* ctx, const bm_climb_opts* o, int64_t* trace_score, double* trace_time, int64_t trace_cap,
              bm_search_result* res) {
     if (!ctx || !o || !res) return BM_ERR_ARG;
@@ -447,10 +450,14 @@ int bm_climb(bm_ctx* ctx, const bm_climb_opts* o, int64_t* trace_score, double* 
         return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: per_round >= 1 and a time or round limit are required");
     if (o->proposal == BM_PROPOSAL_LEARNED && (!o->mlp || o->pool < o->per_round || o->refresh_every < 1))
         return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: the learned proposal needs mlp, pool >= per_round, refresh_every >= 1");
+    if (o->proposal == BM_PROPOSAL_LEARNED && (o->retrain_every < 0 || (o->retrain_every > 0 && !(o->retrain_lr > 0.f))))
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: retrain_every >= 0, and retrain_lr > 0 when retraining");
     if (o->proposal == BM_PROPOSAL_TWOHOP && (o->per_anchor < 1 || o->per_round < o->per_anchor))
         return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: the 2-hop proposal needs 1 <= per_anchor <= per_round");
     if (o->proposal < BM_PROPOSAL_UNIFORM || o->proposal > BM_PROPOSAL_LEARNED)
         return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_climb: unknown proposal");
+    const int learned = o->proposal == BM_PROPOSAL_LEARNED;
+    const int mixed = learned && o->fallback;
     bm_batch* b = NULL;
     int rc = bm_batch_create(ctx, o->per_round, &b);
     if (rc != BM_OK) return rc;
@@ -458,26 +465,54 @@ int bm_climb(bm_ctx* ctx, const bm_climb_opts* o, int64_t* trace_score, double* 
     if ((rc = bm_score(ctx, &score)) != BM_OK) goto done;
     res->initial_score = score;
     const double t0 = now_s();
+    /* moving averages of exact gain per second: [0] learned rounds, [1] uniform rounds */
+    double rate[2] = {0.0, 0.0};
+    int64_t learned_rounds = 0;
     for (int64_t r = 0;; r++) {
         if (o->max_rounds > 0 && r >= o->max_rounds) break;
-        if (o->time_limit_s > 0 && now_s() - t0 >= o->time_limit_s) break;
+        const double tr = now_s();
+        if (o->time_limit_s > 0 && tr - t0 >= o->time_limit_s) break;
         const uint64_t sd = bm_rand_at(o->seed, (uint64_t)r);
-        if (o->proposal == BM_PROPOSAL_UNIFORM) {
-            rc = bm_batch_gen_pairs(b, sd, o->per_round);
-        } else if (o->proposal == BM_PROPOSAL_TWOHOP) {
+        int use_learned = learned;
+        if (mixed) {
+            if (r < 2)
+                use_learned = r == 0;
+            else {
+                use_learned = rate[0] >= rate[1];
+                if (r % 16 == 15) use_learned = !use_learned;
+            }
+        }
+        if (o->proposal == BM_PROPOSAL_TWOHOP) {
             rc = bm_batch_gen_twohop(b, NULL, (int32_t)(o->per_round / o->per_anchor), o->per_anchor, sd);
-        } else {
-            if (r % o->refresh_every == 0 && (rc = bm_mlp_refresh_features(o->mlp)) != BM_OK) goto done;
+        } else if (use_learned) {
+            if (learned_rounds % o->refresh_every == 0 && (rc = bm_mlp_refresh_features(o->mlp)) != BM_OK) goto done;
             rc = bm_batch_gen_learned(b, o->mlp, sd, o->pool, o->per_round);
+        } else {
+            rc = bm_batch_gen_pairs(b, sd, o->per_round);
         }
         if (rc != BM_OK) goto done;
         if ((rc = bm_batch_eval_swaps(b)) != BM_OK) goto done;
+        if (use_learned) {
+            /* the features still describe the matching the deltas were taken on */
+            if (o->retrain_every > 0 && learned_rounds % o->retrain_every == o->retrain_every - 1 &&
+                (rc = bm_mlp_train_batch(o->mlp, b, o->retrain_lr, NULL)) != BM_OK)
+                goto done;
+            learned_rounds++;
+        } else if (learned) {
+            res->fallback_rounds++;
+        }
         int64_t nacc = 0, gain = 0;
         if ((rc = bm_batch_accept_disjoint(b, 0, &nacc, &gain)) != BM_OK) goto done;
         score += gain;
         res->accepted += nacc;
         res->evaluations += bm_batch_size(b);
         res->iterations++;
+        if (mixed) {
+            const double dt = now_s() - tr;
+            const int src = use_learned ? 0 : 1;
+            const double g = dt > 0 ? (double)gain / dt : 0.0;
+            rate[src] = r < 2 ? g : rate[src] + (g - rate[src]) * 0.125;
+        }
         if (o->verify) {
             int64_t s = 0;
             if ((rc = bm_score(ctx, &s)) != BM_OK) goto done;

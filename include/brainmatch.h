@@ -420,6 +420,8 @@ typedef struct bm_search_result {
     int64_t evaluations;      /* candidates evaluated */
     int64_t verify_failures;  /* running score != device recount (must stay 0) */
     double seconds;
+    int64_t fallback_rounds;  /* bm_climb, learned proposal with fallback: rounds that took uniform
+                                 candidates (0 elsewhere) */
 } bm_search_result;
 
 /* The perturb -> evaluate -> accept loop of optimize_mapping
@@ -489,6 +491,12 @@ int bm_mlp_gradient(bm_mlp* m, const int32_t* uv, const float* target, int64_t k
  * stream bm_rand_at(seed, step) labelled by the context's exact deltas;
  * mse_trace (steps floats, may be NULL) gets each batch's loss. */
 int bm_mlp_train(bm_mlp* m, uint64_t seed, int64_t steps, int32_t batch, float lr, float* mse_trace);
+/* One Adam step on the pairs of an evaluated pair batch and their exact
+ * deltas (the candidates a search has just sent to the delta kernels), with
+ * the features of the current matching (refresh them before the batch is
+ * ranked or evaluated).  mse (may be NULL) gets the batch loss before the
+ * step.  BM_ERR_STATE when the batch holds no evaluated pairs. */
+int bm_mlp_train_batch(bm_mlp* m, bm_batch* b, float lr, float* mse);
 /* The `top` best-scored of `pool` uniform pairs of stream `seed` into the
  * batch as a pair list (descending score). */
 int bm_batch_gen_learned(bm_batch* b, bm_mlp* m, uint64_t seed, int64_t pool, int64_t top);
@@ -508,10 +516,20 @@ typedef struct bm_climb_opts {
     bm_mlp* mlp;              /* learned */
     int32_t refresh_every;    /* learned: rounds between feature refreshes */
     int32_t verify;           /* device recount after every round */
+    int32_t retrain_every;    /* learned: one bm_mlp_train_batch step on a learned round's evaluated
+                                 candidates every this many learned rounds (0: never) */
+    float retrain_lr;         /* learned: Adam step size of those steps (> 0 when retrain_every > 0) */
+    int32_t fallback;         /* learned: nonzero lets each round take uniform candidates instead when
+                                 uniform rounds have recently gained more exact score per second */
 } bm_climb_opts;
 
 /* Disjoint-swap climb with one proposal source, for comparing proposals at
- * equal wall clock (see bm_batch_accept_disjoint for the acceptance). */
+ * equal wall clock (see bm_batch_accept_disjoint for the acceptance).  With
+ * the learned proposal and fallback set, each source keeps a moving average
+ * (weight 1/8 per round) of the exact gain per second of its rounds; a round
+ * uses the source with the larger average, every 16th round the other one,
+ * and the first two rounds one of each.  res->evaluations counts every
+ * candidate sent to the delta kernels; res->accepted every applied swap. */
 int bm_climb(bm_ctx* ctx, const bm_climb_opts* opts, int64_t* trace_score, double* trace_time, int64_t trace_cap,
              bm_search_result* res);
 

@@ -292,3 +292,55 @@ def test_best_swap_climb_ends_at_local_optimum(bm, orc):
         q[u], q[v] = q[v], q[u]
         assert o.score(q) == trace[k]
     assert np.array_equal(q, pi_end)
+
+
+def _last_weights(g):
+    """(src, dst) -> weight with duplicate lines keeping the last weight."""
+    w = {}
+    for s, d, x in zip(g.src.tolist(), g.dst.tolist(), g.w.tolist()):
+        w[(s, d)] = x
+    return w
+
+
+@pytest.mark.parametrize("seed", [3, 9])
+def test_gain_row_pinned_by_recounts(bm, orc, seed):
+    """Every entry of or_gain_row against full recounts on the quirky graph
+    (self-loops, duplicate lines, unmatched vertices, non-positive female
+    weights): moving u alone onto f changes the score by
+    G(u, f) - G(u, pi(u)) plus u's self-loop term; the diagonal sums to twice
+    the score without self-loops; and every swap delta equals four table
+    entries plus the corrections for edges between u and v and self-loops."""
+    n = 60
+    male, female, pi = quirky_pair(bm, seed=seed)
+    o = orc.Oracle(n, male, female)
+    wm, wf = _last_weights(male), _last_weights(female)
+
+    def c(w, a, b):
+        if a == 0 or b == 0 or (a, b) not in wf:
+            return 0
+        return min(w, wf[(a, b)])
+
+    G = np.stack([orc.gain_row(o, pi, u) for u in range(n + 1)])
+    assert not G[0].any() and not G[:, 0].any()
+    s0 = o.score(pi)
+    for u in range(1, n + 1):
+        wuu = wm.get((u, u))
+        for f in range(n + 1):
+            q = pi.copy()
+            q[u] = f
+            selfd = 0 if wuu is None else c(wuu, f, f) - c(wuu, pi[u], pi[u])
+            assert o.score(q) - s0 == G[u, f] - G[u, pi[u]] + selfd, (u, f)
+    loops = sum(c(w, pi[s], pi[s]) for (s, d), w in wm.items() if s == d)
+    assert int(G[np.arange(n + 1), pi].sum()) == 2 * (s0 - loops)
+    for u in range(1, n + 1):
+        for v in range(u + 1, n + 1):
+            f1, f2 = int(pi[u]), int(pi[v])
+            d = int(G[u, f2]) - int(G[u, f1]) + int(G[v, f1]) - int(G[v, f2])
+            for (i, j, fi, fj, gi, gj) in ((u, v, f1, f2, f2, f1), (v, u, f2, f1, f1, f2)):
+                if (i, j) in wm:
+                    w = wm[(i, j)]
+                    d += c(w, gi, gj) - c(w, gi, fj) - c(w, fi, gj) + c(w, fi, fj)
+            for x, fo, fn in ((u, f1, f2), (v, f2, f1)):
+                if (x, x) in wm:
+                    d += c(wm[(x, x)], fn, fn) - c(wm[(x, x)], fo, fo)
+            assert d == o.swap_delta_brute(pi, u, v), (u, v)
