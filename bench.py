@@ -160,6 +160,10 @@ def main():
     ap.add_argument("--learned-plain", action="store_true",
                     help="learned climb without online retraining and uniform fallback")
     ap.add_argument("--retrain-lr", type=float, default=2e-4, help="Adam step size of the learned climb's retraining")
+    ap.add_argument("--round-costs", type=int, default=0, metavar="ROUNDS",
+                    help="instead of the swap-delta benchmark: ROUNDS climb rounds of each proposal source on the "
+                         "n=18,524 pair (uniform, learned, learned with retraining) and their wall time per round "
+                         "(one JSON line; run under rocprofv3 for the kernel breakdown)")
     ap.add_argument("--evaluator", choices=["gain", "scan"], default="scan",
                     help="exact delta strategy of the --proposals / --scale climbs (same deltas either way)")
     ap.add_argument("--anneal", type=int, default=0, metavar="LAUNCHES",
@@ -205,6 +209,8 @@ def main():
         return bench_scale(args, bm)
     if args.best_swap > 0:
         return bench_best_swap(args, bm)
+    if args.round_costs > 0:
+        return bench_round_costs(args, bm)
 
     t = time.time()
     male, female, truth, pi_shuffled = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3,
@@ -371,7 +377,7 @@ def main():
         # the second half of the metric: exact score reached in the same wall
         # clock by each proposal source, from the same greedy matching
         sources = tuple(x for x in args.sources.split(",") if x)
-        pr = compare_proposals(bm, ctx, pi0, args.proposal_seconds, args, sources)
+        pr = compare_proposals(bm, ctx, pi0, args.proposal_seconds, args, sources, planted=truth)
         result["proposals"] = dict(metric=f"exact score at {args.proposal_seconds:g} s, learned vs uniform proposals",
                                    **pr)
     ctx.close()
@@ -381,7 +387,7 @@ def main():
 PROPOSAL_SOURCES = ("uniform", "twohop", "learned")
 
 
-def compare_proposals(bm, ctx, pi0, seconds, args, sources=PROPOSAL_SOURCES):
+def compare_proposals(bm, ctx, pi0, seconds, args, sources=PROPOSAL_SOURCES, planted=None):
     """The learned-proposal workload on the n = 18,524 pair, from the greedy
     matching.  The network is trained on train_samples (candidate, exact
     delta) pairs from the context's own exact kernel; then disjoint-swap
@@ -392,7 +398,11 @@ def compare_proposals(bm, ctx, pi0, seconds, args, sources=PROPOSAL_SOURCES):
     on every learned round's 4096 evaluated candidates and their exact deltas
     and falls back to uniform rounds while those gain more per second.  The
     climbs' candidates are evaluated by the scan kernel (--evaluator scan) or
-    the placement-gain table (--evaluator gain); the deltas are the same."""
+    the placement-gain table (--evaluator gain); the deltas are the same.
+    With `planted` (the pair's hidden permutation) each climb also reports
+    when it first reached the planted matching's score, whether it ended on
+    that matching, and the exact best swap over all pairs at its end (<= 0:
+    no single swap improves the matching)."""
     if args.evaluator == "gain":
         ctx.gain_enable(True)
     ctx.set_mapping(pi0)
@@ -415,6 +425,10 @@ def compare_proposals(bm, ctx, pi0, seconds, args, sources=PROPOSAL_SOURCES):
              "twohop": (bm.PROPOSAL_TWOHOP, {"per_anchor": 64}),
              "learned": (bm.PROPOSAL_LEARNED, learned_kw)}
     out = {}
+    s_planted = None
+    if planted is not None:
+        ctx.set_mapping(planted)
+        s_planted = ctx.score()
     for name in sources:
         prop, kw = kinds[name]
         if name == "learned":
@@ -432,6 +446,15 @@ def compare_proposals(bm, ctx, pi0, seconds, args, sources=PROPOSAL_SOURCES):
                      "gain_last_10pct": int(tr[-1] - tr[last][0]) if last.any() else 0}
         if name == "learned":
             out[name]["uniform_fallback_rounds"] = res["fallback_rounds"]
+        if s_planted is not None:
+            hit = np.nonzero(tr >= s_planted)[0]
+            out[name]["reached_planted_score_at_s"] = float(tt[hit[0]]) if len(hit) else None
+            out[name]["ends_on_planted_matching"] = bool(np.array_equal(ctx.get_mapping(), planted))
+            was_on = ctx.gain_enabled()
+            ctx.gain_enable(True)
+            d_best, u_best, v_best = ctx.best_swap()
+            ctx.gain_enable(was_on)
+            out[name]["best_swap_delta_at_end"] = int(d_best)
         log(f"{name}: score {score0} -> {res['final_score']} in {res['seconds']:.1f}s "
             f"({res['iterations']} rounds, {res['accepted']} swaps"
             + (f", {res['fallback_rounds']} uniform fallback rounds" if name == "learned" else "") + ")")
@@ -440,7 +463,7 @@ def compare_proposals(bm, ctx, pi0, seconds, args, sources=PROPOSAL_SOURCES):
     if args.evaluator == "gain":
         ctx.gain_enable(False)
     cfg = {"seconds": seconds, "evaluator": args.evaluator, "initial_matching": "greedy-match",
-           "initial_score": score0, "per_round": 4096, "learned_pool": 1 << 20, "learned_top": 4096,
+           "initial_score": score0, "planted_matching_score": s_planted, "per_round": 4096, "learned_pool": 1 << 20, "learned_top": 4096,
            "learned_online_retraining": not plain, "learned_uniform_fallback": not plain,
            "retrain_lr": None if plain else args.retrain_lr,
            "train_samples": steps * batch, "train_seconds": train_s,
@@ -457,7 +480,7 @@ def bench_proposals(args, bm):
     pi0 = bm.greedy_match(N_VERTICES, male, female)
     ctx = bm.Context(N_VERTICES, male, female)
     sources = tuple(x for x in args.sources.split(",") if x)
-    r = compare_proposals(bm, ctx, pi0, args.proposals, args, sources)
+    r = compare_proposals(bm, ctx, pi0, args.proposals, args, sources, planted=truth)
     out = r["proposals"]
     result = {
         "metric": f"exact score at {args.proposals:g} s, learned vs uniform proposals",
@@ -474,6 +497,44 @@ def bench_proposals(args, bm):
         result["learned_minus_uniform"] = r["learned_minus_uniform"]
     ctx.close()
     print(json.dumps(result), flush=True)
+
+
+def bench_round_costs(args, bm):
+    """Wall time of one disjoint-swap climb round per proposal source on the
+    n = 18,524 pair from the greedy matching (4096 exact candidates a round;
+    learned: the 4096 best of 2^20 ranked by the network, features refreshed
+    every round), after a short training run.  Each source climbs the same
+    number of rounds from the same matching."""
+    male, female, truth, _ = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3, seed=1)
+    pi0 = bm.greedy_match(N_VERTICES, male, female)
+    ctx = bm.Context(N_VERTICES, male, female)
+    ctx.set_mapping(pi0)
+    mlp = bm.Mlp(ctx, seed=1)
+    mlp.refresh_features()
+    mlp.train(seed=11, steps=64, batch=1 << 16, lr=1e-3)
+    trained = mlp.params()
+    R = args.round_costs
+    out = {}
+    for name, prop, kw in (("uniform", bm.PROPOSAL_UNIFORM, {}),
+                           ("learned", bm.PROPOSAL_LEARNED, {"pool": 1 << 20, "mlp": mlp, "refresh_every": 1}),
+                           ("learned_retrain", bm.PROPOSAL_LEARNED,
+                            {"pool": 1 << 20, "mlp": mlp, "refresh_every": 1, "retrain_every": 1,
+                             "retrain_lr": args.retrain_lr})):
+        mlp.set_params(trained)
+        ctx.set_mapping(pi0)
+        ctx.sync()
+        t = time.perf_counter()
+        tr, tt, res = ctx.climb(prop, seed=21, max_rounds=R, per_round=4096, **kw)
+        el = time.perf_counter() - t
+        out[name] = {"rounds": res["iterations"], "ms_per_round": 1000 * el / max(1, res["iterations"]),
+                     "final_score": res["final_score"], "accepted": res["accepted"]}
+        log(f"{name}: {R} rounds in {el:.2f}s ({1000 * el / R:.3f} ms/round), score {tr[0]} -> {tr[-1]}")
+    mlp.close()
+    ctx.close()
+    print(json.dumps({"metric": "climb round wall time", "unit": "ms/round", "n_gpus": 1, "data": "synthetic",
+                      "config": {"workload": "n=18524 ~4M edges, greedy start, 4096 exact candidates per round",
+                                 "learned_pool": 1 << 20, "rounds": R},
+                      "rounds": out}), flush=True)
 
 
 def bench_anneal(args, bm):

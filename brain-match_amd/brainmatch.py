@@ -575,7 +575,7 @@ class Context:
 
 class _AnnealParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("chains_per_instance", C.c_int32), ("t0", C.c_double),
-                ("decay", C.c_double), ("p_rot3", C.c_double), ("traj_cap", C.c_int32)]
+                ("decay", C.c_double), ("p_rot3", C.c_double), ("traj_cap", C.c_int32), ("p_pool", C.c_double)]
 
 
 _sig("bm_anneal_create", C.c_int, [C.POINTER(_c), C.POINTER(_c), C.c_int32, C.POINTER(_AnnealParams)])
@@ -591,6 +591,11 @@ _sig("bm_anneal_algo_bytes", C.c_int, [_c, i64p, i64p])
 _sig("bm_anneal_last_error", C.c_char_p, [_c])
 _sig("bm_anneal_schedule", C.c_int, [_c, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint64)])
 _sig("bm_anneal_reset", C.c_int, [_c])
+_sig("bm_anneal_move_counts", C.c_int, [_c, i64p, i64p])
+_sig("bm_anneal_set_pool", C.c_int, [_c, C.c_int32, i32p, C.c_int64])
+_sig("bm_anneal_pool_learned", C.c_int, [_c, C.c_int32, _c, C.c_uint64, C.c_int64, C.c_int64])
+_sig("bm_anneal_get_pool", C.c_int, [_c, C.c_int32, i32p, i64p])
+_sig("bm_anneal_p_pool", C.c_uint64, [_c])
 _sig("bm_anneal_get_tuning", C.c_int, [_c, C.POINTER(_Tuning)])
 _sig("bm_anneal_set_tuning", C.c_int, [_c, C.POINTER(_Tuning)])
 _sig("bm_anneal_autotune", C.c_int, [_c, C.c_int64, C.POINTER(_Tuning), C.POINTER(C.c_double), C.c_int32,
@@ -602,14 +607,15 @@ class Anneal:
     chains_per_instance chains per context, all advanced by one kernel launch
     replayed from a HIP graph (bm_anneal_* in include/brainmatch.h)."""
 
-    def __init__(self, contexts, chains_per_instance=1, seed=1, t0=50.0, decay=0.9999, p_rot3=0.0, traj_cap=0):
+    def __init__(self, contexts, chains_per_instance=1, seed=1, t0=50.0, decay=0.9999, p_rot3=0.0, traj_cap=0,
+                 p_pool=0.0):
         self.contexts = list(contexts)
         self.seed = seed
         self.ns = [c.n for c in self.contexts]
         self.per_instance = chains_per_instance
         self.traj_cap = traj_cap
         arr = (_c * len(self.contexts))(*[c.h for c in self.contexts])
-        p = _AnnealParams(seed, chains_per_instance, t0, decay, p_rot3, traj_cap)
+        p = _AnnealParams(seed, chains_per_instance, t0, decay, p_rot3, traj_cap, p_pool)
         self.h = _c()
         rc = _lib.bm_anneal_create(C.byref(self.h), arr, len(self.contexts), C.byref(p))
         if rc != 0:
@@ -680,6 +686,33 @@ class Anneal:
 
     def reset(self):
         self._chk(_lib.bm_anneal_reset(self.h))
+
+    def move_counts(self):
+        """Per chain: 3-vertex moves and moves built on a pool pair so far."""
+        r = np.zeros(self.chains, np.int64)
+        p = np.zeros(self.chains, np.int64)
+        self._chk(_lib.bm_anneal_move_counts(self.h, _p64(r), _p64(p)))
+        return r, p
+
+    def set_pool(self, inst, uv):
+        uv = np.ascontiguousarray(uv, dtype=np.int32).reshape(-1, 2)
+        self._chk(_lib.bm_anneal_set_pool(self.h, inst, _p32(uv), uv.shape[0]))
+
+    def pool_learned(self, inst, mlp, seed, pool=1 << 20, top=1 << 16):
+        """Instance inst's pool <- the top best-ranked of `pool` uniform pairs
+        under mlp (a network of that instance's context)."""
+        self._chk(_lib.bm_anneal_pool_learned(self.h, inst, mlp.h, seed, pool, top))
+
+    def get_pool(self, inst):
+        k = C.c_int64(0)
+        self._chk(_lib.bm_anneal_get_pool(self.h, inst, None, C.byref(k)))
+        uv = np.zeros((k.value, 2), np.int32)
+        self._chk(_lib.bm_anneal_get_pool(self.h, inst, _p32(uv), C.byref(k)))
+        return uv
+
+    def p_pool(self):
+        """The pool threshold on 32 random bits, as the kernels use it."""
+        return int(_lib.bm_anneal_p_pool(self.h))
 
     def tuning(self):
         t = _Tuning()

@@ -8,7 +8,9 @@
  *           acceptance counters, stream seed, offset of its matchings);
  *   pool    per chain the current matching and the best-so-far copy,
  *           2 * (n_i + 1) int32 each, chains of one instance contiguous;
- *   traj    chains x traj_cap int64 score samples, one per launch.
+ *   traj    chains x traj_cap int64 score samples, one per launch;
+ *   props   one BmProposalPool per instance, pointing at that instance's
+ *           proposal pairs (a device buffer owned here, grown on demand).
  * A launch advances every chain by a fixed number of steps (k_anneal in
  * bm_delta.hip); it is captured once into a HIP graph and replayed.  The
  * chain state lives only on the device, so a replay needs no host input.
@@ -45,6 +47,10 @@ struct bm_anneal {
     int32_t* d_pool;
     int64_t* d_traj;
     int64_t* d_scores;                /* n_chains, for bm_anneal_verify */
+    BmProposalPool* d_props;          /* n_inst pool descriptors */
+    std::vector<BmProposalPool> h_props;
+    std::vector<int2*> prop_buf;      /* per instance, capacity prop_cap */
+    std::vector<int64_t> prop_cap;
     BmAnnealSched sc;
     BmLaunch L;
     int compact;
@@ -93,6 +99,8 @@ static void free_anneal(bm_anneal* a) {
     (void)hipFree(a->d_traj);
     (void)hipFree(a->d_scores);
     (void)hipFree(a->d_init);
+    (void)hipFree(a->d_props);
+    for (int2* p : a->prop_buf) (void)hipFree(p);
     if (a->stream) (void)hipStreamDestroy(a->stream);
     delete a;
 }
@@ -114,6 +122,7 @@ extern "C" int bm_anneal_create(bm_anneal** out, bm_ctx* const* ctxs, int32_t n_
     if (!(p->decay > 0.0 && p->decay <= 1.0)) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: decay must be in (0, 1]");
     if (!(p->p_rot3 >= 0.0 && p->p_rot3 <= 1.0)) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: p_rot3 must be in [0, 1]");
     if (p->traj_cap < 0) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: traj_cap must be >= 0");
+    if (!(p->p_pool >= 0.0 && p->p_pool <= 1.0)) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: p_pool must be in [0, 1]");
     for (int32_t i = 0; i < n_inst; i++)
         if (!ctxs[i]) return afail(nullptr, BM_ERR_ARG, "bm_anneal_create: context %d is NULL", i);
     const int dev = bm_ctx_device(ctxs[0]);
@@ -198,6 +207,12 @@ extern "C" int bm_anneal_create(bm_anneal** out, bm_ctx* const* ctxs, int32_t n_
     ACHK(hipMalloc((void**)&a->d_chains, sizeof(BmChain) * (size_t)a->n_chains));
     ACHK(hipMalloc((void**)&a->d_pool, sizeof(int32_t) * (size_t)pool));
     ACHK(hipMalloc((void**)&a->d_scores, sizeof(int64_t) * (size_t)a->n_chains));
+    a->h_props.assign((size_t)n_inst, BmProposalPool{nullptr, 0});
+    a->prop_buf.assign((size_t)n_inst, nullptr);
+    a->prop_cap.assign((size_t)n_inst, 0);
+    ACHK(hipMalloc((void**)&a->d_props, sizeof(BmProposalPool) * (size_t)n_inst));
+    ACHK(hipMemcpyAsync(a->d_props, a->h_props.data(), sizeof(BmProposalPool) * (size_t)n_inst, hipMemcpyHostToDevice,
+                        a->stream));
     if (p->traj_cap > 0) ACHK(hipMalloc((void**)&a->d_traj, sizeof(int64_t) * (size_t)a->n_chains * p->traj_cap));
     ACHK(hipMemcpyAsync(a->d_insts, a->graphs.data(), sizeof(BmDevGraph) * (size_t)n_inst, hipMemcpyHostToDevice,
                         a->stream));
@@ -217,6 +232,7 @@ extern "C" int bm_anneal_create(bm_anneal** out, bm_ctx* const* ctxs, int32_t n_
     a->sc.p_rot3 = (uint64_t)llround(p->p_rot3 * 4294967296.0);
     a->sc.traj_cap = p->traj_cap;
     a->sc.pad = 0;
+    a->sc.p_pool = (uint64_t)llround(p->p_pool * 4294967296.0);
     ACHK(bm_anneal_grid(a->compact, a->sc.p_rot3 != 0, a->L, a->n_chains, &a->grid, &a->block, &a->lds));
     ACHK(hipStreamSynchronize(a->stream));
 #undef ACHK
@@ -235,7 +251,7 @@ extern "C" int bm_anneal_run(bm_anneal* a, int64_t steps, int32_t launches) {
         a->graph = nullptr;
         AHIP(a, hipStreamBeginCapture(a->stream, hipStreamCaptureModeThreadLocal));
         hipError_t le = bm_launch_anneal(a->d_insts, a->compact, a->d_chains, a->n_chains, a->d_pool, steps, a->sc,
-                                         a->d_traj, a->L, a->grid, a->block, a->lds, a->stream);
+                                         a->d_traj, a->d_props, a->L, a->grid, a->block, a->lds, a->stream);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(a->stream, &g);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -359,21 +375,96 @@ extern "C" int bm_anneal_algo_bytes(bm_anneal* a, int64_t* bytes, int64_t* steps
     std::vector<BmChain> h;
     int rc = fetch_chains(a, h);
     if (rc != BM_OK) return rc;
-    std::vector<const bm_graph_host*> g((size_t)a->n_inst);
-    for (int32_t i = 0; i < a->n_inst; i++) g[i] = bm_ctx_male(a->ctxs[i]);
-    std::vector<int32_t> inst((size_t)a->n_chains);
-    std::vector<uint64_t> seeds((size_t)a->n_chains);
-    std::vector<int64_t> st((size_t)a->n_chains);
-    int64_t S = 0;
+    /* swap model 48 + 28 D per swap step, triple model 69 + 36 D3 per
+     * 3-vertex step, from the degree sums the chains keep */
+    int64_t B = 0, S = 0;
     for (int64_t c = 0; c < a->n_chains; c++) {
-        inst[c] = h[c].inst;
-        seeds[c] = h[c].seed;
-        st[c] = h[c].step;
-        S += h[c].step;
+        const BmChain& ch = h[(size_t)c];
+        B += 48 * (ch.step - ch.n_rot3) + 28 * ch.deg_swap + 69 * ch.n_rot3 + 36 * ch.deg_rot3;
+        S += ch.step;
     }
-    const int64_t B = bm_chain_algo_bytes(g.data(), inst.data(), seeds.data(), st.data(), a->n_chains, a->sc.p_rot3);
     if (bytes) *bytes = B;
     if (steps) *steps = S;
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_move_counts(bm_anneal* a, int64_t* rot3, int64_t* pooled) {
+    if (!a) return afail(nullptr, BM_ERR_ARG, "bm_anneal_move_counts: NULL object");
+    std::vector<BmChain> h;
+    int rc = fetch_chains(a, h);
+    if (rc != BM_OK) return rc;
+    for (int64_t c = 0; c < a->n_chains; c++) {
+        if (rot3) rot3[c] = h[(size_t)c].n_rot3;
+        if (pooled) pooled[c] = h[(size_t)c].n_learned;
+    }
+    return BM_OK;
+}
+
+/* Device buffer of instance inst's pool holding at least k pairs. */
+static int pool_reserve(bm_anneal* a, int32_t inst, int64_t k) {
+    if (a->prop_cap[(size_t)inst] >= k) return BM_OK;
+    AHIP(a, hipStreamSynchronize(a->stream));
+    (void)hipFree(a->prop_buf[(size_t)inst]);
+    a->prop_buf[(size_t)inst] = nullptr;
+    a->prop_cap[(size_t)inst] = 0;
+    a->h_props[(size_t)inst] = BmProposalPool{nullptr, 0};
+    hipError_t e = hipMalloc((void**)&a->prop_buf[(size_t)inst], sizeof(int2) * (size_t)k);
+    if (e != hipSuccess)
+        return afail(a, e == hipErrorOutOfMemory ? BM_ERR_NOMEM : BM_ERR_HIP, "bm_anneal pool: hipMalloc failed: %s",
+                     hipGetErrorString(e));
+    a->prop_cap[(size_t)inst] = k;
+    return BM_OK;
+}
+
+static int pool_publish(bm_anneal* a, int32_t inst, int64_t k) {
+    a->h_props[(size_t)inst] = BmProposalPool{k ? a->prop_buf[(size_t)inst] : nullptr, k};
+    AHIP(a, hipMemcpyAsync(a->d_props + inst, &a->h_props[(size_t)inst], sizeof(BmProposalPool),
+                           hipMemcpyHostToDevice, a->stream));
+    AHIP(a, hipStreamSynchronize(a->stream));
+    return BM_OK;
+}
+
+extern "C" int bm_anneal_set_pool(bm_anneal* a, int32_t inst, const int32_t* uv, int64_t k) {
+    if (!a) return afail(nullptr, BM_ERR_ARG, "bm_anneal_set_pool: NULL object");
+    if (inst < 0 || inst >= a->n_inst || k < 0 || k > (1LL << 31) || (k > 0 && !uv))
+        return afail(a, BM_ERR_ARG, "bm_anneal_set_pool: instance out of range or k outside 0..2^31");
+    const int32_t n = a->graphs[(size_t)inst].n;
+    for (int64_t i = 0; i < k; i++)
+        if (uv[2 * i] < 1 || uv[2 * i] > n || uv[2 * i + 1] < 1 || uv[2 * i + 1] > n || uv[2 * i] == uv[2 * i + 1])
+            return afail(a, BM_ERR_ARG, "bm_anneal_set_pool: pair %lld is not two distinct vertices in 1..n", (long long)i);
+    AHIP(a, hipSetDevice(a->device));
+    int rc = pool_reserve(a, inst, k);
+    if (rc != BM_OK) return rc;
+    if (k) AHIP(a, hipMemcpyAsync(a->prop_buf[(size_t)inst], uv, sizeof(int2) * (size_t)k, hipMemcpyHostToDevice,
+                                 a->stream));
+    return pool_publish(a, inst, k);
+}
+
+extern "C" int bm_anneal_pool_learned(bm_anneal* a, int32_t inst, bm_mlp* m, uint64_t seed, int64_t pool, int64_t top) {
+    if (!a || !m) return afail(a, BM_ERR_ARG, "bm_anneal_pool_learned: null argument");
+    if (inst < 0 || inst >= a->n_inst || top < 1 || top > pool || top > (1LL << 31))
+        return afail(a, BM_ERR_ARG, "bm_anneal_pool_learned: instance out of range or not 1 <= top <= pool");
+    if (bm_mlp_context(m) != a->ctxs[(size_t)inst])
+        return afail(a, BM_ERR_ARG, "bm_anneal_pool_learned: the network belongs to another instance's context");
+    AHIP(a, hipSetDevice(a->device));
+    int rc = pool_reserve(a, inst, top);
+    if (rc != BM_OK) return rc;
+    rc = bm_mlp_rank_into(m, seed, pool, top, a->prop_buf[(size_t)inst]);
+    if (rc != BM_OK) return afail(a, rc, "bm_anneal_pool_learned: %s", bm_last_error(a->ctxs[(size_t)inst]));
+    return pool_publish(a, inst, top);
+}
+
+extern "C" int bm_anneal_get_pool(bm_anneal* a, int32_t inst, int32_t* uv, int64_t* k) {
+    if (!a || !k) return afail(a, BM_ERR_ARG, "bm_anneal_get_pool: null argument");
+    if (inst < 0 || inst >= a->n_inst) return afail(a, BM_ERR_ARG, "bm_anneal_get_pool: instance out of range");
+    const int64_t cnt = a->h_props[(size_t)inst].count;
+    if (uv && cnt) {
+        AHIP(a, hipSetDevice(a->device));
+        AHIP(a, hipMemcpyAsync(uv, a->prop_buf[(size_t)inst], sizeof(int2) * (size_t)cnt, hipMemcpyDeviceToHost,
+                               a->stream));
+        AHIP(a, hipStreamSynchronize(a->stream));
+    }
+    *k = cnt;
     return BM_OK;
 }
 
@@ -384,6 +475,8 @@ extern "C" int bm_anneal_schedule(const bm_anneal* a, double* beta0, double* lam
     if (p_rot3) *p_rot3 = a->sc.p_rot3;
     return BM_OK;
 }
+
+extern "C" uint64_t bm_anneal_p_pool(const bm_anneal* a) { return a ? a->sc.p_pool : 0; }
 
 /* Chains back to their creation state (matchings, scores, counters). */
 static int reset_chains(bm_anneal* a) {
@@ -484,7 +577,7 @@ extern "C" int bm_anneal_autotune(bm_anneal* a, int64_t steps, bm_tuning* best, 
                 hipError_t e = hipEventRecord(e0, a->stream);
                 if (e == hipSuccess)
                     e = bm_launch_anneal(a->d_insts, a->compact, a->d_chains, a->n_chains, a->d_pool, steps, a->sc,
-                                         nullptr, a->L, a->grid, a->block, a->lds, a->stream);
+                                         nullptr, a->d_props, a->L, a->grid, a->block, a->lds, a->stream);
                 if (e == hipSuccess) e = hipEventRecord(e1, a->stream);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
                 float ms = 0.f;

@@ -695,8 +695,13 @@ __global__ void __launch_bounds__(512) k_score(BmDevGraph g, int64_t* __restrict
 
 /* One wave per chain.  Step s of a chain with stream seed q:
  *   kind:  r_k = rand(q2, s); a 3-vertex move if lo32(r_k) < p_rot3 (n >= 3);
- *   swap:  (u, v) = uniform pair s of stream q0, delta exact;
- *   3-move: (a, b, c) = uniform triple s of stream q3, permutation
+ *   pool:  with a proposal pool of P pairs for the chain's instance,
+ *          r_l = rand(q4, s); the move is built on pool entry
+ *          range(hi32(r_l), P) if lo32(r_l) < p_pool;
+ *   swap:  (u, v) = that pool pair, else uniform pair s of stream q0;
+ *          delta exact;
+ *   3-move: (a, b) = that pool pair and c = bm_third_vertex(q3, s, n, a, b),
+ *          else (a, b, c) = uniform triple s of stream q3; permutation
  *          p = 1 + range(hi32(r_k), 5), delta of that permutation exact;
  *   accept if delta >= 0, else if U < exp(delta * beta_s) with
  *          U = (rand(q1, s) >> 11) * 2^-53 and beta_s = beta0 * exp(s * lam),
@@ -707,11 +712,14 @@ __global__ void __launch_bounds__(512) k_score(BmDevGraph g, int64_t* __restrict
  * maximum).  The Metropolis rule and the geometric schedule follow
  * simulated_annealing (src/best-swap/score-lookup.c:345-400); its delta
  * comes from a lookup table and its stream from xoshiro256**, here the
- * delta is recomputed exactly every step and the streams are counter-based. */
+ * delta is recomputed exactly every step and the streams are counter-based.
+ * Each chain also sums the male degrees of the vertices of its moves (the
+ * byte model of the delta kernels) and counts its 3-vertex and pool moves. */
 template <bool COMPACT, int U, bool ROT>
 __global__ void __launch_bounds__(512) k_anneal(const BmDevGraph* __restrict__ insts, BmChain* __restrict__ chains,
                                                int64_t n_chains, int32_t* __restrict__ pool, int64_t steps,
-                                               BmAnnealSched sc, int64_t* __restrict__ traj, uint32_t cap) {
+                                               BmAnnealSched sc, int64_t* __restrict__ traj, uint32_t cap,
+                                               const BmProposalPool* __restrict__ props) {
     typedef typename Slot<COMPACT>::T S;
     extern __shared__ uint4 smem[];
     const int lane = threadIdx.x & (BM_WAVE - 1);
@@ -728,24 +736,61 @@ __global__ void __launch_bounds__(512) k_anneal(const BmDevGraph* __restrict__ i
         g.pi = pi;
         const uint64_t q = (uint64_t)uni64((int64_t)ch.seed);
         const uint64_t q0 = bm_rand_at(q, 0), q1 = bm_rand_at(q, 1), q2 = bm_rand_at(q, 2), q3 = bm_rand_at(q, 3);
+        const uint64_t q4 = bm_rand_at(q, 4);
         int64_t score = uni64(ch.score), best = uni64(ch.best), nacc = uni64(ch.accepted);
         int at_best = uni(ch.at_best);
         const int64_t s0 = uni64(ch.step);
+        /* the instance's proposal pool (count 0: every move uniform) */
+        const int2* pp = nullptr;
+        int64_t pcount = 0;
+        if (sc.p_pool && props) {
+            pp = reinterpret_cast<const int2*>(uni64((int64_t)reinterpret_cast<uintptr_t>(props[uni(ch.inst)].pairs)));
+            pcount = uni64(props[uni(ch.inst)].count);
+        }
+        int64_t dsw = 0, drt = 0, nrt = 0, npool = 0;
         for (int64_t s = s0; s < s0 + steps; s++) {
             const uint64_t rk = bm_rand_at(q2, (uint64_t)s);
             /* ROT = false: swap-only chains (p_rot3 == 0), built without the
              * 3-vertex path and its registers */
             const bool rot = ROT && n >= 3 && (uint64_t)(uint32_t)rk < sc.p_rot3;
+            /* a move from the pool: entry range(hi32, count) of draw s of stream q4 */
+            int2 from = make_int2(0, 0);
+            if (pcount > 0) {
+                const uint64_t rl = bm_rand_at(q4, (uint64_t)s);
+                if ((uint64_t)(uint32_t)rl < sc.p_pool) {
+                    from = pp[(int64_t)(((uint64_t)(uint32_t)(rl >> 32) * (uint64_t)pcount) >> 32)];
+                    from.x = uni(from.x);
+                    from.y = uni(from.y);
+                    npool++;
+                }
+            }
             int32_t vtx[3] = {0, 0, 0}, f[3] = {0, 0, 0};
             int64_t d;
             int p = 0;
             if (!rot) {
-                bm_uniform_pair(q0, (uint64_t)s, n, &vtx[0], &vtx[1]);
+                if (from.x) {
+                    vtx[0] = from.x;
+                    vtx[1] = from.y;
+                } else {
+                    bm_uniform_pair(q0, (uint64_t)s, n, &vtx[0], &vtx[1]);
+                }
+                dsw += uni64((g.m_out_ptr[vtx[0] + 1] - g.m_out_ptr[vtx[0]]) + (g.m_in_ptr[vtx[0] + 1] - g.m_in_ptr[vtx[0]]) +
+                             (g.m_out_ptr[vtx[1] + 1] - g.m_out_ptr[vtx[1]]) + (g.m_in_ptr[vtx[1] + 1] - g.m_in_ptr[vtx[1]]));
                 f[0] = uni(pi[vtx[0]]);
                 f[1] = uni(pi[vtx[1]]);
                 d = swap_delta_wave<COMPACT, U>(g, vtx[0], vtx[1], f[0], f[1], lds, cap, lane);
             } else {
-                bm_uniform_triple(q3, (uint64_t)s, n, &vtx[0], &vtx[1], &vtx[2]);
+                if (from.x) {
+                    vtx[0] = from.x;
+                    vtx[1] = from.y;
+                    vtx[2] = bm_third_vertex(q3, (uint64_t)s, n, from.x, from.y);
+                } else {
+                    bm_uniform_triple(q3, (uint64_t)s, n, &vtx[0], &vtx[1], &vtx[2]);
+                }
+                nrt++;
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    drt += uni64((g.m_out_ptr[vtx[i] + 1] - g.m_out_ptr[vtx[i]]) + (g.m_in_ptr[vtx[i] + 1] - g.m_in_ptr[vtx[i]]));
 #pragma unroll
                 for (int i = 0; i < 3; i++) f[i] = uni(pi[vtx[i]]);
                 int64_t d5[6];
@@ -792,6 +837,10 @@ __global__ void __launch_bounds__(512) k_anneal(const BmDevGraph* __restrict__ i
             o.accepted = nacc;
             o.step = s0 + steps;
             o.at_best = at_best;
+            o.deg_swap += dsw;
+            o.deg_rot3 += drt;
+            o.n_rot3 += nrt;
+            o.n_learned += npool;
             if (traj && o.nseg < sc.traj_cap) traj[c * sc.traj_cap + o.nseg] = score;
             o.nseg++;
             chains[c] = o;
@@ -889,11 +938,13 @@ hipError_t bm_anneal_grid(int compact, int rot, const BmLaunch& L, int64_t n_cha
 
 hipError_t bm_launch_anneal(const BmDevGraph* insts, int compact, BmChain* chains, int64_t n_chains,
                             int32_t* pool, int64_t steps, const BmAnnealSched& sc, int64_t* traj,
-                            const BmLaunch& L, dim3 grid, dim3 block, size_t lds, hipStream_t s) {
+                            const BmProposalPool* proposals, const BmLaunch& L, dim3 grid, dim3 block, size_t lds,
+                            hipStream_t s) {
     if (n_chains <= 0 || steps <= 0) return hipSuccess;
     const uint32_t cap = (uint32_t)L.lds_slots;
-#define BM_ANNEAL_LAUNCH(C, U, R) \
-    hipLaunchKernelGGL((k_anneal<C, U, R>), grid, block, lds, s, insts, chains, n_chains, pool, steps, sc, traj, cap)
+#define BM_ANNEAL_LAUNCH(C, U, R)                                                                                    \
+    hipLaunchKernelGGL((k_anneal<C, U, R>), grid, block, lds, s, insts, chains, n_chains, pool, steps, sc, traj, cap, \
+                       proposals)
     const bool rot = sc.p_rot3 != 0;
     if (L.unroll == 2) {
         if (compact) {

@@ -155,6 +155,17 @@ struct BmChain {
     int32_t at_best;    /* 1: the current matching is the best one (the copy is stale) */
     int32_t nseg;       /* launches run (trajectory entries) */
     int32_t pad;
+    int64_t deg_swap;   /* sum over the swap moves taken of D(u, v) (byte model) */
+    int64_t deg_rot3;   /* sum over the 3-vertex moves taken of D3 */
+    int64_t n_rot3;     /* 3-vertex moves taken */
+    int64_t n_learned;  /* moves drawn from the instance's proposal pool */
+};
+
+/* Proposal pool of one instance: the chains of that instance draw a fraction
+ * of their moves from these pairs (see k_anneal). */
+struct BmProposalPool {
+    const int2* pairs;  /* count pairs u != v in 1..n */
+    int64_t count;      /* 0: no pool */
 };
 
 struct BmAnnealSched {
@@ -163,6 +174,7 @@ struct BmAnnealSched {
     uint64_t p_rot3;    /* probability of a 3-vertex move, in units of 2^-32 (0..2^32) */
     int32_t traj_cap;   /* trajectory entries kept per chain */
     int32_t pad;
+    uint64_t p_pool;    /* probability that a move is drawn from the proposal pool, units of 2^-32 */
 };
 
 /* Persistent grid for n_chains chains (one wave each); computed once. */
@@ -170,7 +182,8 @@ hipError_t bm_anneal_grid(int compact, int rot, const BmLaunch& L, int64_t n_cha
 /* Advances every chain by `steps` steps.  insts: device array of graphs. */
 hipError_t bm_launch_anneal(const BmDevGraph* insts, int compact, BmChain* chains, int64_t n_chains,
                             int32_t* pool, int64_t steps, const BmAnnealSched& sc, int64_t* traj,
-                            const BmLaunch& L, dim3 grid, dim3 block, size_t lds, hipStream_t s);
+                            const BmProposalPool* proposals, const BmLaunch& L, dim3 grid, dim3 block, size_t lds,
+                            hipStream_t s);
 
 hipError_t bm_launch_gen_pairs(int32_t n, uint64_t seed, int64_t k, int2* cand, hipStream_t s);
 hipError_t bm_launch_gen_triples(int32_t n, uint64_t seed, int64_t k, int4* cand, hipStream_t s);
@@ -252,6 +265,13 @@ struct bm_batch;
 int2* bm_batch_pairs_dev(bm_batch* b);
 int64_t* bm_batch_deltas_dev(bm_batch* b);
 int64_t bm_batch_capacity(const bm_batch* b);
+/* The context a network was created for. */
+typedef struct bm_mlp bm_mlp;
+bm_ctx* bm_mlp_context(const bm_mlp* m);
+/* The `top` best-scored of `pool` uniform pairs of stream `seed` (as
+ * bm_batch_gen_learned) into the device array out; synchronises the
+ * context stream before returning. */
+int bm_mlp_rank_into(bm_mlp* m, uint64_t seed, int64_t pool, int64_t top, int2* out);
 /* Nonzero when the batch holds pairs whose exact deltas have been evaluated. */
 int bm_batch_has_pair_deltas(const bm_batch* b);
 /* The batch now holds k device-generated pairs (not yet evaluated). */

@@ -68,10 +68,6 @@ const bm_graph_host* bm_ctx_male(const bm_ctx* ctx);
 /* Sets the context error message (for host drivers). */
 int bm_ctx_fail(bm_ctx* ctx, int code, const char* msg);
 
-/* Algorithmic bytes of annealing steps (bm_measure.c). */
-int64_t bm_chain_algo_bytes(const bm_graph_host* const* graphs, const int32_t* inst, const uint64_t* seeds,
-                            const int64_t* steps, int64_t n_chains, uint64_t p_rot3);
-
 #ifdef __cplusplus
 }
 #endif

@@ -295,6 +295,21 @@ extern "C" int bm_mlp_train(bm_mlp* m, uint64_t seed, int64_t steps, int32_t bat
     return BM_OK;
 }
 
+bm_ctx* bm_mlp_context(const bm_mlp* m) { return m ? m->ctx : nullptr; }
+
+int bm_mlp_rank_into(bm_mlp* m, uint64_t seed, int64_t pool, int64_t top, int2* out) {
+    if (pool < 1 || top < 1 || top > pool || pool > (1LL << 30))
+        return mfail(m, BM_ERR_ARG, "learned ranking: need 1 <= top <= pool <= 2^30");
+    if (!m->features_ready) return mfail(m, BM_ERR_STATE, "learned ranking: call bm_mlp_refresh_features first");
+    MHIP(m, hipSetDevice(m->device));
+    int rc = grow_bytes(m, &m->d_rank, &m->rank_bytes, bm_mlp_rank_scratch(pool));
+    if (rc != BM_OK) return rc;
+    MHIP(m, bm_launch_mlp_rank(m->d, m->d_vf, seed, m->n, pool, top, m->d_rank, m->rank_bytes, out, m->num_cus,
+                               m->stream));
+    MHIP(m, hipStreamSynchronize(m->stream));
+    return BM_OK;
+}
+
 extern "C" int bm_batch_gen_learned(bm_batch* b, bm_mlp* m, uint64_t seed, int64_t pool, int64_t top) {
     if (!b || !m) return BM_ERR_ARG;
     if (pool < 1 || top < 1 || top > pool || pool > (1LL << 30) || top > bm_batch_capacity(b))

@@ -45,37 +45,6 @@ int bm_uniform_algo_bytes(bm_ctx* ctx, uint64_t seed, int64_t k, int64_t* bytes,
     return BM_OK;
 }
 
-/* Bytes of the annealing steps taken so far: every chain's proposals are
- * replayed from its stream (the same draws as k_anneal) and priced with the
- * swap model above or the triple model 69 + 36 * D3 (6 list bounds, 3 pi, the
- * delta and perm, and per incident edge 12 bytes of male data plus three
- * female probes).  graphs[inst[c]] is chain c's male graph. */
-int64_t bm_chain_algo_bytes(const bm_graph_host* const* graphs, const int32_t* inst, const uint64_t* seeds,
-                            const int64_t* steps, int64_t n_chains, uint64_t p_rot3) {
-    int64_t B = 0;
-#pragma omp parallel for reduction(+ : B) schedule(dynamic, 1)
-    for (int64_t c = 0; c < n_chains; c++) {
-        const bm_graph_host* G = graphs[inst[c]];
-        const int32_t n = G->n;
-        const uint64_t q0 = bm_rand_at(seeds[c], 0), q2 = bm_rand_at(seeds[c], 2), q3 = bm_rand_at(seeds[c], 3);
-#define BM_DEG(x) ((G->out.ptr[(x) + 1] - G->out.ptr[(x)]) + (G->in.ptr[(x) + 1] - G->in.ptr[(x)]))
-        for (int64_t s = 0; s < steps[c]; s++) {
-            const uint64_t rk = bm_rand_at(q2, (uint64_t)s);
-            if (n >= 3 && (uint64_t)(uint32_t)rk < p_rot3) {
-                int32_t x, y, z;
-                bm_uniform_triple(q3, (uint64_t)s, n, &x, &y, &z);
-                B += 69 + 36 * (BM_DEG(x) + BM_DEG(y) + BM_DEG(z));
-            } else {
-                int32_t u, v;
-                bm_uniform_pair(q0, (uint64_t)s, n, &u, &v);
-                B += 48 + 28 * (BM_DEG(u) + BM_DEG(v));
-            }
-        }
-#undef BM_DEG
-    }
-    return B;
-}
-
 /* Swap byte model (48 + 28 * D per pair) summed over the u-major stream
  * (seed, k): the first-vertex counts are recounted from the draws and each
  * partner is regenerated. */

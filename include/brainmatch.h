@@ -556,6 +556,8 @@ typedef struct bm_anneal_params {
     double decay;                 /* per-step factor in (0, 1] (0.9999) */
     double p_rot3;                /* probability of a 3-vertex move in [0, 1] */
     int32_t traj_cap;             /* score samples kept per chain (one per launch) */
+    double p_pool;                /* probability in [0, 1] that a move is built on a pair of its
+                                     instance's proposal pool (when that pool is not empty) */
 } bm_anneal_params;
 
 /* One instance per context (all on the same device); every chain of instance
@@ -580,8 +582,32 @@ int bm_anneal_verify(bm_anneal* a, int64_t* mismatches);
 /* Launches run and their summed duration (HIP events around each launch). */
 int bm_anneal_timing(bm_anneal* a, int64_t* launches, double* total_ms);
 /* Algorithmic bytes of the chain steps run so far, from the byte model of
- * the swap and triple kernels (replays every chain's proposals on the host). */
+ * the swap and triple kernels (48 + 28 D per swap step, 69 + 36 D3 per
+ * 3-vertex step, D summed by the chains from the male degrees). */
 int bm_anneal_algo_bytes(bm_anneal* a, int64_t* bytes, int64_t* steps);
+/* Per chain: 3-vertex moves and moves built on a pool pair, taken so far
+ * (either array may be NULL). */
+int bm_anneal_move_counts(bm_anneal* a, int64_t* rot3, int64_t* pooled);
+
+/* Proposal pools.  With p_pool > 0, step s of a chain of instance i draws
+ * r = bm_rand_at(bm_rand_at(chain seed, 4), s); when lo32(r) < round(p_pool
+ * 2^32) and instance i's pool holds P > 0 pairs, the move is built on pool
+ * entry (hi32(r) P) >> 32: the swap of that pair, or for a 3-vertex move
+ * that pair plus a third vertex uniform over the others (bm_third_vertex in
+ * brain-match_amd/csrc/bm_rng.h); every delta stays exact.  A pool can be
+ * replaced between runs (bm_anneal_run), with no recapture.
+ * bm_anneal_set_pool copies k host pairs (two distinct vertices in 1..n_i
+ * each; k = 0 empties the pool); bm_anneal_pool_learned fills it with the
+ * `top` best-ranked of `pool` uniform pairs of stream `seed` under a network
+ * of instance i's context (as bm_batch_gen_learned, with the features of that
+ * context's matching); bm_anneal_get_pool reads it back (uv may be NULL to
+ * get only *k).  The reference's annealer draws uniform pairs only
+ * (src/best-swap/score-lookup.c:345-400). */
+int bm_anneal_set_pool(bm_anneal* a, int32_t inst, const int32_t* uv, int64_t k);
+int bm_anneal_pool_learned(bm_anneal* a, int32_t inst, bm_mlp* m, uint64_t seed, int64_t pool, int64_t top);
+int bm_anneal_get_pool(bm_anneal* a, int32_t inst, int32_t* uv, int64_t* k);
+/* round(p_pool 2^32), the threshold the kernels compare lo32(r) with. */
+uint64_t bm_anneal_p_pool(const bm_anneal* a);
 /* Chains back to their state at creation. */
 int bm_anneal_reset(bm_anneal* a);
 /* Launch shape of the chain kernel (waves_per_block, lds_slots, blocks_per_cu

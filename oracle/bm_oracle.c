@@ -865,19 +865,35 @@ double or_det_exp(double x) {
     return p * two_k.d;
 }
 
+/* Third vertex of a 3-vertex move on the pair (a, b): uniform over the
+ * n - 2 others, from draw 2s + 1 of stream seed (the draw the uniform triple
+ * takes its third vertex from), skipping a and b in increasing order. */
+int32_t or_third_vertex(uint64_t seed, uint64_t s, int32_t n, int32_t a, int32_t b) {
+    uint32_t d = range32((uint32_t)rand_at(seed, 2 * s + 1), (uint32_t)(n - 2));
+    const uint32_t x = (uint32_t)(a < b ? a : b) - 1, y = (uint32_t)(a < b ? b : a) - 1;
+    if (d >= x) d++;
+    if (d >= y) d++;
+    return (int32_t)d + 1;
+}
+
 /* Replays `steps` steps (from step index s0) of one annealing chain with
  * stream seed q: step s draws its move kind from rand(q2, s) (a 3-vertex move
- * when the low 32 bits are below p_rot3, a 2^-32 fraction, and n >= 3), a
- * uniform pair s of stream q0 or a uniform triple s of stream q3 with
- * permutation 1 + range(high 32 bits, 5), computes the exact delta, and
- * accepts when delta >= 0 or (rand(q1, s) >> 11) * 2^-53 <
+ * when the low 32 bits are below p_rot3, a 2^-32 fraction, and n >= 3); with
+ * a proposal pool of pool_k pairs, r = rand(q4, s) selects pool entry
+ * (hi32(r) pool_k) >> 32 when lo32(r) < p_pool, and the move is that pair's
+ * swap or, for a 3-vertex move, that pair plus third_vertex(q3, s); otherwise
+ * the move is uniform pair s of stream q0 or uniform triple s of stream q3.
+ * A 3-vertex move takes permutation 1 + range(hi32(rand(q2, s)), 5).  The
+ * exact delta is accepted when delta >= 0 or (rand(q1, s) >> 11) * 2^-53 <
  * or_det_exp(delta * beta0 * or_det_exp(s * lam)), q_i = rand(q, i).
  * st = {score, best, accepted, at_best} is updated in place; pi and pib (the
  * latest matching reaching the best score, valid when at_best == 0) too. */
 void or_anneal_chain(const or_graph* g, int32_t* pi, int32_t* pib, uint64_t q, int64_t s0, int64_t steps,
-                     double beta0, double lam, uint64_t p_rot3, int64_t* st) {
+                     double beta0, double lam, uint64_t p_rot3, int64_t* st, const int32_t* pool_uv,
+                     int64_t pool_k, uint64_t p_pool) {
     const int32_t n = g->n;
     const uint64_t q0 = rand_at(q, 0), q1 = rand_at(q, 1), q2 = rand_at(q, 2), q3 = rand_at(q, 3);
+    const uint64_t q4 = rand_at(q, 4);
     int64_t score = st[0], best = st[1], nacc = st[2];
     int at_best = (int)st[3];
     for (int64_t s = s0; s < s0 + steps; s++) {
@@ -886,11 +902,24 @@ void or_anneal_chain(const or_graph* g, int32_t* pi, int32_t* pib, uint64_t q, i
         int32_t v[3];
         int64_t d;
         int p = 0;
+        int pooled = 0;
+        if (pool_k > 0) {
+            const uint64_t r = rand_at(q4, (uint64_t)s);
+            if ((uint64_t)(uint32_t)r < p_pool) {
+                const int64_t j = (int64_t)(((r >> 32) * (uint64_t)pool_k) >> 32);
+                v[0] = pool_uv[2 * j];
+                v[1] = pool_uv[2 * j + 1];
+                pooled = 1;
+            }
+        }
         if (!rot) {
-            or_uniform_pair(q0, (uint64_t)s, n, &v[0], &v[1]);
+            if (!pooled) or_uniform_pair(q0, (uint64_t)s, n, &v[0], &v[1]);
             d = or_swap_delta(g, pi, v[0], v[1]);
         } else {
-            or_uniform_triple(q3, (uint64_t)s, n, &v[0], &v[1], &v[2]);
+            if (pooled)
+                v[2] = or_third_vertex(q3, (uint64_t)s, n, v[0], v[1]);
+            else
+                or_uniform_triple(q3, (uint64_t)s, n, &v[0], &v[1], &v[2]);
             int64_t d5[5];
             or_rot3_all(g, pi, v[0], v[1], v[2], 0, d5);
             p = 1 + (int)range32((uint32_t)(rk >> 32), 5u);

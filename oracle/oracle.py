@@ -331,7 +331,7 @@ def gain_row(o: "Oracle", pi, u):
 _sig(_o, "or_det_exp_many", None, [C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double)])
 _sig(_o, "or_uniform_triple", None, [C.c_uint64, C.c_uint64, C.c_int32, i32p, i32p, i32p])
 _sig(_o, "or_anneal_chain", None, [vp, i32p, i32p, C.c_uint64, C.c_int64, C.c_int64, C.c_double, C.c_double,
-                                   C.c_uint64, i64p])
+                                   C.c_uint64, i64p, i32p, C.c_int64, C.c_uint64])
 _sig(_o, "or_chain_seed", C.c_uint64, [C.c_uint64, C.c_int64])
 
 
@@ -354,19 +354,33 @@ def chain_seed(seed, index):
     return int(_o.or_chain_seed(seed, index))
 
 
+_sig(_o, "or_third_vertex", C.c_int32, [C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, C.c_int32])
+
+
+def third_vertex(seed, s, n, a, b):
+    """Third vertex of a 3-vertex move built on the pair (a, b), step s."""
+    return int(_o.or_third_vertex(seed, s, n, a, b))
+
+
 def p_rot3_fraction(p):
     """Probability of a 3-vertex move as the integer threshold on 32 random bits."""
     return min(1 << 32, max(0, int(round(p * 4294967296.0))))
 
 
-def anneal_chain(o: "Oracle", pi, score, q, s0, steps, beta0, lam, p_rot3, state=None, pib=None):
+def anneal_chain(o: "Oracle", pi, score, q, s0, steps, beta0, lam, p_rot3, state=None, pib=None, pool=None,
+                 p_pool=0):
     """Replays steps s0..s0+steps-1 of one annealing chain on the CPU.
     state = [score, best, accepted, at_best]; returns (pi, pib, state) with pib
-    the latest matching reaching the best score (pi itself when at_best)."""
+    the latest matching reaching the best score (pi itself when at_best).
+    pool (k x 2 pairs) and p_pool (threshold on 32 random bits) give the
+    proposal pool the chain draws from."""
     pi = _a32(pi).copy()
     pib = pi.copy() if pib is None else _a32(pib).copy()
     st = np.array(state if state is not None else [score, score, 0, 1], np.int64)
-    _o.or_anneal_chain(o.h, _p32(pi), _p32(pib), q, s0, steps, beta0, lam, p_rot3, _p64(st))
+    pk = 0 if pool is None else int(np.asarray(pool).reshape(-1, 2).shape[0])
+    pool = _a32(np.zeros((1, 2)) if pool is None else pool).reshape(-1, 2)
+    _o.or_anneal_chain(o.h, _p32(pi), _p32(pib), q, s0, steps, beta0, lam, p_rot3, _p64(st), _p32(pool), pk,
+                       int(p_pool))
     return pi, (pi.copy() if st[3] else pib), st
 
 

@@ -175,3 +175,68 @@ def test_autotune_then_replay(bm, orc):
             assert np.array_equal(ann.mapping(0), pi0)
     finally:
         ctx.close()
+
+
+def test_pool_chains_replay(bm, orc):
+    """Two instances: one with a pool from the learned proposal (the 512
+    best-ranked of 2^14 uniform pairs), one with a pool set from the host;
+    half of the moves come from the pools, 30% are 3-vertex moves.  Every
+    chain replays exactly on the CPU given the device's pools, the chains
+    count their pool moves, and replacing a pool between runs needs no
+    recapture."""
+    ctxs, oracles, starts = [], [], []
+    for n, seed in ((700, 31), (1100, 32)):
+        c, o, p = _mk(bm, orc, n, 15.0, seed)
+        ctxs.append(c)
+        oracles.append(o)
+        starts.append(p)
+    try:
+        with bm.Mlp(ctxs[0], seed=3) as m:
+            m.refresh_features()
+            m.train(seed=5, steps=40, batch=1 << 12, lr=2e-3)
+            with bm.Anneal(ctxs, chains_per_instance=6, seed=41, t0=2.0, decay=0.999, p_rot3=0.3,
+                           traj_cap=4, p_pool=0.5) as ann:
+                ann.pool_learned(0, m, seed=77, pool=1 << 14, top=512)
+                host_pool = orc.uniform_pairs(9, 300, 1100)
+                ann.set_pool(1, host_pool)
+                pools = [ann.get_pool(0), ann.get_pool(1)]
+                assert pools[0].shape == (512, 2) and np.array_equal(pools[1], host_pool)
+                b = ctxs[0].batch(1 << 14)
+                b.gen_learned(m, 77, 1 << 14, 512)
+                assert np.array_equal(b.pairs(), pools[0])
+                b.close()
+                ann.run(150, 2)
+                assert ann.verify() == 0
+                beta0, lam, p3 = ann.schedule()
+                pp = ann.p_pool()
+                assert pp == 1 << 31
+                st = ann.state()
+                rot, pooled = ann.move_counts()
+                assert (pooled > 0.35 * 300).all() and (pooled < 0.65 * 300).all() and (rot > 0).all()
+                for c in range(ann.chains):
+                    i = c // 6
+                    pi, _, state = orc.anneal_chain(oracles[i], starts[i], oracles[i].score(starts[i]),
+                                                    orc.chain_seed(41, c), 0, 300, beta0, lam, p3,
+                                                    pool=pools[i], p_pool=pp)
+                    assert state[0] == st["score"][c] and state[2] == st["accepted"][c], c
+                    assert np.array_equal(pi, ann.mapping(c)), c
+                # a new pool for instance 1, then more steps on the captured graph
+                pool2 = orc.uniform_pairs(10, 50, 1100)
+                ann.set_pool(1, pool2)
+                ann.run(150, 1)
+                assert ann.verify() == 0
+                st2 = ann.state()
+                for c in range(6, 12):
+                    pi, _, state = orc.anneal_chain(oracles[1], starts[1], oracles[1].score(starts[1]),
+                                                    orc.chain_seed(41, c), 0, 300, beta0, lam, p3,
+                                                    pool=pools[1], p_pool=pp)
+                    pi, _, state = orc.anneal_chain(oracles[1], pi, 0, orc.chain_seed(41, c), 300, 150, beta0, lam,
+                                                    p3, state=state, pool=pool2, p_pool=pp)
+                    assert state[0] == st2["score"][c] and np.array_equal(pi, ann.mapping(c)), c
+                with pytest.raises(bm.BMError):
+                    ann.set_pool(1, np.array([[3, 3]], np.int32))
+                with pytest.raises(bm.BMError):
+                    ann.pool_learned(1, m, seed=1, pool=1 << 10, top=16)   # network of another context
+    finally:
+        for c in ctxs:
+            c.close()

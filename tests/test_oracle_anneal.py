@@ -95,3 +95,34 @@ def test_triple_stream_distinct_and_in_range(orc):
         assert len({a, b, d}) == 3 and all(1 <= x <= 5 for x in (a, b, d))
         seen.add((a, b, d))
     assert len(seen) == 60   # every ordered triple of 5 vertices appears
+
+
+def test_pool_moves_touch_only_pool_pairs(bm, orc):
+    """With every move drawn from a one-pair pool, a swap-only chain only
+    ever exchanges that pair's images; with 3-vertex moves the third vertex
+    differs from the pair and is uniform over the others; the running score
+    stays equal to a recount."""
+    male, female, pi = quirky_pair(bm, seed=7)
+    n = 60
+    o = orc.Oracle(n, male, female)
+    s0 = o.score(pi)
+    q = orc.chain_seed(5, 1)
+    full = 1 << 32
+    pool = np.array([[11, 37]], np.int32)
+    p2, _, st = orc.anneal_chain(o, pi, s0, q, 0, 500, 1e-9, 0.0, 0, pool=pool, p_pool=full)
+    moved = np.nonzero(p2 != pi)[0]
+    assert set(moved.tolist()) <= {11, 37} and st[2] >= 499
+    assert st[0] == o.score(p2)
+    # 3-vertex moves on the pool pair: the third vertex is uniform over the other 58
+    thirds = np.array([orc.third_vertex(orc.chain_seed(q, 3), s, n, 11, 37) for s in range(58 * 400)])
+    assert not np.isin(thirds, [11, 37]).any() and thirds.min() >= 1 and thirds.max() <= n
+    h = np.bincount(thirds, minlength=n + 1)[1:]
+    h = np.delete(h, [10, 36])
+    assert np.abs(h - 400).max() < 6 * np.sqrt(400)
+    p3, _, st3 = orc.anneal_chain(o, pi, s0, q, 0, 800, 0.3, 1e-3, orc.p_rot3_fraction(0.5), pool=pool,
+                                  p_pool=full // 2)
+    assert st3[0] == o.score(p3) and st3[1] >= st3[0]
+    # p_pool = 0 ignores the pool: the uniform chain exactly
+    a, _, sa = orc.anneal_chain(o, pi, s0, q, 0, 300, 0.3, 1e-3, orc.p_rot3_fraction(0.5), pool=pool, p_pool=0)
+    b, _, sb = orc.anneal_chain(o, pi, s0, q, 0, 300, 0.3, 1e-3, orc.p_rot3_fraction(0.5))
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
