@@ -499,6 +499,157 @@ __global__ void k_gather_stream_pairs(uint64_t seed, int32_t n, const int32_t* _
     }
 }
 
+/* ---- top of the ranked pool: radix select over distinct 64-bit keys ---- */
+
+/* key(p) = order-preserving bits of score p (the float order, -0 below +0)
+ * in the high half, 2^32 - 1 - p in the low half: a larger key is a larger
+ * score, ties to the smaller pool index, and no two keys are equal -- the
+ * order of a stable descending sort of the scores. */
+__device__ __forceinline__ uint64_t rank_key(float x, int64_t p) {
+    uint32_t u = __float_as_uint(x);
+    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    return ((uint64_t)u << 32) | (uint64_t)(0xFFFFFFFFu - (uint32_t)p);
+}
+
+constexpr int kSelBits = 11, kSelBins = 1 << kSelBits, kSelThreads = 256, kSelPasses = 6;
+constexpr int kSortMax = 8192;       /* largest top sorted in one block */
+
+struct SelState {
+    uint64_t prefix;      /* key bits fixed so far (the bits of the current digit and below are 0) */
+    int64_t remaining;    /* rank (1-based) of the wanted key within the keys sharing the prefix */
+    uint32_t done;        /* blocks of the current pass that have merged their histogram */
+    uint32_t count;       /* keys written by the compaction */
+    uint32_t hist[kSelBins];
+};
+
+__device__ __forceinline__ void sel_digit(int pass, int* shift, int* width) {
+    *width = pass < kSelPasses - 1 ? kSelBits : 64 - kSelBits * (kSelPasses - 1);
+    *shift = 64 - kSelBits * (pass + 1);
+    if (*shift < 0) *shift = 0;
+}
+
+__global__ void k_select_init(SelState* st, int64_t top) {
+    if (threadIdx.x == 0) {
+        st->prefix = 0;
+        st->remaining = top;
+        st->done = 0;
+        st->count = 0;
+    }
+    for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) st->hist[i] = 0;
+}
+
+/* One digit of the select: every block histograms the digit of the keys that
+ * share the prefix (LDS counters, merged into the state with atomics); the
+ * last block to finish finds the digit holding the wanted rank (suffix sums
+ * from the top bin), extends the prefix and clears the histogram. */
+__global__ void __launch_bounds__(kSelThreads) k_select_pass(const float* __restrict__ sc, int64_t P,
+                                                             SelState* st, int pass) {
+    __shared__ uint32_t h[kSelBins];
+    __shared__ uint32_t part[kSelThreads];
+    __shared__ int last;
+    int shift, width;
+    sel_digit(pass, &shift, &width);
+    const int bins = 1 << width;
+    const int top = shift + width;
+    const uint64_t fixed = top >= 64 ? 0ull : (~0ull << top);
+    const uint64_t prefix = st->prefix;
+    for (int i = threadIdx.x; i < bins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += stride) {
+        const uint64_t k = rank_key(sc[p], p);
+        if ((k & fixed) == prefix) atomicAdd(&h[(uint32_t)(k >> shift) & (uint32_t)(bins - 1)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < bins; i += blockDim.x)
+        if (h[i]) atomicAdd(&st->hist[i], h[i]);
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    /* every thread reads the rank before the barriers below; one thread rewrites it */
+    const int64_t rem = st->remaining;
+    /* thread t owns bins [t * per, (t + 1) * per); suffix sums over threads from the top */
+    const int per = bins / kSelThreads;
+    const int t = threadIdx.x;
+    uint32_t mine = 0;
+    for (int j = 0; j < per; j++) {
+        const uint32_t c = __hip_atomic_load(&st->hist[t * per + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        h[t * per + j] = c;
+        mine += c;
+    }
+    part[t] = mine;
+    __syncthreads();
+    for (int o = 1; o < kSelThreads; o <<= 1) {
+        const uint32_t add = t + o < kSelThreads ? part[t + o] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    /* part[t] = keys in bins >= t * per; the digit lies with the thread where
+     * part[t + 1] < remaining <= part[t] */
+    const uint32_t above = t + 1 < kSelThreads ? part[t + 1] : 0u;
+    if ((int64_t)above < rem && rem <= (int64_t)part[t]) {
+        int64_t r = rem - above;
+        int d = t * per + per - 1;
+        for (; d > t * per; d--) {
+            if (r <= (int64_t)h[d]) break;
+            r -= h[d];
+        }
+        st->prefix = prefix | ((uint64_t)d << shift);
+        st->remaining = r;
+    }
+    if (t == 0) st->done = 0;
+    for (int i = t; i < bins; i += blockDim.x) st->hist[i] = 0;
+}
+
+/* Keys >= the selected key (exactly `top` of them: keys are distinct; the
+ * bound only guards the buffer). */
+__global__ void k_select_compact(const float* __restrict__ sc, int64_t P, SelState* st, uint64_t* __restrict__ out,
+                                 uint32_t top) {
+    const uint64_t kstar = st->prefix;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += stride) {
+        const uint64_t k = rank_key(sc[p], p);
+        if (k >= kstar) {
+            const uint32_t at = atomicAdd(&st->count, 1u);
+            if (at < top) out[at] = k;
+        }
+    }
+}
+
+/* One block: the `top` selected keys in descending order (bitonic sort in
+ * LDS, padded with zero keys to a power of two), then their stream pairs. */
+__global__ void __launch_bounds__(1024) k_select_sort(const uint64_t* __restrict__ keys, int32_t top, int32_t padded,
+                                                      uint64_t seed, int32_t n, int2* __restrict__ out) {
+    extern __shared__ uint64_t sk[];
+    for (int i = threadIdx.x; i < padded; i += blockDim.x) sk[i] = i < top ? keys[i] : 0ull;
+    __syncthreads();
+    for (int size = 2; size <= padded; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < padded / 2; i += blockDim.x) {
+                const int lo = 2 * i - (i & (stride - 1));
+                const int hi = lo + stride;
+                const bool desc = (lo & size) == 0;
+                const uint64_t a = sk[lo], b = sk[hi];
+                if ((a < b) == desc) {
+                    sk[lo] = b;
+                    sk[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < top; i += blockDim.x) {
+        const uint32_t p = 0xFFFFFFFFu - (uint32_t)sk[i];
+        int32_t u, v;
+        bm_uniform_pair(seed, (uint64_t)p, n, &u, &v);
+        out[i] = make_int2(u, v);
+    }
+}
+
 int grid_of(int64_t work, int threads, int64_t cap = 65536) {
     int64_t b = (work + threads - 1) / threads;
     if (b < 1) b = 1;
@@ -605,7 +756,9 @@ size_t bm_mlp_rank_scratch(int64_t pool) {
     size_t temp = 0;
     (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp, (const float*)nullptr, (float*)nullptr,
                                                        (const int32_t*)nullptr, (int32_t*)nullptr, (int)pool);
-    return temp + 256 + (size_t)pool * (2 * sizeof(float) + 2 * sizeof(int32_t));
+    const size_t sort_path = temp + 256 + (size_t)pool * (2 * sizeof(float) + 2 * sizeof(int32_t));
+    const size_t select_path = (size_t)pool * sizeof(float) + 512 + sizeof(SelState) + kSortMax * sizeof(uint64_t);
+    return sort_path > select_path ? sort_path : select_path;
 }
 
 hipError_t bm_launch_mlp_rank(const BmMlpDev& d, const float* vf, uint64_t seed, int32_t n, int64_t pool, int64_t top,
@@ -621,6 +774,27 @@ hipError_t bm_launch_mlp_rank(const BmMlpDev& d, const float* vf, uint64_t seed,
     size_t temp_bytes = scratch_bytes - (size_t)(temp - p);
     hipError_t e = bm_launch_mlp_forward(d, vf, nullptr, seed, n, pool, sc, num_cus, s);
     if (e != hipSuccess) return e;
+    if (top <= kSortMax && pool < (1LL << 32)) {
+        /* radix select of the top keys, then one block sorts them */
+        char* q = reinterpret_cast<char*>(sc + pool);
+        q += (256 - (reinterpret_cast<uintptr_t>(q) & 255)) & 255;
+        SelState* st = reinterpret_cast<SelState*>(q);
+        uint64_t* keys = reinterpret_cast<uint64_t*>(st + 1);
+        const int blocks = grid_of(pool, kSelThreads * 16, (int64_t)(num_cus > 0 ? num_cus : 256));
+        hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, top);
+        for (int pass = 0; pass < kSelPasses; pass++)
+            hipLaunchKernelGGL(k_select_pass, dim3(blocks), dim3(kSelThreads), 0, s, sc, pool, st, pass);
+        hipLaunchKernelGGL(k_select_compact, dim3(blocks), dim3(kSelThreads), 0, s, sc, pool, st, keys, (uint32_t)top);
+        int padded = 1;
+        while (padded < top) padded <<= 1;
+        if (padded < 2) padded = 2;
+        e = hipFuncSetAttribute((const void*)k_select_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)((size_t)padded * sizeof(uint64_t)));
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_select_sort, dim3(1), dim3(1024), (size_t)padded * sizeof(uint64_t), s, keys,
+                           (int32_t)top, padded, seed, n, out);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_iota, dim3(grid_of(pool, 256)), dim3(256), 0, s, idx, pool);
     e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, sc, sc_sorted, idx, idx_sorted, (int)pool, 0,
                                                      32, s);

@@ -225,3 +225,34 @@ def test_climb_learned_beats_uniform_at_equal_rounds(bm, c1):
             assert r["verify_failures"] == 0
         assert rl["final_score"] > ru["final_score"]
         assert tl[-1] == rl["final_score"] and tu[-1] == ru["final_score"] and t2[-1] == r2["final_score"]
+
+
+def _rank_keys(s):
+    """Order-preserving integer image of float32 scores (-0 below +0)."""
+    u = np.ascontiguousarray(s, np.float32).view(np.uint32).astype(np.uint64)
+    return np.where(u & 0x80000000, (~u) & 0xFFFFFFFF, u | 0x80000000)
+
+
+@pytest.mark.parametrize("top", [1, 3, 1000, 4096, 8192, 8193])
+def test_learned_ranking_order_exact(bm, c1, orc, top):
+    """The ranked batch is exactly the stable descending order of the pool's
+    scores (ties to the smaller pool index): the radix-select path (top up to
+    8,192) and the full-sort path agree with a host sort of the same scores;
+    with all-zero weights every score ties and the first pool pairs win."""
+    ctx, n = c1["ctx"], c1["n"]
+    ctx.set_mapping(c1["pi0"])
+    pool = 1 << 15
+    with bm.Mlp(ctx, seed=9) as m:
+        m.refresh_features()
+        m.train(seed=2, steps=20, batch=1 << 12, lr=2e-3)
+        uv = orc.uniform_pairs(123, pool, n)
+        s = m.score_pairs(uv)
+        order = np.lexsort((np.arange(pool), -_rank_keys(s).astype(np.float64)))
+        # lexsort on float64 of 32-bit keys is exact (< 2^53)
+        b = ctx.batch(top)
+        b.gen_learned(m, 123, pool, top)
+        assert np.array_equal(b.pairs(), uv[order[:top]])
+        m.set_params(np.zeros(m.N_PARAMS, np.float32))
+        b.gen_learned(m, 123, pool, top)
+        assert np.array_equal(b.pairs(), uv[:top])
+        b.close()
