@@ -174,6 +174,10 @@ def main():
                     help="instead of the swap-delta benchmark: the n=100,000 ~20M-edge pair with 40%% of the matching "
                          "shuffled: delta throughput of both strategies, then learned vs uniform climbs and "
                          "swap + 3-vertex annealing for SECONDS each (one JSON line)")
+    ap.add_argument("--p-pool", type=float, default=0.5,
+                    help="--scale: fraction of the learned annealing chains' moves drawn from the ranked pool")
+    ap.add_argument("--scale-climbs", action="store_true",
+                    help="--scale: also run uniform and learned disjoint-swap climbs for SECONDS each")
     ap.add_argument("--best-swap", type=int, default=0, metavar="SWAPS",
                     help="instead of the swap-delta benchmark: steepest ascent on the n=18,524 pair from the greedy "
                          "matching, SWAPS best swaps over all 171.6M pairs each (one JSON line)")
@@ -616,27 +620,87 @@ def bench_anneal(args, bm):
     print(json.dumps(result), flush=True)
 
 
+def anneal_for(bm, ctx, pi0, seconds, chains, p_pool, mlp=None, refresh_s=2.0, steps_per_launch=500, seed=9):
+    """Annealing chains (swaps and 25% 3-vertex moves, T_s = 50 * 0.9999^s)
+    on one instance for `seconds` of wall clock from pi0.  With mlp, a
+    fraction p_pool of the moves is drawn from a proposal pool: the 2^16
+    best-ranked of 2^20 uniform pairs under the network, with the features
+    of the best chain's current matching, re-ranked every refresh_s seconds
+    (the ranking time is inside the wall clock).  Returns the trajectory of
+    the best score over chains (seconds, score) and the step counts."""
+    ctx.set_mapping(pi0)
+    ann = bm.Anneal([ctx], chains_per_instance=chains, seed=seed, t0=50.0, decay=0.9999, p_rot3=0.25,
+                    p_pool=p_pool if mlp is not None else 0.0)
+    score0 = ctx.score()
+    traj = [(0.0, score0)]
+    refreshes = 0
+    rank_s = 0.0
+    t0 = time.perf_counter()
+    last_refresh = -1e9
+    launches = 0
+    while True:
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+        if mlp is not None and el - last_refresh >= refresh_s:
+            tr0 = time.perf_counter()
+            if launches:
+                st = ann.state()
+                ctx.set_mapping(ann.mapping(int(np.argmax(st["score"]))))
+            mlp.refresh_features()
+            ann.pool_learned(0, mlp, seed=1000 + refreshes, pool=1 << 20, top=1 << 16)
+            rank_s += time.perf_counter() - tr0
+            refreshes += 1
+            last_refresh = time.perf_counter() - t0
+        ann.run(steps_per_launch, 1)
+        launches += 1
+        if launches % 20 == 0:
+            traj.append((time.perf_counter() - t0, int(ann.state()["best"].max())))
+    el = time.perf_counter() - t0
+    st = ann.state()
+    traj.append((el, int(st["best"].max())))
+    kernel_launches, kernel_ms = ann.timing()
+    algo, steps = ann.algo_bytes()
+    rot, pooled = ann.move_counts()
+    mism = int(ann.verify())
+    out = {"chains": ann.chains, "seconds": el, "steps_per_chain": int(st["steps"][0]),
+           "steps_per_s": int(steps) / el, "kernel_steps_per_s": int(steps) / (kernel_ms / 1000),
+           "exact_deltas_per_s": (int(steps) + 4 * int(rot.sum())) / el,
+           "algorithmic_GBps_kernel": algo / (kernel_ms / 1000) / 1e9, "kernel_ms": kernel_ms,
+           "best_score": int(st["best"].max()), "median_best": int(np.median(st["best"])),
+           "three_vertex_fraction": float(rot.sum() / max(1, steps)),
+           "pool_fraction": float(pooled.sum() / max(1, steps)), "recount_mismatches": mism,
+           "pool_refreshes": refreshes, "pool_ranking_seconds": rank_s,
+           "trajectory": [(round(a, 2), b) for a, b in traj[::max(1, len(traj) // 40)]] + [(round(el, 2), traj[-1][1])]}
+    ann.close()
+    ctx.set_mapping(pi0)
+    return out
+
+
 def bench_scale(args, bm):
-    """The scale workload: n = 100,000, ~20M directed edges, lognormal weights, 40% of
-    the true matching shuffled.  Exact swap-delta throughput of both
+    """The scale workload: n = 100,000, ~20M directed edges, lognormal weights,
+    40% of the true matching shuffled.  Exact swap-delta throughput of both
     strategies on 2^28 u-major pairs per step, the step of the n = 18,524
     benchmark (the gain table is 40 GB here, rebuilt every step in column
-    windows, with per-pair lookups), then, for SECONDS each: a uniform
-    and a learned-proposal climb (4096 exact candidates per round, evaluated
-    from the gain table with --evaluator gain, by the scan kernel with
-    --evaluator scan), and annealing chains with swaps and 3-vertex moves
-    (uniform proposals)."""
+    windows, with per-pair lookups); then swap + 3-vertex annealing (2048
+    chains) for SECONDS each with learned proposal pools and with uniform
+    proposals only; with --scale-climbs also a uniform and a learned
+    disjoint-swap climb for SECONDS each."""
     n = 100_000
     t = time.perf_counter()
     male, female, truth, pi0 = bm.synth_pair(n, 200.0, bm.WEIGHTS_LOGNORMAL, 0.4, seed=5)
     ctx = bm.Context(n, male, female)
     ctx.set_mapping(pi0)
     score0 = ctx.score()
-    log(f"scale pair: n={n}, {male.m} male edges, initial score {score0} ({time.perf_counter() - t:.1f}s)")
+    ctx.set_mapping(truth)
+    planted = ctx.score()
+    ctx.set_mapping(pi0)
+    log(f"scale pair: n={n}, {male.m} male edges, initial score {score0}, planted matching {planted} "
+        f"({time.perf_counter() - t:.1f}s)")
     K = 1 << 28
     b = ctx.batch(K)
     ctx.timing_enable(True)
-    out = {"initial_score": score0, "male_edges": male.m}
+    out = {"initial_score": score0, "planted_score": planted, "male_edges": male.m}
     b.eval_swaps_umajor(1, 1 << 20)
     wall, kt = timed(ctx, (bm.KERNEL_SWAP,), lambda i: b.eval_swaps_umajor(77, K), 1)
     algo, _ = ctx.umajor_algo_bytes(77, K)
@@ -649,54 +713,60 @@ def bench_scale(args, bm):
     wall, kt = timed(ctx, (bm.KERNEL_GAIN_BUILD, bm.KERNEL_GAIN_SWAP, bm.KERNEL_AUX),
                      lambda i: (ctx.gain_rebuild(), b.eval_swaps_umajor(77, K)), 2)
     same = bool((b.deltas(0, 1 << 16) == head).all())
+    mo, mi = ctx.degrees(female=False)
+    fo, fi = ctx.degrees(female=True)
+    pa = np.asarray(pi0, np.int64)
+    expanded = int((mi.astype(np.int64) * fi[pa]).sum() + (mo.astype(np.int64) * fo[pa]).sum())
+    build_ms = per_launch_ms(kt, bm.KERNEL_GAIN_BUILD)
     out["gain"] = {"deltas_per_s": K * 2 / wall, "ms_per_step": 1000 * wall / 2,
-                   "build_ms": per_launch_ms(kt, bm.KERNEL_GAIN_BUILD),
+                   "build_ms": build_ms, "expanded_entries": expanded,
+                   "build_adds_per_s": expanded / (build_ms / 1000),
                    "lookup_ms": per_launch_ms(kt, bm.KERNEL_GAIN_SWAP), "matches_scan": same,
                    "table_GB": (n + 1) ** 2 * 4 / 1e9}
     log(f"scale deltas: scan {out['scan']['deltas_per_s']:.4g}/s, gain {out['gain']['deltas_per_s']:.4g}/s "
-        f"(build {out['gain']['build_ms']:.1f} ms, lookups {out['gain']['lookup_ms']:.1f} ms); identical: {same}")
+        f"(build {build_ms:.1f} ms over {expanded:.3g} expanded entries, lookups {out['gain']['lookup_ms']:.1f} ms); "
+        f"identical: {same}")
     b.close()
     ctx.timing_enable(False)
-    ctx.gain_enable(args.evaluator == "gain")   # the climbs' exact deltas: table or scan
+    ctx.gain_enable(False)
     mlp = bm.Mlp(ctx, seed=2)
     mlp.refresh_features()
     t = time.perf_counter()
     losses = mlp.train(seed=12, steps=300, batch=1 << 16, lr=1e-3)
     out["train"] = {"samples": 300 << 16, "seconds": time.perf_counter() - t,
                     "mse_first": float(losses[:10].mean()), "mse_last": float(losses[-10:].mean())}
-    for name, prop, kw in (("uniform", bm.PROPOSAL_UNIFORM, {}),
-                           ("learned", bm.PROPOSAL_LEARNED, {"pool": 1 << 20, "mlp": mlp, "refresh_every": 1})):
-        ctx.set_mapping(pi0)
-        tr, tt, res = ctx.climb(prop, seed=31, time_limit_s=args.scale, per_round=4096, **kw)
-        marks = {}
-        for frac in (0.1, 0.25, 0.5, 1.0):
-            i = int(np.searchsorted(tt, frac * args.scale))
-            marks[f"{frac * args.scale:g}s"] = int(tr[min(i, len(tr) - 1)]) if len(tr) else score0
-        out["climb_" + name] = {"final_score": res["final_score"], "rounds": res["iterations"],
-                                "accepted": res["accepted"], "verify_failures": res["verify_failures"],
-                                "score_at": marks}
-        log(f"climb {name}: {score0} -> {res['final_score']} in {res['seconds']:.0f}s")
+    for name, p_pool in (("anneal_learned", args.p_pool), ("anneal_uniform", 0.0)):
+        r = anneal_for(bm, ctx, pi0, args.scale, 2048, p_pool, mlp if p_pool > 0 else None)
+        out[name] = r
+        log(f"{name}: best {r['best_score']} after {r['seconds']:.0f}s, {r['steps_per_s']:.4g} steps/s, "
+            f"{r['algorithmic_GBps_kernel']:.0f} GB/s algorithmic, recount mismatches {r['recount_mismatches']}")
+    if args.scale_climbs:
+        ctx.gain_enable(args.evaluator == "gain")   # the climbs' exact deltas: table or scan
+        for name, prop, kw in (("uniform", bm.PROPOSAL_UNIFORM, {}),
+                               ("learned", bm.PROPOSAL_LEARNED,
+                                {"pool": 1 << 20, "mlp": mlp, "refresh_every": 1, "retrain_every": 1,
+                                 "retrain_lr": args.retrain_lr, "fallback": True})):
+            ctx.set_mapping(pi0)
+            tr, tt, res = ctx.climb(prop, seed=31, time_limit_s=args.scale, per_round=4096, **kw)
+            marks = {}
+            for frac in (0.1, 0.25, 0.5, 1.0):
+                i = int(np.searchsorted(tt, frac * args.scale))
+                marks[f"{frac * args.scale:g}s"] = int(tr[min(i, len(tr) - 1)]) if len(tr) else score0
+            out["climb_" + name] = {"final_score": res["final_score"], "rounds": res["iterations"],
+                                    "accepted": res["accepted"], "verify_failures": res["verify_failures"],
+                                    "score_at": marks}
+            log(f"climb {name}: {score0} -> {res['final_score']} in {res['seconds']:.0f}s")
+        ctx.gain_enable(False)
     mlp.close()
-    ctx.gain_enable(False)
-    ctx.set_mapping(pi0)
-    ann = bm.Anneal([ctx], chains_per_instance=2048, seed=9, t0=50.0, decay=0.9999, p_rot3=0.25)
-    t = time.perf_counter()
-    launches = 0
-    while time.perf_counter() - t < args.scale:
-        ann.run(500, 1)
-        launches += 1
-    el = time.perf_counter() - t
-    st = ann.state()
-    out["anneal"] = {"chains": ann.chains, "steps_per_chain": int(st["steps"][0]), "seconds": el,
-                     "steps_per_s": ann.chains * 500 * launches / el, "best_score": int(st["best"].max()),
-                     "median_best": int(np.median(st["best"])), "recount_mismatches": int(ann.verify())}
-    log(f"anneal: {out['anneal']['steps_per_s']:.4g} steps/s, best {out['anneal']['best_score']}")
-    ann.close()
     ctx.close()
-    result = {"metric": f"exact score at {args.scale:g} s on the n=100,000 pair, learned vs uniform proposals",
-              "value": out["climb_learned"]["final_score"], "unit": "score", "n_gpus": 1, "higher_is_better": True,
-              "data": "synthetic", "config": {"workload": "n=100000, ~20M edges, 40% shuffled", "seconds": args.scale,
-                                              "evaluator": args.evaluator},
+    result = {"metric": f"exact score at {args.scale:g} s on the n=100,000 pair, swap + 3-vertex annealing with "
+                        "learned proposals vs uniform proposals",
+              "value": out["anneal_learned"]["best_score"], "unit": "score", "n_gpus": 1, "higher_is_better": True,
+              "data": "synthetic",
+              "config": {"workload": "n=100000, ~20M edges, 40% shuffled", "seconds": args.scale,
+                         "chains": 2048, "p_rot3": 0.25, "p_pool": args.p_pool, "t0": 50.0, "decay": 0.9999,
+                         "evaluator": args.evaluator},
+              "learned_minus_uniform": out["anneal_learned"]["best_score"] - out["anneal_uniform"]["best_score"],
               "results": out}
     print(json.dumps(result), flush=True)
 

@@ -242,7 +242,7 @@ hipError_t bm_launch_mlp_forward(const BmMlpDev& d, const float* vf, const int2*
                                  int64_t k, float* score, int32_t num_cus, hipStream_t s);
 /* Gradient of the mean squared error over k pairs, added into d.grad; the
  * summed squared error is added into *loss.  act: bm_mlp_train_bytes(k). */
-size_t bm_mlp_train_bytes(int64_t B);
+size_t bm_mlp_train_bytes(int64_t k);
 hipError_t bm_launch_mlp_gradient(const BmMlpDev& d, const float* vf, const int2* list, uint64_t seed, int32_t n,
                                   int64_t k, const float* target, void* act, float* loss, int32_t num_cus,
                                   hipStream_t s);

@@ -228,7 +228,7 @@ extern "C" int bm_mlp_score_pairs(bm_mlp* m, const int32_t* uv, int64_t k, float
 /* Gradient of one batch into m->d.grad (which the caller has cleared); the
  * squared error into *d_loss (cleared here). */
 static int batch_gradient(bm_mlp* m, const int2* list, uint64_t seed, int64_t k, const float* d_target) {
-    int rc = grow_bytes(m, &m->d_act, &m->act_bytes, bm_mlp_train_bytes((k + 15) / 16 * 16));
+    int rc = grow_bytes(m, &m->d_act, &m->act_bytes, bm_mlp_train_bytes(k));
     if (rc != BM_OK) return rc;
     MHIP(m, hipMemsetAsync(m->d_loss, 0, sizeof(float), m->stream));
     MHIP(m, bm_launch_mlp_gradient(m->d, m->d_vf, list, seed, m->n, k, d_target, m->d_act, m->d_loss, m->num_cus,

@@ -1,12 +1,12 @@
 /*
  * Learned swap proposal (gfx950): a 16 -> 128 -> 128 -> 1 perceptron in BF16
- * with FP32 accumulation on the matrix cores (v_mfma_f32_16x16x16 bf16), with
- * hand-written forward and backward kernels.  It scores candidate pairs from
- * per-vertex features of the current matching; a search sends only the top
- * candidates of a large pool to the exact delta kernels, so the score stays
- * exact whatever the network predicts.  There is no counterpart in the
- * reference (its proposals are uniform, score-weighted or 2-hop:
- * src/graph-re-alignment-gpu-swap.cu:816-887, src/refine-twohop-random.c:676-693).
+ * with FP32 accumulation on the matrix cores, with hand-written forward and
+ * backward kernels.  It scores candidate pairs from per-vertex features of
+ * the current matching; a search sends only the top candidates of a large
+ * pool to the exact delta kernels, so the score stays exact whatever the
+ * network predicts.  There is no counterpart in the reference (its proposals
+ * are uniform, score-weighted or 2-hop: src/graph-re-alignment-gpu-swap.cu:816-887,
+ * src/refine-twohop-random.c:676-693).
  *
  * Model (x = the 16 pair features, bf() = rounding to bfloat16, nearest even):
  *   z1 = bf(x) @ bf(W1) + b1          W1: 16 x 128
@@ -18,16 +18,24 @@
  *   dy = 2 (y - t) / B;  dw3 = sum h2 dy;  db3 = sum dy
  *   g2 = bf(dy w3 * [z2 > 0]);  dW2 = h1^T g2;  db2 = sum g2 (unrounded)
  *   g1 = bf((g2 @ bf(W2)^T) * [z1 > 0]);  dW1 = bf(x)^T g1;  db1 = sum g1 (unrounded)
- * All matrix products run on MFMA tiles of 16 x 16 x 16; the weight-gradient
+ * Products with K = 16 (layer 1) run on v_mfma_f32_16x16x16_bf16, those with
+ * K = 128 or K = the batch (layer 2, the backward through W2, the weight
+ * gradients) on gfx950's v_mfma_f32_16x16x32_bf16.  The scoring kernel
+ * computes the transposed products (z^T = W^T x^T: the weights are the A
+ * operand), so each lane ends a layer holding four consecutive hidden units of
+ * one pair: h1 goes to LDS in 8-byte stores and the output layer's sum over
+ * hidden units is one in-lane sum plus two shuffles.  The weight-gradient
  * products use activations stored transposed (feature-major) so that every
- * operand fragment is one 8-byte load.
+ * operand fragment is one 16-byte load.
  *
- * Fragment layout of v_mfma_f32_16x16x16 bf16 (64 lanes, l = lane):
- *   A (16 x 16): lane l holds A[l % 16][4 (l / 16) + i], i = 0..3
- *   B (16 x 16): lane l holds B[4 (l / 16) + i][l % 16]
- *   D (16 x 16): lane l holds D[4 (l / 16) + j][l % 16], j = 0..3
- * Weights are kept in "fragment order": for a K x N matrix W, fragment (s, t)
- * (rows 16 s.., columns 16 t..) stores lane l's four values contiguously.
+ * Fragment layouts (64 lanes, l = lane, q = l / 16):
+ *   16x16x16: A lane l holds A[l % 16][4q + i], B holds B[4q + i][l % 16], i < 4;
+ *   16x16x32: A lane l holds A[l % 16][8q + i], B holds B[8q + i][l % 16], i < 8;
+ *   both:     D lane l holds D[4q + j][l % 16], j < 4.
+ * Weights are kept in "fragment order": for a K x N matrix W and k-steps of
+ * KB = 16 or 32 rows, fragment (s, t) (rows KB s.., columns 16 t..) stores
+ * lane l's KB / 4 values W[KB s + (KB / 4) q + i][16 t + l % 16] contiguously.
+ * The same array serves as the B operand of W and as the A operand of W^T.
  *
  * Per-vertex features (8 per vertex; a pair's 16 are u's then v's):
  *   log1p d_out, log1p d_in, log1p (weight sum), L / (weight sum + 1),
@@ -46,21 +54,25 @@ using namespace bmd;
 namespace {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kF = 16;     /* features per pair */
 constexpr int kH = 128;    /* hidden width */
 constexpr int kT = kH / 16;   /* 16-wide tiles per hidden layer */
+constexpr int kS = kH / 32;   /* 32-deep k-steps over a hidden layer */
 
-__device__ __forceinline__ short bf16_of(float x) {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (short)(u >> 16);   /* inf / nan: truncate */
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
+/* bfloat16 bits of x, rounded to nearest even (v_cvt_pk_bf16_f32). */
+__device__ __forceinline__ short bf16_of(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v4f mfma32(v8s a, v8s b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0,
+                                                    0);
 }
 
 /* w_F(a, b) through a row hash table (descriptor (offset << 6) | bits). */
@@ -129,29 +141,27 @@ __global__ void __launch_bounds__(256) k_vertex_features(BmDevGraph g, float4* _
     }
 }
 
-/* Weights in fragment order: frag[(s * (N / 16) + t) * 64 + l] holds
- * W[16 s + 4 (l / 16) + i][16 t + l % 16], i = 0..3 (W is K x N, row-major);
- * with transpose, the fragments of W^T (N x K). */
-__global__ void k_mlp_pack(const float* __restrict__ W, int K, int N, int transpose, v4s* __restrict__ frag) {
+/* Weights in fragment order with k-steps of KB rows (16 or 32):
+ * frag[((s * (NN / 16) + t) * 64 + l) * (KB / 4) + i] = M[KB s + (KB / 4) (l / 16) + i][16 t + l % 16]
+ * for the KK x NN matrix M = W (K x N, row-major) or, with transpose, W^T. */
+__global__ void k_mlp_pack(const float* __restrict__ W, int K, int N, int transpose, int KB, short* __restrict__ frag) {
     const int KK = transpose ? N : K, NN = transpose ? K : N;   /* shape of the packed matrix */
-    const int total = (KK / 16) * (NN / 16) * 64;
+    const int per = KB / 4;
+    const int total = (KK / KB) * (NN / 16) * 64;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += gridDim.x * blockDim.x) {
         const int l = q & 63, f = q >> 6;
         const int s = f / (NN / 16), t = f % (NN / 16);
-        v4s v;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int r = 16 * s + 4 * (l / 16) + i, c = 16 * t + (l % 16);
-            v[i] = bf16_of(transpose ? W[c * N + r] : W[r * N + c]);
+        for (int i = 0; i < per; i++) {
+            const int r = KB * s + per * (l / 16) + i, c = 16 * t + (l % 16);
+            frag[(size_t)q * per + i] = bf16_of(transpose ? W[c * N + r] : W[r * N + c]);
         }
-        frag[q] = v;
     }
 }
 
 struct MlpDev {
-    const v4s* W1f;     /* 1 x 8 fragments of W1 (16 x 128) */
-    const v4s* W2f;     /* 8 x 8 fragments of W2 (128 x 128) */
-    const v4s* W2Tf;    /* 8 x 8 fragments of W2^T */
+    const v4s* W1f;     /* 1 x 8 fragments (k-step 16) of W1 (16 x 128) */
+    const v8s* W2f;     /* 4 x 8 fragments (k-step 32) of W2 (128 x 128) */
+    const v8s* W2Tf;    /* 4 x 8 fragments (k-step 32) of W2^T */
     const float* b1;
     const float* b2;
     const float* w3;
@@ -174,18 +184,18 @@ __device__ __forceinline__ int2 pair_at(const PairSrc& ps, int64_t p) {
 
 constexpr int kFwdThreads = 256;
 constexpr int kFwdWaves = kFwdThreads / BM_WAVE;
-constexpr int kHPad = kH + 8;   /* padded bf16 row of a staged 16 x 128 tile */
+constexpr int kHPad = kH + 8;   /* padded bf16 row of a staged 16 x 128 tile (272 bytes: 16-byte reads hit distinct banks) */
 
 struct FwdShared {
     v4s W1[kT * 64];
-    v4s W2[kT * kT * 64];
+    v8s W2[kS * kT * 64];
     float b1[kH], b2[kH], w3[kH];
     short Ht[kFwdWaves][16][kHPad];
 };
 
 __device__ __forceinline__ void stage_weights(FwdShared& S, const MlpDev& M) {
     for (int q = threadIdx.x; q < kT * 64; q += blockDim.x) S.W1[q] = M.W1f[q];
-    for (int q = threadIdx.x; q < kT * kT * 64; q += blockDim.x) S.W2[q] = M.W2f[q];
+    for (int q = threadIdx.x; q < kS * kT * 64; q += blockDim.x) S.W2[q] = M.W2f[q];
     for (int q = threadIdx.x; q < kH; q += blockDim.x) {
         S.b1[q] = M.b1[q];
         S.b2[q] = M.b2[q];
@@ -194,7 +204,8 @@ __device__ __forceinline__ void stage_weights(FwdShared& S, const MlpDev& M) {
 }
 
 /* Input fragment of a 16-pair tile: lane l holds features 4 (l / 16) .. +3
- * of pair l % 16 (u's features 0..7, v's 8..15). */
+ * of pair l % 16 (u's features 0..7, v's 8..15) -- the A operand of x W1
+ * and, equally, the B operand of W1^T x^T. */
 __device__ __forceinline__ v4s input_frag(const float4* __restrict__ vf, int2 uv, bool valid, int lane) {
     const int q = lane / 16;
     float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -207,8 +218,9 @@ __device__ __forceinline__ v4s input_frag(const float4* __restrict__ vf, int2 uv
     return a;
 }
 
-/* Layer 1 for one tile: z1 (D layout, kept for the backward mask) and h1 in
- * the wave's LDS tile; returns nothing, z1 in z1[t]. */
+/* Layer 1 for one tile in the pair-major form (training): z1 (D layout:
+ * lane l has pairs 4 (l / 16) + j of hidden unit 16 t + l % 16, kept for the
+ * backward mask) and h1 into the wave's LDS tile. */
 __device__ __forceinline__ void layer1(const FwdShared& S, short (*Ht)[kHPad], v4s xa, v4f* z1, int lane) {
 #pragma unroll
     for (int t = 0; t < kT; t++) {
@@ -224,13 +236,19 @@ __device__ __forceinline__ void layer1(const FwdShared& S, short (*Ht)[kHPad], v
     }
 }
 
-/* A fragments (8 k-steps) of the wave's staged 16 x 128 bf16 tile. */
-__device__ __forceinline__ void load_a(const short (*Ht)[kHPad], v4s* a, int lane) {
+/* 32-deep fragments (kS k-steps) of the wave's staged 16 x 128 bf16 tile:
+ * lane l gets row l % 16, columns 32 s + 8 (l / 16) .. +7 (one 16-byte read). */
+__device__ __forceinline__ void load_tile32(const short (*Ht)[kHPad], v8s* a, int lane) {
 #pragma unroll
-    for (int s = 0; s < kT; s++) a[s] = *reinterpret_cast<const v4s*>(&Ht[lane % 16][16 * s + 4 * (lane / 16)]);
+    for (int s = 0; s < kS; s++) a[s] = *reinterpret_cast<const v8s*>(&Ht[lane % 16][32 * s + 8 * (lane / 16)]);
 }
 
-/* Forward: scores of k pairs (one wave per 16-pair tile). */
+/* Forward: scores of k pairs (one wave per 16-pair tile), in the transposed
+ * form: z1^T = W1^T x^T (A = W1^T fragments, B = the input fragment) leaves
+ * lane l with hidden units 16 t + 4 (l / 16) .. +3 of pair l % 16, stored to
+ * the pair's LDS row in one 8-byte write per t; z2^T = W2^T h1^T takes h1^T
+ * fragments as 16-byte reads of those rows; the output layer sums the lane's
+ * 32 hidden units, then over the four lanes of the pair. */
 __global__ void __launch_bounds__(kFwdThreads) k_mlp_forward(const float4* __restrict__ vf, PairSrc ps, int64_t k,
                                                              MlpDev M, float* __restrict__ score) {
     extern __shared__ uint4 smem[];
@@ -238,6 +256,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_mlp_forward(const float4* __res
     stage_weights(S, M);
     __syncthreads();
     const int lane = threadIdx.x & (BM_WAVE - 1), wave = threadIdx.x / BM_WAVE;
+    const int q = lane / 16;
     short(*Ht)[kHPad] = S.Ht[wave];
     const float b3 = M.b3[0];
     const int64_t tiles = (k + 15) / 16;
@@ -245,41 +264,43 @@ __global__ void __launch_bounds__(kFwdThreads) k_mlp_forward(const float4* __res
         const int64_t p = tile * 16 + (lane % 16);
         const bool valid = p < k;
         const int2 uv = valid ? pair_at(ps, p) : make_int2(0, 0);
-        const v4s xa = input_frag(vf, uv, valid, lane);
-        v4f z1[kT];
-        layer1(S, Ht, xa, z1, lane);
+        const v4s xb = input_frag(vf, uv, valid, lane);
+#pragma unroll
+        for (int t = 0; t < kT; t++) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(S.W1[t * 64 + lane], xb, acc);
+            v4s h;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float z = acc[j] + S.b1[16 * t + 4 * q + j];
+                h[j] = bf16_of(z > 0.f ? z : 0.f);
+            }
+            *reinterpret_cast<v4s*>(&Ht[lane % 16][16 * t + 4 * q]) = h;
+        }
         wave_sync();
-        v4s a[kT];
-        load_a(Ht, a, lane);
-        float part[4] = {0.f, 0.f, 0.f, 0.f};
+        v8s hb[kS];
+        load_tile32(Ht, hb, lane);
+        float part = 0.f;
 #pragma unroll
         for (int t = 0; t < kT; t++) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < kT; s++) acc = mfma(a[s], S.W2[(s * kT + t) * 64 + lane], acc);
-            const int col = 16 * t + (lane % 16);
+            for (int s2 = 0; s2 < kS; s2++) acc = mfma32(S.W2[(s2 * kT + t) * 64 + lane], hb[s2], acc);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const float h = acc[j] + S.b2[col];
-                part[j] += (h > 0.f ? h : 0.f) * S.w3[col];
+                const int u = 16 * t + 4 * q + j;
+                const float h2 = acc[j] + S.b2[u];
+                part += (h2 > 0.f ? h2 : 0.f) * S.w3[u];
             }
         }
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) part[j] += __shfl_xor(part[j], o, BM_WAVE);
-        if ((lane % 16) == 0) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int64_t q = tile * 16 + 4 * (lane / 16) + j;
-                if (q < k) score[q] = part[j] + b3;
-            }
-        }
+        part += __shfl_xor(part, 16, BM_WAVE);
+        part += __shfl_xor(part, 32, BM_WAVE);
+        if (lane < 16 && valid) score[p] = part + b3;
         wave_sync();
     }
 }
 
-/* Training pass over a batch of B pairs (B a multiple of 16): forward, loss,
+/* Training pass over a batch of B pairs (B a multiple of 32): forward, loss,
  * and the backward through the activations.  Stores bf(x)^T (16 x B), h1^T,
  * g1^T and g2^T (128 x B) for the weight-gradient products and accumulates
  * the bias gradients, dw3 and the summed squared error with atomics. */
@@ -298,7 +319,7 @@ constexpr int kOffW1 = 0, kOffB1 = kF * kH, kOffW2 = kOffB1 + kH, kOffB2 = kOffW
 
 struct TrainShared {
     FwdShared f;
-    v4s W2T[kT * kT * 64];
+    v8s W2T[kS * kT * 64];
 };
 
 __global__ void __launch_bounds__(kFwdThreads) k_mlp_train_act(const float4* __restrict__ vf, PairSrc ps, int64_t k,
@@ -306,7 +327,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_mlp_train_act(const float4* __r
     extern __shared__ uint4 smem[];
     TrainShared& S = *reinterpret_cast<TrainShared*>(smem);
     stage_weights(S.f, M);
-    for (int q = threadIdx.x; q < kT * kT * 64; q += blockDim.x) S.W2T[q] = M.W2Tf[q];
+    for (int q = threadIdx.x; q < kS * kT * 64; q += blockDim.x) S.W2T[q] = M.W2Tf[q];
     __syncthreads();
     const int lane = threadIdx.x & (BM_WAVE - 1), wave = threadIdx.x / BM_WAVE;
     short(*Ht)[kHPad] = S.f.Ht[wave];
@@ -325,8 +346,8 @@ __global__ void __launch_bounds__(kFwdThreads) k_mlp_train_act(const float4* __r
         v4f z1[kT];
         layer1(S.f, Ht, xa, z1, lane);
         wave_sync();
-        v4s a[kT];
-        load_a(Ht, a, lane);
+        v8s a[kS];
+        load_tile32(Ht, a, lane);
         /* h1^T: rows 4 (l/16) + j of column c are 4 consecutive pairs */
 #pragma unroll
         for (int t = 0; t < kT; t++) {
@@ -342,7 +363,7 @@ __global__ void __launch_bounds__(kFwdThreads) k_mlp_train_act(const float4* __r
         for (int t = 0; t < kT; t++) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < kT; s++) acc = mfma(a[s], S.f.W2[(s * kT + t) * 64 + lane], acc);
+            for (int s = 0; s < kS; s++) acc = mfma32(a[s], S.f.W2[(s * kT + t) * 64 + lane], acc);
             const int col = 16 * t + (lane % 16);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -401,13 +422,13 @@ __global__ void __launch_bounds__(kFwdThreads) k_mlp_train_act(const float4* __r
         }
         wave_sync();
         /* g1 = bf((g2 @ W2^T) [z1 > 0]) */
-        v4s ga[kT];
-        load_a(Ht, ga, lane);
+        v8s ga[kS];
+        load_tile32(Ht, ga, lane);
 #pragma unroll
         for (int t = 0; t < kT; t++) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < kT; s++) acc = mfma(ga[s], S.W2T[(s * kT + t) * 64 + lane], acc);
+            for (int s = 0; s < kS; s++) acc = mfma32(ga[s], S.W2T[(s * kT + t) * 64 + lane], acc);
             const int col = 16 * t + (lane % 16);
             float gsum = 0.f;
             v4s gv;
@@ -426,16 +447,16 @@ __global__ void __launch_bounds__(kFwdThreads) k_mlp_train_act(const float4* __r
     }
 }
 
-/* Weight gradients: dW1 = bf(x)^T g1 (16 x 128), dW2 = h1^T g2 (128 x 128).
- * Work item = (output tile, slice of the batch); one wave each, summed into
- * the gradient with atomics. */
+/* Weight gradients: dW1 = bf(x)^T g1 (16 x 128), dW2 = h1^T g2 (128 x 128),
+ * 32 batch entries per MFMA.  Work item = (output tile, slice of the batch);
+ * one wave each, summed into the gradient with atomics. */
 constexpr int kWgTiles = 1 * kT + kT * kT;
 
 __global__ void __launch_bounds__(256) k_mlp_wgrad(TrainBuf T, int32_t slices) {
     const int lane = threadIdx.x & (BM_WAVE - 1);
     const int64_t items = (int64_t)kWgTiles * slices;
     const int64_t B = T.B;
-    const int64_t steps = B / 16;
+    const int64_t steps = B / 32;
     for (int64_t it = (int64_t)blockIdx.x * (blockDim.x / BM_WAVE) + threadIdx.x / BM_WAVE; it < items;
          it += (int64_t)gridDim.x * (blockDim.x / BM_WAVE)) {
         const int tile = (int)(it % kWgTiles), slice = (int)(it / kWgTiles);
@@ -447,10 +468,10 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(TrainBuf T, int32_t slices) {
         v4f acc = {0.f, 0.f, 0.f, 0.f};
         const int64_t arow = (int64_t)(16 * mt + lane % 16) * B, brow = (int64_t)(16 * nt + lane % 16) * B;
         for (int64_t s = s0; s < s1; s++) {
-            const int64_t kk = 16 * s + 4 * (lane / 16);
-            const v4s a = *reinterpret_cast<const v4s*>(&AT[arow + kk]);
-            const v4s b = *reinterpret_cast<const v4s*>(&BT[brow + kk]);
-            acc = mfma(a, b, acc);
+            const int64_t kk = 32 * s + 8 * (lane / 16);
+            const v8s a = *reinterpret_cast<const v8s*>(&AT[arow + kk]);
+            const v8s b = *reinterpret_cast<const v8s*>(&BT[brow + kk]);
+            acc = mfma32(a, b, acc);
         }
         const int off = first ? kOffW1 : kOffW2;
 #pragma unroll
@@ -512,14 +533,18 @@ __device__ __forceinline__ uint64_t rank_key(float x, int64_t p) {
 }
 
 constexpr int kSelBits = 11, kSelBins = 1 << kSelBits, kSelThreads = 256, kSelPasses = 6;
+constexpr int kSelCopies = 16;       /* copies of the global histogram (block b adds to copy b % 16) */
 constexpr int kSortMax = 8192;       /* largest top sorted in one block */
 
 struct SelState {
     uint64_t prefix;      /* key bits fixed so far (the bits of the current digit and below are 0) */
+    uint64_t fixed;       /* mask of the fixed bits */
     int64_t remaining;    /* rank (1-based) of the wanted key within the keys sharing the prefix */
     uint32_t done;        /* blocks of the current pass that have merged their histogram */
     uint32_t count;       /* keys written by the compaction */
-    uint32_t hist[kSelBins];
+    uint32_t settled;     /* 1: the keys >= prefix under the mask are exactly the top */
+    uint32_t pad;
+    uint32_t hist[kSelCopies][kSelBins];
 };
 
 __device__ __forceinline__ void sel_digit(int pass, int* shift, int* width) {
@@ -531,38 +556,66 @@ __device__ __forceinline__ void sel_digit(int pass, int* shift, int* width) {
 __global__ void k_select_init(SelState* st, int64_t top) {
     if (threadIdx.x == 0) {
         st->prefix = 0;
+        st->fixed = 0;
         st->remaining = top;
         st->done = 0;
         st->count = 0;
+        st->settled = 0;
     }
-    for (int i = threadIdx.x; i < kSelBins; i += blockDim.x) st->hist[i] = 0;
+    for (int i = threadIdx.x; i < kSelCopies * kSelBins; i += blockDim.x) (&st->hist[0][0])[i] = 0;
+}
+
+/* Adds 1 to bin b of the LDS histogram for every lane with `valid`: lanes
+ * sharing the first valid lane's bin (the common case: scores crowd into a
+ * few bins) add together in one atomic. */
+__device__ __forceinline__ void wave_hist_add(uint32_t* h, bool valid, uint32_t b, int lane) {
+    const uint64_t act = __ballot(valid);
+    if (!act) return;
+    const int lead = __ffsll((unsigned long long)act) - 1;
+    const uint32_t lb = (uint32_t)__shfl((int)b, lead, BM_WAVE);
+    const uint64_t same = __ballot(valid && b == lb);
+    if (lane == lead) atomicAdd(&h[lb], (uint32_t)__popcll(same));
+    if (valid && b != lb) atomicAdd(&h[b], 1u);
 }
 
 /* One digit of the select: every block histograms the digit of the keys that
- * share the prefix (LDS counters, merged into the state with atomics); the
- * last block to finish finds the digit holding the wanted rank (suffix sums
- * from the top bin), extends the prefix and clears the histogram. */
+ * share the prefix (LDS counters, merged into one of kSelCopies global
+ * copies with atomics); the last block to finish sums the copies, finds the
+ * digit holding the wanted rank (suffix sums from the top bin), extends the
+ * prefix and clears the histograms.  When the wanted rank is the last key of
+ * its digit's bin, every key of that bin and above belongs to the top: the
+ * select is settled and the remaining passes do nothing. */
 __global__ void __launch_bounds__(kSelThreads) k_select_pass(const float* __restrict__ sc, int64_t P,
                                                              SelState* st, int pass) {
     __shared__ uint32_t h[kSelBins];
     __shared__ uint32_t part[kSelThreads];
     __shared__ int last;
+    if (st->settled) return;
     int shift, width;
     sel_digit(pass, &shift, &width);
     const int bins = 1 << width;
     const int top = shift + width;
     const uint64_t fixed = top >= 64 ? 0ull : (~0ull << top);
     const uint64_t prefix = st->prefix;
+    const int lane = threadIdx.x & (BM_WAVE - 1);
     for (int i = threadIdx.x; i < bins; i += blockDim.x) h[i] = 0;
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += stride) {
-        const uint64_t k = rank_key(sc[p], p);
-        if ((k & fixed) == prefix) atomicAdd(&h[(uint32_t)(k >> shift) & (uint32_t)(bins - 1)], 1u);
+    /* every lane of a wave runs the same iterations (the ballots need them all) */
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(BM_WAVE - 1)); base < P; base += stride) {
+        const int64_t p = base + lane;
+        uint64_t k = 0;
+        bool valid = false;
+        if (p < P) {
+            k = rank_key(sc[p], p);
+            valid = (k & fixed) == prefix;
+        }
+        wave_hist_add(h, valid, (uint32_t)(k >> shift) & (uint32_t)(bins - 1), lane);
     }
     __syncthreads();
+    uint32_t* gh = st->hist[blockIdx.x % kSelCopies];
     for (int i = threadIdx.x; i < bins; i += blockDim.x)
-        if (h[i]) atomicAdd(&st->hist[i], h[i]);
+        if (h[i]) atomicAdd(&gh[i], h[i]);
     __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
@@ -576,7 +629,9 @@ __global__ void __launch_bounds__(kSelThreads) k_select_pass(const float* __rest
     const int t = threadIdx.x;
     uint32_t mine = 0;
     for (int j = 0; j < per; j++) {
-        const uint32_t c = __hip_atomic_load(&st->hist[t * per + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t c = 0;
+        for (int cp = 0; cp < kSelCopies; cp++)
+            c += __hip_atomic_load(&st->hist[cp][t * per + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         h[t * per + j] = c;
         mine += c;
     }
@@ -599,22 +654,37 @@ __global__ void __launch_bounds__(kSelThreads) k_select_pass(const float* __rest
             r -= h[d];
         }
         st->prefix = prefix | ((uint64_t)d << shift);
+        st->fixed = top - width >= 64 ? ~0ull : (~0ull << (top - width));
         st->remaining = r;
+        st->settled = r == (int64_t)h[d];
     }
     if (t == 0) st->done = 0;
-    for (int i = t; i < bins; i += blockDim.x) st->hist[i] = 0;
+    for (int cp = 0; cp < kSelCopies; cp++)
+        for (int i = t; i < bins; i += blockDim.x) st->hist[cp][i] = 0;
 }
 
-/* Keys >= the selected key (exactly `top` of them: keys are distinct; the
- * bound only guards the buffer). */
+/* Keys at or above the selected prefix under its mask (exactly `top` of
+ * them; the bound only guards the buffer).  One global atomic per wave. */
 __global__ void k_select_compact(const float* __restrict__ sc, int64_t P, SelState* st, uint64_t* __restrict__ out,
                                  uint32_t top) {
-    const uint64_t kstar = st->prefix;
+    const uint64_t kstar = st->prefix, fixed = st->fixed;
+    const int lane = threadIdx.x & (BM_WAVE - 1);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += stride) {
-        const uint64_t k = rank_key(sc[p], p);
-        if (k >= kstar) {
-            const uint32_t at = atomicAdd(&st->count, 1u);
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(BM_WAVE - 1)); base < P; base += stride) {
+        const int64_t p = base + lane;
+        uint64_t k = 0;
+        bool in = false;
+        if (p < P) {
+            k = rank_key(sc[p], p);
+            in = (k & fixed) >= kstar;
+        }
+        const uint64_t m = __ballot(in);
+        if (!m) continue;
+        uint32_t at0 = 0;
+        if (lane == 0) at0 = atomicAdd(&st->count, (uint32_t)__popcll(m));
+        at0 = (uint32_t)__shfl((int)at0, 0, BM_WAVE);
+        if (in) {
+            const uint32_t at = at0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             if (at < top) out[at] = k;
         }
     }
@@ -660,8 +730,8 @@ int grid_of(int64_t work, int threads, int64_t cap = 65536) {
 MlpDev mlpdev(const BmMlpDev& d) {
     MlpDev M;
     M.W1f = reinterpret_cast<const v4s*>(d.W1f);
-    M.W2f = reinterpret_cast<const v4s*>(d.W2f);
-    M.W2Tf = reinterpret_cast<const v4s*>(d.W2Tf);
+    M.W2f = reinterpret_cast<const v8s*>(d.W2f);
+    M.W2Tf = reinterpret_cast<const v8s*>(d.W2Tf);
     M.b1 = d.params + kOffB1;
     M.b2 = d.params + kOffB2;
     M.w3 = d.params + kOffW3;
@@ -682,12 +752,12 @@ hipError_t bm_launch_vertex_features(const BmDevGraph& g, float* vf, int32_t num
 }
 
 hipError_t bm_launch_mlp_pack(const BmMlpDev& d, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_pack, dim3(8), dim3(256), 0, s, d.params + kOffW1, kF, kH, 0,
-                       reinterpret_cast<v4s*>(d.W1f));
-    hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, s, d.params + kOffW2, kH, kH, 0,
-                       reinterpret_cast<v4s*>(d.W2f));
-    hipLaunchKernelGGL(k_mlp_pack, dim3(64), dim3(256), 0, s, d.params + kOffW2, kH, kH, 1,
-                       reinterpret_cast<v4s*>(d.W2Tf));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(2), dim3(256), 0, s, d.params + kOffW1, kF, kH, 0, 16,
+                       static_cast<short*>(d.W1f));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(8), dim3(256), 0, s, d.params + kOffW2, kH, kH, 0, 32,
+                       static_cast<short*>(d.W2f));
+    hipLaunchKernelGGL(k_mlp_pack, dim3(8), dim3(256), 0, s, d.params + kOffW2, kH, kH, 1, 32,
+                       static_cast<short*>(d.W2Tf));
     return hipGetLastError();
 }
 
@@ -707,14 +777,14 @@ hipError_t bm_launch_mlp_forward(const BmMlpDev& d, const float* vf, const int2*
     return hipGetLastError();
 }
 
-size_t bm_mlp_train_bytes(int64_t B) { return (size_t)B * (kF + 3 * kH) * sizeof(short); }
+size_t bm_mlp_train_bytes(int64_t k) { return (size_t)((k + 31) / 32 * 32) * (kF + 3 * kH) * sizeof(short); }
 
 hipError_t bm_launch_mlp_gradient(const BmMlpDev& d, const float* vf, const int2* list, uint64_t seed, int32_t n,
                                   int64_t k, const float* target, void* act, float* loss, int32_t num_cus,
                                   hipStream_t s) {
     if (k <= 0) return hipSuccess;
     TrainBuf T;
-    T.B = (k + 15) / 16 * 16;
+    T.B = (k + 31) / 32 * 32;
     T.XT = reinterpret_cast<short*>(act);
     T.H1T = T.XT + (size_t)kF * T.B;
     T.G1T = T.H1T + (size_t)kH * T.B;
@@ -731,7 +801,7 @@ hipError_t bm_launch_mlp_gradient(const BmMlpDev& d, const float* vf, const int2
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_mlp_train_act, dim3((unsigned)blocks), dim3(kFwdThreads), sizeof(TrainShared), s,
                        reinterpret_cast<const float4*>(vf), ps, k, target, mlpdev(d), T);
-    int32_t slices = (int32_t)(tiles / 64);
+    int32_t slices = (int32_t)(tiles / 128);
     if (slices < 1) slices = 1;
     if (slices > 64) slices = 64;
     const int64_t items = (int64_t)kWgTiles * slices;
@@ -780,7 +850,7 @@ hipError_t bm_launch_mlp_rank(const BmMlpDev& d, const float* vf, uint64_t seed,
         q += (256 - (reinterpret_cast<uintptr_t>(q) & 255)) & 255;
         SelState* st = reinterpret_cast<SelState*>(q);
         uint64_t* keys = reinterpret_cast<uint64_t*>(st + 1);
-        const int blocks = grid_of(pool, kSelThreads * 16, (int64_t)(num_cus > 0 ? num_cus : 256));
+        const int blocks = grid_of(pool, kSelThreads * 8, (int64_t)(num_cus > 0 ? num_cus : 256));
         hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, top);
         for (int pass = 0; pass < kSelPasses; pass++)
             hipLaunchKernelGGL(k_select_pass, dim3(blocks), dim3(kSelThreads), 0, s, sc, pool, st, pass);
