@@ -71,6 +71,7 @@ struct bm_ctx {
     int32_t* d_f_self;
     int4* d_moves;
     int64_t moves_cap;
+    int64_t* d_patch_plan;  /* 4 moves_cap + 1 work-item offsets of a table patch */
     int64_t* d_nmoves;
     int64_t gain_version;   /* bumped by every build and patch of the table */
     /* u-major evaluation (bm_batch_eval_swaps_umajor) */
@@ -248,6 +249,7 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_m_self);
     (void)hipFree(c->d_f_self);
     (void)hipFree(c->d_moves);
+    (void)hipFree(c->d_patch_plan);
     (void)hipFree(c->d_nmoves);
     (void)hipFree(c->d_um_counts);
     (void)hipFree(c->d_um_off);
@@ -306,9 +308,12 @@ static int gain_moves_cap(bm_ctx* c, int64_t need) {
     if (c->moves_cap >= need) return BM_OK;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(c->d_moves);
+    (void)hipFree(c->d_patch_plan);
     c->d_moves = nullptr;
+    c->d_patch_plan = nullptr;
     c->moves_cap = 0;
     HIPCHK(c, hipMalloc((void**)&c->d_moves, (size_t)need * sizeof(int4)));
+    HIPCHK(c, hipMalloc((void**)&c->d_patch_plan, (size_t)(4 * need + 1) * sizeof(int64_t)));
     c->moves_cap = need;
     return BM_OK;
 }
@@ -316,7 +321,10 @@ static int gain_moves_cap(bm_ctx* c, int64_t need) {
 static int gain_patch(bm_ctx* c, const int64_t* d_count, int64_t max_moves) {
     KTiming t;
     timing_begin(c, BM_KERNEL_GAIN_UPDATE, &t);
-    HIPCHK(c, bm_launch_gain_update(devgraph(c), devgain(c), c->d_moves, d_count, max_moves, c->stream));
+    if (max_moves > c->moves_cap) return fail(c, BM_ERR_STATE, "gain patch: %lld moves exceed the move list",
+                                              (long long)max_moves);
+    HIPCHK(c, bm_launch_gain_update(devgraph(c), devgain(c), c->d_moves, d_count, max_moves, c->d_patch_plan,
+                                    c->num_cus, c->stream));
     timing_end(c, &t);
     c->gain_version++;
     return BM_OK;

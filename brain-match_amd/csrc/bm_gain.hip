@@ -482,60 +482,98 @@ __global__ void k_pack_rows(const int2* __restrict__ in, int64_t m, int layout, 
 }
 
 /* Patch the rows of the neighbours of each moved vertex x (pi(x): old -> new).
- * moves[i] = {x, old, new, 0}, *count moves; each move's male edges are
- * split over `slices` consecutive blocks (a block's waves take every
- * slices-th group of 64 edges), so a few moves of high-degree vertices still
- * fill the device.  An in-edge
- * y -> x contributes c(w, f, pi(x)) to G(y, f) over the female in-row of pi(x);
- * an out-edge x -> y contributes c(w, pi(x), f) over the female out-row.  The
- * old female's terms are subtracted and the new one's added; self-loops are
- * not in the table.  Updates of different moves commute (integer atomics). */
+ * moves[i] = {x, old, new, 0}, *count moves.  An in-edge y -> x contributes
+ * c(w, f, pi(x)) to G(y, f) over the female in-row of pi(x); an out-edge
+ * x -> y contributes c(w, pi(x), f) over the female out-row.  The old
+ * female's terms are subtracted and the new one's added; self-loops are not
+ * in the table.  Updates of different moves commute (integer atomics).
+ *
+ * Work items: (move, side = old / new female, direction, group of 64 of x's
+ * edges, chunk of 64 entries of the female row).  k_gain_patch_plan counts
+ * each (move, side, direction)'s items and scans them into offsets; the
+ * patch kernel's waves take items from the whole index space, so the many
+ * items of one high-degree vertex spread over the device.  An item: lanes
+ * hold 64 edges and 64 row entries, and each entry is applied for every edge
+ * in turn (the edge's row y and weight read out of its lane as scalars). */
+constexpr int kPlanThreads = 1024;
+
+__device__ __forceinline__ int64_t patch_items(const BmDevGraph& g, int4 mv, int side, int dir) {
+    const int32_t gx = side ? mv.z : mv.y;
+    if (mv.y == mv.z || gx == 0 || mv.x == 0) return 0;
+    const int64_t* mptr = dir ? g.m_out_ptr : g.m_in_ptr;
+    const int64_t* fptr = dir ? g.f_out_ptr : g.f_in_ptr;
+    const int64_t deg = mptr[mv.x + 1] - mptr[mv.x], len = fptr[gx + 1] - fptr[gx];
+    return ((deg + BM_WAVE - 1) / BM_WAVE) * ((len + BM_WAVE - 1) / BM_WAVE);
+}
+
+/* One block: off[s] = items of the slots before s, s = 4 move + 2 side + dir,
+ * off[4 nm] = all items. */
+__global__ void __launch_bounds__(kPlanThreads) k_gain_patch_plan(BmDevGraph g, const int4* __restrict__ moves,
+                                                                  const int64_t* __restrict__ count,
+                                                                  int64_t* __restrict__ off) {
+    __shared__ int64_t part[kPlanThreads];
+    const int64_t slots = 4 * *count;
+    const int t = threadIdx.x;
+    const int64_t per = (slots + kPlanThreads - 1) / kPlanThreads;
+    const int64_t s0 = t * per, s1 = s0 + per < slots ? s0 + per : slots;
+    int64_t mine = 0;
+    for (int64_t sl = s0; sl < s1; sl++) mine += patch_items(g, moves[sl >> 2], (int)((sl >> 1) & 1), (int)(sl & 1));
+    part[t] = mine;
+    __syncthreads();
+    for (int o = 1; o < kPlanThreads; o <<= 1) {
+        const int64_t add = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int64_t run = t ? part[t - 1] : 0;
+    for (int64_t sl = s0; sl < s1; sl++) {
+        off[sl] = run;
+        run += patch_items(g, moves[sl >> 2], (int)((sl >> 1) & 1), (int)(sl & 1));
+    }
+    if (t == kPlanThreads - 1) off[slots] = part[t];
+}
+
 __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmGain G, const int4* __restrict__ moves,
-                                                               const int64_t* __restrict__ count, int32_t slices) {
-    const int tid = threadIdx.x;
-    const int lane = tid & (BM_WAVE - 1);
-    const int wave = tid / BM_WAVE;
-    const int64_t nm = *count;
-    const int32_t slice = (int32_t)(blockIdx.x % (uint32_t)slices);
-    const int64_t first = (int64_t)((slice * kPatchWaves + wave) * BM_WAVE);
-    const int64_t step = (int64_t)slices * kPatchWaves * BM_WAVE;
-    for (int64_t mi = blockIdx.x / (uint32_t)slices; mi < nm; mi += gridDim.x / (uint32_t)slices) {
-        const int4 mv = moves[mi];
-        const int32_t x = mv.x;
-        if (mv.y == mv.z) continue;
-        for (int side = 0; side < 2; side++) {
-            const int32_t gx = side ? mv.z : mv.y;
-            if (gx == 0) continue;
-            const int32_t sign = side ? 1 : -1;
-            for (int dir = 0; dir < 2; dir++) {
-                const int64_t* mptr = dir ? g.m_out_ptr : g.m_in_ptr;
-                const int2* madj = dir ? g.m_out : g.m_in;
-                const int64_t* fptr = dir ? g.f_out_ptr : g.f_in_ptr;
-                const int2* fadj = dir ? g.f_out : g.f_in;
-                const int64_t b = mptr[x], e = mptr[x + 1];
-                const int64_t rb = fptr[gx], re = fptr[gx + 1];
-                /* every edge of x expands the same female row (that of gx):
-                 * lanes hold 64 of x's edges, then 64 row entries at a time,
-                 * and each entry is applied for every edge in turn (the edge's
-                 * row y and weight read out of its lane as scalars) */
-                for (int64_t e0 = b + first; e0 < e; e0 += step) {
-                    const int64_t i = e0 + lane;
-                    const int2 ed = i < e ? madj[i] : make_int2(0, 0);
-                    const int32_t yl = ed.x != x ? ed.x : 0;   /* 0: past the list or a self-loop */
-                    const int32_t nv = e - e0 < BM_WAVE ? (int32_t)(e - e0) : BM_WAVE;
-                    for (int64_t r0 = rb; r0 < re; r0 += BM_WAVE) {
-                        const int64_t ri = r0 + lane;
-                        const bool rin = ri < re;
-                        const int2 ent = rin ? fadj[ri] : make_int2(0, 0);
-                        for (int32_t k = 0; k < nv; k++) {
-                            const int32_t y = __builtin_amdgcn_readlane(yl, k);
-                            if (y == 0) continue;
-                            const int32_t w = __builtin_amdgcn_readlane(ed.y, k);
-                            if (rin) atomicAdd(&G.G[(int64_t)y * G.stride + ent.x], sign * (w < ent.y ? w : ent.y));
-                        }
-                    }
-                }
-            }
+                                                               const int64_t* __restrict__ count,
+                                                               const int64_t* __restrict__ off) {
+    const int lane = threadIdx.x & (BM_WAVE - 1);
+    const int64_t slots = 4 * *count;
+    const int64_t total = off[slots];
+    const int64_t nw = (int64_t)gridDim.x * kPatchWaves;
+    for (int64_t item = (int64_t)blockIdx.x * kPatchWaves + threadIdx.x / BM_WAVE; item < total; item += nw) {
+        /* the slot holding the item: the last s with off[s] <= item */
+        int64_t lo = 0, hi = slots;   /* off[lo] <= item < off[hi] */
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (off[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        const int4 mv = moves[lo >> 2];
+        const int side = (int)((lo >> 1) & 1), dir = (int)(lo & 1);
+        const int32_t x = mv.x, gx = side ? mv.z : mv.y;
+        const int32_t sign = side ? 1 : -1;
+        const int64_t* mptr = dir ? g.m_out_ptr : g.m_in_ptr;
+        const int2* madj = dir ? g.m_out : g.m_in;
+        const int64_t* fptr = dir ? g.f_out_ptr : g.f_in_ptr;
+        const int2* fadj = dir ? g.f_out : g.f_in;
+        const int64_t b = mptr[x], e = mptr[x + 1];
+        const int64_t rb = fptr[gx], re = fptr[gx + 1];
+        const int64_t nchunk = (re - rb + BM_WAVE - 1) / BM_WAVE;
+        const int64_t local = item - off[lo];
+        const int64_t e0 = b + (local / nchunk) * BM_WAVE;
+        const int64_t i = e0 + lane;
+        const int2 ed = i < e ? madj[i] : make_int2(0, 0);
+        const int32_t yl = ed.x != x ? ed.x : 0;   /* 0: past the list or a self-loop */
+        const int32_t nv = e - e0 < BM_WAVE ? (int32_t)(e - e0) : BM_WAVE;
+        const int64_t ri = rb + (local % nchunk) * BM_WAVE + lane;
+        const bool rin = ri < re;
+        const int2 ent = rin ? fadj[ri] : make_int2(0, 0);
+        for (int32_t k = 0; k < nv; k++) {
+            const int32_t y = __builtin_amdgcn_readlane(yl, k);
+            if (y == 0) continue;
+            const int32_t w = __builtin_amdgcn_readlane(ed.y, k);
+            if (rin) atomicAdd(&G.G[(int64_t)y * G.stride + ent.x], sign * (w < ent.y ? w : ent.y));
         }
     }
 }
@@ -1244,15 +1282,12 @@ hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t nu
 }
 
 hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int4* moves, const int64_t* count,
-                                 int64_t max_moves, hipStream_t s) {
+                                 int64_t max_moves, int64_t* plan, int32_t num_cus, hipStream_t s) {
     if (max_moves <= 0) return hipSuccess;
-    const int64_t per_move = max_moves < 4096 ? max_moves : 4096;
-    /* about 2048 blocks in all, at least one per move slot */
-    int32_t slices = (int32_t)(2048 / per_move);
-    if (slices < 1) slices = 1;
-    if (slices > 256) slices = 256;
-    hipLaunchKernelGGL(k_gain_update, dim3((unsigned)(per_move * slices)), dim3(kPatchThreads), 0, s, g, G, moves,
-                       count, slices);
+    hipLaunchKernelGGL(k_gain_patch_plan, dim3(1), dim3(kPlanThreads), 0, s, g, moves, count, plan);
+    /* persistent grid: every wave walks the item space */
+    const int64_t blocks = (int64_t)(num_cus > 0 ? num_cus : 256) * 4;
+    hipLaunchKernelGGL(k_gain_update, dim3((unsigned)blocks), dim3(kPatchThreads), 0, s, g, G, moves, count, plan);
     return hipGetLastError();
 }
 
