@@ -530,6 +530,13 @@ class Context:
         self._chk(_lib.bm_twohop_set(self.h, v, _p32(out), C.byref(cnt)))
         return out[:cnt.value].copy()
 
+    def greedy_match(self):
+        """The greedy initial matching computed on the device (the same as
+        greedy_match on the context's edges)."""
+        pi = np.zeros(self.n + 1, np.int32)
+        self._chk(_lib.bm_greedy_match_device(self.h, _p32(pi)))
+        return pi
+
     def gain_enable(self, on=True):
         """Switch evaluation to the placement-gain table (or back to edge scans)."""
         self._chk(_lib.bm_gain_enable(self.h, 1 if on else 0))
@@ -578,6 +585,7 @@ class _AnnealParams(C.Structure):
                 ("decay", C.c_double), ("p_rot3", C.c_double), ("traj_cap", C.c_int32), ("p_pool", C.c_double)]
 
 
+_sig("bm_greedy_match_device", C.c_int, [_c, i32p])
 _sig("bm_anneal_create", C.c_int, [C.POINTER(_c), C.POINTER(_c), C.c_int32, C.POINTER(_AnnealParams)])
 _sig("bm_anneal_destroy", None, [_c])
 _sig("bm_anneal_num_chains", C.c_int64, [_c])

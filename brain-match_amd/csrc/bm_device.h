@@ -237,7 +237,12 @@ struct BmMlpDev {
     void* W2Tf;
 };
 int32_t bm_mlp_param_count(void);
-hipError_t bm_launch_vertex_features(const BmDevGraph& g, float* vf, int32_t num_cus, hipStream_t s);
+/* Per-vertex features from edge chunks {x, out (1) or in (0), first, count}
+ * of the male lists (at most bm_feature_chunk_edges() edges each); acc
+ * holds (n+1) x 32 zeroed bytes of accumulators and is left zeroed. */
+int32_t bm_feature_chunk_edges(void);
+hipError_t bm_launch_vertex_features(const BmDevGraph& g, const int4* chunks, int64_t nchunks, void* acc, float* vf,
+                                     int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_mlp_pack(const BmMlpDev& d, hipStream_t s);
 /* Scores of k pairs (list, or uniform stream `seed` when list is NULL). */
 hipError_t bm_launch_mlp_forward(const BmMlpDev& d, const float* vf, const int2* list, uint64_t seed, int32_t n,
@@ -252,6 +257,13 @@ hipError_t bm_launch_mlp_adam(const BmMlpDev& d, float lr, int64_t step, hipStre
 hipError_t bm_launch_mlp_targets(const int64_t* delta, int64_t k, float* t, hipStream_t s);
 /* Top `top` of `pool` uniform-stream pairs by score, as a pair list. */
 size_t bm_mlp_rank_scratch(int64_t pool);
+/* Scratch bytes of the select-only ranking of a pool (bm_launch_mlp_rank_select)
+ * and the largest top it sorts. */
+size_t bm_mlp_select_bytes(int64_t pool);
+int64_t bm_mlp_select_max_top(void);
+hipError_t bm_launch_mlp_rank_select(const BmMlpDev& d, const float* vf, uint64_t seed, int32_t n, int64_t pool,
+                                     int64_t top, void* scratch, size_t scratch_bytes, int2* out, int32_t num_cus,
+                                     hipStream_t s);
 hipError_t bm_launch_mlp_rank(const BmMlpDev& d, const float* vf, uint64_t seed, int32_t n, int64_t pool, int64_t top,
                               void* scratch, size_t scratch_bytes, int2* out, int32_t num_cus, hipStream_t s);
 

@@ -20,6 +20,7 @@
 
 #include <vector>
 
+
 struct bm_mlp {
     bm_ctx* ctx;
     int device;
@@ -28,6 +29,9 @@ struct bm_mlp {
     hipStream_t stream;
     BmMlpDev d;
     float* d_vf;
+    int4* d_chunks;         /* male edge chunks of the feature pass */
+    int64_t nchunks;
+    void* d_feat_acc;       /* (n+1) x 32 bytes of feature accumulators, zero between passes */
     int features_ready;
     float* d_loss;
     float* d_target;
@@ -92,6 +96,8 @@ extern "C" void bm_mlp_destroy(bm_mlp* m) {
     (void)hipFree(m->d.W2f);
     (void)hipFree(m->d.W2Tf);
     (void)hipFree(m->d_vf);
+    (void)hipFree(m->d_chunks);
+    (void)hipFree(m->d_feat_acc);
     (void)hipFree(m->d_loss);
     (void)hipFree(m->d_target);
     (void)hipFree(m->d_act);
@@ -172,6 +178,27 @@ extern "C" int bm_mlp_create(bm_ctx* ctx, uint64_t seed, bm_mlp** out) {
     alloc(&m->d.W2Tf, 128 * 128 * 2);
     alloc((void**)&m->d_vf, ((size_t)m->n + 1) * 8 * sizeof(float));
     alloc((void**)&m->d_loss, sizeof(float));
+    alloc(&m->d_feat_acc, ((size_t)m->n + 1) * 32);
+    if (rc == BM_OK && hipMemset(m->d_feat_acc, 0, ((size_t)m->n + 1) * 32) != hipSuccess)
+        rc = bm_ctx_set_error(ctx, BM_ERR_HIP, "bm_mlp_create: hipMemset failed");
+    if (rc == BM_OK) {
+        /* chunks of at most bm_feature_chunk_edges() edges of every male list */
+        const bm_graph_host* g = bm_ctx_male(ctx);
+        const int64_t cap = bm_feature_chunk_edges();
+        std::vector<int4> ch;
+        for (int32_t x = 1; x <= m->n; x++)
+            for (int dir = 0; dir < 2; dir++) {
+                const bm_csr_host& L = dir ? g->out : g->in;
+                const int64_t deg = L.ptr[x + 1] - L.ptr[x];
+                for (int64_t e0 = 0; e0 < deg; e0 += cap)
+                    ch.push_back(make_int4(x, dir, (int32_t)e0, (int32_t)(deg - e0 < cap ? deg - e0 : cap)));
+            }
+        m->nchunks = (int64_t)ch.size();
+        alloc((void**)&m->d_chunks, ch.size() * sizeof(int4) + sizeof(int4));
+        if (rc == BM_OK && !ch.empty() &&
+            hipMemcpy(m->d_chunks, ch.data(), ch.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+            rc = bm_ctx_set_error(ctx, BM_ERR_HIP, "bm_mlp_create: chunk upload failed");
+    }
     if (rc == BM_OK) {
         std::vector<float> p(np);
         init_params(p, seed);
@@ -188,7 +215,8 @@ extern "C" int bm_mlp_create(bm_ctx* ctx, uint64_t seed, bm_mlp** out) {
 extern "C" int bm_mlp_refresh_features(bm_mlp* m) {
     if (!m) return BM_ERR_ARG;
     MHIP(m, hipSetDevice(m->device));
-    MHIP(m, bm_launch_vertex_features(bm_ctx_devgraph(m->ctx), m->d_vf, m->num_cus, m->stream));
+    MHIP(m, bm_launch_vertex_features(bm_ctx_devgraph(m->ctx), m->d_chunks, m->nchunks, m->d_feat_acc, m->d_vf,
+                                      m->num_cus, m->stream));
     m->features_ready = 1;
     return BM_OK;
 }

@@ -93,51 +93,67 @@ __device__ __forceinline__ int32_t row_w(const uint64_t* __restrict__ desc, cons
     }
 }
 
-/* One wave per vertex x: 8 features (two float4) into vf[2x], vf[2x+1]. */
-__global__ void __launch_bounds__(256) k_vertex_features(BmDevGraph g, float4* __restrict__ vf) {
+/* Per-vertex features in two passes.  k_feature_chunks: one wave per chunk
+ * {x, direction, first, count} of at most kFeatChunk edges of one of x's male
+ * lists (the chunks depend only on the male graph), so a hub's lists spread
+ * over many waves; each edge's contribution under pi (a probe of the female
+ * row of pi(x)) is summed with its weight and whether it is 0, and the wave
+ * adds the sums to x's accumulators {L, weight sum, zero count} (a self-loop
+ * counts once, with the out-edges).  k_feature_finish: one thread per vertex
+ * turns the exact integer sums into the 8 floats and clears the
+ * accumulators. */
+__global__ void __launch_bounds__(256) k_feature_chunks(BmDevGraph g, const int4* __restrict__ chunks, int64_t nchunks,
+                                                        longlong4* __restrict__ acc) {
     const int lane = threadIdx.x & (BM_WAVE - 1);
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / BM_WAVE);
-    for (int64_t xx = (int64_t)blockIdx.x * (blockDim.x / BM_WAVE) + threadIdx.x / BM_WAVE; xx <= g.n; xx += nw) {
-        const int32_t x = (int32_t)xx;
-        if (x == 0) {
-            if (lane == 0) {
-                vf[0] = make_float4(0.f, 0.f, 0.f, 0.f);
-                vf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            continue;
-        }
+    for (int64_t ci = (int64_t)blockIdx.x * (blockDim.x / BM_WAVE) + threadIdx.x / BM_WAVE; ci < nchunks; ci += nw) {
+        const int4 ch = chunks[ci];
+        const int32_t x = uni(ch.x);
+        const int out = uni(ch.y);
         const int32_t fx = uni(g.pi[x]);
-        const int64_t ob = g.m_out_ptr[x], oe = g.m_out_ptr[x + 1];
-        const int64_t ib = g.m_in_ptr[x], ie = g.m_in_ptr[x + 1];
-        int64_t L = 0, wsum = 0, miss = 0;
-        for (int64_t e = ob + lane; e < oe; e += BM_WAVE) {
-            const int2 ed = g.m_out[e];
+        const int64_t base = (out ? g.m_out_ptr[x] : g.m_in_ptr[x]) + uni(ch.z);
+        const int2* adj = out ? g.m_out : g.m_in;
+        const uint64_t* desc = out ? g.f_out_desc : g.f_in_desc;
+        const int2* tab = out ? g.f_out_tab : g.f_in_tab;
+        int64_t L = 0, ws = 0, miss = 0;
+        for (int32_t i = lane; i < ch.w; i += BM_WAVE) {
+            const int2 ed = adj[base + i];
+            if (!out && ed.x == x) continue;   /* a self-loop counted once, with the out-edges */
             const int32_t fy = ed.x == x ? fx : g.pi[ed.x];
-            const int32_t c = contrib(ed.y, row_w(g.f_out_desc, g.f_out_tab, fx, fy));
+            const int32_t c = contrib(ed.y, row_w(desc, tab, fx, fy));
             L += c;
-            wsum += ed.y;
-            miss += c == 0;
-        }
-        for (int64_t e = ib + lane; e < ie; e += BM_WAVE) {
-            const int2 ed = g.m_in[e];
-            if (ed.x == x) continue;   /* a self-loop counted once, with the out-edges */
-            const int32_t c = contrib(ed.y, row_w(g.f_in_desc, g.f_in_tab, fx, g.pi[ed.x]));
-            L += c;
-            wsum += ed.y;
+            ws += ed.y;
             miss += c == 0;
         }
         L = wave_sum64(L);
-        wsum = wave_sum64(wsum);
+        ws = wave_sum64(ws);
         miss = wave_sum64(miss);
         if (lane == 0) {
-            const float dout = (float)(oe - ob), din = (float)(ie - ib);
-            const float fo = fx ? (float)(g.f_out_ptr[fx + 1] - g.f_out_ptr[fx]) : 0.f;
-            const float fi = fx ? (float)(g.f_in_ptr[fx + 1] - g.f_in_ptr[fx]) : 0.f;
-            const float ws = (float)(wsum > 0 ? wsum : 0);
-            const float Lf = (float)(L > 0 ? L : 0);
-            vf[2 * xx] = make_float4(log1pf(dout), log1pf(din), log1pf(ws), Lf / (ws + 1.f));
-            vf[2 * xx + 1] = make_float4((float)miss / (dout + din + 1.f), log1pf(fo), log1pf(fi), log1pf(Lf));
+            atomicAdd(reinterpret_cast<unsigned long long*>(&acc[x].x), (unsigned long long)L);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&acc[x].y), (unsigned long long)ws);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&acc[x].z), (unsigned long long)miss);
         }
+    }
+}
+
+__global__ void k_feature_finish(BmDevGraph g, longlong4* __restrict__ acc, float4* __restrict__ vf) {
+    for (int32_t x = blockIdx.x * blockDim.x + threadIdx.x; x <= g.n; x += gridDim.x * blockDim.x) {
+        if (x == 0) {
+            vf[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+            vf[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        const longlong4 a = acc[x];
+        acc[x] = make_longlong4(0, 0, 0, 0);
+        const int64_t L = a.x, wsum = a.y, miss = a.z;
+        const int32_t fx = g.pi[x];
+        const float dout = (float)(g.m_out_ptr[x + 1] - g.m_out_ptr[x]), din = (float)(g.m_in_ptr[x + 1] - g.m_in_ptr[x]);
+        const float fo = fx ? (float)(g.f_out_ptr[fx + 1] - g.f_out_ptr[fx]) : 0.f;
+        const float fi = fx ? (float)(g.f_in_ptr[fx + 1] - g.f_in_ptr[fx]) : 0.f;
+        const float ws = (float)(wsum > 0 ? wsum : 0);
+        const float Lf = (float)(L > 0 ? L : 0);
+        vf[2 * (int64_t)x] = make_float4(log1pf(dout), log1pf(din), log1pf(ws), Lf / (ws + 1.f));
+        vf[2 * (int64_t)x + 1] = make_float4((float)miss / (dout + din + 1.f), log1pf(fo), log1pf(fi), log1pf(Lf));
     }
 }
 
@@ -743,11 +759,18 @@ MlpDev mlpdev(const BmMlpDev& d) {
 
 int32_t bm_mlp_param_count(void) { return kParams; }
 
-hipError_t bm_launch_vertex_features(const BmDevGraph& g, float* vf, int32_t num_cus, hipStream_t s) {
-    int64_t blocks = ((int64_t)g.n + 1 + 3) / 4;
-    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 8;
+int32_t bm_feature_chunk_edges(void) { return 256; }
+
+hipError_t bm_launch_vertex_features(const BmDevGraph& g, const int4* chunks, int64_t nchunks, void* acc, float* vf,
+                                     int32_t num_cus, hipStream_t s) {
+    const int64_t cap = (int64_t)(num_cus > 0 ? num_cus : 256) * 16;
+    int64_t blocks = (nchunks + 3) / 4;
     if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(k_vertex_features, dim3((unsigned)blocks), dim3(256), 0, s, g, reinterpret_cast<float4*>(vf));
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_feature_chunks, dim3((unsigned)blocks), dim3(256), 0, s, g, chunks, nchunks,
+                       static_cast<longlong4*>(acc));
+    hipLaunchKernelGGL(k_feature_finish, dim3(grid_of((int64_t)g.n + 1, 256)), dim3(256), 0, s, g,
+                       static_cast<longlong4*>(acc), reinterpret_cast<float4*>(vf));
     return hipGetLastError();
 }
 
@@ -822,12 +845,51 @@ hipError_t bm_launch_mlp_targets(const int64_t* delta, int64_t k, float* t, hipS
     return hipGetLastError();
 }
 
+size_t bm_mlp_select_bytes(int64_t pool) {
+    return (size_t)pool * sizeof(float) + 512 + sizeof(SelState) + kSortMax * sizeof(uint64_t);
+}
+
+int64_t bm_mlp_select_max_top(void) { return kSortMax; }
+
+/* Scores of the pool's pairs, radix select of the `top` largest keys and a
+ * one-block sort of them into pairs; scratch holds the scores, the select
+ * state and the selected keys (bm_mlp_select_bytes(pool)), and the state's
+ * counters and histograms are cleared by k_select_init, so the call
+ * allocates nothing and needs no memset. */
+hipError_t bm_launch_mlp_rank_select(const BmMlpDev& d, const float* vf, uint64_t seed, int32_t n, int64_t pool,
+                                     int64_t top, void* scratch, size_t scratch_bytes, int2* out, int32_t num_cus,
+                                     hipStream_t s) {
+    if (pool <= 0 || top <= 0) return hipSuccess;
+    if (top > kSortMax || pool >= (1LL << 32) || scratch_bytes < bm_mlp_select_bytes(pool)) return hipErrorInvalidValue;
+    float* sc = static_cast<float*>(scratch);
+    hipError_t e = bm_launch_mlp_forward(d, vf, nullptr, seed, n, pool, sc, num_cus, s);
+    if (e != hipSuccess) return e;
+    char* q = reinterpret_cast<char*>(sc + pool);
+    q += (256 - (reinterpret_cast<uintptr_t>(q) & 255)) & 255;
+    SelState* st = reinterpret_cast<SelState*>(q);
+    uint64_t* keys = reinterpret_cast<uint64_t*>(st + 1);
+    const int blocks = grid_of(pool, kSelThreads * 8, (int64_t)(num_cus > 0 ? num_cus : 256));
+    hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, top);
+    for (int pass = 0; pass < kSelPasses; pass++)
+        hipLaunchKernelGGL(k_select_pass, dim3(blocks), dim3(kSelThreads), 0, s, sc, pool, st, pass);
+    hipLaunchKernelGGL(k_select_compact, dim3(blocks), dim3(kSelThreads), 0, s, sc, pool, st, keys, (uint32_t)top);
+    int padded = 1;
+    while (padded < top) padded <<= 1;
+    if (padded < 2) padded = 2;
+    e = hipFuncSetAttribute((const void*)k_select_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)((size_t)padded * sizeof(uint64_t)));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_select_sort, dim3(1), dim3(1024), (size_t)padded * sizeof(uint64_t), s, keys, (int32_t)top,
+                       padded, seed, n, out);
+    return hipGetLastError();
+}
+
 size_t bm_mlp_rank_scratch(int64_t pool) {
     size_t temp = 0;
     (void)hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp, (const float*)nullptr, (float*)nullptr,
                                                        (const int32_t*)nullptr, (int32_t*)nullptr, (int)pool);
     const size_t sort_path = temp + 256 + (size_t)pool * (2 * sizeof(float) + 2 * sizeof(int32_t));
-    const size_t select_path = (size_t)pool * sizeof(float) + 512 + sizeof(SelState) + kSortMax * sizeof(uint64_t);
+    const size_t select_path = bm_mlp_select_bytes(pool);
     return sort_path > select_path ? sort_path : select_path;
 }
 
@@ -842,29 +904,11 @@ hipError_t bm_launch_mlp_rank(const BmMlpDev& d, const float* vf, uint64_t seed,
     char* temp = reinterpret_cast<char*>(idx_sorted + pool);
     temp += (256 - (reinterpret_cast<uintptr_t>(temp) & 255)) & 255;
     size_t temp_bytes = scratch_bytes - (size_t)(temp - p);
+    /* radix select of the top keys, then one block sorts them */
+    if (top <= kSortMax && pool < (1LL << 32))
+        return bm_launch_mlp_rank_select(d, vf, seed, n, pool, top, scratch, scratch_bytes, out, num_cus, s);
     hipError_t e = bm_launch_mlp_forward(d, vf, nullptr, seed, n, pool, sc, num_cus, s);
     if (e != hipSuccess) return e;
-    if (top <= kSortMax && pool < (1LL << 32)) {
-        /* radix select of the top keys, then one block sorts them */
-        char* q = reinterpret_cast<char*>(sc + pool);
-        q += (256 - (reinterpret_cast<uintptr_t>(q) & 255)) & 255;
-        SelState* st = reinterpret_cast<SelState*>(q);
-        uint64_t* keys = reinterpret_cast<uint64_t*>(st + 1);
-        const int blocks = grid_of(pool, kSelThreads * 8, (int64_t)(num_cus > 0 ? num_cus : 256));
-        hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, top);
-        for (int pass = 0; pass < kSelPasses; pass++)
-            hipLaunchKernelGGL(k_select_pass, dim3(blocks), dim3(kSelThreads), 0, s, sc, pool, st, pass);
-        hipLaunchKernelGGL(k_select_compact, dim3(blocks), dim3(kSelThreads), 0, s, sc, pool, st, keys, (uint32_t)top);
-        int padded = 1;
-        while (padded < top) padded <<= 1;
-        if (padded < 2) padded = 2;
-        e = hipFuncSetAttribute((const void*)k_select_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)((size_t)padded * sizeof(uint64_t)));
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_select_sort, dim3(1), dim3(1024), (size_t)padded * sizeof(uint64_t), s, keys,
-                           (int32_t)top, padded, seed, n, out);
-        return hipGetLastError();
-    }
     hipLaunchKernelGGL(k_iota, dim3(grid_of(pool, 256)), dim3(256), 0, s, idx, pool);
     e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, sc, sc_sorted, idx, idx_sorted, (int)pool, 0,
                                                      32, s);

@@ -138,6 +138,13 @@ int bm_synth_pair(const bm_synth_params* p, bm_edge_list* male, bm_edge_list* fe
  * distance.  Ties are broken by the smaller vertex id (the reference's qsort
  * leaves them to libc).  pi gets n+1 ints. */
 int bm_greedy_match(int32_t n, const bm_edges* male, const bm_edges* female, int32_t* pi);
+/* The same greedy matching computed on the device from a context's resident
+ * graphs (bm_create's edges): the 2-hop counts with an LDS bitmap per vertex,
+ * a radix sort of both vertex lists, and one 1024-thread block for the greedy
+ * walk (the O(n^2) loop of src/greedy-match.c:495-522).  pi gets n+1 ints;
+ * the context matching is not changed.  BM_ERR_ARG when the two graphs have
+ * different numbers of vertices with an incident edge. */
+int bm_greedy_match_device(bm_ctx* ctx, int32_t* pi);
 
 /* ------------------------------------------------------------------ */
 /* Context: graphs + matching resident in HBM                          */
