@@ -769,6 +769,7 @@ _sig("bm_mlp_gradient", C.c_int, [_c, i32p, _fp, C.c_int64, _fp, _fp])
 _sig("bm_mlp_train", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int32, C.c_float, _fp])
 _sig("bm_mlp_train_batch", C.c_int, [_c, _c, C.c_float, _fp])
 _sig("bm_batch_gen_learned", C.c_int, [_c, _c, C.c_uint64, C.c_int64, C.c_int64])
+_sig("bm_batch_gen_learned_noalloc", C.c_int, [_c, _c, C.c_uint64, C.c_int64, C.c_int64])
 _sig("bm_climb", C.c_int, [_c, C.POINTER(_ClimbOpts), i64p, _dp, C.c_int64, C.POINTER(_SearchResult)])
 
 
@@ -866,7 +867,13 @@ def _batch_gen_learned(self, mlp, seed, pool, top):
     self._chk(_lib.bm_batch_gen_learned(self.h, mlp.h, seed, pool, top))
 
 
+def _batch_gen_learned_noalloc(self, mlp, seed, pool, top):
+    """gen_learned with no device allocation (fixed library scratch)."""
+    self._chk(_lib.bm_batch_gen_learned_noalloc(self.h, mlp.h, seed, pool, top))
+
+
 Batch.gen_learned = _batch_gen_learned
+Batch.gen_learned_noalloc = _batch_gen_learned_noalloc
 
 
 def _ctx_climb(self, proposal, seed=1, time_limit_s=0.0, max_rounds=0, per_round=4096, pool=1 << 20,

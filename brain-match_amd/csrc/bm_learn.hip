@@ -18,8 +18,34 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
+/* Device scratch of bm_batch_gen_learned_noalloc: the pool's scores, the
+ * select state and the selected keys for pools of up to BM_NOALLOC_POOL pairs
+ * (bm_mlp_select_bytes(BM_NOALLOC_POOL) = 4,391,464 bytes fit).  Each device
+ * holds its own copy for the life of the process; host code uses the address
+ * hipGetSymbolAddress returns for the device (g_scratch_dev), never the
+ * array's name.  Every call on a device shares that device's copy. */
+#define BM_NOALLOC_SCRATCH_BYTES ((4u << 20) + (256u << 10))
+static __device__ __align__(256) unsigned char scratch[BM_NOALLOC_SCRATCH_BYTES];
+
+static constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+/* Device address of the scratch array on device dev (current), taken once
+ * per device. */
+static hipError_t scratch_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[dev]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[dev] = ptr;
+    return hipSuccess;
+}
 
 struct bm_mlp {
     bm_ctx* ctx;
@@ -169,6 +195,8 @@ extern "C" int bm_mlp_create(bm_ctx* ctx, uint64_t seed, bm_mlp** out) {
             rc = bm_ctx_set_error(ctx, BM_ERR_NOMEM, "bm_mlp_create: device allocation failed");
     };
     if (hipSetDevice(m->device) != hipSuccess) rc = bm_ctx_set_error(ctx, BM_ERR_HIP, "bm_mlp_create: hipSetDevice");
+    if (rc == BM_OK && scratch_setup(m->device) != hipSuccess)
+        rc = bm_ctx_set_error(ctx, BM_ERR_HIP, "bm_mlp_create: hipGetSymbolAddress of the ranking scratch failed");
     alloc((void**)&m->d.params, np * sizeof(float));
     alloc((void**)&m->d.grad, np * sizeof(float));
     alloc((void**)&m->d.adam_m, np * sizeof(float));
@@ -373,5 +401,25 @@ extern "C" int bm_mlp_train_batch(bm_mlp* m, bm_batch* b, float lr, float* mse) 
         MHIP(m, hipStreamSynchronize(m->stream));
         *mse = se / (float)k;
     }
+    return BM_OK;
+}
+
+extern "C" int bm_batch_gen_learned_noalloc(bm_batch* b, bm_mlp* m, uint64_t seed, int64_t pool, int64_t top) {
+    if (!b || !m) return BM_ERR_ARG;
+    if (pool < 1 || top < 1 || top > pool || pool > BM_NOALLOC_POOL || top > bm_mlp_select_max_top() ||
+        top > bm_batch_capacity(b))
+        return mfail(m, BM_ERR_ARG,
+                     "bm_batch_gen_learned_noalloc: need 1 <= top <= pool, pool <= 2^20, top <= 8192 and top <= "
+                     "batch capacity");
+    if (!m->features_ready)
+        return mfail(m, BM_ERR_STATE, "bm_batch_gen_learned_noalloc: call bm_mlp_refresh_features first");
+    if (bm_mlp_select_bytes(pool) > (size_t)(BM_NOALLOC_SCRATCH_BYTES))
+        return mfail(m, BM_ERR_ARG, "bm_batch_gen_learned_noalloc: pool exceeds the scratch");
+    void* ptr = (m->device >= 0 && m->device < kMaxDevices) ? g_scratch_dev[m->device] : nullptr;
+    if (!ptr) return mfail(m, BM_ERR_STATE, "bm_batch_gen_learned_noalloc: no scratch address for this device");
+    MHIP(m, hipSetDevice(m->device));
+    MHIP(m, bm_launch_mlp_rank_select(m->d, m->d_vf, seed, m->n, pool, top, ptr, (size_t)(BM_NOALLOC_SCRATCH_BYTES),
+                                      bm_batch_pairs_dev(b), m->num_cus, m->stream));
+    bm_batch_mark_pairs(b, top);
     return BM_OK;
 }

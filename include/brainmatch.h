@@ -508,6 +508,17 @@ int bm_mlp_train_batch(bm_mlp* m, bm_batch* b, float lr, float* mse);
  * batch as a pair list (descending score). */
 int bm_batch_gen_learned(bm_batch* b, bm_mlp* m, uint64_t seed, int64_t pool, int64_t top);
 
+/* bm_batch_gen_learned without any device allocation: the pool's scores
+ * and the select state live in a fixed device array of the library (one per
+ * device, 4.25 MiB, held until the process ends; its address is taken in
+ * bm_mlp_create), so the call can run inside HIP graph capture or a serving
+ * loop that must not allocate.  Limits: pool <= BM_NOALLOC_POOL, top <= 8192
+ * (BM_ERR_ARG beyond).  The same pairs as bm_batch_gen_learned.  Calls that
+ * overlap on one device race on the shared array.  No counterpart in the
+ * reference: it is for callers that cannot allocate. */
+#define BM_NOALLOC_POOL (1 << 20)
+int bm_batch_gen_learned_noalloc(bm_batch* b, bm_mlp* m, uint64_t seed, int64_t pool, int64_t top);
+
 #define BM_PROPOSAL_UNIFORM 0
 #define BM_PROPOSAL_TWOHOP 1
 #define BM_PROPOSAL_LEARNED 2
