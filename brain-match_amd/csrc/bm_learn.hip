@@ -23,7 +23,7 @@
 
 /* Device scratch of bm_batch_gen_learned_noalloc: the pool's scores, the
  * select state and the selected keys for pools of up to BM_NOALLOC_POOL pairs
- * (bm_mlp_select_bytes(BM_NOALLOC_POOL) = 4,391,464 bytes fit).  Each device
+ * (bm_mlp_select_bytes(BM_NOALLOC_POOL) = 4,325,928 bytes fit).  Each device
  * holds its own copy for the life of the process; host code uses the address
  * hipGetSymbolAddress returns for the device (g_scratch_dev), never the
  * array's name.  Every call on a device shares that device's copy. */

@@ -549,7 +549,8 @@ __device__ __forceinline__ uint64_t rank_key(float x, int64_t p) {
 }
 
 constexpr int kSelBits = 11, kSelBins = 1 << kSelBits, kSelThreads = 256, kSelPasses = 6;
-constexpr int kSelCopies = 16;       /* copies of the global histogram (block b adds to copy b % 16) */
+constexpr int kSelPer = 16;          /* consecutive keys per lane, loaded before any is used */
+constexpr int kSelCopies = 8;        /* copies of the global histogram (block b adds to copy b % 8) */
 constexpr int kSortMax = 8192;       /* largest top sorted in one block */
 
 struct SelState {
@@ -616,17 +617,33 @@ __global__ void __launch_bounds__(kSelThreads) k_select_pass(const float* __rest
     const int lane = threadIdx.x & (BM_WAVE - 1);
     for (int i = threadIdx.x; i < bins; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    /* every lane of a wave runs the same iterations (the ballots need them all) */
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(BM_WAVE - 1)); base < P; base += stride) {
-        const int64_t p = base + lane;
-        uint64_t k = 0;
-        bool valid = false;
-        if (p < P) {
-            k = rank_key(sc[p], p);
-            valid = (k & fixed) == prefix;
+    /* a wave takes 64 kSelPer consecutive keys, kSelPer per lane, loaded
+     * before any is used; every lane of a wave runs the same iterations (the
+     * ballots need them all) */
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / BM_WAVE) * BM_WAVE * kSelPer;
+    for (int64_t w0 = ((int64_t)blockIdx.x * (blockDim.x / BM_WAVE) + threadIdx.x / BM_WAVE) * BM_WAVE * kSelPer;
+         w0 < P; w0 += wstride) {
+        const int64_t p0 = w0 + (int64_t)lane * kSelPer;
+        float v[kSelPer];
+        if (p0 + kSelPer <= P) {
+#pragma unroll
+            for (int j = 0; j < kSelPer; j += 4) {
+                const float4 f = *reinterpret_cast<const float4*>(&sc[p0 + j]);
+                v[j] = f.x;
+                v[j + 1] = f.y;
+                v[j + 2] = f.z;
+                v[j + 3] = f.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kSelPer; j++) v[j] = p0 + j < P ? sc[p0 + j] : 0.f;
         }
-        wave_hist_add(h, valid, (uint32_t)(k >> shift) & (uint32_t)(bins - 1), lane);
+#pragma unroll
+        for (int j = 0; j < kSelPer; j++) {
+            const uint64_t k = rank_key(v[j], p0 + j);
+            const bool valid = p0 + j < P && (k & fixed) == prefix;
+            wave_hist_add(h, valid, (uint32_t)(k >> shift) & (uint32_t)(bins - 1), lane);
+        }
     }
     __syncthreads();
     uint32_t* gh = st->hist[blockIdx.x % kSelCopies];
@@ -685,23 +702,37 @@ __global__ void k_select_compact(const float* __restrict__ sc, int64_t P, SelSta
                                  uint32_t top) {
     const uint64_t kstar = st->prefix, fixed = st->fixed;
     const int lane = threadIdx.x & (BM_WAVE - 1);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(BM_WAVE - 1)); base < P; base += stride) {
-        const int64_t p = base + lane;
-        uint64_t k = 0;
-        bool in = false;
-        if (p < P) {
-            k = rank_key(sc[p], p);
-            in = (k & fixed) >= kstar;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / BM_WAVE) * BM_WAVE * kSelPer;
+    for (int64_t w0 = ((int64_t)blockIdx.x * (blockDim.x / BM_WAVE) + threadIdx.x / BM_WAVE) * BM_WAVE * kSelPer;
+         w0 < P; w0 += wstride) {
+        const int64_t p0 = w0 + (int64_t)lane * kSelPer;
+        float v[kSelPer];
+        if (p0 + kSelPer <= P) {
+#pragma unroll
+            for (int j = 0; j < kSelPer; j += 4) {
+                const float4 f = *reinterpret_cast<const float4*>(&sc[p0 + j]);
+                v[j] = f.x;
+                v[j + 1] = f.y;
+                v[j + 2] = f.z;
+                v[j + 3] = f.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kSelPer; j++) v[j] = p0 + j < P ? sc[p0 + j] : 0.f;
         }
-        const uint64_t m = __ballot(in);
-        if (!m) continue;
-        uint32_t at0 = 0;
-        if (lane == 0) at0 = atomicAdd(&st->count, (uint32_t)__popcll(m));
-        at0 = (uint32_t)__shfl((int)at0, 0, BM_WAVE);
-        if (in) {
-            const uint32_t at = at0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (at < top) out[at] = k;
+#pragma unroll
+        for (int j = 0; j < kSelPer; j++) {
+            const uint64_t k = rank_key(v[j], p0 + j);
+            const bool in = p0 + j < P && (k & fixed) >= kstar;
+            const uint64_t m = __ballot(in);
+            if (!m) continue;
+            uint32_t at0 = 0;
+            if (lane == 0) at0 = atomicAdd(&st->count, (uint32_t)__popcll(m));
+            at0 = (uint32_t)__shfl((int)at0, 0, BM_WAVE);
+            if (in) {
+                const uint32_t at = at0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (at < top) out[at] = k;
+            }
         }
     }
 }
@@ -868,7 +899,7 @@ hipError_t bm_launch_mlp_rank_select(const BmMlpDev& d, const float* vf, uint64_
     q += (256 - (reinterpret_cast<uintptr_t>(q) & 255)) & 255;
     SelState* st = reinterpret_cast<SelState*>(q);
     uint64_t* keys = reinterpret_cast<uint64_t*>(st + 1);
-    const int blocks = grid_of(pool, kSelThreads * 8, (int64_t)(num_cus > 0 ? num_cus : 256));
+    const int blocks = grid_of(pool, kSelThreads * kSelPer, (int64_t)(num_cus > 0 ? num_cus : 256) * 4);
     hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, top);
     for (int pass = 0; pass < kSelPasses; pass++)
         hipLaunchKernelGGL(k_select_pass, dim3(blocks), dim3(kSelThreads), 0, s, sc, pool, st, pass);
