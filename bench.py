@@ -656,6 +656,9 @@ def anneal_for(bm, ctx, pi0, seconds, chains, p_pool, mlp=None, refresh_s=2.0, s
         launches += 1
         if launches % 20 == 0:
             traj.append((time.perf_counter() - t0, int(ann.state()["best"].max())))
+            if len(traj) % 25 == 0:
+                log(f"  anneal ({'learned pool' if mlp is not None else 'uniform'}): {traj[-1][0]:.0f}s, "
+                    f"best {traj[-1][1]}")
     el = time.perf_counter() - t0
     st = ann.state()
     traj.append((el, int(st["best"].max())))
