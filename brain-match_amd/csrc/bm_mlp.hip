@@ -657,16 +657,43 @@ __global__ void __launch_bounds__(kSelThreads) k_select_pass(const float* __rest
     __threadfence();
     /* every thread reads the rank before the barriers below; one thread rewrites it */
     const int64_t rem = st->remaining;
-    /* thread t owns bins [t * per, (t + 1) * per); suffix sums over threads from the top */
+    /* thread t owns bins [t * per, (t + 1) * per) (per = 8 or 2); the
+     * device-scope fence above makes the other blocks' counts visible to
+     * plain loads, all issued before any is summed */
     const int per = bins / kSelThreads;
     const int t = threadIdx.x;
     uint32_t mine = 0;
-    for (int j = 0; j < per; j++) {
-        uint32_t c = 0;
-        for (int cp = 0; cp < kSelCopies; cp++)
-            c += __hip_atomic_load(&st->hist[cp][t * per + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        h[t * per + j] = c;
-        mine += c;
+    if (per == 8) {
+        uint4 a[kSelCopies][2];
+#pragma unroll
+        for (int cp = 0; cp < kSelCopies; cp++) {
+            a[cp][0] = *reinterpret_cast<const uint4*>(&st->hist[cp][t * 8]);
+            a[cp][1] = *reinterpret_cast<const uint4*>(&st->hist[cp][t * 8 + 4]);
+        }
+        uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int cp = 0; cp < kSelCopies; cp++) {
+            c[0] += a[cp][0].x;
+            c[1] += a[cp][0].y;
+            c[2] += a[cp][0].z;
+            c[3] += a[cp][0].w;
+            c[4] += a[cp][1].x;
+            c[5] += a[cp][1].y;
+            c[6] += a[cp][1].z;
+            c[7] += a[cp][1].w;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            h[t * 8 + j] = c[j];
+            mine += c[j];
+        }
+    } else {
+        for (int j = 0; j < per; j++) {
+            uint32_t c = 0;
+            for (int cp = 0; cp < kSelCopies; cp++) c += st->hist[cp][t * per + j];
+            h[t * per + j] = c;
+            mine += c;
+        }
     }
     part[t] = mine;
     __syncthreads();
@@ -693,19 +720,22 @@ __global__ void __launch_bounds__(kSelThreads) k_select_pass(const float* __rest
     }
     if (t == 0) st->done = 0;
     for (int cp = 0; cp < kSelCopies; cp++)
-        for (int i = t; i < bins; i += blockDim.x) st->hist[cp][i] = 0;
+        for (int i = 4 * t; i < bins; i += 4 * blockDim.x) *reinterpret_cast<uint4*>(&st->hist[cp][i]) = make_uint4(0, 0, 0, 0);
 }
 
 /* Keys at or above the selected prefix under its mask (exactly `top` of
- * them; the bound only guards the buffer).  One global atomic per wave. */
-__global__ void k_select_compact(const float* __restrict__ sc, int64_t P, SelState* st, uint64_t* __restrict__ out,
-                                 uint32_t top) {
+ * them; the bound only guards the buffer).  Each wave ballots its lanes'
+ * keys, the block scans its waves' counts in LDS, and one global atomic per
+ * block reserves the block's slots. */
+__global__ void __launch_bounds__(kSelThreads) k_select_compact(const float* __restrict__ sc, int64_t P, SelState* st,
+                                                                uint64_t* __restrict__ out, uint32_t top) {
+    __shared__ uint32_t wcount[kSelThreads / BM_WAVE];
+    __shared__ uint32_t base;
     const uint64_t kstar = st->prefix, fixed = st->fixed;
-    const int lane = threadIdx.x & (BM_WAVE - 1);
-    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x / BM_WAVE) * BM_WAVE * kSelPer;
-    for (int64_t w0 = ((int64_t)blockIdx.x * (blockDim.x / BM_WAVE) + threadIdx.x / BM_WAVE) * BM_WAVE * kSelPer;
-         w0 < P; w0 += wstride) {
-        const int64_t p0 = w0 + (int64_t)lane * kSelPer;
+    const int lane = threadIdx.x & (BM_WAVE - 1), wave = threadIdx.x / BM_WAVE;
+    const int64_t bstride = (int64_t)gridDim.x * blockDim.x * kSelPer;
+    for (int64_t b0 = (int64_t)blockIdx.x * blockDim.x * kSelPer; b0 < P; b0 += bstride) {
+        const int64_t p0 = b0 + ((int64_t)wave * BM_WAVE + lane) * kSelPer;
         float v[kSelPer];
         if (p0 + kSelPer <= P) {
 #pragma unroll
@@ -720,20 +750,40 @@ __global__ void k_select_compact(const float* __restrict__ sc, int64_t P, SelSta
 #pragma unroll
             for (int j = 0; j < kSelPer; j++) v[j] = p0 + j < P ? sc[p0 + j] : 0.f;
         }
+        uint32_t inmask = 0;   /* bit j: this lane's key j is in */
 #pragma unroll
         for (int j = 0; j < kSelPer; j++) {
             const uint64_t k = rank_key(v[j], p0 + j);
-            const bool in = p0 + j < P && (k & fixed) >= kstar;
-            const uint64_t m = __ballot(in);
-            if (!m) continue;
-            uint32_t at0 = 0;
-            if (lane == 0) at0 = atomicAdd(&st->count, (uint32_t)__popcll(m));
-            at0 = (uint32_t)__shfl((int)at0, 0, BM_WAVE);
-            if (in) {
-                const uint32_t at = at0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                if (at < top) out[at] = k;
-            }
+            if (p0 + j < P && (k & fixed) >= kstar) inmask |= 1u << j;
         }
+        uint32_t mine = (uint32_t)__popc(inmask);
+        /* exclusive prefix of the lanes' counts within the wave */
+        uint32_t incl = mine;
+#pragma unroll
+        for (int o = 1; o < BM_WAVE; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, o, BM_WAVE);
+            if (lane >= o) incl += y;
+        }
+        if (lane == BM_WAVE - 1) wcount[wave] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int w = 0; w < kSelThreads / BM_WAVE; w++) {
+                const uint32_t c = wcount[w];
+                wcount[w] = tot;
+                tot += c;
+            }
+            base = tot ? atomicAdd(&st->count, tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t at = base + wcount[wave] + incl - mine;
+#pragma unroll
+        for (int j = 0; j < kSelPer; j++)
+            if (inmask & (1u << j)) {
+                if (at < top) out[at] = rank_key(v[j], p0 + j);
+                at++;
+            }
+        __syncthreads();
     }
 }
 
