@@ -452,8 +452,10 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
          * before the barrier that publishes the row */
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         __syncthreads();
+        /* the finished row streams to HBM without taking L2 lines from the
+         * female rows that later rows re-read */
         int32_t* out = G.G + (int64_t)u * G.stride + col0;
-        for (int32_t c = tid; c < ncols; c += kBuildThreads) out[c] = row[c];
+        for (int32_t c = tid; c < ncols; c += kBuildThreads) __builtin_nontemporal_store(row[c], &out[c]);
         __syncthreads();
     }
 }

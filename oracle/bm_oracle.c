@@ -56,17 +56,28 @@
  *                               src/re-alignment-parallel.c:601, used as the
  *                               timed CPU baseline ("port").
  *  or_gain_row                  a placement-gain table row from its
- *                               definition.  Pinned: G(u, pi(v)) sums give
- *                               the exact swap delta (tests).
+ *                               definition.  Pinned by full recounts: moving
+ *                               u alone onto f changes the score by
+ *                               G(u, f) - G(u, pi(u)) plus its self-loop
+ *                               term, for every u and f of the quirky graph;
+ *                               the diagonal sums to twice the score without
+ *                               self-loops; every swap delta equals four
+ *                               entries plus the corrections (tests).
  *  or_det_exp                   deterministic exp (Cody & Waite reduction,
  *                               Taylor polynomial).  Pinned against libm exp
  *                               to 2 ulp over [-700, 700].
  *  or_anneal_chain              replay of an annealing chain: the Metropolis
  *                               rule and geometric schedule of
  *                               simulated_annealing (src/best-swap/score-lookup.c:345-400)
- *                               over exact deltas.  Pinned by the score
+ *                               over exact deltas, with moves optionally built
+ *                               on pairs of a proposal pool.  Pinned by the score
  *                               invariant (running score == or_score of the
- *                               final matching) and a brute-force step check.
+ *                               final matching), a brute-force step check, and
+ *                               for pools: a one-pair pool moves only that
+ *                               pair, p_pool = 0 gives the uniform chain.
+ *  or_third_vertex              third vertex of a move on a pool pair;
+ *                               pinned by distinctness and uniformity over
+ *                               the other n - 2 vertices.
  *  or_twohop_set / _pairs       collect_neighborhood's 2-hop set
  *                               (src/refine-twohop-random.c:582-625) and the
  *                               K-smallest-key sample of it.  Pinned by set
