@@ -31,10 +31,16 @@ value = candidates / second over the K timed steps of the headline strategy
 every step).  roofline = the step's dominant kernel: algorithmic bytes per
 launch / mean launch time from HIP events on the library stream; traffic = HBM
 bytes per launch measured with rocprofv3 (profiles/swap_n18524_traffic.json).
+lds_roofline = the same kernel against the LDS atomic-add rate it is bound
+by (one add per male edge x female row entry expansion).
 cpu_baseline = the reference's own calculate_swap_delta
 (src/graph-alignment-optimized.c:321-408, compiled from its sources into
 oracle/_ref) over an evenly strided sample of the last timed stream, OpenMP
 dynamic schedule, on this host; its deltas are compared with the device's.
+proposals = the second half of the metric: from the pair's greedy matching,
+disjoint-swap climbs with uniform, 2-hop and learned proposals run for the
+same wall clock (--proposal-seconds, 60 s each; 0 leaves it out) and their
+exact scores are compared.
 """
 import argparse
 import json
@@ -48,6 +54,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "brain-match_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# LDS: 32 banks of 4 bytes per CU (128 bytes per clock), so at most 32 four-byte
+# atomic adds per CU per clock; 256 CUs at the 2.4 GHz peak engine clock
+LDS_ADD_PEAK_PER_S = 32 * 256 * 2.4e9
 N_VERTICES = 18524
 N_EDGES = 4_000_000
 
@@ -270,12 +279,20 @@ def main():
                                "expanded_entries": entries, "female_entry_bytes": entry_bytes},
             "lookup_roofline": {"bound": "hbm", "achieved": lb / look_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": lb / look_s / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": lb},
+            # the build's work is one LDS add per (male edge, female row entry) expansion
+            "build_lds_roofline": {"bound": "lds", "achieved": entries / build_s, "peak": LDS_ADD_PEAK_PER_S,
+                                   "unit": "atomic adds/s", "frac": entries / build_s / LDS_ADD_PEAK_PER_S,
+                                   "adds_per_launch": entries,
+                                   "peak_basis": "32 four-byte LDS atomics per CU per clock x 256 CUs x 2.4 GHz"},
         }
         # the per-candidate byte model of a list scan (4 male lists, 28 B per
         # incident edge): what the same stream would cost a kernel that reads
         # every candidate's lists, and the rate such a kernel reaches at HBM peak
         pcb, pcd = stream_bytes(seeds[-1], K)
         gain["per_candidate_model"] = {
+            "applies_to_this_strategy": False,
+            "why": "the table strategy reads four table entries per pair, not the pair's male lists; the model "
+                   "prices a list scan and applies to the scan kernel (scan_kernel.roofline)",
             "bytes_per_step": pcb, "mean_D": pcd / K,
             "deltas_per_s_at_hbm_peak": K * HBM_PEAK_GBS * 1e9 / pcb,
             "equivalent_GBps": pcb / (wall / args.steps) / 1e9,
@@ -327,9 +344,12 @@ def main():
 
     traffic = measured_traffic(args)
 
+    lds_rf = None
     if args.headline == "gain":
         dom = "lookup" if gain["lookup_ms"] >= gain["build_ms"] else "build"
         rf = dict(gain[dom + "_roofline"])
+        if dom == "build":
+            lds_rf = dict(gain["build_lds_roofline"], kernel="k_gain_build")
         rf["kernel"] = ("k_gain_swap_umajor" if umajor else "k_gain_swap<uniform>") if dom == "lookup" \
             else "k_gain_edge_desc+k_gain_build"
     else:
@@ -364,6 +384,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": rf,
+        "lds_roofline": lds_rf,
         "gain_table": gain,
         "scan_kernel": scan,
         "delta_stats_last_step": {k: stats[k] for k in ("sum", "max", "n_pos", "n_zero")},
