@@ -229,14 +229,26 @@ def main():
     male, female, truth, pi_shuffled = bm.synth_pair(N_VERTICES, N_EDGES / N_VERTICES, bm.WEIGHTS_LOGNORMAL, 0.3,
                                                      seed=1)
     log(f"synthetic pair: n={N_VERTICES}, male edges {male.m}, female edges {female.m} ({time.time() - t:.1f}s)")
+    greedy = None
     if args.pi0 == "greedy":
         t = time.time()
         pi0 = bm.greedy_match(N_VERTICES, male, female)
-        log(f"greedy initial matching ({time.time() - t:.1f}s)")
+        host_s = time.time() - t
+        log(f"greedy initial matching ({host_s:.1f}s)")
     else:
         pi0 = pi_shuffled
 
     ctx = bm.Context(N_VERTICES, male, female)
+    if args.pi0 == "greedy":
+        ctx.greedy_match()                     # warm-up (kernel loads)
+        ctx.sync()
+        t = time.perf_counter()
+        pi_dev = ctx.greedy_match()
+        dev_s = time.perf_counter() - t
+        greedy = {"host_seconds": host_s, "device_seconds": dev_s, "identical": bool(np.array_equal(pi_dev, pi0))}
+        log(f"greedy matching on the device: {1000 * dev_s:.1f} ms, identical to the host's: {greedy['identical']}")
+        if not greedy["identical"]:
+            raise SystemExit("device and host greedy matchings differ")
     ctx.set_mapping(pi0)
     score0 = ctx.score()
     log(f"device {ctx.device_info()}; initial score {score0}")
@@ -388,6 +400,7 @@ def main():
         "gain_table": gain,
         "scan_kernel": scan,
         "delta_stats_last_step": {k: stats[k] for k in ("sum", "max", "n_pos", "n_zero")},
+        "greedy_initial_matching": greedy,
     }
 
     if not args.no_rot3:
