@@ -416,8 +416,14 @@ extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm
         free_ctx(c);
         return fail(nullptr, BM_ERR_NOMEM, "bm_create: out of host memory building female tables");
     }
-    /* Compact LDS slots need keys below 2^17 and female weights in 0..32767. */
+    /* Compact LDS slots need keys below 2^17 and female weights in 0..32767;
+     * compact kernels also sum a lane's contributions in 32 bits, which needs
+     * male weights within +-32767 (every contribution is then within
+     * +-32767, and a lane takes at most 2^12 + 8 edges of a list shorter
+     * than 2^18). */
     c->compact = n < (1 << 17);
+    for (int64_t e = 0; e < c->male.m && c->compact; e++)
+        if (c->male.out.adj[e].y < -32767 || c->male.out.adj[e].y > 32767) c->compact = 0;
     for (int32_t r = 1; r <= n; r++) {
         c->f_out_deg[r] = (int32_t)(fg.out.ptr[r + 1] - fg.out.ptr[r]);
         c->f_in_deg[r] = (int32_t)(fg.in.ptr[r + 1] - fg.in.ptr[r]);

@@ -23,6 +23,8 @@
  * src/graph-re-alignment-gpu-3-swap.cu:186-248); here the work is
  * O(deg) per candidate and exact for any int32 weights.
  */
+#include <type_traits>
+
 #include "bm_detexp.h"
 #include "bm_device.h"
 #include "bm_devutil.h"
@@ -47,9 +49,8 @@ struct Tab {
 __device__ __forceinline__ uint32_t lds_bits_for(int64_t deg) {
     if (deg <= 0) return 0;
     const int64_t need = (deg * 4 + 2) / 3;   /* ceil(4/3 deg) slots */
-    uint32_t b = 2;
-    while (((int64_t)1 << b) < need) b++;
-    return b;
+    /* the smallest b >= 2 with 2^b >= need */
+    return need > 4 ? 64u - (uint32_t)__clzll((unsigned long long)(need - 1)) : 2u;
 }
 
 __device__ __forceinline__ Tab make_tab(const uint64_t* __restrict__ desc, const int2* __restrict__ gtab,
@@ -371,13 +372,17 @@ __device__ __forceinline__ void build(typename Slot<COMPACT>::T* lds, const Tab*
  * A = female row of f1 = pi(u), B = row of f2 = pi(v).
  * Out-phase: an edge to u or v uses that endpoint's post-swap female (this
  * covers u<->v edges and self-loops).  In-phase: edges from u or v were
- * counted by the out-phase and are skipped. */
+ * counted by the out-phase and are skipped.  Compact graphs sum a lane's
+ * contributions in 32 bits: each is within +-65534 and a lane takes at most
+ * 2^12 + U edges (deduplicated lists of a graph with n < 2^17 hold fewer
+ * than 2^18 entries together). */
 template <bool COMPACT, bool OUTP, int U>
 __device__ __forceinline__ int64_t swap_scan(const int2* __restrict__ adj, const int32_t* __restrict__ pi,
                                              int64_t b0, int64_t L0, int64_t b1, int64_t L1,
                                              const typename Slot<COMPACT>::T* lds, const Tab& A, const Tab& B,
                                              int32_t u, int32_t v, int32_t f1, int32_t f2, int lane) {
-    int64_t acc = 0;
+    typedef typename std::conditional<COMPACT, int32_t, int64_t>::type Acc;
+    Acc acc = 0;
     /* list positions in 32 bits (a list is shorter than 2^31) */
     const int32_t M0 = (int32_t)L0, M = (int32_t)(L0 + L1);
     for (int32_t base = 0; base < M; base += BM_WAVE * U) {
@@ -414,10 +419,10 @@ __device__ __forceinline__ int64_t swap_scan(const int2* __restrict__ adj, const
             const int32_t w = ed[j].y;
             const int32_t cA = contrib(w, rA[j]);
             const int32_t cB = contrib(w, rB[j]);
-            acc += side ? (int64_t)cA - cB : (int64_t)cB - cA;
+            acc += side ? (Acc)cA - (Acc)cB : (Acc)cB - (Acc)cA;
         }
     }
-    return acc;
+    return (int64_t)acc;
 }
 
 /* Exact delta of swapping the images of u != v (f1 = pi(u), f2 = pi(v)) in
