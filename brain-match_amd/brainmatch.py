@@ -141,6 +141,7 @@ _sig("bm_search", C.c_int, [_c, C.POINTER(_SearchOpts), i32p, C.POINTER(_SearchR
 _sig("bm_hillclimb", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int64, C.c_int32, i64p, C.POINTER(_SearchResult)])
 _sig("bm_gain_enable", C.c_int, [_c, C.c_int])
 _sig("bm_gain_enabled", C.c_int, [_c])
+_sig("bm_gain_dense_rows", C.c_int, [_c])
 _sig("bm_gain_rebuild", C.c_int, [_c])
 _sig("bm_batch_gen_amajor", C.c_int, [_c, C.c_uint64, C.c_int64])
 _sig("bm_batch_eval_rot3_amajor", C.c_int, [_c, C.c_uint64, C.c_int64])
@@ -543,6 +544,10 @@ class Context:
 
     def gain_enabled(self):
         return bool(_lib.bm_gain_enabled(self.h))
+
+    def gain_dense_rows(self):
+        """Female rows the table build reads as dense 16-bit column arrays."""
+        return int(_lib.bm_gain_dense_rows(self.h))
 
     def gain_rebuild(self):
         self._chk(_lib.bm_gain_rebuild(self.h))

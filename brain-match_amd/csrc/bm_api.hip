@@ -90,6 +90,13 @@ struct bm_ctx {
     int32_t* d_f_in_bnd;    /* column-window boundaries of the female rows (windowed builds only) */
     int32_t* d_f_out_bnd;
     int32_t nwin;
+    uint32_t* d_dense;      /* dense copies of the longest female rows (BmGain::dense) */
+    uint32_t* d_dense_w;    /* dense tile weights (BmGain::dense_w) */
+    int32_t* d_f_out_dense; /* female vertex -> its dense out-row, or -1 */
+    int32_t* d_f_in_dense;
+    int32_t ndense;
+    int32_t male_nonneg;    /* every male weight is >= 0 */
+    int32_t pi_distinct;    /* no two vertices share a nonzero image (moves keep this; bm_set_mapping checks it) */
     int64_t n_echunks;
     int64_t gt_version;
     int4* d_best;           /* best-swap result, then kBestSwapParts per-block results */
@@ -261,6 +268,10 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_echunks);
     (void)hipFree(c->d_f_in_bnd);
     (void)hipFree(c->d_f_out_bnd);
+    (void)hipFree(c->d_dense);
+    (void)hipFree(c->d_dense_w);
+    (void)hipFree(c->d_f_out_dense);
+    (void)hipFree(c->d_f_in_dense);
     (void)hipFree(c->d_m_adj);
     (void)hipFree(c->d_best);
     bm_graph_free(&c->male);
@@ -291,6 +302,14 @@ static BmGain devgain(const bm_ctx* c) {
     G.m_adj = c->d_m_adj;
     G.m_adj_words = c->m_adj_words;
     G.build_unroll = c->tune[BM_KERNEL_GAIN_BUILD].unroll;
+    /* a male row lists each dense female row at most once per side only
+     * while images are distinct, which sizes the build's dense-edge list */
+    const bool dense = c->ndense > 0 && c->pi_distinct;
+    G.dense = dense ? c->d_dense : nullptr;
+    G.f_out_dense = c->d_f_out_dense;
+    G.f_in_dense = c->d_f_in_dense;
+    G.ndense = dense ? c->ndense : 0;
+    G.dense_w = c->d_dense_w;
     return G;
 }
 
@@ -356,7 +375,7 @@ static void default_tuning(bm_ctx* c) {
     c->tune[BM_KERNEL_SWAP] = BmLaunch{8, s, 8, cus, 2};
     c->tune[BM_KERNEL_ROT3] = BmLaunch{8, 2 * s, 8, cus, 2};
     c->tune[BM_KERNEL_SCORE] = BmLaunch{4, s, 8, cus, 8};
-    c->tune[BM_KERNEL_GAIN_BUILD] = BmLaunch{4, s, 8, cus, 8};   /* only unroll is used */
+    c->tune[BM_KERNEL_GAIN_BUILD] = BmLaunch{4, s, 8, cus, 4};   /* only unroll is used */
     for (int i = 0; i < BM_KERNEL_COUNT; i++) {
         c->group_lanes[i] = 64;
         c->user_tuned[i] = 0;
@@ -424,6 +443,9 @@ extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm
     c->compact = n < (1 << 17);
     for (int64_t e = 0; e < c->male.m && c->compact; e++)
         if (c->male.out.adj[e].y < -32767 || c->male.out.adj[e].y > 32767) c->compact = 0;
+    c->male_nonneg = 1;
+    for (int64_t e = 0; e < c->male.m; e++)
+        if (c->male.out.adj[e].y < 0) c->male_nonneg = 0;
     for (int32_t r = 1; r <= n; r++) {
         c->f_out_deg[r] = (int32_t)(fg.out.ptr[r + 1] - fg.out.ptr[r]);
         c->f_in_deg[r] = (int32_t)(fg.in.ptr[r + 1] - fg.in.ptr[r]);
@@ -483,6 +505,7 @@ extern "C" int bm_create(bm_ctx** out, int32_t n, const bm_edges* male, const bm
     if (rc == BM_OK) rc = h2d(c, c->d_f_out, fg.out.adj, (size_t)fg.m);
     if (rc == BM_OK) rc = h2d(c, c->d_f_in_ptr, fg.in.ptr, (size_t)n + 2);
     if (rc == BM_OK) rc = h2d(c, c->d_f_in, fg.in.adj, (size_t)fg.m);
+    c->pi_distinct = 1;   /* all unmatched */
     if (rc == BM_OK && hipMemsetAsync(c->d_pi, 0, ((size_t)n + 1) * sizeof(int32_t), c->stream) != hipSuccess)
         rc = fail(c, BM_ERR_HIP, "hipMemsetAsync failed");
     if (rc == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, BM_ERR_HIP, "hipStreamSynchronize failed");
@@ -518,6 +541,14 @@ extern "C" int bm_set_mapping(bm_ctx* c, const int32_t* pi) {
     for (int32_t i = 1; i <= c->n; i++)
         if (pi[i] < 0 || pi[i] > c->n)
             return fail(c, BM_ERR_ARG, "bm_set_mapping: pi[%d] = %d outside 0..%d", i, pi[i], c->n);
+    {
+        std::vector<uint8_t> seen((size_t)c->n + 1, 0);
+        c->pi_distinct = 1;
+        for (int32_t i = 1; i <= c->n && c->pi_distinct; i++) {
+            if (pi[i] != 0 && seen[pi[i]]) c->pi_distinct = 0;
+            seen[pi[i]] = 1;
+        }
+    }
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(c->d_pi + 1, pi + 1, (size_t)c->n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
     if (c->gain_on) {
@@ -1215,6 +1246,8 @@ extern "C" int64_t bm_gain_table_bytes(int32_t n) {
 
 extern "C" int bm_gain_enabled(const bm_ctx* c) { return c ? c->gain_on : 0; }
 
+extern "C" int bm_gain_dense_rows(const bm_ctx* c) { return c && c->gain_on && c->pi_distinct ? c->ndense : 0; }
+
 extern "C" int bm_gain_enable(bm_ctx* c, int on) {
     if (!c) return BM_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -1278,6 +1311,59 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         c->row_layout = c->max_abs_fw <= 8191 ? BM_ROWS_P19 : BM_ROWS_P15;
         HIPCHK(c, bm_launch_pack_rows(c->d_f_out, c->female_m, c->row_layout, c->d_f_out_p, c->stream));
         HIPCHK(c, bm_launch_pack_rows(c->d_f_in, c->female_m, c->row_layout, c->d_f_in_p, c->stream));
+    }
+    /* Dense copies of the longest female rows: a row of at least a quarter
+     * of the columns costs at most 8 bytes per entry read as 16-bit columns
+     * (against 4 packed) but is summed in registers without LDS atomics.
+     * Exact when the graph is compact (weights fit 16 bits) and every male
+     * weight is >= 0 (an absent column then adds min(w, 0) = 0, as a
+     * missing female edge does).  The longest rows qualify, ties to the
+     * lower id, at most BM_DENSE_MAX and at most 65535 / max |w_F| (the dense
+     * sums are kept in 16 bits). */
+    if (c->compact && c->male_nonneg && !c->d_f_out_dense && c->female_m > 0) {
+        const int64_t stride = bm_gain_stride(c->n);
+        const int32_t thr = (int32_t)(stride / 4) > 1 ? (int32_t)(stride / 4) : 1;
+        std::vector<int2> cand;   /* {degree, side * (n + 1) + id}, side 0 = out-row */
+        for (int32_t r = 1; r <= c->n; r++) {
+            if (c->f_out_deg[r] >= thr) cand.push_back(make_int2(c->f_out_deg[r], r));
+            if (c->f_in_deg[r] >= thr) cand.push_back(make_int2(c->f_in_deg[r], (c->n + 1) + r));
+        }
+        std::sort(cand.begin(), cand.end(), [](const int2& a, const int2& b) {
+            return a.x != b.x ? a.x > b.x : a.y < b.y;
+        });
+        const size_t cap16 = (size_t)(65535 / (c->max_abs_fw > 1 ? c->max_abs_fw : 1));
+        const size_t keep = cap16 < (size_t)BM_DENSE_MAX ? cap16 : (size_t)BM_DENSE_MAX;
+        if (cand.size() > keep) cand.resize(keep);
+        if (!cand.empty()) {
+            std::vector<int32_t> hout((size_t)c->n + 1, -1), hin((size_t)c->n + 1, -1);
+            std::vector<int2> rows(cand.size());
+            for (size_t k = 0; k < cand.size(); k++) {
+                const bool in = cand[k].y > c->n;
+                const int32_t r = in ? cand[k].y - (c->n + 1) : cand[k].y;
+                (in ? hin : hout)[r] = (int32_t)k;
+                rows[k] = make_int2(r, in ? 0 : 1);
+            }
+            int2* d_rows = nullptr;
+            int r2 = dalloc(c, &c->d_f_out_dense, (size_t)c->n + 1);
+            if (r2 == BM_OK) r2 = dalloc(c, &c->d_f_in_dense, (size_t)c->n + 1);
+            if (r2 == BM_OK) r2 = dalloc(c, &c->d_dense, cand.size() * (size_t)(stride / 2));
+            const size_t wcount = (size_t)((c->n + 16) / 16) * cand.size() * 16;
+            if (r2 == BM_OK) r2 = dalloc(c, &c->d_dense_w, wcount);
+            if (r2 == BM_OK) r2 = dalloc(c, &d_rows, rows.size());
+            if (r2 == BM_OK) r2 = h2d(c, c->d_f_out_dense, hout.data(), hout.size());
+            if (r2 == BM_OK) r2 = h2d(c, c->d_f_in_dense, hin.data(), hin.size());
+            if (r2 == BM_OK) r2 = h2d(c, d_rows, rows.data(), rows.size());
+            if (r2 == BM_OK && hipMemsetAsync(c->d_dense, 0, cand.size() * (size_t)stride * 2, c->stream) != hipSuccess)
+                r2 = fail(c, BM_ERR_HIP, "bm_gain_enable: clearing the dense rows failed");
+            if (r2 == BM_OK && bm_launch_dense_fill(c->d_f_out_ptr, c->d_f_out, c->d_f_in_ptr, c->d_f_in, d_rows,
+                                                    (int32_t)rows.size(), stride, (uint16_t*)c->d_dense,
+                                                    c->stream) != hipSuccess)
+                r2 = fail(c, BM_ERR_HIP, "bm_gain_enable: filling the dense rows failed");
+            if (r2 == BM_OK && hipStreamSynchronize(c->stream) != hipSuccess) r2 = fail(c, BM_ERR_HIP, "sync failed");
+            (void)hipFree(d_rows);
+            if (r2 != BM_OK) return r2;
+            c->ndense = (int32_t)rows.size();
+        }
     }
     /* one spare entry: the build's look-ahead reads entry base(u) of a row with no edges */
     int rc = dalloc(c, &c->d_edesc, (size_t)(2 * c->male.m + 1));

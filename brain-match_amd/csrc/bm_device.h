@@ -56,7 +56,25 @@ struct BmGain {
     const int32_t* f_in_bnd;
     const int32_t* f_out_bnd;
     int32_t nwin;
+    /* Dense copies of the longest female rows (compact graphs whose male
+     * weights are all >= 0): ndense rows of stride / 2 words, word i holding
+     * the weights of columns 2i (low half) and 2i + 1 (high half), 0 where
+     * the row has no entry.  f_out_dense / f_in_dense give a female vertex's
+     * row in it, or -1.  A separate pass sums the dense edges of 16 male
+     * rows at a time and the row build starts from its result; their
+     * work-list entries carry BM_EDESC_DENSE.  NULL / 0 when no row is
+     * dense. */
+    const uint32_t* dense;
+    const int32_t* f_out_dense;
+    const int32_t* f_in_dense;
+    int32_t ndense;             /* at most 65535 / max female weight, so 16-bit sums cannot overflow */
+    uint32_t* dense_w;          /* per tile of 16 male rows: ndense x 16 packed weights (rebuilt by every build) */
 };
+
+/* Work-list entry flags (high word of the start offset). */
+#define BM_EDESC_IN 0x80000000u      /* in-edge: expands the female out-row */
+#define BM_EDESC_DENSE 0x40000000u   /* dense row: {row in G.dense, flags, 0, w} */
+#define BM_DENSE_MAX 128             /* at most this many dense female rows */
 
 /* Row stride of the gain table and its transpose: n + 1 rounded up to a
  * multiple of 32 int32 (128 bytes), so every row starts on a cache line and
@@ -78,6 +96,11 @@ hipError_t bm_launch_adj_bits(const BmDevGraph& g, uint32_t* bits, int64_t words
 hipError_t bm_launch_pack_rows(const int2* in, int64_t m, int layout, uint32_t* out, hipStream_t s);
 /* Column-window width of the table build for a row stride. */
 int32_t bm_gain_chunk_cols(int64_t stride);
+/* Fills the zeroed dense rows: dense row k is female vertex rows[k].x's
+ * out-row (rows[k].y = 1) or in-row (0); pool holds ndense x stride uint16. */
+hipError_t bm_launch_dense_fill(const int64_t* out_ptr, const int2* out_rows, const int64_t* in_ptr,
+                                const int2* in_rows, const int2* rows, int32_t ndense, int64_t stride,
+                                uint16_t* pool, hipStream_t s);
 /* Window boundaries of every female row for a build in nwin windows of
  * width cols (see BmGain::f_in_bnd): bnd[r * (nwin + 1) + k], rows 0..n. */
 hipError_t bm_launch_row_bounds(const int64_t* ptr, const int2* rows, int32_t n, int32_t cols, int32_t nwin,

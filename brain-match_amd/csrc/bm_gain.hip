@@ -23,10 +23,15 @@
  * device.
  *
  * Build: a work list first gives every male edge the female row it expands
- * (one lane per edge, so the dependent loads of all edges overlap); then one
- * block per male row accumulates a column chunk of the row in LDS with atomic
- * adds, its waves taking the row's edges one at a time with the next edge's
- * entry already loading, and writes the chunk out coalesced.  Lookups: one
+ * (one lane per edge, so the dependent loads of all edges overlap).  The
+ * longest female rows (hubs adjacent to a large share of the vertices, which
+ * carry most of the expanded entries) are also kept as dense 16-bit column
+ * arrays: a tile pass sums their edges for 16 male rows at a time with
+ * packed 16-bit min/add, one load of a dense row serving the 16 rows, and
+ * writes the sums to the table.  Then one block per male row starts a column
+ * chunk of the row in LDS from those sums (or zero) and adds the other edges
+ * with atomic adds, its waves taking the row's edges one at a time with the
+ * next edge's entry already loading, and writes the chunk out coalesced.  Lookups: one
  * thread per candidate, or one block per first vertex of a u-major stream.
  * Patches after moves: (male edge, female-row entry) pairs flattened per wave
  * (64 edges at a time, prefix sum of their female-row lengths), with global
@@ -44,6 +49,8 @@ constexpr int kBuildThreads = 1024;   /* build: 16 waves per row */
 constexpr int kDescChunk = BM_DESC_CHUNK;   /* work-list slots per wave of k_gain_edge_desc */
 constexpr int kDescLong = 1024;             /* rows longer than this lead their work-list chunk */
 constexpr int kPatchThreads = 512;
+constexpr int kTileRows = 16;       /* dense tile kernel: male rows per block */
+constexpr int kTileThreads = 256;
 constexpr int kPatchWaves = kPatchThreads / BM_WAVE;
 
 /* Word of the dense adjacency bitmap holding the pair {a, b}: bit (max & 31)
@@ -247,7 +254,7 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
         const int64_t t0 = uni(ch.y), t1 = t0 + kDescChunk < L ? t0 + kDescChunk : L;
         int4* out = G.edesc + ob + ib;
         int2 ed[kJ];
-        int32_t gx[kJ], lo32[kJ], hi32[kJ];
+        int32_t gx[kJ], lo32[kJ], hi32[kJ], di[kJ];
         int64_t rb[kJ], re[kJ];
 #pragma unroll
         for (int j = 0; j < kJ; j++) {
@@ -263,6 +270,7 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
             const int64_t* fptr = outp ? g.f_in_ptr : g.f_out_ptr;
             rb[j] = gx[j] ? fptr[gx[j]] : 0;
             re[j] = gx[j] ? fptr[gx[j] + 1] : 0;
+            di[j] = (G.ndense > 0 && gx[j]) ? (outp ? G.f_in_dense : G.f_out_dense)[gx[j]] : -1;
             if (windowed) {   /* the row's part inside the window, from its precomputed boundaries */
                 const int32_t* bnd = (outp ? G.f_in_bnd : G.f_out_bnd) + (int64_t)gx[j] * (G.nwin + 1) + win;
                 lo32[j] = gx[j] ? bnd[0] : 0;
@@ -285,7 +293,7 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
 #pragma unroll
         for (int j = 0; j < kJ; j++) {
             const bool valid = t0 + lane + BM_WAVE * j < t1;
-            const bool lng = valid && re[j] - rb[j] > kDescLong;
+            const bool lng = valid && di[j] < 0 && re[j] - rb[j] > kDescLong;
             mlong[j] = __ballot(lng);
             mshort[j] = __ballot(valid && !lng);
             nlong += __popcll(mlong[j]);
@@ -300,8 +308,14 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
                 const int32_t pos = lng ? before_l + __popcll(mlong[j] & below)
                                         : nlong + before_s + __popcll(mshort[j] & below);
                 const bool outp = t < nout;
-                const uint32_t hi = (uint32_t)((uint64_t)rb[j] >> 32) | (outp ? 0u : 0x80000000u);
-                out[t0 + pos] = make_int4((int32_t)(uint32_t)rb[j], (int32_t)hi, (int32_t)(re[j] - rb[j]), ed[j].y);
+                const uint32_t side = outp ? 0u : BM_EDESC_IN;
+                const uint32_t hi = (uint32_t)((uint64_t)rb[j] >> 32) | side;
+                if (di[j] >= 0 && win == 0)   /* the dense tile kernel's weight for (u, dense row) */
+                    G.dense_w[((int64_t)(u / kTileRows) * G.ndense + di[j]) * kTileRows + (u % kTileRows)] =
+                        (uint32_t)ed[j].y | ((uint32_t)ed[j].y << 16);
+                out[t0 + pos] = di[j] >= 0 ? make_int4(di[j], (int32_t)(side | BM_EDESC_DENSE), 0, ed[j].y)
+                                           : make_int4((int32_t)(uint32_t)rb[j], (int32_t)hi, (int32_t)(re[j] - rb[j]),
+                                                       ed[j].y);
             }
             before_l += __popcll(mlong[j]);
             before_s += __popcll(mshort[j]);
@@ -309,14 +323,114 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
     }
 }
 
+/* Dense part of the table: G(u, f) restricted to u's edges whose female
+ * row is dense, the sum over them of min(w, dense weight of column f).  One
+ * block per tile of 16 male rows over every column.  The work list has put
+ * each such edge's weight, packed w | w << 16, at dense_w[tile][dense row]
+ * [row in tile] (0 where the male row has no edge to that dense row), so a
+ * dense row's 16 weights are one scalar load; a thread owns two column pairs
+ * (a 32-bit word of every dense row each) and keeps their 16 rows' sums as
+ * packed 16-bit halves, one load of a dense-row word serving 16 rows.  The
+ * sums stay below 2^16: there are at most 65535 / max weight dense rows.
+ * The result is written to G (the row build starts from it and adds the
+ * other edges). */
+typedef unsigned short Us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pk_min_add(uint32_t acc, uint32_t v, uint32_t wpk) {
+    const Us2 m = __builtin_elementwise_min(__builtin_bit_cast(Us2, v), __builtin_bit_cast(Us2, wpk));
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(Us2, acc) + m);
+}
+
+/* Four packed weights read as constant data (a scalar load for a uniform
+ * address). */
+__device__ __forceinline__ uint4 load_weights(const uint32_t* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef __attribute__((address_space(4))) const uint4 ConstUint4;
+    return *(const ConstUint4*)p;
+#else
+    return *(const uint4*)p;
+#endif
+}
+
+/* The tile's word loop (restrict-qualified, so the weight reads are known
+ * not to alias the table writes and become scalar loads). */
+__device__ __forceinline__ void dense_tile_words(const uint32_t* __restrict__ wt, const uint32_t* __restrict__ dense,
+                                                 int32_t* __restrict__ gout, int64_t stride, int64_t hw, int32_t nwords,
+                                                 int32_t nd, int32_t u0, int32_t n, int tid) {
+    /* the word loop and everything in it are block-uniform (scalar
+     * weight loads); a thread's words past the row are clamped to the
+     * last word for loading and not stored */
+    for (int32_t wb = 0; wb < nwords; wb += 2 * kTileThreads) {
+        const int32_t w0 = wb + tid, w1 = w0 + kTileThreads;
+        const int32_t l0 = w0 < nwords ? w0 : nwords - 1, l1 = w1 < nwords ? w1 : nwords - 1;
+        uint32_t a0[kTileRows], a1[kTileRows];
+#pragma unroll
+        for (int r = 0; r < kTileRows; r++) {
+            a0[r] = 0u;
+            a1[r] = 0u;
+        }
+        /* dense row k's two words are loaded one iteration ahead */
+        uint32_t v0 = dense[l0], v1 = dense[l1];
+        for (int32_t k = 0; k < nd; k++) {
+            const uint32_t* dn = dense + (int64_t)(k + 1 < nd ? k + 1 : k) * hw;
+            const uint32_t n0 = dn[l0], n1 = dn[l1];
+            const uint32_t* wk = wt + k * kTileRows;
+            const uint4 q0 = load_weights(wk), q1 = load_weights(wk + 4), q2 = load_weights(wk + 8),
+                        q3 = load_weights(wk + 12);
+            const uint32_t any = q0.x | q0.y | q0.z | q0.w | q1.x | q1.y | q1.z | q1.w | q2.x | q2.y | q2.z |
+                                 q2.w | q3.x | q3.y | q3.z | q3.w;
+            if (any != 0u) {   /* some row of the tile uses dense row k */
+                const uint32_t w[kTileRows] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                               q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+#pragma unroll
+                for (int r = 0; r < kTileRows; r++) {
+                    a0[r] = pk_min_add(a0[r], v0, w[r]);
+                    a1[r] = pk_min_add(a1[r], v1, w[r]);
+                }
+            }
+            v0 = n0;
+            v1 = n1;
+        }
+#pragma unroll
+        for (int r = 0; r < kTileRows; r++) {
+            const int32_t u = u0 + r;
+            if (u >= 1 && u <= n) {
+                int2* out = reinterpret_cast<int2*>(gout + (int64_t)u * stride);
+                if (w0 < nwords) out[w0] = make_int2((int32_t)(a0[r] & 0xffffu), (int32_t)(a0[r] >> 16));
+                if (w1 < nwords) out[w1] = make_int2((int32_t)(a1[r] & 0xffffu), (int32_t)(a1[r] >> 16));
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kTileThreads) k_gain_dense_tile(BmDevGraph g, BmGain G) {
+    const int tid = threadIdx.x;
+    const int32_t ntiles = (g.n + kTileRows) / kTileRows;   /* rows 0..n */
+    const int64_t hw = G.stride >> 1;
+    const int32_t nwords = (int32_t)hw;
+    const int32_t nd = G.ndense;
+    for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int32_t u0 = tile * kTileRows;
+        const uint32_t* wt = G.dense_w + (int64_t)tile * nd * kTileRows;
+        dense_tile_words(wt, G.dense, G.G, G.stride, hw, nwords, nd, u0, g.n, tid);
+    }
+}
+
+/* Clears n words (the dense tile weights before the work list fills them). */
+__global__ void __launch_bounds__(256) k_zero_words(uint32_t* __restrict__ p, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
 /* Row u: the edges of u's work list are taken one at a time by the block's
  * waves from an LDS counter (so a long female row does not hold up the
  * block).  A wave walks its edges' female rows in chunks of 64 x U entries
- * and keeps two chunks in flight: the next chunk (of the same edge or the
- * next one) is loaded while the current one's min(w, fw) are added into the
- * LDS copy of the row's column window.  Work-list entries are read through
- * the constant address space (scalar loads), so taking a new edge does not
- * wait on the row loads in flight; the entry after next is always loading. */
+ * and keeps S chunks in flight: while the oldest chunk's min(w, fw) are
+ * added into the LDS copy of the row's column window, the next S - 1 (of the
+ * same edge or the next ones) are loading.  Work-list entries are read
+ * through the constant address space (scalar loads), so taking a new edge
+ * does not wait on the row loads in flight; the entry after next is always
+ * loading. */
 /* One work-list entry, read as constant data (a scalar load for a uniform
  * index).  The host pass of the compiler only type-checks kernel bodies. */
 __device__ __forceinline__ int4 load_entry(const int4* p) {
@@ -328,14 +442,13 @@ __device__ __forceinline__ int4 load_entry(const int4* p) {
 #endif
 }
 
-template <int LAYOUT, int U, bool WINDOWED>
+template <int LAYOUT, int U, int S, bool WINDOWED>
 __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGain G, int32_t col0_arg, int32_t ncols) {
     typedef Row<LAYOUT> R;
     constexpr bool PACKED = LAYOUT != BM_ROWS_INT2;
     typedef typename R::T E;
-    /* the row window sits at LDS address 0 (the edge counter after it), and
-     * a single window starts at column 0, so a column's LDS address is just
-     * 4 f */
+    /* the row window sits at LDS address 0, and a single window starts at
+     * column 0, so a column's LDS address is just 4 f */
     extern __shared__ int32_t row[];
     int32_t& next_edge = row[ncols];
     const int32_t col0 = WINDOWED ? col0_arg : 0;
@@ -345,12 +458,18 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
     const E* __restrict__ f_in_rows = PACKED ? (const E*)G.f_in_p : (const E*)g.f_in;
     const E* __restrict__ f_out_rows = PACKED ? (const E*)G.f_out_p : (const E*)g.f_out;
     for (int32_t u = blockIdx.x + 1; u <= g.n; u += gridDim.x) {
-        for (int32_t c = tid; c < ncols; c += kBuildThreads) row[c] = 0;
-        if (tid == 0) next_edge = 0;
-        __syncthreads();
         const int64_t base = g.m_out_ptr[u] + g.m_in_ptr[u];
         const int32_t L = (int32_t)(g.m_out_ptr[u + 1] + g.m_in_ptr[u + 1] - base);
         const int4* list = G.edesc + base;
+        /* the row starts from its dense part (k_gain_dense_tile) or zero */
+        if (G.ndense > 0) {
+            const int32_t* src = G.G + (int64_t)u * G.stride + col0;
+            for (int32_t c = tid; c < ncols; c += kBuildThreads) row[c] = src[c];
+        } else {
+            for (int32_t c = tid; c < ncols; c += kBuildThreads) row[c] = 0;
+        }
+        if (tid == 0) next_edge = 0;
+        __syncthreads();
         /* the list index is clamped so the look-ahead load is unconditional
          * (its value is used only while the index is < L) */
         const int32_t last = L > 0 ? L - 1 : 0;
@@ -365,14 +484,15 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
         int4 d = load_entry(list + (t < last ? t : last));
         int32_t tn = grab();
         int4 dn = load_entry(list + (tn < last ? tn : last));
-        const E* frow = nullptr;
+        const E* frow = f_in_rows;
         int32_t len = 0, w = 0, j0 = 0;
-        auto decode = [&](const int4 e) {
-            const uint32_t hi = (uint32_t)uni(e.y);
-            const int64_t start = (int64_t)((uint64_t)(uint32_t)uni(e.x) | ((uint64_t)(hi & 0x7fffffffu) << 32));
-            frow = ((hi >> 31) ? f_out_rows : f_in_rows) + start;
-            len = uni(e.z);
-            w = uni(e.w);
+        auto decode = [&](const int4 en) {
+            const uint32_t hi = (uint32_t)uni(en.y);
+            const int64_t start =
+                (int64_t)((uint64_t)(uint32_t)uni(en.x) | ((uint64_t)(hi & ~(BM_EDESC_IN | BM_EDESC_DENSE)) << 32));
+            frow = ((hi & BM_EDESC_IN) ? f_out_rows : f_in_rows) + start;
+            len = (hi & BM_EDESC_DENSE) ? 0 : uni(en.z);
+            w = uni(en.w);
             j0 = 0;
         };
         if (t < L) decode(d);
@@ -400,9 +520,9 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
             }
         };
         /* Chunk loads are unconditional (buffer loads bounded by the chunk)
-         * and the loop is unrolled twice so the two register sets swap roles
+         * and the loop is unrolled S times so the register sets rotate
          * without copies: both keep the compiler's load counting exact, so an
-         * atomic waits only on the older chunk while the newer one loads. */
+         * add waits only on its own chunk while the newer ones load. */
         auto load_chunk = [&](E (&r)[U], const E* p, int32_t cnt) {
             /* cnt = 0 gives a resource of no records: every load returns
              * zero without touching memory, so the load count never depends
@@ -433,20 +553,52 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
                 }
             }
         };
-        E a[U], b[U];
+        /* named register sets (arrays of them cost registers) */
+        E ra[U], rb[U];
         const E* pa;
         const E* pb;
         int32_t ca, wa, cb, wb;
-        next_chunk(pa, ca, wa);
-        load_chunk(a, pa, ca);
-        while (ca > 0) {
-            next_chunk(pb, cb, wb);
-            load_chunk(b, pb, cb);
-            add_chunk(a, ca, wa);
-            if (cb == 0) break;
+        if constexpr (S == 2) {
             next_chunk(pa, ca, wa);
-            load_chunk(a, pa, ca);
-            add_chunk(b, cb, wb);
+            load_chunk(ra, pa, ca);
+            while (ca > 0) {
+                next_chunk(pb, cb, wb);
+                load_chunk(rb, pb, cb);
+                add_chunk(ra, ca, wa);
+                if (cb == 0) break;
+                next_chunk(pa, ca, wa);
+                load_chunk(ra, pa, ca);
+                add_chunk(rb, cb, wb);
+            }
+        } else {
+            static_assert(S == 4, "the build pipelines 2 or 4 chunks");
+            E rc[U], rd[U];
+            const E* pc;
+            const E* pd;
+            int32_t cc, wc, cd, wd;
+            next_chunk(pa, ca, wa);
+            load_chunk(ra, pa, ca);
+            next_chunk(pb, cb, wb);
+            load_chunk(rb, pb, cb);
+            next_chunk(pc, cc, wc);
+            load_chunk(rc, pc, cc);
+            while (ca > 0) {
+                next_chunk(pd, cd, wd);
+                load_chunk(rd, pd, cd);
+                add_chunk(ra, ca, wa);
+                if (cb == 0) break;
+                next_chunk(pa, ca, wa);
+                load_chunk(ra, pa, ca);
+                add_chunk(rb, cb, wb);
+                if (cc == 0) break;
+                next_chunk(pb, cb, wb);
+                load_chunk(rb, pb, cb);
+                add_chunk(rc, cc, wc);
+                if (cd == 0) break;
+                next_chunk(pc, cc, wc);
+                load_chunk(rc, pc, cc);
+                add_chunk(rd, cd, wd);
+            }
         }
         /* the compiler does not count the inline-asm LDS adds: wait for them
          * before the barrier that publishes the row */
@@ -481,6 +633,20 @@ __global__ void k_pack_rows(const int2* __restrict__ in, int64_t m, int layout, 
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
         out[i] = layout == BM_ROWS_P19 ? ((uint32_t)in[i].y << 19) | ((uint32_t)in[i].x << 2)
                                        : ((uint32_t)in[i].x << 15) | (uint32_t)in[i].y;
+}
+
+/* Dense row k: one block over the entries of female row rows[k].x (its
+ * out-row when rows[k].y, else its in-row), each weight stored at its
+ * column. */
+__global__ void __launch_bounds__(256) k_dense_fill(const int64_t* __restrict__ out_ptr, const int2* __restrict__ out_rows,
+                                                    const int64_t* __restrict__ in_ptr, const int2* __restrict__ in_rows,
+                                                    const int2* __restrict__ rows, int64_t stride,
+                                                    uint16_t* __restrict__ pool) {
+    const int2 r = rows[blockIdx.x];
+    const int64_t* ptr = r.y ? out_ptr : in_ptr;
+    const int2* adj = r.y ? out_rows : in_rows;
+    uint16_t* dst = pool + (int64_t)blockIdx.x * stride;
+    for (int64_t e = ptr[r.x] + threadIdx.x; e < ptr[r.x + 1]; e += blockDim.x) dst[adj[e].x] = (uint16_t)adj[e].y;
 }
 
 /* Patch the rows of the neighbours of each moved vertex x (pi(x): old -> new).
@@ -1217,6 +1383,15 @@ int32_t bm_gain_chunk_cols(int64_t stride) {
     return (int32_t)(w < stride ? w : stride);
 }
 
+hipError_t bm_launch_dense_fill(const int64_t* out_ptr, const int2* out_rows, const int64_t* in_ptr,
+                                const int2* in_rows, const int2* rows, int32_t ndense, int64_t stride,
+                                uint16_t* pool, hipStream_t s) {
+    if (ndense <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dense_fill, dim3((unsigned)ndense), dim3(256), 0, s, out_ptr, out_rows, in_ptr, in_rows, rows,
+                       stride, pool);
+    return hipGetLastError();
+}
+
 hipError_t bm_launch_row_bounds(const int64_t* ptr, const int2* rows, int32_t n, int32_t cols, int32_t nwin,
                                 int32_t* bnd, hipStream_t s) {
     if (n < 0 || cols <= 0 || nwin < 1) return hipErrorInvalidValue;
@@ -1229,11 +1404,12 @@ hipError_t bm_launch_row_bounds(const int64_t* ptr, const int2* rows, int32_t n,
 
 template <int LAYOUT, int U>
 static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s) {
+    constexpr int S = U == 4 ? 4 : 2;   /* chunks in flight per wave */
     const int32_t chunk = bm_gain_chunk_cols(G.stride);
     const bool windowed = chunk < G.stride;
     const size_t lds = ((size_t)chunk + 1) * sizeof(int32_t);   /* row window + edge counter */
     const void* kern =
-        windowed ? (const void*)k_gain_build<LAYOUT, U, true> : (const void*)k_gain_build<LAYOUT, U, false>;
+        windowed ? (const void*)k_gain_build<LAYOUT, U, S, true> : (const void*)k_gain_build<LAYOUT, U, S, false>;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int occ = 0;
@@ -1249,15 +1425,26 @@ static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_
     if (dblocks < 1) dblocks = 1;
     if (windowed && (!G.f_in_bnd || !G.f_out_bnd || G.nwin != (int32_t)((G.stride + chunk - 1) / chunk)))
         return hipErrorInvalidValue;
+    if (G.ndense > 0) {   /* the work list sets the dense tile weights that are in use */
+        const int64_t nw = (int64_t)((g.n + kTileRows) / kTileRows) * G.ndense * kTileRows;
+        int64_t zb = (nw + 255) / 256;
+        if (zb > cus * 8) zb = cus * 8;
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)zb), dim3(256), 0, s, G.dense_w, nw);
+    }
     for (int64_t c0 = 0, win = 0; c0 < G.stride; c0 += chunk, win++) {
         const int32_t nc = (int32_t)(G.stride - c0 < chunk ? G.stride - c0 : chunk);
         hipLaunchKernelGGL(k_gain_edge_desc<LAYOUT>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc,
                            (int32_t)win);
+        if (win == 0 && G.ndense > 0) {   /* every column of the dense part, from the first work list */
+            int64_t tblocks = (g.n + kTileRows) / kTileRows;
+            if (tblocks > cus * 16) tblocks = cus * 16;
+            hipLaunchKernelGGL(k_gain_dense_tile, dim3((unsigned)tblocks), dim3(kTileThreads), 0, s, g, G);
+        }
         if (windowed)
-            hipLaunchKernelGGL((k_gain_build<LAYOUT, U, true>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
+            hipLaunchKernelGGL((k_gain_build<LAYOUT, U, S, true>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
                                G, (int32_t)c0, nc);
         else
-            hipLaunchKernelGGL((k_gain_build<LAYOUT, U, false>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
+            hipLaunchKernelGGL((k_gain_build<LAYOUT, U, S, false>), dim3((unsigned)blocks), dim3(kBuildThreads), lds, s, g,
                                G, (int32_t)c0, nc);
     }
     return hipGetLastError();

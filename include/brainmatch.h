@@ -330,6 +330,15 @@ int bm_batch_accept_disjoint(bm_batch* b, int64_t max_accept, int64_t* n_accepte
  * int32. */
 int bm_gain_enable(bm_ctx* ctx, int on);
 int bm_gain_enabled(const bm_ctx* ctx);
+/* Number of female rows whose part of the table the build sums as dense
+ * 16-bit column arrays, 16 male rows at a time, instead of expanding them
+ * edge by edge: the longest rows of at least a quarter of the columns, at
+ * most 128 and at most 65535 / the largest female weight; only for graphs
+ * whose weights fit 16 bits and whose male weights are all >= 0, and only
+ * while no two male vertices share a female.  0 when the table is off or no
+ * row qualifies.  An introspection call with no counterpart in the
+ * reference: the table it describes is the one above. */
+int bm_gain_dense_rows(const bm_ctx* ctx);
 int bm_gain_rebuild(bm_ctx* ctx);
 int64_t bm_gain_table_bytes(int32_t n);
 /* Copy row u (n+1 entries) of the table to the host. */
