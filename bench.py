@@ -31,8 +31,9 @@ value = candidates / second over the K timed steps of the headline strategy
 every step).  roofline = the step's dominant kernel: algorithmic bytes per
 launch / mean launch time from HIP events on the library stream; traffic = HBM
 bytes per launch measured with rocprofv3 (profiles/swap_n18524_traffic.json).
-lds_roofline = the same kernel against the LDS atomic-add rate it is bound
-by (one add per male edge x female row entry expansion).
+lds_roofline = the row build against the LDS atomic-add rate (one add per
+male edge x entry of a sparse female row; the hub rows kept dense are summed
+by the tile pass, gain_table.dense_rows).
 cpu_baseline = the reference's own calculate_swap_delta
 (src/graph-alignment-optimized.c:321-408, compiled from its sources into
 oracle/_ref) over an evenly strided sample of the last timed stream, OpenMP
@@ -93,6 +94,39 @@ def gain_build_bytes(ctx, pi, entry_bytes):
     m = int(mo.sum())
     mf = int(fo.sum())
     return 2 * m * 28 + 2 * mf * entry_bytes + (ctx.n + 1) * gain_stride(ctx.n) * 4, entries
+
+
+def dense_split(ctx, pi, female, n_dense):
+    """The table build's split of the expansions between the dense tile pass
+    and the LDS row build.  Dense rows follow the library's rule: female rows
+    of at least a quarter of the row stride, longest first (ties: lower id,
+    out-rows before in-rows), at most min(128, 65535 // max female weight);
+    their count is checked against the library's.  Returns (expanded entries
+    of dense rows, (male row, dense row) pairs)."""
+    import numpy as np
+    if n_dense == 0:
+        return 0, 0
+    n = ctx.n
+    mo, mi = ctx.degrees(female=False)
+    fo, fi = ctx.degrees(female=True)
+    thr = max(1, gain_stride(n) // 4)
+    cand = [(-int(fo[r]), r) for r in range(1, n + 1) if fo[r] >= thr]
+    cand += [(-int(fi[r]), n + 1 + r) for r in range(1, n + 1) if fi[r] >= thr]
+    cand.sort()
+    keep = min(128, 65535 // max(1, int(np.abs(female.w).max())))
+    cand = cand[:keep]
+    if len(cand) != n_dense:
+        raise SystemExit(f"dense-row rule disagrees with the library ({len(cand)} against {n_dense})")
+    dout = np.zeros(n + 1, bool)
+    din = np.zeros(n + 1, bool)
+    for _, key in cand:
+        (din if key > n else dout)[key - (n + 1) if key > n else key] = True
+    pi = np.asarray(pi, dtype=np.int64)
+    # an out-edge u -> x expands the in-row of pi(x); an in-edge x -> u the out-row of pi(x)
+    uses_in, uses_out = din[pi] & (pi > 0), dout[pi] & (pi > 0)
+    entries = int((mi.astype(np.int64) * fi[pi] * uses_in).sum() + (mo.astype(np.int64) * fo[pi] * uses_out).sum())
+    pairs = int((mi.astype(np.int64) * uses_in).sum() + (mo.astype(np.int64) * uses_out).sum())
+    return entries, pairs
 
 
 def gain_lookup_bytes(ctx, k):
@@ -278,6 +312,9 @@ def main():
         look_s = per_launch_ms(kt, bm.KERNEL_GAIN_SWAP) / 1000
         aux_s = per_launch_ms(kt, bm.KERNEL_AUX, steps=args.steps) / 1000
         bb, entries = gain_build_bytes(ctx, pi0, entry_bytes)
+        n_dense = ctx.gain_dense_rows()
+        dense_entries, dense_pairs = dense_split(ctx, pi0, female, n_dense)
+        adds = entries - dense_entries
         lb = gain_lookup_bytes(ctx, K)
         gain = {
             "value": K * args.steps / wall,
@@ -291,11 +328,16 @@ def main():
                                "expanded_entries": entries, "female_entry_bytes": entry_bytes},
             "lookup_roofline": {"bound": "hbm", "achieved": lb / look_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": lb / look_s / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": lb},
-            # the build's work is one LDS add per (male edge, female row entry) expansion
-            "build_lds_roofline": {"bound": "lds", "achieved": entries / build_s, "peak": LDS_ADD_PEAK_PER_S,
-                                   "unit": "atomic adds/s", "frac": entries / build_s / LDS_ADD_PEAK_PER_S,
-                                   "adds_per_launch": entries,
+            # the row build adds one LDS atomic per (male edge, female row entry)
+            # expansion of a sparse female row; the dense rows' expansions are
+            # summed by the tile pass in packed 16-bit vector arithmetic
+            "build_lds_roofline": {"bound": "lds", "achieved": adds / build_s, "peak": LDS_ADD_PEAK_PER_S,
+                                   "unit": "atomic adds/s", "frac": adds / build_s / LDS_ADD_PEAK_PER_S,
+                                   "adds_per_launch": adds,
                                    "peak_basis": "32 four-byte LDS atomics per CU per clock x 256 CUs x 2.4 GHz"},
+            "dense_rows": {"rows": n_dense, "expanded_entries": dense_entries, "row_pairs": dense_pairs,
+                           "column_terms": dense_pairs * gain_stride(N_VERTICES),
+                           "share_of_expansions": dense_entries / entries if entries else 0.0},
         }
         # the per-candidate byte model of a list scan (4 male lists, 28 B per
         # incident edge): what the same stream would cost a kernel that reads
@@ -361,9 +403,9 @@ def main():
         dom = "lookup" if gain["lookup_ms"] >= gain["build_ms"] else "build"
         rf = dict(gain[dom + "_roofline"])
         if dom == "build":
-            lds_rf = dict(gain["build_lds_roofline"], kernel="k_gain_build")
+            lds_rf = dict(gain["build_lds_roofline"], kernel="k_gain_build (sparse female rows)")
         rf["kernel"] = ("k_gain_swap_umajor" if umajor else "k_gain_swap<uniform>") if dom == "lookup" \
-            else "k_gain_edge_desc+k_gain_build"
+            else "k_gain_edge_desc+k_gain_dense_tile+k_gain_build"
     else:
         rf = dict(scan["roofline"])
         rf["kernel"] = "k_swap<umajor>" if umajor else "k_swap<uniform>"
