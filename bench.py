@@ -797,13 +797,19 @@ def bench_scale(args, bm):
     pa = np.asarray(pi0, np.int64)
     expanded = int((mi.astype(np.int64) * fi[pa]).sum() + (mo.astype(np.int64) * fo[pa]).sum())
     build_ms = per_launch_ms(kt, bm.KERNEL_GAIN_BUILD)
+    n_dense = ctx.gain_dense_rows()
+    dense_entries, dense_pairs = dense_split(ctx, pi0, female, n_dense)
     out["gain"] = {"deltas_per_s": K * 2 / wall, "ms_per_step": 1000 * wall / 2,
                    "build_ms": build_ms, "expanded_entries": expanded,
-                   "build_adds_per_s": expanded / (build_ms / 1000),
+                   "build_expansions_per_s": expanded / (build_ms / 1000),
+                   "dense_rows": n_dense, "dense_expanded_entries": dense_entries,
+                   "dense_column_terms": dense_pairs * gain_stride(n),
+                   "lds_adds": expanded - dense_entries,
                    "lookup_ms": per_launch_ms(kt, bm.KERNEL_GAIN_SWAP), "matches_scan": same,
                    "table_GB": (n + 1) ** 2 * 4 / 1e9}
     log(f"scale deltas: scan {out['scan']['deltas_per_s']:.4g}/s, gain {out['gain']['deltas_per_s']:.4g}/s "
-        f"(build {build_ms:.1f} ms over {expanded:.3g} expanded entries, lookups {out['gain']['lookup_ms']:.1f} ms); "
+        f"(build {build_ms:.1f} ms over {expanded:.3g} expanded entries, {n_dense} dense rows holding "
+        f"{dense_entries:.3g}, lookups {out['gain']['lookup_ms']:.1f} ms); "
         f"identical: {same}")
     b.close()
     ctx.timing_enable(False)

@@ -74,7 +74,13 @@ class _SearchOpts(C.Structure):
 class _SearchResult(C.Structure):
     _fields_ = [("initial_score", C.c_int64), ("best_score", C.c_int64), ("final_score", C.c_int64),
                 ("iterations", C.c_int64), ("accepted", C.c_int64), ("evaluations", C.c_int64),
-                ("verify_failures", C.c_int64), ("seconds", C.c_double), ("fallback_rounds", C.c_int64)]
+                ("verify_failures", C.c_int64), ("seconds", C.c_double), ("fallback_rounds", C.c_int64),
+                ("restarts", C.c_int64)]
+
+
+class _SweepOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("initial_swaps", C.c_int32), ("restart_swaps", C.c_int32),
+                ("batch", C.c_int32), ("verify", C.c_int32), ("max_passes", C.c_int64), ("time_limit_s", C.c_double)]
 
 
 def _sig(name, res, args):
@@ -149,6 +155,8 @@ _sig("bm_gain_table_bytes", C.c_int64, [C.c_int32])
 _sig("bm_gain_row", C.c_int, [_c, C.c_int32, i32p])
 _sig("bm_best_swap", C.c_int, [_c, i64p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)])
 _sig("bm_best_swap_climb", C.c_int, [_c, C.c_int64, C.c_int64, i64p, C.POINTER(_SearchResult)])
+_sig("bm_best_partners", C.c_int, [_c, C.c_int32, C.c_int32, i64p, i32p])
+_sig("bm_sweep", C.c_int, [_c, C.POINTER(_SweepOpts), i32p, i64p, C.POINTER(_SearchResult)])
 _sig("bm_twohop_set", C.c_int, [_c, C.c_int32, i32p, C.POINTER(C.c_int32)])
 _sig("bm_batch_gen_twohop", C.c_int, [_c, i32p, C.c_int32, C.c_int32, C.c_uint64])
 _sig("bm_hillclimb_twohop", C.c_int, [_c, C.c_uint64, C.c_int64, C.c_int32, C.c_int32, C.c_int32, i64p,
@@ -570,6 +578,26 @@ class Context:
         self._chk(_lib.bm_best_swap_climb(self.h, max_iters, verify_every, _p64(trace), C.byref(r)))
         res = {k: getattr(r, k) for k, _ in _SearchResult._fields_}
         return trace[:res["iterations"] + 1], res
+
+    def best_partners(self, u0, count):
+        """(delta, v) of the best partner v > u of each u in u0 .. u0+count-1."""
+        d = np.zeros(count, np.int64)
+        v = np.zeros(count, np.int32)
+        self._chk(_lib.bm_best_partners(self.h, u0, count, _p64(d), _p32(v)))
+        return d, v
+
+    def sweep(self, seed=0, max_passes=0, time_limit_s=0.0, initial_swaps=50, restart_swaps=200, batch=0,
+              verify=True):
+        """The per-vertex best-partner sweep with restarts; returns (best
+        matching, best score after each pass, result)."""
+        o = _SweepOpts(seed, initial_swaps, restart_swaps, batch, 1 if verify else 0, max_passes, time_limit_s)
+        best = np.zeros(self.n + 1, np.int32)
+        trace = np.zeros(max(max_passes, 1), np.int64)
+        r = _SearchResult()
+        self._chk(_lib.bm_sweep(self.h, C.byref(o), _p32(best), _p64(trace) if max_passes > 0 else None,
+                                C.byref(r)))
+        res = {k: getattr(r, k) for k, _ in _SearchResult._fields_}
+        return best, trace[:res["iterations"]] if max_passes > 0 else trace[:0], res
 
     def hillclimb_twohop(self, seed, rounds, anchors, per_anchor, verify=True):
         trace = np.zeros(rounds + 1, np.int64)

@@ -100,6 +100,8 @@ struct bm_ctx {
     int64_t n_echunks;
     int64_t gt_version;
     int4* d_best;           /* best-swap result, then kBestSwapParts per-block results */
+    int4* d_partners;       /* best partner per vertex of a bm_best_partners range */
+    int32_t partners_cap;
     char err[512];
 };
 
@@ -274,6 +276,7 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_f_in_dense);
     (void)hipFree(c->d_m_adj);
     (void)hipFree(c->d_best);
+    (void)hipFree(c->d_partners);
     bm_graph_free(&c->male);
     free(c->f_out_deg);
     free(c->f_in_deg);
@@ -1221,6 +1224,47 @@ extern "C" int bm_best_swap(bm_ctx* c, int64_t* delta, int32_t* u, int32_t* v) {
     return BM_OK;
 }
 
+/* Best partner of every u in u0 .. u0 + count - 1 over v = u+1..n (the
+ * inner loop of the OpenMP tool's sweep, src/re-alignment-parallel.c:598-621,
+ * which keeps the largest calculate_swap_delta over node_m2 > node_m1). */
+extern "C" int bm_best_partners(bm_ctx* c, int32_t u0, int32_t count, int64_t* delta, int32_t* v) {
+    if (!c || !delta || !v) return fail(c, BM_ERR_ARG, "bm_best_partners: null argument");
+    if (count < 0 || (count > 0 && (u0 < 1 || (int64_t)u0 + count - 1 > c->n)))
+        return fail(c, BM_ERR_ARG, "bm_best_partners: range %d + %d outside 1..%d", u0, count, c->n);
+    if (!c->gain_on) return fail(c, BM_ERR_ARG, "bm_best_partners: needs the placement-gain table (bm_gain_enable)");
+    if (count == 0) return BM_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = gain_transposed(c);
+    if (rc != BM_OK) return rc;
+    if (c->partners_cap < count) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->d_partners);
+        c->d_partners = nullptr;
+        c->partners_cap = 0;
+        if ((rc = dalloc(c, &c->d_partners, (size_t)count)) != BM_OK) return rc;
+        c->partners_cap = count;
+    }
+    KTiming t;
+    timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
+    HIPCHK(c, bm_launch_gain_pis(devgraph(c), devgain(c), c->d_pis, c->stream));
+    HIPCHK(c, bm_launch_gain_best_partners(devgraph(c), devgain(c), c->d_GT, c->d_pis, u0, count, c->d_partners,
+                                           c->num_cus, c->stream));
+    timing_end(c, &t);
+    std::vector<int4> h((size_t)count);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_partners, (size_t)count * sizeof(int4), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int32_t i = 0; i < count; i++) {
+        if (h[i].w == INT32_MAX) {   /* u = n: no partner */
+            delta[i] = INT64_MIN;
+            v[i] = 0;
+        } else {
+            delta[i] = (int64_t)(((uint64_t)(uint32_t)h[i].y << 32) | (uint64_t)(uint32_t)h[i].x);
+            v[i] = h[i].w;
+        }
+    }
+    return BM_OK;
+}
+
 extern "C" int bm_apply_perm3(bm_ctx* c, int32_t a, int32_t b, int32_t d, int32_t p) {
     if (!c || a < 1 || a > c->n || b < 1 || b > c->n || d < 1 || d > c->n || a == b || a == d || b == d || p < 0 || p > 5)
         return fail(c, BM_ERR_ARG, "bm_apply_perm3: invalid triple or permutation");
@@ -1320,7 +1364,12 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
      * missing female edge does).  The longest rows qualify, ties to the
      * lower id, at most BM_DENSE_MAX and at most 65535 / max |w_F| (the dense
      * sums are kept in 16 bits). */
-    if (c->compact && c->male_nonneg && !c->d_f_out_dense && c->female_m > 0) {
+    /* Only for a table built in one column window: the tile pass writes the
+     * table once more and the row build reads it back, 2.7 GB at n = 18,524
+     * but 80 GB at n = 100,000, where that costs more than the dense rows
+     * save (an 89.5 ms build against 81.6 ms). */
+    if (c->compact && c->male_nonneg && !c->d_f_out_dense && c->female_m > 0 &&
+        bm_gain_chunk_cols(bm_gain_stride(c->n)) >= bm_gain_stride(c->n)) {
         const int64_t stride = bm_gain_stride(c->n);
         const int32_t thr = (int32_t)(stride / 4) > 1 ? (int32_t)(stride / 4) : 1;
         std::vector<int2> cand;   /* {degree, side * (n + 1) + id}, side 0 = out-row */

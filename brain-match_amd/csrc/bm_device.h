@@ -135,6 +135,11 @@ hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, cons
 constexpr int32_t kBestSwapParts = 2048;
 hipError_t bm_launch_gain_best_swap(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
                                     int4* part, int32_t max_parts, int4* out, int32_t num_cus, hipStream_t s);
+/* Best partner of each u in u0 .. u0 + count - 1 over v = u+1..n: out[i] =
+ * {delta lo, delta hi, u0 + i, v} (ties to the smaller v; delta INT64_MIN and
+ * v INT32_MAX for u = n). */
+hipError_t bm_launch_gain_best_partners(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
+                                        int32_t u0, int32_t count, int4* out, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_rot3(const BmDevGraph& g, const BmGain& G, const int4* cand, int64_t k, int64_t* best,
                                uint8_t* perm, int4* cand_out_perm, hipStream_t s);
 hipError_t bm_launch_moves_from_swap(const int32_t* pi_before, int32_t u, int32_t v, int4* moves, int64_t* count,

@@ -859,53 +859,77 @@ __device__ __forceinline__ void best_block_reduce(int64_t& d, int32_t& u, int32_
     }
 }
 
-__global__ void __launch_bounds__(kBestThreads) k_gain_best_swap(BmDevGraph g, BmGain G, const int32_t* __restrict__ GT,
-                                                                 const int4* __restrict__ pis, int4* __restrict__ part) {
+/* The block's threads scan the partners v = u+1..n of u and keep their best
+ * (d, u, v) under best_before. */
+__device__ __forceinline__ void best_partner_scan(const BmDevGraph& g, const BmGain& G, const int32_t* __restrict__ GT,
+                                                  const int4* __restrict__ pis, int32_t u, int64_t& bd, int32_t& bu,
+                                                  int32_t& bv) {
     const int32_t n = g.n;
-    int64_t bd = INT64_MIN;
-    int32_t bu = INT32_MAX, bv = INT32_MAX;
-    for (int32_t u = blockIdx.x + 1; u < n; u += gridDim.x) {
-        const int4 pu = pis[u];
-        const int32_t f1 = pu.x;
-        const int32_t* Gu = G.G + (int64_t)u * G.stride;
-        const int32_t* Tf1 = GT + (int64_t)f1 * G.stride;
-        for (int32_t v0 = u + 1 + (int32_t)threadIdx.x; v0 <= n; v0 += kBestThreads * kBestPairs) {
-            /* every load of kBestPairs partners in flight before any use */
-            int4 pv[kBestPairs];
-            int32_t tv[kBestPairs], gu[kBestPairs];
-            bool mm[kBestPairs];
+    const int4 pu = pis[u];
+    const int32_t f1 = pu.x;
+    const int32_t* Gu = G.G + (int64_t)u * G.stride;
+    const int32_t* Tf1 = GT + (int64_t)f1 * G.stride;
+    for (int32_t v0 = u + 1 + (int32_t)threadIdx.x; v0 <= n; v0 += kBestThreads * kBestPairs) {
+        /* every load of kBestPairs partners in flight before any use */
+        int4 pv[kBestPairs];
+        int32_t tv[kBestPairs], gu[kBestPairs];
+        bool mm[kBestPairs];
 #pragma unroll
-            for (int j = 0; j < kBestPairs; j++) {
-                const int32_t v = v0 + kBestThreads * j;
-                const bool in = v <= n;
-                pv[j] = in ? pis[v] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
-                tv[j] = in ? Tf1[v] : 0;
-                mm[j] = in && male_maybe(G, u, v);
+        for (int j = 0; j < kBestPairs; j++) {
+            const int32_t v = v0 + kBestThreads * j;
+            const bool in = v <= n;
+            pv[j] = in ? pis[v] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
+            tv[j] = in ? Tf1[v] : 0;
+            mm[j] = in && male_maybe(G, u, v);
+        }
+#pragma unroll
+        for (int j = 0; j < kBestPairs; j++) gu[j] = Gu[pv[j].x];
+#pragma unroll
+        for (int j = 0; j < kBestPairs; j++) {
+            const int32_t v = v0 + kBestThreads * j;
+            if (v > n) continue;
+            int64_t d;
+            if (mm[j]) {   /* a male edge between u and v may exist (rare): the full correction */
+                d = gain_swap_delta_from(g, G, u, v, f1, pv[j].x, true, gu[j], pu.y, tv[j], pv[j].y);
+            } else {       /* only self-loops, whose weights the per-vertex words carry */
+                d = (int64_t)gu[j] - pu.y + tv[j] - pv[j].y;
+                if (pu.z != BM_ABSENT) d += contrib(pu.z, pv[j].w) - (int64_t)contrib(pu.z, pu.w);
+                if (pv[j].z != BM_ABSENT) d += contrib(pv[j].z, pu.w) - (int64_t)contrib(pv[j].z, pv[j].w);
             }
-#pragma unroll
-            for (int j = 0; j < kBestPairs; j++) gu[j] = Gu[pv[j].x];
-#pragma unroll
-            for (int j = 0; j < kBestPairs; j++) {
-                const int32_t v = v0 + kBestThreads * j;
-                if (v > n) continue;
-                int64_t d;
-                if (mm[j]) {   /* a male edge between u and v may exist (rare): the full correction */
-                    d = gain_swap_delta_from(g, G, u, v, f1, pv[j].x, true, gu[j], pu.y, tv[j], pv[j].y);
-                } else {       /* only self-loops, whose weights the per-vertex words carry */
-                    d = (int64_t)gu[j] - pu.y + tv[j] - pv[j].y;
-                    if (pu.z != BM_ABSENT) d += contrib(pu.z, pv[j].w) - (int64_t)contrib(pu.z, pu.w);
-                    if (pv[j].z != BM_ABSENT) d += contrib(pv[j].z, pu.w) - (int64_t)contrib(pv[j].z, pv[j].w);
-                }
-                if (best_before(d, u, v, bd, bu, bv)) { bd = d; bu = u; bv = v; }
-            }
+            if (best_before(d, u, v, bd, bu, bv)) { bd = d; bu = u; bv = v; }
         }
     }
+}
+
+__global__ void __launch_bounds__(kBestThreads) k_gain_best_swap(BmDevGraph g, BmGain G, const int32_t* __restrict__ GT,
+                                                                 const int4* __restrict__ pis, int4* __restrict__ part) {
+    int64_t bd = INT64_MIN;
+    int32_t bu = INT32_MAX, bv = INT32_MAX;
+    for (int32_t u = blockIdx.x + 1; u < g.n; u += gridDim.x) best_partner_scan(g, G, GT, pis, u, bd, bu, bv);
     best_block_reduce(bd, bu, bv);
     if (threadIdx.x == 0)
         part[blockIdx.x] = make_int4((int32_t)(uint32_t)(uint64_t)bd, (int32_t)(uint32_t)((uint64_t)bd >> 32), bu, bv);
 }
 
-/* One block: the best of nparts block results (nparts >= 1). */
+/* Best partner of each u in u0 .. u0 + count - 1: block i takes u0 + i (and
+ * every gridDim.x-th after it) and writes {delta lo, delta hi, u, v} for it
+ * (delta INT64_MIN, v INT32_MAX when u has no partner v > u). */
+__global__ void __launch_bounds__(kBestThreads) k_gain_best_partners(BmDevGraph g, BmGain G,
+                                                                     const int32_t* __restrict__ GT,
+                                                                     const int4* __restrict__ pis, int32_t u0,
+                                                                     int32_t count, int4* __restrict__ out) {
+    for (int32_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const int32_t u = u0 + i;
+        int64_t bd = INT64_MIN;
+        int32_t bu = u, bv = INT32_MAX;
+        best_partner_scan(g, G, GT, pis, u, bd, bu, bv);
+        best_block_reduce(bd, bu, bv);
+        if (threadIdx.x == 0)
+            out[i] = make_int4((int32_t)(uint32_t)(uint64_t)bd, (int32_t)(uint32_t)((uint64_t)bd >> 32), u, bv);
+        __syncthreads();   /* the reduction's shared words are reused for the next u */
+    }
+}
+
 __global__ void __launch_bounds__(kBestThreads) k_best_swap_final(const int4* __restrict__ part, int32_t nparts,
                                                                   int4* __restrict__ out) {
     int64_t bd = INT64_MIN;
@@ -1552,6 +1576,18 @@ hipError_t bm_launch_gain_best_swap(const BmDevGraph& g, const BmGain& G, const 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_best_swap_final, dim3(1), dim3(kBestThreads), 0, s, part, (int32_t)blocks, out);
+    return hipGetLastError();
+}
+
+hipError_t bm_launch_gain_best_partners(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
+                                        int32_t u0, int32_t count, int4* out, int32_t num_cus, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    if (u0 < 1 || (int64_t)u0 + count - 1 > g.n) return hipErrorInvalidValue;
+    int64_t blocks = count;
+    const int64_t cap = (int64_t)8 * (num_cus > 0 ? num_cus : 256);
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL(k_gain_best_partners, dim3((unsigned)blocks), dim3(kBestThreads), 0, s, g, G, GT, pis, u0, count,
+                       out);
     return hipGetLastError();
 }
 

@@ -15,6 +15,12 @@
  * recount, where the reference recounts on the host after every move (:1249).
  * The random streams are seeded xoshiro256**, not time-seeded rand().
  *
+ * bm_sweep is the OpenMP tool's loop (src/re-alignment-parallel.c:556-676):
+ * passes over the vertices, each swapped with its best partner when that
+ * delta is positive, restarts from a perturbed best matching after a pass
+ * without a swap; the best partners of a run of consecutive vertices come
+ * from one device launch, and the run is cut at the first swap applied.
+ *
  * bm_hillclimb is the greedy disjoint-swap climb: each round evaluates a
  * batch of uniform pairs and applies a disjoint set of positive swaps.
  * bm_hillclimb_twohop does the same with 2-hop proposals, the device-wide
@@ -423,6 +429,142 @@ int bm_best_swap_climb(bm_ctx* ctx, int64_t max_iters, int64_t verify_every, int
     res->best_score = res->final_score = score;
     res->seconds = now_s() - t0;
     return BM_OK;
+}
+
+/* Random disjoint swaps of perturbation p: 2k distinct vertices drawn from
+ * the stream bm_rand_at(bm_rand_at(seed, p), j), the i-th swapped with the
+ * (i + k)-th (random_swap_k_vertices, src/re-alignment-parallel.c:495-537,
+ * draws them with rand() % n). */
+static int sweep_perturb(int32_t* pi, int32_t n, uint64_t seed, uint64_t p, int32_t k, uint8_t* used, int32_t* sel) {
+    if (k <= 0) return 0;
+    if ((int64_t)2 * k > n) return -1;
+    memset(used, 0, (size_t)n + 1);
+    const uint64_t st = bm_rand_at(seed, p);
+    int32_t cnt = 0;
+    for (uint64_t j = 0; cnt < 2 * k; j++) {
+        const int32_t x = 1 + (int32_t)bm_range32((uint32_t)(bm_rand_at(st, j) >> 32), (uint32_t)n);
+        if (!used[x]) {
+            used[x] = 1;
+            sel[cnt++] = x;
+        }
+    }
+    for (int32_t i = 0; i < k; i++) {
+        const int32_t t = pi[sel[i]];
+        pi[sel[i]] = pi[sel[i + k]];
+        pi[sel[i + k]] = t;
+    }
+    return 0;
+}
+
+int bm_sweep(bm_ctx* ctx, const bm_sweep_opts* o, int32_t* best_pi, int64_t* pass_trace, bm_search_result* res) {
+    if (!ctx || !o || !res) return BM_ERR_ARG;
+    memset(res, 0, sizeof *res);
+    if (!bm_gain_enabled(ctx))
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_sweep: needs the placement-gain table (bm_gain_enable)");
+    if (o->max_passes < 0 || o->time_limit_s < 0 || (o->max_passes == 0 && o->time_limit_s <= 0))
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_sweep: set max_passes > 0 or time_limit_s > 0 (both >= 0)");
+    if (o->initial_swaps < 0 || o->restart_swaps < 0 || o->batch < 0)
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_sweep: initial_swaps, restart_swaps and batch must be >= 0");
+    const int32_t n = bm_num_vertices(ctx);
+    if ((int64_t)2 * o->initial_swaps > n || (int64_t)2 * o->restart_swaps > n)
+        return bm_ctx_fail(ctx, BM_ERR_ARG, "bm_sweep: a perturbation of k swaps needs 2k <= n vertices");
+    const int32_t batch = o->batch > 0 ? o->batch : 512;
+    const double t0 = now_s();
+    int32_t* pi = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int32_t* best = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int32_t* sel = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    uint8_t* used = (uint8_t*)malloc((size_t)n + 1);
+    int64_t* dl = (int64_t*)malloc(sizeof(int64_t) * (size_t)batch);
+    int32_t* vl = (int32_t*)malloc(sizeof(int32_t) * (size_t)batch);
+    int rc = BM_OK;
+    if (!pi || !best || !sel || !used || !dl || !vl) {
+        rc = bm_ctx_fail(ctx, BM_ERR_NOMEM, "bm_sweep: out of host memory");
+        goto done;
+    }
+    int64_t score = 0;
+    if ((rc = bm_score(ctx, &score)) != BM_OK) goto done;
+    res->initial_score = score;
+    if ((rc = bm_get_mapping(ctx, pi)) != BM_OK) goto done;
+    uint64_t nperturb = 0;
+    sweep_perturb(pi, n, o->seed, nperturb++, o->initial_swaps, used, sel);
+    if (o->initial_swaps > 0) {
+        if ((rc = bm_set_mapping(ctx, pi)) != BM_OK || (rc = bm_score(ctx, &score)) != BM_OK) goto done;
+    }
+    int64_t best_score = score;
+    memcpy(best, pi, sizeof(int32_t) * ((size_t)n + 1));
+    int64_t pass = 0, improvements = 0, last = 0, passes = 0;
+    int out_of_time = 0;
+    while (!out_of_time && (o->max_passes == 0 || passes < o->max_passes)) {
+        for (int32_t u = 1; u <= n;) {
+            if (o->time_limit_s > 0 && now_s() - t0 >= o->time_limit_s) {
+                out_of_time = 1;
+                break;
+            }
+            const int32_t cnt = n - u + 1 < batch ? n - u + 1 : batch;
+            if ((rc = bm_best_partners(ctx, u, cnt, dl, vl)) != BM_OK) goto done;
+            int32_t i = 0;
+            while (i < cnt && !(dl[i] > 0)) {
+                res->evaluations += n - (u + i);
+                i++;
+            }
+            if (i == cnt) {
+                u += cnt;
+                continue;
+            }
+            res->evaluations += n - (u + i);
+            const int32_t a = u + i, b = vl[i];
+            if ((rc = bm_apply_swap(ctx, a, b)) != BM_OK) goto done;
+            const int32_t t = pi[a];
+            pi[a] = pi[b];
+            pi[b] = t;
+            score += dl[i];
+            improvements++;
+            res->accepted++;
+            if (score > best_score) {
+                best_score = score;
+                memcpy(best, pi, sizeof(int32_t) * ((size_t)n + 1));
+            }
+            if (o->verify) {
+                int64_t s2 = 0;
+                if ((rc = bm_score(ctx, &s2)) != BM_OK) goto done;
+                if (s2 != score) {
+                    res->verify_failures++;
+                    score = s2;
+                }
+            }
+            u = a + 1;
+        }
+        if (out_of_time) break;
+        pass++;
+        passes++;
+        if (pass_trace && o->max_passes > 0) pass_trace[passes - 1] = best_score;
+        if (pass > 2 && improvements == last) {   /* no swap in this pass: restart from the best, perturbed */
+            memcpy(pi, best, sizeof(int32_t) * ((size_t)n + 1));
+            sweep_perturb(pi, n, o->seed, nperturb++, o->restart_swaps, used, sel);
+            if ((rc = bm_set_mapping(ctx, pi)) != BM_OK || (rc = bm_score(ctx, &score)) != BM_OK) goto done;
+            improvements = last = pass = 0;
+            res->restarts++;
+        }
+        if (last < improvements) last = improvements;
+    }
+    res->iterations = passes;
+    if ((rc = bm_set_mapping(ctx, best)) != BM_OK) goto done;
+    res->best_score = res->final_score = best_score;
+    if (o->verify) {
+        int64_t s2 = 0;
+        if ((rc = bm_score(ctx, &s2)) != BM_OK) goto done;
+        if (s2 != best_score) res->verify_failures++;
+    }
+    if (best_pi) memcpy(best_pi, best, sizeof(int32_t) * ((size_t)n + 1));
+    res->seconds = now_s() - t0;
+done:
+    free(pi);
+    free(best);
+    free(sel);
+    free(used);
+    free(dl);
+    free(vl);
+    return rc;
 }
 
 int bm_hillclimb_twohop(bm_ctx* ctx, uint64_t seed, int64_t rounds, int32_t anchors, int32_t per_anchor,

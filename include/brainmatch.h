@@ -334,8 +334,9 @@ int bm_gain_enabled(const bm_ctx* ctx);
  * 16-bit column arrays, 16 male rows at a time, instead of expanding them
  * edge by edge: the longest rows of at least a quarter of the columns, at
  * most 128 and at most 65535 / the largest female weight; only for graphs
- * whose weights fit 16 bits and whose male weights are all >= 0, and only
- * while no two male vertices share a female.  0 when the table is off or no
+ * whose weights fit 16 bits and whose male weights are all >= 0, whose
+ * table rows fit one LDS column window (n + 1 <= 40,768), and only while no
+ * two male vertices share a female.  0 when the table is off or no
  * row qualifies.  An introspection call with no counterpart in the
  * reference: the table it describes is the one above. */
 int bm_gain_dense_rows(const bm_ctx* ctx);
@@ -350,6 +351,14 @@ int bm_gain_row(bm_ctx* ctx, int32_t u, int32_t* row);
  * stale and no delta is truncated to int16.  Needs the table (BM_ERR_ARG
  * otherwise) and, once, a transposed copy of it (bm_gain_table_bytes more). */
 int bm_best_swap(bm_ctx* ctx, int64_t* delta, int32_t* u, int32_t* v);
+/* Best partner of every u in u0 .. u0 + count - 1 under the context
+ * matching: delta[i] = the largest exact delta of swapping u = u0 + i with a
+ * v in u+1..n, v[i] = that v (ties to the smaller v); delta[i] = INT64_MIN and
+ * v[i] = 0 for u = n.  Replaces the inner loop of the OpenMP tool's sweep
+ * (src/re-alignment-parallel.c:598-621: the largest calculate_swap_delta
+ * over node_m2 > node_m1, the first found among equal ones).  Needs the
+ * placement-gain table (BM_ERR_ARG otherwise) and its transposed copy. */
+int bm_best_partners(bm_ctx* ctx, int32_t u0, int32_t count, int64_t* delta, int32_t* v);
 
 /* ------------------------------------------------------------------ */
 /* Measurement                                                         */
@@ -438,6 +447,7 @@ typedef struct bm_search_result {
     double seconds;
     int64_t fallback_rounds;  /* bm_climb, learned proposal with fallback: rounds that took uniform
                                  candidates (0 elsewhere) */
+    int64_t restarts;         /* bm_sweep: restarts from a perturbed best matching (0 elsewhere) */
 } bm_search_result;
 
 /* The perturb -> evaluate -> accept loop of optimize_mapping
@@ -465,6 +475,40 @@ int bm_hillclimb(bm_ctx* ctx, uint64_t seed, int64_t rounds, int64_t batch, int3
  * max_iters stop.  Needs the placement-gain table. */
 int bm_best_swap_climb(bm_ctx* ctx, int64_t max_iters, int64_t verify_every, int64_t* score_trace,
                        bm_search_result* res);
+/* Options of bm_sweep.  The random disjoint swaps of perturbation p (0 =
+ * before the first pass, 1, 2, ... = the restarts) pick 2k distinct vertices
+ * of 1..n from the stream bm_rand_at(bm_rand_at(seed, p), j), j = 0, 1, ...
+ * (vertex 1 + the high 32 bits of each draw times n, >> 32; repeats skipped)
+ * and swap the images of the i-th and (i + k)-th. */
+typedef struct bm_sweep_opts {
+    uint64_t seed;
+    int32_t initial_swaps;    /* random disjoint swaps before the first pass (the tool: 50) */
+    int32_t restart_swaps;    /* random disjoint swaps of the best matching at a restart (the tool: 200) */
+    int32_t batch;            /* vertices whose best partners one launch finds (0: 512) */
+    int32_t verify;           /* recount the score after every applied swap, as the tool does */
+    int64_t max_passes;       /* passes over the vertices, restarts included (0: no limit) */
+    double time_limit_s;      /* checked between launches (0: no limit); one of the two limits is required */
+} bm_sweep_opts;
+/* The OpenMP tool's optimisation loop, optimize_mapping of
+ * src/re-alignment-parallel.c:556-676: after a random perturbation, passes
+ * over node_m1 = 1..n, each swapping node_m1 with its best partner
+ * (bm_best_partners) when that delta is > 0 and keeping the best matching;
+ * from the third pass on, a pass without an applied swap restarts from the
+ * best matching perturbed by restart_swaps random swaps.  Each vertex is
+ * evaluated under the matching left by the swaps before it, as in the tool:
+ * a launch finds the best partners of `batch` consecutive vertices, the first
+ * of them with a positive delta is swapped, and the next launch starts after
+ * it.  Differences from the tool, all deliberate: its perturbation draws
+ * rand() % n (so it can swap vertex 0's slot and never vertex n) and is seeded
+ * by the clock; ties among best partners go to the smaller v (its OpenMP
+ * critical section keeps whichever thread reports first); it runs until
+ * killed.  On return the context holds the best matching (res->final_score =
+ * res->best_score), best_pi (n+1 ints, may be NULL) receives it, pass_trace
+ * (max_passes entries, may be NULL; only with max_passes > 0) the best score
+ * after each pass; res->iterations = passes, res->accepted = swaps applied,
+ * res->evaluations = pairs evaluated, res->restarts.  Needs the
+ * placement-gain table. */
+int bm_sweep(bm_ctx* ctx, const bm_sweep_opts* opts, int32_t* best_pi, int64_t* pass_trace, bm_search_result* res);
 /* The same climb over 2-hop proposals: round r evaluates
  * bm_batch_gen_twohop(NULL, anchors, per_anchor, seed + r) (refine-twohop-random's
  * proposal, src/refine-twohop-random.c:676-693, with 10 anchors x 1000

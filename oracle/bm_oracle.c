@@ -49,6 +49,19 @@
  *                               graphs; the climb's score trace equals
  *                               or_score after every swap and ends with no
  *                               positive pair left.
+ *  or_best_partner / or_sweep   the OpenMP tool's sweep (src/re-alignment-parallel.c:556-676):
+ *                               best partner v > u of each u in turn (ties to
+ *                               the smaller v), swapped when its delta is > 0;
+ *                               a restart from the best matching perturbed by
+ *                               k disjoint swaps after a pass without a swap
+ *                               from the third pass on; the perturbation's
+ *                               vertex stream as bm_sweep documents it.
+ *                               Pinned: best partners equal the argmax of
+ *                               brute-force recounts; the pass trace equals
+ *                               or_score of the best matching; a pass that
+ *                               applies no swap leaves no positive pair; a
+ *                               perturbation moves exactly 2k distinct
+ *                               vertices and is a permutation of the images.
  *  or_uniform_pair              the candidate stream definition; pinned by
  *                               comparing with pairs downloaded from the device.
  *  or_cpu_baseline_*            the reference CPU arithmetic (a4) driven by an
@@ -1165,6 +1178,103 @@ void or_best_swap(const or_graph* g, const int32_t* pi, int64_t* delta, int32_t*
     *delta = best;
     *bu = ru;
     *bv = rv;
+}
+
+/* Best partner v > u of u: the largest or_swap_delta, ties to the smaller v
+ * (INT64_MIN and 0 when u = n). */
+void or_best_partner(const or_graph* g, const int32_t* pi, int32_t u, int64_t* delta, int32_t* bv) {
+    int64_t best = INT64_MIN;
+    int32_t rv = 0;
+    for (int32_t v = u + 1; v <= g->n; v++) {
+        const int64_t d = swap_delta_fast(g, pi, u, v);
+        if (rv == 0 || d > best) {
+            best = d;
+            rv = v;
+        }
+    }
+    *delta = best;
+    *bv = rv;
+}
+
+static uint64_t or_rand_at(uint64_t seed, uint64_t c) {
+    return mix64(seed * 0x9E3779B97F4A7C15ULL + (c + 1) * 0xD1B54A32D192ED03ULL);
+}
+
+/* Perturbation p of seed: 2k distinct vertices of 1..n from the stream
+ * or_rand_at(or_rand_at(seed, p), j) (vertex 1 + high 32 bits x n >> 32,
+ * repeats skipped), the i-th image swapped with the (i + k)-th.  sel gets the
+ * 2k vertices. */
+void or_sweep_perturb(int32_t* pi, int32_t n, uint64_t seed, uint64_t p, int32_t k, int32_t* sel) {
+    if (k <= 0) return;
+    uint8_t* used = (uint8_t*)calloc((size_t)n + 1, 1);
+    const uint64_t st = or_rand_at(seed, p);
+    int32_t cnt = 0;
+    for (uint64_t j = 0; cnt < 2 * k; j++) {
+        const uint32_t hi = (uint32_t)(or_rand_at(st, j) >> 32);
+        const int32_t x = 1 + (int32_t)(((uint64_t)hi * (uint64_t)(uint32_t)n) >> 32);
+        if (!used[x]) {
+            used[x] = 1;
+            sel[cnt++] = x;
+        }
+    }
+    for (int32_t i = 0; i < k; i++) {
+        const int32_t t = pi[sel[i]];
+        pi[sel[i]] = pi[sel[i + k]];
+        pi[sel[i + k]] = t;
+    }
+    free(used);
+}
+
+/* The sweep over max_passes passes (restarts included).  pi: start in, best
+ * out.  trace[p] = best score after pass p; out = {best score, swaps applied,
+ * restarts, pairs evaluated}. */
+void or_sweep(const or_graph* g, int32_t* pi, uint64_t seed, int64_t max_passes, int32_t initial_swaps,
+              int32_t restart_swaps, int64_t* trace, int64_t* out) {
+    const int32_t n = g->n;
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    int32_t* sel = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    memcpy(cur, pi, sizeof(int32_t) * ((size_t)n + 1));
+    uint64_t np = 0;
+    or_sweep_perturb(cur, n, seed, np++, initial_swaps, sel);
+    int64_t score = or_score(g, cur), best = score, accepted = 0, restarts = 0, evals = 0;
+    memcpy(pi, cur, sizeof(int32_t) * ((size_t)n + 1));
+    int64_t pass = 0, improvements = 0, last = 0;
+    for (int64_t passes = 0; passes < max_passes; passes++) {
+        for (int32_t u = 1; u <= n; u++) {
+            int64_t d;
+            int32_t v;
+            or_best_partner(g, cur, u, &d, &v);
+            evals += n - u;
+            if (d > 0) {
+                const int32_t t = cur[u];
+                cur[u] = cur[v];
+                cur[v] = t;
+                score += d;
+                improvements++;
+                accepted++;
+                if (score > best) {
+                    best = score;
+                    memcpy(pi, cur, sizeof(int32_t) * ((size_t)n + 1));
+                }
+            }
+        }
+        pass++;
+        trace[passes] = best;
+        if (pass > 2 && improvements == last) {
+            memcpy(cur, pi, sizeof(int32_t) * ((size_t)n + 1));
+            or_sweep_perturb(cur, n, seed, np++, restart_swaps, sel);
+            score = or_score(g, cur);
+            improvements = last = pass = 0;
+            restarts++;
+        }
+        if (last < improvements) last = improvements;
+    }
+    out[0] = best;
+    out[1] = accepted;
+    out[2] = restarts;
+    out[3] = evals;
+    free(cur);
+    free(sel);
 }
 
 /* Steepest ascent: apply the best pair while its delta is > 0, at most

@@ -350,6 +350,37 @@ def uniform_triple(seed, c, n):
     return a.value, b.value, d.value
 
 
+_sig(_o, "or_best_partner", None, [vp, i32p, C.c_int32, i64p, i32p])
+_sig(_o, "or_sweep_perturb", None, [i32p, C.c_int32, C.c_uint64, C.c_uint64, C.c_int32, i32p])
+_sig(_o, "or_sweep", None, [vp, i32p, C.c_uint64, C.c_int64, C.c_int32, C.c_int32, i64p, i64p])
+
+
+def best_partner(o: "Oracle", pi, u):
+    """(delta, v) of the best partner v > u of u (ties to the smaller v)."""
+    pi = _a32(pi)
+    d, v = C.c_int64(), C.c_int32()
+    _o.or_best_partner(o.h, _p32(pi), u, C.byref(d), C.byref(v))
+    return d.value, v.value
+
+
+def sweep_perturb(pi, n, seed, p, k):
+    """The sweep's perturbation p of pi (a copy) and the 2k vertices it moved."""
+    pi = _a32(pi).copy()
+    sel = np.zeros(max(2 * k, 1), np.int32)
+    _o.or_sweep_perturb(_p32(pi), n, seed, p, k, _p32(sel))
+    return pi, sel[:2 * k]
+
+
+def sweep(o: "Oracle", pi, seed, max_passes, initial_swaps=50, restart_swaps=200):
+    """Replay of the per-vertex best-partner sweep: (best matching, best score
+    after each pass, {best, accepted, restarts, evaluations})."""
+    pi = _a32(pi).copy()
+    trace = np.zeros(max_passes, np.int64)
+    out = np.zeros(4, np.int64)
+    _o.or_sweep(o.h, _p32(pi), seed, max_passes, initial_swaps, restart_swaps, _p64(trace), _p64(out))
+    return pi, trace, dict(zip(("best_score", "accepted", "restarts", "evaluations"), (int(x) for x in out)))
+
+
 def chain_seed(seed, index):
     return int(_o.or_chain_seed(seed, index))
 

@@ -294,6 +294,55 @@ def test_best_swap_climb_ends_at_local_optimum(bm, orc):
     assert np.array_equal(q, pi_end)
 
 
+@pytest.mark.parametrize("seed", [3, 8])
+def test_best_partner_is_brute_force_argmax(bm, orc, seed):
+    """or_best_partner = the argmax over v > u of full recounts (ties to the
+    smaller v) for every u of the quirky graph; u = n has no partner."""
+    male, female, pi = quirky_pair(bm, seed=seed)
+    o = orc.Oracle(60, male, female)
+    for u in range(1, 60):
+        want = max(((o.swap_delta_brute(pi, u, v), -v) for v in range(u + 1, 61)))
+        assert orc.best_partner(o, pi, u) == (want[0], -want[1]), u
+    assert orc.best_partner(o, pi, 60) == (-(1 << 63), 0)
+
+
+def test_sweep_perturbation_moves_2k_distinct_vertices(orc):
+    """A perturbation of k swaps exchanges the images of 2k distinct vertices
+    of 1..n in pairs (so the images stay a permutation); k = 0 moves nothing;
+    the vertices drawn over many perturbations cover 1..n evenly."""
+    n = 50
+    pi = np.concatenate([[0], np.arange(101, 151)]).astype(np.int32)
+    counts = np.zeros(n + 1, np.int64)
+    for p in range(400):
+        q, sel = orc.sweep_perturb(pi, n, 17, p, 10)
+        assert len(set(sel.tolist())) == 20 and sel.min() >= 1 and sel.max() <= n
+        assert np.array_equal(np.sort(q), np.sort(pi)) and q[0] == 0
+        moved = np.nonzero(q != pi)[0]
+        assert np.array_equal(np.sort(moved), np.sort(sel))
+        assert np.array_equal(q[sel[:10]], pi[sel[10:]]) and np.array_equal(q[sel[10:]], pi[sel[:10]])
+        counts[sel] += 1
+    assert counts[0] == 0 and np.abs(counts[1:] - 160).max() < 6 * np.sqrt(160)
+    q, sel = orc.sweep_perturb(pi, n, 17, 0, 0)
+    assert np.array_equal(q, pi) and len(sel) == 0
+
+
+def test_sweep_trace_and_local_optimum(bm, orc):
+    """The sweep's pass trace is the best score so far and equals a recount of
+    the returned matching at the end; with restarts of 0 swaps (a restart
+    then resumes from the best matching itself) the sweep converges to a
+    matching with no positive swap over all pairs; with the tool's restarts
+    the best score never falls below the converged one's start."""
+    male, female, pi = quirky_pair(bm, seed=4)
+    o = orc.Oracle(60, male, female)
+    best, trace, r = orc.sweep(o, pi, seed=5, max_passes=12, initial_swaps=5, restart_swaps=0)
+    assert np.all(np.diff(trace) >= 0) and trace[-1] == r["best_score"] == o.score(best)
+    assert r["restarts"] >= 1 and r["accepted"] > 0
+    assert o.best_swap(best)[0] <= 0
+    assert r["evaluations"] == 12 * 60 * 59 // 2
+    best2, trace2, r2 = orc.sweep(o, pi, seed=5, max_passes=12, initial_swaps=5, restart_swaps=10)
+    assert trace2[-1] == o.score(best2) and trace2[2] == trace[2]
+
+
 def _last_weights(g):
     """(src, dst) -> weight with duplicate lines keeping the last weight."""
     w = {}
