@@ -340,15 +340,38 @@ static int gain_moves_cap(bm_ctx* c, int64_t need) {
     return BM_OK;
 }
 
-static int gain_patch(bm_ctx* c, const int64_t* d_count, int64_t max_moves) {
+/* Male degree (out + in) of vertex x. */
+static int64_t male_degree(const bm_ctx* c, int32_t x) {
+    return (c->male.out.ptr[x + 1] - c->male.out.ptr[x]) + (c->male.in.ptr[x + 1] - c->male.in.ptr[x]);
+}
+
+/* Whether a patch of moves of the vertices xs (k of them) should also patch
+ * the transposed copy: its additions land in as many rows as the moved
+ * vertices have male edges, one 4-byte atomic per row and female entry, so it
+ * pays against a 0.54 ms transpose (n = 18,524) only for moves of vertices of
+ * ordinary degree (together at most 8x the mean); hub moves leave the copy to
+ * be rebuilt. */
+static bool gt_patch_pays(const bm_ctx* c, const int32_t* xs, int k) {
+    int64_t deg = 0;
+    for (int i = 0; i < k; i++) deg += male_degree(c, xs[i]);
+    const int64_t mean2 = c->n > 0 ? 2 * c->male.m / c->n : 0;
+    return deg <= 8 * (mean2 > 1 ? mean2 : 1);
+}
+
+static int gain_patch(bm_ctx* c, const int64_t* d_count, int64_t max_moves, bool gt_allowed) {
     KTiming t;
     timing_begin(c, BM_KERNEL_GAIN_UPDATE, &t);
     if (max_moves > c->moves_cap) return fail(c, BM_ERR_STATE, "gain patch: %lld moves exceed the move list",
                                               (long long)max_moves);
-    HIPCHK(c, bm_launch_gain_update(devgraph(c), devgain(c), c->d_moves, d_count, max_moves, c->d_patch_plan,
-                                    c->num_cus, c->stream));
+    /* a transposed copy that is current is patched alongside (the same
+     * additions at transposed positions) instead of being rebuilt later when
+     * the caller found that cheaper (gt_patch_pays) */
+    const bool gt = gt_allowed && c->d_GT && c->gt_version == c->gain_version;
+    HIPCHK(c, bm_launch_gain_update(devgraph(c), devgain(c), gt ? c->d_GT : nullptr, c->d_moves, d_count, max_moves,
+                                    c->d_patch_plan, c->num_cus, c->stream));
     timing_end(c, &t);
     c->gain_version++;
+    if (gt) c->gt_version = c->gain_version;
     return BM_OK;
 }
 
@@ -1080,7 +1103,7 @@ static int accept_impl(bm_batch* b, int64_t max_accept, int32_t* accepted, int64
                                b->d_acc_scratch, b->acc_scratch_bytes, b->d_acc_idx, b->d_out,
                                c->gain_on ? c->d_moves : nullptr, c->stream));
     if (c->gain_on && b->k > 0) {
-        int rc = gain_patch(c, b->d_out + 2, 2 * b->k);
+        int rc = gain_patch(c, b->d_out + 2, 2 * b->k, false);
         if (rc != BM_OK) return rc;
     }
     int64_t meta[2];
@@ -1188,7 +1211,10 @@ extern "C" int bm_apply_swap(bm_ctx* c, int32_t u, int32_t v) {
         HIPCHK(c, bm_launch_moves_from_swap(c->d_pi, u, v, c->d_moves, c->d_nmoves, c->stream));
     }
     HIPCHK(c, bm_launch_apply_swap(c->d_pi, u, v, c->stream));
-    if (c->gain_on) return gain_patch(c, c->d_nmoves, 2);
+    if (c->gain_on) {
+        const int32_t xs[2] = {u, v};
+        return gain_patch(c, c->d_nmoves, 2, gt_patch_pays(c, xs, 2));
+    }
     return BM_OK;
 }
 
@@ -1275,7 +1301,10 @@ extern "C" int bm_apply_perm3(bm_ctx* c, int32_t a, int32_t b, int32_t d, int32_
         HIPCHK(c, bm_launch_moves_from_perm3(c->d_pi, a, b, d, p, c->d_moves, c->d_nmoves, c->stream));
     }
     HIPCHK(c, bm_launch_apply_perm3(c->d_pi, a, b, d, p, c->stream));
-    if (c->gain_on) return gain_patch(c, c->d_nmoves, 3);
+    if (c->gain_on) {
+        const int32_t xs[3] = {a, b, d};
+        return gain_patch(c, c->d_nmoves, 3, gt_patch_pays(c, xs, 3));
+    }
     return BM_OK;
 }
 

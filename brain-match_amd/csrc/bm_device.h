@@ -110,8 +110,9 @@ hipError_t bm_launch_row_bounds(const int64_t* ptr, const int2* rows, int32_t n,
  * rows.  G.edesc must hold 2 x male edges + 1 entries. */
 hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t num_cus, hipStream_t s);
 /* Table patch after moves (moves[0..*count), *count <= max_moves); plan
- * holds 4 max_moves + 1 int64 of scratch for the work-item offsets. */
-hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int4* moves, const int64_t* count,
+ * holds 4 max_moves + 1 int64 of scratch for the work-item offsets; GT (may
+ * be NULL) is a transposed copy patched alongside. */
+hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, int32_t* GT, const int4* moves, const int64_t* count,
                                  int64_t max_moves, int64_t* plan, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, const int64_t* off,
                                uint64_t seed, int64_t k, int64_t* delta, hipStream_t s);

@@ -704,7 +704,8 @@ __global__ void __launch_bounds__(kPlanThreads) k_gain_patch_plan(BmDevGraph g, 
 
 __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmGain G, const int4* __restrict__ moves,
                                                                const int64_t* __restrict__ count,
-                                                               const int64_t* __restrict__ off) {
+                                                               const int64_t* __restrict__ off,
+                                                               int32_t* __restrict__ GT) {
     const int lane = threadIdx.x & (BM_WAVE - 1);
     const int64_t slots = 4 * *count;
     const int64_t total = off[slots];
@@ -741,7 +742,11 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
             const int32_t y = __builtin_amdgcn_readlane(yl, k);
             if (y == 0) continue;
             const int32_t w = __builtin_amdgcn_readlane(ed.y, k);
-            if (rin) atomicAdd(&G.G[(int64_t)y * G.stride + ent.x], sign * (w < ent.y ? w : ent.y));
+            if (rin) {
+                const int32_t d = sign * (w < ent.y ? w : ent.y);
+                atomicAdd(&G.G[(int64_t)y * G.stride + ent.x], d);
+                if (GT) atomicAdd(&GT[(int64_t)ent.x * G.stride + y], d);   /* the transposed copy, when kept */
+            }
         }
     }
 }
@@ -1494,13 +1499,13 @@ hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t nu
     }
 }
 
-hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, const int4* moves, const int64_t* count,
+hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, int32_t* GT, const int4* moves, const int64_t* count,
                                  int64_t max_moves, int64_t* plan, int32_t num_cus, hipStream_t s) {
     if (max_moves <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gain_patch_plan, dim3(1), dim3(kPlanThreads), 0, s, g, moves, count, plan);
     /* persistent grid: every wave walks the item space */
     const int64_t blocks = (int64_t)(num_cus > 0 ? num_cus : 256) * 4;
-    hipLaunchKernelGGL(k_gain_update, dim3((unsigned)blocks), dim3(kPatchThreads), 0, s, g, G, moves, count, plan);
+    hipLaunchKernelGGL(k_gain_update, dim3((unsigned)blocks), dim3(kPatchThreads), 0, s, g, G, moves, count, plan, GT);
     return hipGetLastError();
 }
 

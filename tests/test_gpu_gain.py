@@ -106,6 +106,44 @@ def test_patches_equal_rebuild(g1, orc):
     ctx.set_mapping(pi0)
 
 
+def test_transposed_copy_follows_patches(g1, orc):
+    """u-major lookups read the transposed table.  Swaps and 3-permutations
+    of ordinary vertices applied while it is current patch it alongside the
+    table; a hub's swap and a large acceptance batch leave it to be rebuilt.
+    After each, u-major deltas and best partners equal the oracle."""
+    ctx, o, n = g1["ctx"], g1["o"], g1["n"]
+    pi0 = g1["pi0"]
+    ctx.set_mapping(pi0)
+    k = 1 << 15
+    b = ctx.batch(k)
+
+    def check(seed):
+        pi = ctx.get_mapping()
+        b.eval_swaps_umajor(seed, k)
+        assert np.array_equal(b.deltas(), o.swap_deltas(pi, orc.umajor_pairs(seed, k, n))), seed
+        d, v = ctx.best_partners(1, 40)
+        assert [(int(x), int(y)) for x, y in zip(d, v)] == [orc.best_partner(o, pi, u) for u in range(1, 41)]
+
+    od, idg = ctx.degrees()
+    deg = od + idg
+    order = np.argsort(deg[1:]) + 1
+    low = [int(x) for x in order[:6]]          # ordinary vertices: the copy is patched
+    hub = int(order[-1])                       # a hub: the copy is rebuilt
+    check(1)
+    ctx.apply_swap(low[0], low[1])
+    check(2)
+    ctx.apply_perm3(low[2], low[3], low[4], 3)
+    ctx.apply_swap(low[4], low[5])
+    check(3)
+    ctx.apply_swap(hub, low[0])
+    check(5)
+    uv = orc.uniform_pairs(43, 20000, n)
+    ctx.accept_disjoint(uv, ctx.swap_deltas(uv))
+    check(4)
+    b.close()
+    ctx.set_mapping(pi0)
+
+
 def test_hillclimb_with_gain_matches_replay(g1, orc):
     ctx, o, pi = g1["ctx"], g1["o"], g1["pi0"]
     ctx.set_mapping(pi)
