@@ -1488,7 +1488,9 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
             HIPCHK(c, bm_launch_adj_bits(devgraph(c), c->d_m_adj, words, c->num_cus, c->stream));
         }
     }
-    rc = dalloc(c, &c->d_G, bytes / sizeof(int32_t));
+    /* 16 rows of padding past row n: the dense tile pass stores whole tiles of
+     * 16 male rows */
+    rc = dalloc(c, &c->d_G, bytes / sizeof(int32_t) + (size_t)16 * (size_t)bm_gain_stride(c->n));
     if (rc != BM_OK) return rc;
     /* row 0 is never written by the build; clear it once */
     HIPCHK(c, hipMemsetAsync(c->d_G, 0, (size_t)bm_gain_stride(c->n) * sizeof(int32_t), c->stream));

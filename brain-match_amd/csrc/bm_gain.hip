@@ -356,7 +356,7 @@ __device__ __forceinline__ uint4 load_weights(const uint32_t* p) {
  * not to alias the table writes and become scalar loads). */
 __device__ __forceinline__ void dense_tile_words(const uint32_t* __restrict__ wt, const uint32_t* __restrict__ dense,
                                                  int32_t* __restrict__ gout, int64_t stride, int64_t hw, int32_t nwords,
-                                                 int32_t nd, int32_t u0, int32_t n, int tid) {
+                                                 int32_t nd, int32_t u0, int tid) {
     /* the word loop and everything in it are block-uniform (scalar
      * weight loads); a thread's words past the row are clamped to the
      * last word for loading and not stored */
@@ -391,14 +391,14 @@ __device__ __forceinline__ void dense_tile_words(const uint32_t* __restrict__ wt
             v0 = n0;
             v1 = n1;
         }
+        /* every row of the tile is stored: row 0's sums are zero (vertex 0
+         * has no edges), and rows past n land in the table's 16 padding rows */
+        const bool s0 = w0 < nwords, s1 = w1 < nwords;
 #pragma unroll
         for (int r = 0; r < kTileRows; r++) {
-            const int32_t u = u0 + r;
-            if (u >= 1 && u <= n) {
-                int2* out = reinterpret_cast<int2*>(gout + (int64_t)u * stride);
-                if (w0 < nwords) out[w0] = make_int2((int32_t)(a0[r] & 0xffffu), (int32_t)(a0[r] >> 16));
-                if (w1 < nwords) out[w1] = make_int2((int32_t)(a1[r] & 0xffffu), (int32_t)(a1[r] >> 16));
-            }
+            int2* out = reinterpret_cast<int2*>(gout + (int64_t)(u0 + r) * stride);
+            if (s0) out[w0] = make_int2((int32_t)(a0[r] & 0xffffu), (int32_t)(a0[r] >> 16));
+            if (s1) out[w1] = make_int2((int32_t)(a1[r] & 0xffffu), (int32_t)(a1[r] >> 16));
         }
     }
 }
@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(kTileThreads) k_gain_dense_tile(BmDevGraph g, 
     for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int32_t u0 = tile * kTileRows;
         const uint32_t* wt = G.dense_w + (int64_t)tile * nd * kTileRows;
-        dense_tile_words(wt, G.dense, G.G, G.stride, hw, nwords, nd, u0, g.n, tid);
+        dense_tile_words(wt, G.dense, G.G, G.stride, hw, nwords, nd, u0, tid);
     }
 }
 
