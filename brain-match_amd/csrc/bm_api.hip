@@ -92,6 +92,7 @@ struct bm_ctx {
     int32_t nwin;
     uint32_t* d_dense;      /* dense copies of the longest female rows (BmGain::dense) */
     uint32_t* d_dense_w;    /* dense tile weights (BmGain::dense_w) */
+    int32_t* d_row_fill;    /* work-list entries per male row (BmGain::row_fill) */
     int32_t* d_f_out_dense; /* female vertex -> its dense out-row, or -1 */
     int32_t* d_f_in_dense;
     int32_t ndense;
@@ -272,6 +273,7 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_f_out_bnd);
     (void)hipFree(c->d_dense);
     (void)hipFree(c->d_dense_w);
+    (void)hipFree(c->d_row_fill);
     (void)hipFree(c->d_f_out_dense);
     (void)hipFree(c->d_f_in_dense);
     (void)hipFree(c->d_m_adj);
@@ -313,6 +315,7 @@ static BmGain devgain(const bm_ctx* c) {
     G.f_in_dense = c->d_f_in_dense;
     G.ndense = dense ? c->ndense : 0;
     G.dense_w = c->d_dense_w;
+    G.row_fill = c->d_row_fill;
     return G;
 }
 
@@ -1446,6 +1449,7 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
     /* one spare entry: the build's look-ahead reads entry base(u) of a row with no edges */
     int rc = dalloc(c, &c->d_edesc, (size_t)(2 * c->male.m + 1));
     if (rc != BM_OK) return rc;
+    if (!c->d_row_fill && (rc = dalloc(c, &c->d_row_fill, (size_t)c->n + 1)) != BM_OK) return rc;
     if (!c->d_echunks) {
         std::vector<int2> ch;
         for (int32_t u = 1; u <= c->n; u++) {

@@ -226,13 +226,13 @@ __global__ void k_row_bounds(const int64_t* __restrict__ ptr, const int2* __rest
     }
 }
 
-/* Work list of the build, one entry per male edge slot of row u (out-edges,
- * then in-edges, at base(u) = m_out_ptr[u] + m_in_ptr[u]): the female row the
+/* Work list of the build: for row u, at base(u) = m_out_ptr[u] +
+ * m_in_ptr[u], one entry per male edge whose female row is sparse and has
+ * entries in the column window (row_fill[u] of them): the female row the
  * edge expands (the in-row of the neighbour's image for an out-edge, its
- * out-row for an in-edge) clipped to the column window (the row's window
+ * out-row for an in-edge) clipped to the window (the row's window
  * boundaries are precomputed, k_row_bounds), as {start lo, start hi |
- * in-edge flag << 31, length, w}; length 0 for self-loops and unmatched
- * neighbours.
+ * in-edge flag << 31, length, w}.
  * One wave per chunk of at most kDescChunk slots of one vertex's list (the
  * chunks are fixed by the male graph, so a hub's list is spread over many
  * waves), kDescChunk / 64 slots per lane with each level of the dependent
@@ -284,38 +284,45 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
                 rb[j] += lo32[j];
             }
         }
-        /* Within the chunk, rows longer than kDescLong entries are written
-         * first (the build takes entries in list order, so long rows start
-         * early and a male row's last entries are short ones); the order
-         * within each class follows the slots. */
+        /* Only edges that expand a non-empty sparse row go on the list: a
+         * chunk reserves places for them at the front of u's list with one
+         * atomic on the row's fill count (the row walk takes that many), and
+         * within the chunk rows longer than kDescLong entries go first (the
+         * build takes entries in list order, so long rows start early and a
+         * male row's last entries are short ones).  Dense rows' edges only
+         * set their tile weight; self-loops, unmatched neighbours and rows
+         * with no entry in the window are dropped. */
         uint64_t mlong[kJ], mshort[kJ];
-        int32_t nlong = 0;
+        int32_t nlong = 0, nshort = 0;
 #pragma unroll
         for (int j = 0; j < kJ; j++) {
-            const bool valid = t0 + lane + BM_WAVE * j < t1;
-            const bool lng = valid && di[j] < 0 && re[j] - rb[j] > kDescLong;
+            const bool keep = t0 + lane + BM_WAVE * j < t1 && di[j] < 0 && re[j] > rb[j];
+            const bool lng = keep && re[j] - rb[j] > kDescLong;
             mlong[j] = __ballot(lng);
-            mshort[j] = __ballot(valid && !lng);
+            mshort[j] = __ballot(keep && !lng);
             nlong += __popcll(mlong[j]);
+            nshort += __popcll(mshort[j]);
         }
+        int32_t first = 0;
+        if (lane == 0 && nlong + nshort > 0) first = atomicAdd(&G.row_fill[u], nlong + nshort);
+        first = uni(first);
         const uint64_t below = (1ull << lane) - 1ull;
         int32_t before_l = 0, before_s = 0;
 #pragma unroll
         for (int j = 0; j < kJ; j++) {
             const int64_t t = t0 + lane + BM_WAVE * j;
             if (t < t1) {
-                const bool lng = (mlong[j] >> lane) & 1ull;
-                const int32_t pos = lng ? before_l + __popcll(mlong[j] & below)
-                                        : nlong + before_s + __popcll(mshort[j] & below);
-                const bool outp = t < nout;
-                const uint32_t side = outp ? 0u : BM_EDESC_IN;
-                const uint32_t hi = (uint32_t)((uint64_t)rb[j] >> 32) | side;
                 if (di[j] >= 0 && win == 0)   /* the dense tile kernel's weight for (u, dense row) */
                     G.dense_w[((int64_t)(u / kTileRows) * G.ndense + di[j]) * kTileRows + (u % kTileRows)] =
                         (uint32_t)ed[j].y | ((uint32_t)ed[j].y << 16);
-                out[t0 + pos] = di[j] >= 0 ? make_int4(di[j], (int32_t)(side | BM_EDESC_DENSE), 0, ed[j].y)
-                                           : make_int4((int32_t)(uint32_t)rb[j], (int32_t)hi, (int32_t)(re[j] - rb[j]),
-                                                       ed[j].y);
+                const bool lng = (mlong[j] >> lane) & 1ull, shrt = (mshort[j] >> lane) & 1ull;
+                if (lng || shrt) {
+                    const int32_t pos = first + (lng ? before_l + __popcll(mlong[j] & below)
+                                                     : nlong + before_s + __popcll(mshort[j] & below));
+                    const uint32_t side = t < nout ? 0u : BM_EDESC_IN;
+                    const uint32_t hi = (uint32_t)((uint64_t)rb[j] >> 32) | side;
+                    out[pos] = make_int4((int32_t)(uint32_t)rb[j], (int32_t)hi, (int32_t)(re[j] - rb[j]), ed[j].y);
+                }
             }
             before_l += __popcll(mlong[j]);
             before_s += __popcll(mshort[j]);
@@ -459,7 +466,7 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
     const E* __restrict__ f_out_rows = PACKED ? (const E*)G.f_out_p : (const E*)g.f_out;
     for (int32_t u = blockIdx.x + 1; u <= g.n; u += gridDim.x) {
         const int64_t base = g.m_out_ptr[u] + g.m_in_ptr[u];
-        const int32_t L = (int32_t)(g.m_out_ptr[u + 1] + g.m_in_ptr[u + 1] - base);
+        const int32_t L = uni(G.row_fill[u]);   /* the list's entries: edges of non-empty sparse rows */
         const int4* list = G.edesc + base;
         /* the row starts from its dense part (k_gain_dense_tile) or zero */
         if (G.ndense > 0) {
@@ -1460,8 +1467,12 @@ static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_
         if (zb > cus * 8) zb = cus * 8;
         hipLaunchKernelGGL(k_zero_words, dim3((unsigned)zb), dim3(256), 0, s, G.dense_w, nw);
     }
+    const int64_t fb = ((int64_t)g.n + 1 + 255) / 256;
     for (int64_t c0 = 0, win = 0; c0 < G.stride; c0 += chunk, win++) {
         const int32_t nc = (int32_t)(G.stride - c0 < chunk ? G.stride - c0 : chunk);
+        /* each window's work list counts its rows' entries afresh */
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)fb), dim3(256), 0, s, reinterpret_cast<uint32_t*>(G.row_fill),
+                           (int64_t)g.n + 1);
         hipLaunchKernelGGL(k_gain_edge_desc<LAYOUT>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc,
                            (int32_t)win);
         if (win == 0 && G.ndense > 0) {   /* every column of the dense part, from the first work list */
