@@ -1478,6 +1478,11 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
             HIPCHK(c, bm_launch_row_bounds(c->d_f_out_ptr, c->d_f_out, c->n, cols, nwin, c->d_f_out_bnd, c->stream));
         }
         c->nwin = nwin;
+        /* a table built in column windows (long rows, one block per CU) walks
+         * its rows in fewer, larger chunks: 8 entries per lane, two chunks in
+         * flight (n = 100,000: 80.3 ms against 95.3 ms with 4 entries and four
+         * chunks, which suit the single window of n = 18,524) */
+        if (!c->user_tuned[BM_KERNEL_GAIN_BUILD]) c->tune[BM_KERNEL_GAIN_BUILD].unroll = nwin > 1 ? 8 : 4;
     }
     /* dense male adjacency for the table lookups' edge tests, when it fits
      * 2 GiB (43 MB at n = 18,524); otherwise they probe the hash tables */
