@@ -332,15 +332,24 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
 
 /* Dense part of the table: G(u, f) restricted to u's edges whose female
  * row is dense, the sum over them of min(w, dense weight of column f).  One
- * block per tile of 16 male rows over every column.  The work list has put
- * each such edge's weight, packed w | w << 16, at dense_w[tile][dense row]
- * [row in tile] (0 where the male row has no edge to that dense row), so a
- * dense row's 16 weights are one scalar load; a thread owns two column pairs
- * (a 32-bit word of every dense row each) and keeps their 16 rows' sums as
- * packed 16-bit halves, one load of a dense-row word serving 16 rows.  The
- * sums stay below 2^16: there are at most 65535 / max weight dense rows.
- * The result is written to G (the row build starts from it and adds the
- * other edges). */
+ * block per work item, a tile of 16 male rows by 512 words (1,024 columns):
+ * the items are many more than the CUs' block slots, so the dispatcher
+ * evens out the CUs (one block per tile over every column left the CUs
+ * holding five tiles a quarter longer than those holding four).  The work
+ * list has put each such edge's weight, packed w | w << 16, at dense_w[tile]
+ * [dense row][row in tile] (0 where the male row has no edge to that dense
+ * row), so a dense row's 16 weights are one scalar load; a thread owns two
+ * column pairs (a 32-bit word of every dense row each) and keeps their 16
+ * rows' sums as packed 16-bit halves, one load of a dense-row word serving
+ * 16 rows; a row whose weight is 0 is skipped (a scalar branch: about 40% of
+ * the rows of a tile that uses a dense row do not).  The sums stay below
+ * 2^16: there are at most 65535 / max weight dense rows.  The result stays
+ * packed: word w of the second half of G's row u holds columns 2w (low
+ * half) and 2w + 1, and the row build unpacks it into its LDS row, adds the
+ * other edges and overwrites the row.  In the second half a column window's
+ * packed words are never under an earlier window's stores (window [c0, c1)
+ * stores bytes [4 c0, 4 c1) of the row, and its packed words start at byte
+ * 2 stride + 2 c0 >= 4 c1 of every window before it). */
 typedef unsigned short Us2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t pk_min_add(uint32_t acc, uint32_t v, uint32_t wpk) {
@@ -359,15 +368,16 @@ __device__ __forceinline__ uint4 load_weights(const uint32_t* p) {
 #endif
 }
 
-/* The tile's word loop (restrict-qualified, so the weight reads are known
- * not to alias the table writes and become scalar loads). */
+/* One work item: the tile's words [wb, wb + 512) (restrict-qualified, so
+ * the weight reads are known not to alias the table writes and become
+ * scalar loads). */
 __device__ __forceinline__ void dense_tile_words(const uint32_t* __restrict__ wt, const uint32_t* __restrict__ dense,
                                                  int32_t* __restrict__ gout, int64_t stride, int64_t hw, int32_t nwords,
-                                                 int32_t nd, int32_t u0, int tid) {
-    /* the word loop and everything in it are block-uniform (scalar
+                                                 int32_t nd, int32_t u0, int tid, int32_t wb) {
+    /* the dense-row loop and everything in it are block-uniform (scalar
      * weight loads); a thread's words past the row are clamped to the
      * last word for loading and not stored */
-    for (int32_t wb = 0; wb < nwords; wb += 2 * kTileThreads) {
+    {
         const int32_t w0 = wb + tid, w1 = w0 + kTileThreads;
         const int32_t l0 = w0 < nwords ? w0 : nwords - 1, l1 = w1 < nwords ? w1 : nwords - 1;
         uint32_t a0[kTileRows], a1[kTileRows];
@@ -391,21 +401,24 @@ __device__ __forceinline__ void dense_tile_words(const uint32_t* __restrict__ wt
                                                q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
 #pragma unroll
                 for (int r = 0; r < kTileRows; r++) {
-                    a0[r] = pk_min_add(a0[r], v0, w[r]);
-                    a1[r] = pk_min_add(a1[r], v1, w[r]);
+                    if (w[r] != 0u) {
+                        a0[r] = pk_min_add(a0[r], v0, w[r]);
+                        a1[r] = pk_min_add(a1[r], v1, w[r]);
+                    }
                 }
             }
             v0 = n0;
             v1 = n1;
         }
-        /* every row of the tile is stored: row 0's sums are zero (vertex 0
-         * has no edges), and rows past n land in the table's 16 padding rows */
+        /* every row of the tile is stored, packed: row 0's sums are zero
+         * (vertex 0 has no edges), and rows past n land in the table's 16
+         * padding rows */
         const bool s0 = w0 < nwords, s1 = w1 < nwords;
 #pragma unroll
         for (int r = 0; r < kTileRows; r++) {
-            int2* out = reinterpret_cast<int2*>(gout + (int64_t)(u0 + r) * stride);
-            if (s0) out[w0] = make_int2((int32_t)(a0[r] & 0xffffu), (int32_t)(a0[r] >> 16));
-            if (s1) out[w1] = make_int2((int32_t)(a1[r] & 0xffffu), (int32_t)(a1[r] >> 16));
+            uint32_t* out = reinterpret_cast<uint32_t*>(gout + (int64_t)(u0 + r) * stride + hw);
+            if (s0) out[w0] = a0[r];
+            if (s1) out[w1] = a1[r];
         }
     }
 }
@@ -416,10 +429,14 @@ __global__ void __launch_bounds__(kTileThreads) k_gain_dense_tile(BmDevGraph g, 
     const int64_t hw = G.stride >> 1;
     const int32_t nwords = (int32_t)hw;
     const int32_t nd = G.ndense;
-    for (int32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    /* items column-block-major: the blocks running at one time share the
+     * dense rows' 2 KB column blocks in L2 */
+    const int32_t nwb = (nwords + 2 * kTileThreads - 1) / (2 * kTileThreads);
+    for (int64_t item = blockIdx.x; item < (int64_t)ntiles * nwb; item += gridDim.x) {
+        const int32_t tile = (int32_t)(item % ntiles), wbi = (int32_t)(item / ntiles);
         const int32_t u0 = tile * kTileRows;
         const uint32_t* wt = G.dense_w + (int64_t)tile * nd * kTileRows;
-        dense_tile_words(wt, G.dense, G.G, G.stride, hw, nwords, nd, u0, tid);
+        dense_tile_words(wt, G.dense, G.G, G.stride, hw, nwords, nd, u0, tid, wbi * 2 * kTileThreads);
     }
 }
 
@@ -468,10 +485,16 @@ __global__ void __launch_bounds__(kBuildThreads) k_gain_build(BmDevGraph g, BmGa
         const int64_t base = g.m_out_ptr[u] + g.m_in_ptr[u];
         const int32_t L = uni(G.row_fill[u]);   /* the list's entries: edges of non-empty sparse rows */
         const int4* list = G.edesc + base;
-        /* the row starts from its dense part (k_gain_dense_tile) or zero */
+        /* the row starts from its dense part or zero; k_gain_dense_tile
+         * leaves the dense part packed in the row's second half, columns 2w
+         * and 2w + 1 in word w (col0 and ncols are even) */
         if (G.ndense > 0) {
-            const int32_t* src = G.G + (int64_t)u * G.stride + col0;
-            for (int32_t c = tid; c < ncols; c += kBuildThreads) row[c] = src[c];
+            const uint32_t* src =
+                reinterpret_cast<const uint32_t*>(G.G + (int64_t)u * G.stride + (G.stride >> 1)) + (col0 >> 1);
+            for (int32_t c = tid; c < (ncols >> 1); c += kBuildThreads) {
+                const uint32_t v = src[c];
+                reinterpret_cast<int2*>(row)[c] = make_int2((int32_t)(v & 0xffffu), (int32_t)(v >> 16));
+            }
         } else {
             for (int32_t c = tid; c < ncols; c += kBuildThreads) row[c] = 0;
         }
@@ -1476,8 +1499,9 @@ static hipError_t launch_build_rows(const BmDevGraph& g, const BmGain& G, int32_
         hipLaunchKernelGGL(k_gain_edge_desc<LAYOUT>, dim3((unsigned)dblocks), dim3(256), 0, s, g, G, (int32_t)c0, nc,
                            (int32_t)win);
         if (win == 0 && G.ndense > 0) {   /* every column of the dense part, from the first work list */
-            int64_t tblocks = (g.n + kTileRows) / kTileRows;
-            if (tblocks > cus * 16) tblocks = cus * 16;
+            const int64_t nwb = (G.stride / 2 + 2 * kTileThreads - 1) / (2 * kTileThreads);
+            int64_t tblocks = (g.n + kTileRows) / kTileRows * nwb;   /* one block per work item */
+            if (tblocks > cus * 128) tblocks = cus * 128;
             hipLaunchKernelGGL(k_gain_dense_tile, dim3((unsigned)tblocks), dim3(kTileThreads), 0, s, g, G);
         }
         if (windowed)

@@ -215,7 +215,9 @@ def test_full_size_gain_equals_scan(bm, orc):
 
 def test_multi_chunk_build_equals_scan(bm, orc):
     """n=45,000: the table is built in two column windows (each row's
-    female entries split at precomputed window boundaries); rows match their definition,
+    female entries split at precomputed window boundaries) on top of the
+    dense tile pass's sums for the pair's four hub rows, unpacked per window;
+    rows match their definition,
     table deltas match the scan kernels on uniform and u-major streams (the
     u-major stream falls back to per-pair lookups at this n), and a
     rebuild after moves matches the patched table."""
@@ -234,6 +236,7 @@ def test_multi_chunk_build_equals_scan(bm, orc):
         assert np.array_equal(b.deltas(), d_scan)
         b.eval_swaps_umajor(8, k)
         assert np.array_equal(b.deltas(), d_scan_um)
+        assert ctx.gain_dense_rows() == 4
         o = orc.Oracle(n, male, female)
         hubs = np.argsort(-(np.bincount(male.src, minlength=n + 1) + np.bincount(male.dst, minlength=n + 1)))[:3]
         for u in [1, 15007, 15008, 30015, 30016, n] + [int(x) for x in hubs if x > 0]:
@@ -250,7 +253,7 @@ def test_multi_chunk_build_equals_scan(bm, orc):
 def test_scale_pair_windowed_build(bm, orc):
     """n=100,000, ~20M edges per graph, 40% of the matching shuffled (the
     scale stress pair): the 40 GB table is built in three column windows at
-    one block per CU.  Whole rows (every column, so every window) of five
+    one block per CU, on top of the dense tile pass's sums for 57 hub rows.  Whole rows (every column, so every window) of five
     vertices up to degree 1,500 match their definition; the two largest hubs
     (adjacent to nearly every vertex, rows too costly for the oracle's
     O(n * degree) row) are checked through 2048 swap deltas each against the
@@ -269,7 +272,7 @@ def test_scale_pair_windowed_build(bm, orc):
         b.eval_swaps_uniform(31, k)
         d_scan = b.deltas()
         ctx.gain_enable(True)
-        assert ctx.gain_enabled()
+        assert ctx.gain_enabled() and ctx.gain_dense_rows() == 57
         b.eval_swaps_uniform(31, k)
         d_gain = b.deltas()
         assert np.array_equal(d_gain, d_scan)
