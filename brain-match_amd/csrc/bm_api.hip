@@ -1324,6 +1324,9 @@ extern "C" int bm_gain_enabled(const bm_ctx* c) { return c ? c->gain_on : 0; }
 
 extern "C" int bm_gain_dense_rows(const bm_ctx* c) { return c && c->gain_on && c->pi_distinct ? c->ndense : 0; }
 
+/* Dense rows: female rows of at least stride / kDenseDiv entries. */
+static constexpr int64_t kDenseDiv = 4;
+
 extern "C" int bm_gain_enable(bm_ctx* c, int on) {
     if (!c) return BM_ERR_ARG;
     HIPCHK(c, hipSetDevice(c->device));
@@ -1388,21 +1391,24 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         HIPCHK(c, bm_launch_pack_rows(c->d_f_out, c->female_m, c->row_layout, c->d_f_out_p, c->stream));
         HIPCHK(c, bm_launch_pack_rows(c->d_f_in, c->female_m, c->row_layout, c->d_f_in_p, c->stream));
     }
-    /* Dense copies of the longest female rows: a row of at least a quarter
-     * of the columns costs at most 8 bytes per entry read as 16-bit columns
-     * (against 4 packed) but is summed in registers without LDS atomics.
-     * Exact when the graph is compact (weights fit 16 bits) and every male
-     * weight is >= 0 (an absent column then adds min(w, 0) = 0, as a
-     * missing female edge does).  The longest rows qualify, ties to the
-     * lower id, at most BM_DENSE_MAX and at most 65535 / max |w_F| (the dense
-     * sums are kept in 16 bits).  The tile pass leaves its sums packed in
-     * the second half of each row (half the table's bytes written and read
-     * back by the row build), which also serves a table built in column
-     * windows: at n = 100,000 a 74.5 ms build with 57 dense rows against
-     * 81.0 ms without. */
+    /* Dense copies of the longest female rows, summed 16 male rows at a time
+     * in packed 16-bit registers: rows of at least stride / kDenseDiv
+     * entries qualify, longest first (ties to the lower id), at most
+     * BM_DENSE_MAX.  A dense row costs the tile pass about 10 us at
+     * n = 18,524 whatever its length, so only the longest pay: 48 to 100
+     * rows build in 3.80-3.88 ms there, 197 in 4.42 ms, 406 in 5.84 ms.
+     * Exact when the graph is compact (female weights in 0..32767) and
+     * every male weight is >= 0: an absent column then adds min(w, 0) = 0,
+     * as a missing female edge does, and each term min(w, fw) is at most
+     * fw, so a dense sum never exceeds its column's sum of dense-row
+     * weights.  A row is taken only while every column's sum stays
+     * <= 65535, which keeps the packed 16-bit sums exact.  The tile pass
+     * leaves its sums packed in the second half of each row (half the
+     * table's bytes written and read back by the row build), which also
+     * serves a table built in column windows. */
     if (c->compact && c->male_nonneg && !c->d_f_out_dense && c->female_m > 0) {
         const int64_t stride = bm_gain_stride(c->n);
-        const int32_t thr = (int32_t)(stride / 4) > 1 ? (int32_t)(stride / 4) : 1;
+        const int32_t thr = (int32_t)(stride / kDenseDiv) > 1 ? (int32_t)(stride / kDenseDiv) : 1;
         std::vector<int2> cand;   /* {degree, side * (n + 1) + id}, side 0 = out-row */
         for (int32_t r = 1; r <= c->n; r++) {
             if (c->f_out_deg[r] >= thr) cand.push_back(make_int2(c->f_out_deg[r], r));
@@ -1411,9 +1417,28 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
         std::sort(cand.begin(), cand.end(), [](const int2& a, const int2& b) {
             return a.x != b.x ? a.x > b.x : a.y < b.y;
         });
-        const size_t cap16 = (size_t)(65535 / (c->max_abs_fw > 1 ? c->max_abs_fw : 1));
-        const size_t keep = cap16 < (size_t)BM_DENSE_MAX ? cap16 : (size_t)BM_DENSE_MAX;
-        if (cand.size() > keep) cand.resize(keep);
+        if (!cand.empty()) {   /* the column sums, from each candidate's entries {column, weight} */
+            std::vector<int64_t> ptr_out((size_t)c->n + 2), ptr_in((size_t)c->n + 2);
+            HIPCHK(c, hipMemcpy(ptr_out.data(), c->d_f_out_ptr, ptr_out.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+            HIPCHK(c, hipMemcpy(ptr_in.data(), c->d_f_in_ptr, ptr_in.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+            std::vector<int32_t> colsum((size_t)c->n + 1, 0);
+            std::vector<int2> ent, kept;
+            for (const int2& cd : cand) {
+                if (kept.size() == (size_t)BM_DENSE_MAX) break;
+                const bool in = cd.y > c->n;
+                const int32_t r = in ? cd.y - (c->n + 1) : cd.y;
+                const int64_t* p = in ? ptr_in.data() : ptr_out.data();
+                ent.resize((size_t)(p[r + 1] - p[r]));
+                HIPCHK(c, hipMemcpy(ent.data(), (in ? c->d_f_in : c->d_f_out) + p[r], ent.size() * sizeof(int2),
+                                    hipMemcpyDeviceToHost));
+                bool fits = true;
+                for (const int2& e : ent) fits = fits && colsum[e.x] + e.y <= 65535;
+                if (!fits) continue;
+                for (const int2& e : ent) colsum[e.x] += e.y;
+                kept.push_back(cd);
+            }
+            cand.swap(kept);
+        }
         if (!cand.empty()) {
             std::vector<int32_t> hout((size_t)c->n + 1, -1), hin((size_t)c->n + 1, -1);
             std::vector<int2> rows(cand.size());

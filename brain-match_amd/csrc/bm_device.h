@@ -67,7 +67,7 @@ struct BmGain {
     const uint32_t* dense;
     const int32_t* f_out_dense;
     const int32_t* f_in_dense;
-    int32_t ndense;             /* at most 65535 / max female weight, so 16-bit sums cannot overflow */
+    int32_t ndense;             /* every column's dense-row weights sum to <= 65535: 16-bit sums cannot overflow */
     uint32_t* dense_w;          /* per tile of 16 male rows: ndense x 16 packed weights (rebuilt by every build) */
     int32_t* row_fill;          /* per male row: work-list entries of the current window (n + 1, rebuilt) */
 };

@@ -343,7 +343,8 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
  * rows' sums as packed 16-bit halves, one load of a dense-row word serving
  * 16 rows; a row whose weight is 0 is skipped (a scalar branch: about 40% of
  * the rows of a tile that uses a dense row do not).  The sums stay below
- * 2^16: there are at most 65535 / max weight dense rows.  The result stays
+ * 2^16: each term min(w, fw) is at most fw, and the dense rows are chosen
+ * so that every column's weights sum to at most 65535.  The result stays
  * packed: word w of the second half of G's row u holds columns 2w (low
  * half) and 2w + 1, and the row build unpacks it into its LDS row, adds the
  * other edges and overwrites the row.  In the second half a column window's

@@ -56,23 +56,25 @@ def test_power_law_pair_uses_dense_rows(bm, orc):
 
 def test_dense_sums_at_the_16_bit_limit(bm, orc):
     """Three female hubs with an edge to every other vertex of weight 21,845
-    (so at most 65,535 / 21,845 = 3 dense rows) and three male vertices
-    mapped onto them with edges of weight 30,000 to every vertex: a column's
-    dense sum reaches exactly 65,535 and every row matches its definition."""
+    and three male vertices mapped onto them with edges of weight 30,000 to
+    every vertex: a column's dense sum reaches exactly 65,535.  A fourth hub
+    of the same degree with weights 1 would take columns to 65,536, so it is
+    left to the row build, which adds it on top of the 16-bit sums; every
+    row matches its definition."""
     n = 64
     rng = np.random.default_rng(11)
     src, dst, w = [], [], []
     fsrc, fdst, fw = [], [], []
-    for h in (1, 2, 3):   # female hubs 1..3, male vertices 1..3 (identity matching)
+    for h, hw in ((1, 21845), (2, 21845), (3, 21845), (4, 1)):   # female hubs, male 1..4 mapped onto them
         for v in range(1, n + 1):
             if v != h:
-                fsrc.append(h), fdst.append(v), fw.append(21845)
+                fsrc.append(h), fdst.append(v), fw.append(hw)
                 src.append(h), dst.append(v), w.append(30000)
     for _ in range(300):   # sparse rest, weights below the hubs'
-        a, c = (int(x) for x in rng.integers(4, n + 1, 2))
+        a, c = (int(x) for x in rng.integers(5, n + 1, 2))
         if a != c:
             src.append(a), dst.append(c), w.append(int(rng.integers(1, 500)))
-            b, d = (int(x) for x in rng.integers(4, n + 1, 2))
+            b, d = (int(x) for x in rng.integers(5, n + 1, 2))
             if b != d:
                 fsrc.append(b), fdst.append(d), fw.append(int(rng.integers(1, 500)))
     male = bm.Graph(np.array(src, np.int32), np.array(dst, np.int32), np.array(w, np.int32), n)
@@ -84,7 +86,7 @@ def test_dense_sums_at_the_16_bit_limit(bm, orc):
         ctx.gain_enable(True)
         assert ctx.gain_dense_rows() == 3
         row = ctx.gain_row(10)
-        assert row[20] >= 3 * 21845
+        assert row[20] >= 3 * 21845 + 1
         _all_rows_match(ctx, orc, o, pi, n)
 
 
