@@ -767,7 +767,8 @@ def bench_scale(args, bm):
     40% of the true matching shuffled.  Exact swap-delta throughput of both
     strategies on 2^28 u-major pairs per step, the step of the n = 18,524
     benchmark (the gain table is 40 GB here, rebuilt every step in column
-    windows, with per-pair lookups); then swap + 3-vertex annealing (2048
+    windows; the lookups stage each first vertex's bitmaps and gather the
+    table entries); then swap + 3-vertex annealing (2048
     chains) for SECONDS each with learned proposal pools and with uniform
     proposals only; with --scale-climbs also a uniform and a learned
     disjoint-swap climb for SECONDS each."""

@@ -942,10 +942,15 @@ extern "C" int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k)
     if (rc != BM_OK) return rc;
     KTiming t;
     if (c->gain_on) {
+        /* rows and bitmaps staged per first vertex (n <= 18,975), bitmaps only
+         * with the two table reads gathered (no transposed table), or per-pair
+         * lookups when not even the bitmaps fit */
         const bool staged = bm_gain_umajor_lds(c->n) <= 160 * 1024;
+        const bool bitmaps = !staged && bm_gain_umajor_bitmap_lds(c->n) <= 160 * 1024;
         if (staged && (rc = gain_transposed(c)) != BM_OK) return rc;
+        if (bitmaps && !c->d_pis && (rc = dalloc(c, &c->d_pis, (size_t)c->n + 1)) != BM_OK) return rc;
         timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
-        if (staged) {
+        if (staged || bitmaps) {
             /* pi and the diagonal G(v, pi(v)) are cheap: refreshed every call */
             HIPCHK(c, bm_launch_gain_pis(devgraph(c), devgain(c), c->d_pis, c->stream));
             HIPCHK(c, bm_launch_gain_swap_umajor(devgraph(c), devgain(c), c->d_GT, c->d_pis, c->d_um_off, seed,

@@ -117,9 +117,13 @@ hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, int32_t* 
                                  int64_t max_moves, int64_t* plan, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, const int64_t* off,
                                uint64_t seed, int64_t k, int64_t* delta, hipStream_t s);
-/* u-major evaluation: GT = transposed table, pis = {pi, G(v, pi(v)), self-loop, 0}
- * per vertex, off = stream offsets.  Needs bm_gain_umajor_lds(n) bytes of LDS. */
+/* u-major evaluation: pis = {pi, G(v, pi(v)), self-loops} per vertex, off =
+ * stream offsets.  With bm_gain_umajor_lds(n) <= 160 KiB both rows of a first
+ * vertex are staged in LDS (GT = the transposed table, required); otherwise
+ * only its bitmaps, bm_gain_umajor_bitmap_lds(n) bytes (GT unused), and the
+ * launch fails if those do not fit either. */
 size_t bm_gain_umajor_lds(int32_t n);
+size_t bm_gain_umajor_bitmap_lds(int32_t n);
 hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t nrows, int32_t* GT, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_pis(const BmDevGraph& g, const BmGain& G, int4* pis, hipStream_t s);
 /* a-major 3-vertex lookups (best delta and permutation per stream entry);

@@ -1044,6 +1044,7 @@ __global__ void k_gain_pis(BmDevGraph g, BmGain G, int4* __restrict__ pis) {
  * weight (w_M(u, v), w_M(v, u), w_F(f1, f2), w_F(f2, f1)) is one direct load;
  * self-loops add their exact terms. */
 constexpr int kUmThreads = 1024;
+constexpr size_t kLdsBytes = 160 * 1024;
 constexpr int kUmStage = 5;    /* 16-byte row words per thread and row in flight while staging (5 x 1024 x 4
                                   * covers a row of up to 20,480 entries in one round) */
 constexpr int kUmPairs = 4;    /* pairs per thread in flight */
@@ -1059,7 +1060,8 @@ __device__ __forceinline__ int32_t list_id(const int2* base, int64_t len, int32_
 }
 
 /* Rank of x in a bitmap (uint32 words) with prefix counts per 64-bit word. */
-__device__ __forceinline__ int32_t bit_rank(const uint32_t* bm, const uint16_t* pre, int32_t x) {
+template <typename P>
+__device__ __forceinline__ int32_t bit_rank(const uint32_t* bm, const P* pre, int32_t x) {
     const int32_t w = x >> 6;
     const uint64_t word = (uint64_t)bm[2 * w] | ((uint64_t)bm[2 * w + 1] << 32);
     return (int32_t)pre[w] + __popcll(word & ((1ull << (x & 63)) - 1ull));
@@ -1067,6 +1069,13 @@ __device__ __forceinline__ int32_t bit_rank(const uint32_t* bm, const uint16_t* 
 
 __device__ __forceinline__ bool bit_test(const uint32_t* bm, int32_t x) { return (bm[x >> 5] >> (x & 31)) & 1u; }
 
+/* ROWS = false is the form for tables whose two rows do not fit in LDS
+ * (n >= 18,976): only the bitmaps are staged (with 32-bit prefix counts, as
+ * a hub's list may pass 65,535 entries), G(u, f2) is gathered from row u
+ * (which the block's loads share in L2) and G(v, f1) from v's row, so a pair
+ * costs one random table read besides its per-vertex words; no transposed
+ * table is needed. */
+template <bool ROWS, typename P>
 __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, BmGain G, const int32_t* __restrict__ GT,
                                                                  const int4* __restrict__ pis,
                                                                  const int64_t* __restrict__ off, uint64_t seed,
@@ -1078,12 +1087,13 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
     const int32_t w64 = (n + 1 + 63) >> 6;             /* prefix entries per bitmap */
     int32_t* rowG = reinterpret_cast<int32_t*>(sm4);
     int32_t* rowT = rowG + stride;
-    uint32_t* bits = reinterpret_cast<uint32_t*>(rowT + stride);   /* 4 bitmaps of 2 * w64 words */
+    uint32_t* bits = ROWS ? reinterpret_cast<uint32_t*>(rowT + stride)   /* 4 bitmaps of 2 * w64 words */
+                          : reinterpret_cast<uint32_t*>(sm4);
     uint32_t* mo = bits;                               /* u -> v in M */
     uint32_t* mi = bits + 2 * w64;                     /* v -> u in M */
     uint32_t* fo = bits + 4 * w64;                     /* f1 -> f in F */
     uint32_t* fi = bits + 6 * w64;                     /* f -> f1 in F */
-    uint16_t* pre = reinterpret_cast<uint16_t*>(bits + 8 * w64);   /* 4 x w64 prefix counts */
+    P* pre = reinterpret_cast<P*>(bits + 8 * w64);     /* 4 x w64 prefix counts */
     const int tid = threadIdx.x;
     const int lane = tid & (BM_WAVE - 1), wave = tid / BM_WAVE;
     for (int32_t u = blockIdx.x + 1; u <= n; u += gridDim.x) {
@@ -1106,16 +1116,17 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
         const int32_t x_mi = list_id(g.m_in + mib, mie - mib, tid);
         const int32_t x_fo = list_id(g.f_out + fob, foe - fob, tid);
         const int32_t x_fi = list_id(g.f_in + fib, fie - fib, tid);
-        const __amdgpu_buffer_rsrc_t rG =
-            __builtin_amdgcn_make_buffer_rsrc(G.G + (int64_t)u * stride, 0, (int32_t)(stride * 4), kBufWord3);
-        const __amdgpu_buffer_rsrc_t rT =
-            __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(GT) + (int64_t)f1 * stride, 0, (int32_t)(stride * 4),
-                                              kBufWord3);
         V4u a[kUmStage], t[kUmStage];
+        if constexpr (ROWS) {
+            const __amdgpu_buffer_rsrc_t rG =
+                __builtin_amdgcn_make_buffer_rsrc(G.G + (int64_t)u * stride, 0, (int32_t)(stride * 4), kBufWord3);
+            const __amdgpu_buffer_rsrc_t rT = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<int32_t*>(GT) + (int64_t)f1 * stride, 0, (int32_t)(stride * 4), kBufWord3);
 #pragma unroll
-        for (int j = 0; j < kUmStage; j++) {
-            a[j] = __builtin_amdgcn_raw_buffer_load_b128(rG, (tid + kUmThreads * j) * 16, 0, 0);
-            t[j] = __builtin_amdgcn_raw_buffer_load_b128(rT, (tid + kUmThreads * j) * 16, 0, 0);
+            for (int j = 0; j < kUmStage; j++) {
+                a[j] = __builtin_amdgcn_raw_buffer_load_b128(rG, (tid + kUmThreads * j) * 16, 0, 0);
+                t[j] = __builtin_amdgcn_raw_buffer_load_b128(rT, (tid + kUmThreads * j) * 16, 0, 0);
+            }
         }
         if (x_mo) atomicOr(&mo[x_mo >> 5], 1u << (x_mo & 31));   /* ids are >= 1: 0 marks nothing */
         if (x_mi) atomicOr(&mi[x_mi >> 5], 1u << (x_mi & 31));
@@ -1137,20 +1148,22 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
             const int32_t y = g.f_in[e].x;
             atomicOr(&fi[y >> 5], 1u << (y & 31));
         }
-        V4u* sG = reinterpret_cast<V4u*>(rowG);
-        V4u* sT = reinterpret_cast<V4u*>(rowT);
+        if constexpr (ROWS) {
+            V4u* sG = reinterpret_cast<V4u*>(rowG);
+            V4u* sT = reinterpret_cast<V4u*>(rowT);
 #pragma unroll
-        for (int j = 0; j < kUmStage; j++) {
-            const int32_t c = tid + kUmThreads * j;
-            if (c < s4) {
-                sG[c] = a[j];
-                sT[c] = t[j];
+            for (int j = 0; j < kUmStage; j++) {
+                const int32_t c = tid + kUmThreads * j;
+                if (c < s4) {
+                    sG[c] = a[j];
+                    sT[c] = t[j];
+                }
             }
         }
         __syncthreads();
         if (wave < 4) {   /* exclusive prefix popcounts, one wave per bitmap */
             const uint32_t* bm = bits + 2 * w64 * wave;
-            uint16_t* pr = pre + w64 * wave;
+            P* pr = pre + w64 * wave;
             const int32_t per = (w64 + BM_WAVE - 1) / BM_WAVE;
             const int32_t lo = lane * per, hi = lo + per < w64 ? lo + per : w64;
             int32_t cnt = 0;
@@ -1163,19 +1176,25 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
             }
             int32_t run = incl - cnt;
             for (int32_t w = lo; w < hi; w++) {
-                pr[w] = (uint16_t)run;
+                pr[w] = (P)run;
                 run += __popc(bm[2 * w]) + __popc(bm[2 * w + 1]);
             }
         }
         __syncthreads();
+        const int32_t* Gu = G.G + (int64_t)u * stride;
         for (int64_t ib = i0 + tid; ib < i1; ib += (int64_t)kUmThreads * kUmPairs) {
-            int32_t v[kUmPairs];
+            int32_t v[kUmPairs], gu[kUmPairs], gt[kUmPairs];
             int4 pv[kUmPairs];
 #pragma unroll
             for (int j = 0; j < kUmPairs; j++) {   /* all gathers in flight before any use */
                 const int64_t i = ib + (int64_t)kUmThreads * j;
                 v[j] = i < i1 ? bm_umajor_partner(seed, (uint64_t)i, u, n) : 0;
                 pv[j] = i < i1 ? pis[v[j]] : make_int4(0, 0, BM_ABSENT, 0);
+                if constexpr (!ROWS) gt[j] = G.G[(int64_t)v[j] * stride + f1];   /* G(v, f1); row 0 when past the range */
+            }
+            if constexpr (!ROWS) {
+#pragma unroll
+                for (int j = 0; j < kUmPairs; j++) gu[j] = Gu[pv[j].x];   /* G(u, f2) */
             }
 #pragma unroll
             for (int j = 0; j < kUmPairs; j++) {
@@ -1183,7 +1202,8 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_swap_umajor(BmDevGraph g, B
                 if (i >= i1) continue;
                 const int32_t f2 = pv[j].x, x = v[j];
                 const int32_t F22 = pv[j].w;                /* w_F(f2, f2) */
-                int64_t d = (int64_t)rowG[f2] - pu.y + rowT[x] - pv[j].y;
+                const int32_t g_uf2 = ROWS ? rowG[f2] : gu[j], g_vf1 = ROWS ? rowT[x] : gt[j];
+                int64_t d = (int64_t)g_uf2 - pu.y + g_vf1 - pv[j].y;
                 const bool uv = bit_test(mo, x);
                 const bool vu = bit_test(mi, x);
                 if (uv || vu) {
@@ -1573,6 +1593,11 @@ size_t bm_gain_umajor_lds(int32_t n) {
     return 2 * (size_t)bm_gain_stride(n) * 4 + 4 * w64 * 8 + 4 * w64 * 2;   /* rows, 4 bitmaps, 4 prefix arrays */
 }
 
+size_t bm_gain_umajor_bitmap_lds(int32_t n) {
+    const size_t w64 = ((size_t)n + 1 + 63) >> 6;
+    return 4 * w64 * 8 + 4 * w64 * 4;   /* 4 bitmaps, 4 arrays of 32-bit prefix counts */
+}
+
 hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t nrows, int32_t* GT, int32_t num_cus, hipStream_t s) {
     if (G.stride & 31) return hipErrorInvalidValue;   /* 16-byte accesses need 128-byte rows */
     int64_t blocks = ((G.stride + 63) / 64) * (((int64_t)nrows + 63) / 64);
@@ -1592,18 +1617,25 @@ hipError_t bm_launch_gain_pis(const BmDevGraph& g, const BmGain& G, int4* pis, h
 hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
                                       const int64_t* off, uint64_t seed, int64_t* delta, int32_t num_cus,
                                       hipStream_t s) {
-    const size_t lds = bm_gain_umajor_lds(g.n);
-    hipError_t e = hipFuncSetAttribute((const void*)k_gain_swap_umajor, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
+    /* rows staged when they fit (GT required), else bitmaps only */
+    const bool rows = bm_gain_umajor_lds(g.n) <= kLdsBytes;
+    const size_t lds = rows ? bm_gain_umajor_lds(g.n) : bm_gain_umajor_bitmap_lds(g.n);
+    if (lds > kLdsBytes || (rows && !GT)) return hipErrorInvalidValue;
+    const void* kern = rows ? (const void*)k_gain_swap_umajor<true, uint16_t> : (const void*)k_gain_swap_umajor<false, uint32_t>;
+    hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     int occ = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_gain_swap_umajor, kUmThreads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kUmThreads, lds);
     if (e != hipSuccess) return e;
     if (occ < 1) occ = 1;
     int64_t blocks = (int64_t)occ * (num_cus > 0 ? num_cus : 256);
     if (blocks > g.n) blocks = g.n;
-    hipLaunchKernelGGL(k_gain_swap_umajor, dim3((unsigned)blocks), dim3(kUmThreads), lds, s, g, G, GT, pis, off, seed,
-                       delta);
+    if (rows)
+        hipLaunchKernelGGL((k_gain_swap_umajor<true, uint16_t>), dim3((unsigned)blocks), dim3(kUmThreads), lds, s, g, G,
+                           GT, pis, off, seed, delta);
+    else
+        hipLaunchKernelGGL((k_gain_swap_umajor<false, uint32_t>), dim3((unsigned)blocks), dim3(kUmThreads), lds, s, g, G,
+                           GT, pis, off, seed, delta);
     return hipGetLastError();
 }
 

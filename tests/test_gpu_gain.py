@@ -218,8 +218,9 @@ def test_multi_chunk_build_equals_scan(bm, orc):
     female entries split at precomputed window boundaries) on top of the
     dense tile pass's sums for the pair's four hub rows, unpacked per window;
     rows match their definition,
-    table deltas match the scan kernels on uniform and u-major streams (the
-    u-major stream falls back to per-pair lookups at this n), and a
+    table deltas match the scan kernels on uniform and u-major streams (at
+    this n the u-major lookups stage only the first vertex's bitmaps and
+    gather both table entries), and a
     rebuild after moves matches the patched table."""
     n = 45000
     male, female, truth, pi0 = bm.synth_pair(n, 12.0, bm.WEIGHTS_LOGNORMAL, 0.3, seed=5)
@@ -253,7 +254,9 @@ def test_multi_chunk_build_equals_scan(bm, orc):
 def test_scale_pair_windowed_build(bm, orc):
     """n=100,000, ~20M edges per graph, 40% of the matching shuffled (the
     scale stress pair): the 40 GB table is built in three column windows at
-    one block per CU, on top of the dense tile pass's sums for 57 hub rows.  Whole rows (every column, so every window) of five
+    one block per CU, on top of the dense tile pass's sums for 57 hub rows;
+    u-major table deltas (bitmaps staged per first vertex) equal the scan
+    kernel on 2^22 pairs.  Whole rows (every column, so every window) of five
     vertices up to degree 1,500 match their definition; the two largest hubs
     (adjacent to nearly every vertex, rows too costly for the oracle's
     O(n * degree) row) are checked through 2048 swap deltas each against the
@@ -271,11 +274,20 @@ def test_scale_pair_windowed_build(bm, orc):
         b = ctx.batch(k)
         b.eval_swaps_uniform(31, k)
         d_scan = b.deltas()
+        ku = 1 << 22
+        bu = ctx.batch(ku)
+        bu.eval_swaps_umajor(32, ku)
+        d_scan_um = bu.deltas()
         ctx.gain_enable(True)
         assert ctx.gain_enabled() and ctx.gain_dense_rows() == 57
         b.eval_swaps_uniform(31, k)
         d_gain = b.deltas()
         assert np.array_equal(d_gain, d_scan)
+        bu.eval_swaps_umajor(32, ku)
+        assert np.array_equal(bu.deltas(), d_scan_um)
+        bu.close()
+        print("u-major table lookups (bitmaps staged, entries gathered) equal the scan kernel on 2^22 pairs",
+              flush=True)
         uv = orc.uniform_pairs(31, k, n)
         rng = np.random.default_rng(2)
         idx = rng.choice(k, 4096, replace=False)
