@@ -100,9 +100,10 @@ def dense_split(ctx, pi, female, n_dense):
     """The table build's split of the expansions between the dense tile pass
     and the LDS row build.  Dense rows follow the library's rule: female rows
     of at least a quarter of the row stride, longest first (ties: lower id,
-    out-rows before in-rows), at most min(128, 65535 // max female weight);
-    their count is checked against the library's.  Returns (expanded entries
-    of dense rows, (male row, dense row) pairs)."""
+    out-rows before in-rows), each taken only while every column's sum of
+    dense-row weights stays <= 65535, at most 128; their count is checked
+    against the library's.  Returns (expanded entries of dense rows,
+    (male row, dense row) pairs)."""
     import numpy as np
     if n_dense == 0:
         return 0, 0
@@ -113,8 +114,26 @@ def dense_split(ctx, pi, female, n_dense):
     cand = [(-int(fo[r]), r) for r in range(1, n + 1) if fo[r] >= thr]
     cand += [(-int(fi[r]), n + 1 + r) for r in range(1, n + 1) if fi[r] >= thr]
     cand.sort()
-    keep = min(128, 65535 // max(1, int(np.abs(female.w).max())))
-    cand = cand[:keep]
+    # female edges, the last line of each (src, dst) kept (the library's rule)
+    key = female.src.astype(np.int64)[::-1] * (n + 1) + female.dst.astype(np.int64)[::-1]
+    key, first = np.unique(key, return_index=True)
+    w = female.w[::-1][first].astype(np.int64)
+    src, dst = key // (n + 1), key % (n + 1)
+    by_src, by_dst = np.argsort(src, kind="stable"), np.argsort(dst, kind="stable")
+    src_ptr = np.searchsorted(src[by_src], np.arange(n + 2))
+    dst_ptr = np.searchsorted(dst[by_dst], np.arange(n + 2))
+    colsum = np.zeros(n + 1, np.int64)
+    kept = []
+    for c in cand:
+        if len(kept) == 128:
+            break
+        r = c[1] - (n + 1) if c[1] > n else c[1]
+        e = by_dst[dst_ptr[r]:dst_ptr[r + 1]] if c[1] > n else by_src[src_ptr[r]:src_ptr[r + 1]]
+        cols = src[e] if c[1] > n else dst[e]
+        if (colsum[cols] + w[e]).max(initial=0) <= 65535:
+            colsum[cols] += w[e]
+            kept.append(c)
+    cand = kept
     if len(cand) != n_dense:
         raise SystemExit(f"dense-row rule disagrees with the library ({len(cand)} against {n_dense})")
     dout = np.zeros(n + 1, bool)
