@@ -1311,7 +1311,10 @@ __global__ void __launch_bounds__(256) k_gain_rot3(BmDevGraph g, BmGain G, int4*
  * only G(b, f_c) and G(c, f_b) from the table in HBM; male edges between a
  * and b or c are tested in the bitmaps, between b and c in the dense
  * adjacency bitmap, and a present edge's weight and the female weights its
- * correction needs come from the hash tables (rare). */
+ * correction needs come from the hash tables, whose row descriptors are
+ * loaded only for a triple with such an edge (about 7% at n = 18,524;
+ * loading them with the other gathers for every triple cost 10% of the
+ * kernel). */
 constexpr int kR3Triples = 2;   /* triples per thread in flight */
 
 __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, BmGain G, const int32_t* __restrict__ GT,
@@ -1373,11 +1376,47 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
             }
         }
         __syncthreads();
+        /* the five permutations' best for entry i from its gathered values;
+         * the edge corrections (descriptor loads and hash probes) only when
+         * edges says so */
+        auto finish = [&](int64_t i, int32_t b, int32_t c, const int4& pb, const int4& pc, uint32_t wbc, int32_t gbc,
+                          int32_t gcb, bool edges) {
+            const int32_t f[3] = {fa, pb.x, pc.x};
+            const int64_t Gv[3][3] = {{pa.y, rowG[f[1]], rowG[f[2]]}, {rowT[b], pb.y, gbc}, {rowT[c], gcb, pc.y}};
+            const int32_t sv[3] = {pa.z, pb.z, pc.z};
+            const int32_t fs[3] = {pa.w, pb.w, pc.w};
+            int32_t w[3][3], F[3][3];
+#pragma unroll
+            for (int x = 0; x < 3; x++)
+#pragma unroll
+                for (int y = 0; y < 3; y++) {
+                    w[x][y] = BM_ABSENT;
+                    F[x][y] = x == y ? fs[x] : BM_ABSENT;
+                }
+            if (edges) {
+                const bool ab = (mo[b >> 5] >> (b & 31)) & 1u, ba = (mi[b >> 5] >> (b & 31)) & 1u;
+                const bool ac = (mo[c >> 5] >> (c & 31)) & 1u, ca = (mi[c >> 5] >> (c & 31)) & 1u;
+                const bool bc = (wbc >> ((b < c ? c : b) & 31)) & 1u;   /* either direction */
+                const int32_t vtx[3] = {a, b, c};
+                const bool have[3][3] = {{false, ab, ac}, {ba, false, bc}, {ca, bc, false}};
+                const uint64_t md[3] = {mda, G.m_out_desc[b], G.m_out_desc[c]};
+                const uint64_t fd[3] = {fda, g.f_out_desc[pb.x], g.f_out_desc[pc.x]};
+                triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
+            }
+            int64_t bd;
+            int bp;
+            rot3_from_gains(Gv, w, F, sv, fs, bd, bp);
+            best[i] = bd;
+            perm[i] = (uint8_t)bp;
+        };
+        auto has_edge = [&](int32_t b, int32_t c, uint32_t wbc) {
+            return (((mo[b >> 5] | mi[b >> 5]) >> (b & 31)) & 1u) || (((mo[c >> 5] | mi[c >> 5]) >> (c & 31)) & 1u) ||
+                   ((wbc >> ((b < c ? c : b) & 31)) & 1u);
+        };
         for (int64_t ib = i0 + tid; ib < i1; ib += (int64_t)kUmThreads * kR3Triples) {
             int32_t vb[kR3Triples], vc[kR3Triples];
             int4 pb[kR3Triples], pc[kR3Triples];
             uint32_t wbc[kR3Triples];   /* dense-bitmap word of the pair {b, c} */
-            uint64_t mdb[kR3Triples], mdc[kR3Triples], fdb[kR3Triples], fdc[kR3Triples];
             int32_t gbc[kR3Triples], gcb[kR3Triples];
 #pragma unroll
             for (int j = 0; j < kR3Triples; j++) {   /* every gather of a level in flight before any use */
@@ -1387,40 +1426,17 @@ __global__ void __launch_bounds__(kUmThreads) k_gain_rot3_amajor(BmDevGraph g, B
                 pb[j] = i < i1 ? pis[vb[j]] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
                 pc[j] = i < i1 ? pis[vc[j]] : make_int4(0, 0, BM_ABSENT, BM_ABSENT);
                 wbc[j] = G.m_adj ? adj_word(G, vb[j], vc[j]) : ~0u;
-                mdb[j] = G.m_out_desc[vb[j]];
-                mdc[j] = G.m_out_desc[vc[j]];
             }
 #pragma unroll
             for (int j = 0; j < kR3Triples; j++) {
                 gbc[j] = G.G[(int64_t)vb[j] * stride + pc[j].x];   /* G(b, f_c) */
                 gcb[j] = G.G[(int64_t)vc[j] * stride + pb[j].x];   /* G(c, f_b) */
-                fdb[j] = g.f_out_desc[pb[j].x];
-                fdc[j] = g.f_out_desc[pc[j].x];
             }
 #pragma unroll
             for (int j = 0; j < kR3Triples; j++) {
                 const int64_t i = ib + (int64_t)kUmThreads * j;
                 if (i >= i1) continue;
-                const int32_t b = vb[j], c = vc[j];
-                const int32_t f[3] = {fa, pb[j].x, pc[j].x};
-                const int64_t Gv[3][3] = {{pa.y, rowG[f[1]], rowG[f[2]]},
-                                          {rowT[b], pb[j].y, gbc[j]},
-                                          {rowT[c], gcb[j], pc[j].y}};
-                const bool ab = (mo[b >> 5] >> (b & 31)) & 1u, ba = (mi[b >> 5] >> (b & 31)) & 1u;
-                const bool ac = (mo[c >> 5] >> (c & 31)) & 1u, ca = (mi[c >> 5] >> (c & 31)) & 1u;
-                const bool bc = (wbc[j] >> ((b < c ? c : b) & 31)) & 1u;   /* either direction */
-                const int32_t vtx[3] = {a, b, c};
-                const bool have[3][3] = {{false, ab, ac}, {ba, false, bc}, {ca, bc, false}};
-                const uint64_t md[3] = {mda, mdb[j], mdc[j]}, fd[3] = {fda, fdb[j], fdc[j]};
-                const int32_t sv[3] = {pa.z, pb[j].z, pc[j].z};
-                const int32_t fs[3] = {pa.w, pb[j].w, pc[j].w};
-                int32_t w[3][3], F[3][3];
-                triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
-                int64_t bd;
-                int bp;
-                rot3_from_gains(Gv, w, F, sv, fs, bd, bp);
-                best[i] = bd;
-                perm[i] = (uint8_t)bp;
+                finish(i, vb[j], vc[j], pb[j], pc[j], wbc[j], gbc[j], gcb[j], has_edge(vb[j], vc[j], wbc[j]));
             }
         }
         __syncthreads();
@@ -1668,6 +1684,7 @@ hipError_t bm_launch_gain_rot3_amajor(const BmDevGraph& g, const BmGain& G, cons
                                       const int64_t* off, uint64_t seed, int64_t* best, uint8_t* perm, int32_t num_cus,
                                       hipStream_t s) {
     const size_t lds = bm_gain_rot3_amajor_lds(g.n);
+    if (lds > kLdsBytes) return hipErrorInvalidValue;
     const void* kern = (const void*)k_gain_rot3_amajor;
     hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
