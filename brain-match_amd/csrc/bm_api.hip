@@ -102,6 +102,9 @@ struct bm_ctx {
     int64_t gt_version;
     int4* d_best;           /* best-swap result, then kBestSwapParts per-block results */
     int4* d_partners;       /* best partner per vertex of a bm_best_partners range */
+    int4* d_r3q;            /* a-major 3-vertex lookups: queued triples with an edge (3 x int4 each) */
+    int64_t r3q_entries;
+    int32_t* d_r3q_counts;  /* per queue region (kR3QueueRegions) */
     int32_t partners_cap;
     char err[512];
 };
@@ -265,6 +268,8 @@ static void free_ctx(bm_ctx* c) {
     (void)hipFree(c->d_um_off);
     (void)hipFree(c->d_GT);
     (void)hipFree(c->d_pis);
+    (void)hipFree(c->d_r3q);
+    (void)hipFree(c->d_r3q_counts);
     (void)hipFree(c->d_f_out_p);
     (void)hipFree(c->d_f_in_p);
     (void)hipFree(c->d_edesc);
@@ -1008,6 +1013,10 @@ extern "C" int bm_batch_gen_amajor(bm_batch* b, uint64_t seed, int64_t k) {
     return BM_OK;
 }
 
+/* Regions of the a-major 3-vertex queue: at most one per block of that
+ * launch (one block per CU there). */
+static constexpr int32_t kR3QueueRegions = 4096;
+
 extern "C" int bm_batch_eval_rot3_amajor(bm_batch* b, uint64_t seed, int64_t k) {
     if (!b) return BM_ERR_ARG;
     bm_ctx* c = b->ctx;
@@ -1022,11 +1031,24 @@ extern "C" int bm_batch_eval_rot3_amajor(bm_batch* b, uint64_t seed, int64_t k) 
     int rc = umajor_offsets(c, bm_amajor_seed(seed), k);
     if (rc == BM_OK) rc = gain_transposed(c);
     if (rc != BM_OK) return rc;
+    /* the queue of triples with an edge: an eighth of the batch (about 7% of
+     * the triples have one at n = 18,524; a block whose region fills
+     * finishes the rest in place) */
+    const int64_t want = k / 8 > 65536 ? k / 8 : 65536;
+    if (!c->d_r3q_counts && (rc = dalloc(c, &c->d_r3q_counts, (size_t)kR3QueueRegions)) != BM_OK) return rc;
+    if (c->r3q_entries < want) {
+        (void)hipFree(c->d_r3q);
+        c->d_r3q = nullptr;
+        c->r3q_entries = 0;
+        if ((rc = dalloc(c, &c->d_r3q, (size_t)want * 3)) != BM_OK) return rc;
+        c->r3q_entries = want;
+    }
     KTiming t;
     timing_begin(c, BM_KERNEL_GAIN_ROT3, &t);
     HIPCHK(c, bm_launch_gain_pis(devgraph(c), devgain(c), c->d_pis, c->stream));
     HIPCHK(c, bm_launch_gain_rot3_amajor(devgraph(c), devgain(c), c->d_GT, c->d_pis, c->d_um_off, seed, b->d_delta,
-                                         b->d_perm, c->num_cus, c->stream));
+                                         b->d_perm, c->d_r3q, c->r3q_entries, c->d_r3q_counts, kR3QueueRegions,
+                                         c->num_cus, c->stream));
     timing_end(c, &t);
     b->k = k;
     b->kind = 4; /* best deltas and permutations only: triples implicit in (seed, index) */

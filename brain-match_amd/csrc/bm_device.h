@@ -127,11 +127,16 @@ size_t bm_gain_umajor_bitmap_lds(int32_t n);
 hipError_t bm_launch_gain_transpose(const BmGain& G, int32_t nrows, int32_t* GT, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_pis(const BmDevGraph& g, const BmGain& G, int4* pis, hipStream_t s);
 /* a-major 3-vertex lookups (best delta and permutation per stream entry);
- * needs bm_gain_rot3_amajor_lds(n) bytes of LDS. */
+ * needs bm_gain_rot3_amajor_lds(n) bytes of LDS.  Triples with an edge are
+ * queued (queue_entries entries of 3 x int4, split into one region per
+ * block; queue_counts holds a count per region, at most queue_regions
+ * blocks) and finished by a second kernel; queue = NULL finishes them in
+ * place. */
 size_t bm_gain_rot3_amajor_lds(int32_t n);
 hipError_t bm_launch_gain_rot3_amajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
-                                      const int64_t* off, uint64_t seed, int64_t* best, uint8_t* perm, int32_t num_cus,
-                                      hipStream_t s);
+                                      const int64_t* off, uint64_t seed, int64_t* best, uint8_t* perm, int4* queue,
+                                      int64_t queue_entries, int32_t* queue_counts, int32_t queue_regions,
+                                      int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_swap_umajor(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
                                       const int64_t* off, uint64_t seed, int64_t* delta, int32_t num_cus,
                                       hipStream_t s);
