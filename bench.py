@@ -1021,7 +1021,7 @@ def bench_rot3(ctx, bm, args):
                    + T * (2 * 4 + 2 * 4 + 9))
         else:
             ab3, _ = b3.algo_bytes(generated=False)
-        kern = "k_gain_rot3_amajor" if mode == "gain" else "k_rot3"
+        kern = "k_gain_rot3_amajor+k_gain_rot3_edges" if mode == "gain" else "k_rot3"
         out[mode] = {"value": T * steps / wall, "ms_per_step": 1000 * wall / steps, "launch_ms": 1000 * l3,
                      "roofline": {"bound": "hbm", "achieved": ab3 / l3 / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": ab3 / l3 / 1e9 / HBM_PEAK_GBS, "algo_bytes_per_launch": ab3,
