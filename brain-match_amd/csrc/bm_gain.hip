@@ -793,22 +793,25 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
 __device__ __forceinline__ int64_t gain_swap_delta_from(const BmDevGraph& g, const BmGain& G, int32_t u, int32_t v,
                                                         int32_t f1, int32_t f2, bool e_uv, int64_t guf2, int64_t guf1,
                                                         int64_t gvf1, int64_t gvf2) {
-    const bool have[3][3] = {{false, e_uv, false}, {e_uv, false, false}, {false, false, false}};
-    const uint64_t md[3] = {G.m_out_desc[u], G.m_out_desc[v], 0ull};
     int64_t d = guf2 - guf1 + gvf1 - gvf2;
-    const uint64_t fd[3] = {g.f_out_desc[f1], g.f_out_desc[f2], 0ull};
-    /* male edges u -> v and v -> u; their corrections need the four female
-     * weights among f1 and f2 (rare: only with an edge) */
-    const int32_t vtx[3] = {u, v, 0}, f[3] = {f1, f2, 0};
     const int32_t fs[3] = {G.f_self[f1], G.f_self[f2], BM_ABSENT};
-    int32_t w[3][3], F[3][3];
-    triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
-    const int32_t w1 = w[0][1], w2 = w[1][0];
-    /* u -> v lands on f2 -> f1; v -> u on f1 -> f2 */
-    if (w1 != BM_ABSENT)
-        d += (int64_t)contrib(w1, F[1][0]) - contrib(w1, F[1][1]) - contrib(w1, F[0][0]) + contrib(w1, F[0][1]);
-    if (w2 != BM_ABSENT)
-        d += (int64_t)contrib(w2, F[0][1]) - contrib(w2, F[0][0]) - contrib(w2, F[1][1]) + contrib(w2, F[1][0]);
+    if (e_uv) {
+        /* male edges u -> v and v -> u; their corrections need the four
+         * female weights among f1 and f2 (rare: the hash-row descriptors are
+         * loaded only here) */
+        const bool have[3][3] = {{false, true, false}, {true, false, false}, {false, false, false}};
+        const uint64_t md[3] = {G.m_out_desc[u], G.m_out_desc[v], 0ull};
+        const uint64_t fd[3] = {g.f_out_desc[f1], g.f_out_desc[f2], 0ull};
+        const int32_t vtx[3] = {u, v, 0}, f[3] = {f1, f2, 0};
+        int32_t w[3][3], F[3][3];
+        triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
+        const int32_t w1 = w[0][1], w2 = w[1][0];
+        /* u -> v lands on f2 -> f1; v -> u on f1 -> f2 */
+        if (w1 != BM_ABSENT)
+            d += (int64_t)contrib(w1, F[1][0]) - contrib(w1, F[1][1]) - contrib(w1, F[0][0]) + contrib(w1, F[0][1]);
+        if (w2 != BM_ABSENT)
+            d += (int64_t)contrib(w2, F[0][1]) - contrib(w2, F[0][0]) - contrib(w2, F[1][1]) + contrib(w2, F[1][0]);
+    }
     const int32_t su = G.m_self[u], sv = G.m_self[v];
     if (su != BM_ABSENT) d += contrib(su, fs[1]) - (int64_t)contrib(su, fs[0]);
     if (sv != BM_ABSENT) d += contrib(sv, fs[0]) - (int64_t)contrib(sv, fs[1]);
