@@ -846,6 +846,20 @@ extern "C" int bm_batch_gen_changed_triples(bm_batch* b, const int32_t* changed,
     return BM_OK;
 }
 
+/* The per-vertex words for per-pair table lookups of k pairs: refreshed
+ * (one light launch) and used when the batch has at least n pairs, where two
+ * random table reads per pair saved outweigh it; NULL otherwise. */
+static const int4* pair_pis(bm_ctx* c, int64_t k, int* rc) {
+    *rc = BM_OK;
+    if (k < c->n) return nullptr;
+    if (!c->d_pis && (*rc = dalloc(c, &c->d_pis, (size_t)c->n + 1)) != BM_OK) return nullptr;
+    if (bm_launch_gain_pis(devgraph(c), devgain(c), c->d_pis, c->stream) != hipSuccess) {
+        *rc = fail(c, BM_ERR_HIP, "refreshing the per-vertex words failed");
+        return nullptr;
+    }
+    return c->d_pis;
+}
+
 extern "C" int bm_batch_eval_swaps(bm_batch* b) {
     if (!b) return BM_ERR_ARG;
     bm_ctx* c = b->ctx;
@@ -854,8 +868,14 @@ extern "C" int bm_batch_eval_swaps(bm_batch* b) {
     KTiming t;
     if (c->gain_on) {
         timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
+        int rc = BM_OK;
+        const int4* pis = pair_pis(c, b->k, &rc);
+        if (rc != BM_OK) {
+            timing_end(c, &t);
+            return rc;
+        }
         HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_LIST, (const int2*)b->d_cand, nullptr, 0, b->k,
-                                      b->d_delta, c->stream));
+                                      b->d_delta, pis, c->stream));
     } else {
         timing_begin(c, BM_KERNEL_SWAP, &t);
         HIPCHK(c, bm_launch_swap(devgraph(c), scan_launch(c, BM_KERNEL_SWAP, b->k), BM_CAND_LIST, (const int2*)b->d_cand, nullptr, 0, b->k,
@@ -874,8 +894,14 @@ extern "C" int bm_batch_eval_swaps_uniform(bm_batch* b, uint64_t seed, int64_t k
     KTiming t;
     if (c->gain_on) {
         timing_begin(c, BM_KERNEL_GAIN_SWAP, &t);
+        int rc = BM_OK;
+        const int4* pis = pair_pis(c, k, &rc);
+        if (rc != BM_OK) {
+            timing_end(c, &t);
+            return rc;
+        }
         HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_UNIFORM, nullptr, nullptr, seed, k, b->d_delta,
-                                      c->stream));
+                                      pis, c->stream));
     } else {
         timing_begin(c, BM_KERNEL_SWAP, &t);
         HIPCHK(c, bm_launch_swap(devgraph(c), scan_launch(c, BM_KERNEL_SWAP, k), BM_CAND_UNIFORM, nullptr, nullptr, seed, k, b->d_delta,
@@ -961,8 +987,13 @@ extern "C" int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k)
             HIPCHK(c, bm_launch_gain_swap_umajor(devgraph(c), devgain(c), c->d_GT, c->d_pis, c->d_um_off, seed,
                                                  b->d_delta, c->num_cus, c->stream));
         } else {
+            const int4* pis = pair_pis(c, k, &rc);
+            if (rc != BM_OK) {
+                timing_end(c, &t);
+                return rc;
+            }
             HIPCHK(c, bm_launch_gain_swap(devgraph(c), devgain(c), BM_CAND_UMAJOR, nullptr, c->d_um_off, seed, k,
-                                          b->d_delta, c->stream));
+                                          b->d_delta, pis, c->stream));
         }
     } else {
         timing_begin(c, BM_KERNEL_SWAP, &t);

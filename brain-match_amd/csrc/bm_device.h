@@ -115,8 +115,10 @@ hipError_t bm_launch_gain_build(const BmDevGraph& g, const BmGain& G, int32_t nu
  * be NULL) is a transposed copy patched alongside. */
 hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, int32_t* GT, const int4* moves, const int64_t* count,
                                  int64_t max_moves, int64_t* plan, int32_t num_cus, hipStream_t s);
+/* Per-pair table lookups; with pis (the per-vertex words, refreshed) the
+ * diagonal entries and self-loops come from them. */
 hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, const int64_t* off,
-                               uint64_t seed, int64_t k, int64_t* delta, hipStream_t s);
+                               uint64_t seed, int64_t k, int64_t* delta, const int4* pis, hipStream_t s);
 /* u-major evaluation: pis = {pi, G(v, pi(v)), self-loops} per vertex, off =
  * stream offsets.  With bm_gain_umajor_lds(n) <= 160 KiB both rows of a first
  * vertex are staged in LDS (GT = the transposed table, required); otherwise

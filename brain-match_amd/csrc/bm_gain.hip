@@ -792,9 +792,10 @@ __global__ void __launch_bounds__(kPatchThreads) k_gain_update(BmDevGraph g, BmG
  * the female weights its correction needs are hash-probed only then. */
 __device__ __forceinline__ int64_t gain_swap_delta_from(const BmDevGraph& g, const BmGain& G, int32_t u, int32_t v,
                                                         int32_t f1, int32_t f2, bool e_uv, int64_t guf2, int64_t guf1,
-                                                        int64_t gvf1, int64_t gvf2) {
+                                                        int64_t gvf1, int64_t gvf2, int32_t su, int32_t sv,
+                                                        int32_t fs1, int32_t fs2) {
     int64_t d = guf2 - guf1 + gvf1 - gvf2;
-    const int32_t fs[3] = {G.f_self[f1], G.f_self[f2], BM_ABSENT};
+    const int32_t fs[3] = {fs1, fs2, BM_ABSENT};
     if (e_uv) {
         /* male edges u -> v and v -> u; their corrections need the four
          * female weights among f1 and f2 (rare: the hash-row descriptors are
@@ -812,7 +813,6 @@ __device__ __forceinline__ int64_t gain_swap_delta_from(const BmDevGraph& g, con
         if (w2 != BM_ABSENT)
             d += (int64_t)contrib(w2, F[0][1]) - contrib(w2, F[0][0]) - contrib(w2, F[1][1]) + contrib(w2, F[1][0]);
     }
-    const int32_t su = G.m_self[u], sv = G.m_self[v];
     if (su != BM_ABSENT) d += contrib(su, fs[1]) - (int64_t)contrib(su, fs[0]);
     if (sv != BM_ABSENT) d += contrib(sv, fs[0]) - (int64_t)contrib(sv, fs[1]);
     return d;
@@ -823,13 +823,25 @@ __device__ __forceinline__ int64_t gain_swap_delta(const BmDevGraph& g, const Bm
     const int32_t f1 = g.pi[u], f2 = g.pi[v];
     const int32_t* Gu = G.G + (int64_t)u * G.stride;
     const int32_t* Gv = G.G + (int64_t)v * G.stride;
-    return gain_swap_delta_from(g, G, u, v, f1, f2, male_maybe(G, u, v), Gu[f2], Gu[f1], Gv[f1], Gv[f2]);
+    return gain_swap_delta_from(g, G, u, v, f1, f2, male_maybe(G, u, v), Gu[f2], Gu[f1], Gv[f1], Gv[f2], G.m_self[u],
+                                G.m_self[v], G.f_self[f1], G.f_self[f2]);
 }
 
-template <int MODE>
+/* The same from the per-vertex words {pi, G(v, pi(v)), male self-loop,
+ * female self-loop of pi(v)}: two random table reads per pair instead of
+ * four. */
+__device__ __forceinline__ int64_t gain_swap_delta_pis(const BmDevGraph& g, const BmGain& G, const int4* pis, int32_t u,
+                                                       int32_t v) {
+    if (u == v) return 0;
+    const int4 pu = pis[u], pv = pis[v];
+    return gain_swap_delta_from(g, G, u, v, pu.x, pv.x, male_maybe(G, u, v), G.G[(int64_t)u * G.stride + pv.x], pu.y,
+                                G.G[(int64_t)v * G.stride + pu.x], pv.y, pu.z, pv.z, pu.w, pv.w);
+}
+
+template <int MODE, bool PIS>
 __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const int2* __restrict__ cand,
                                                    const int64_t* __restrict__ off, uint64_t seed, int64_t k,
-                                                   int64_t* __restrict__ delta) {
+                                                   int64_t* __restrict__ delta, const int4* __restrict__ pis) {
     if (MODE == BM_CAND_UMAJOR) {
         /* a u-major stream is read in one contiguous range per block, so a
          * thread's next entry (blockDim later) almost always has the same
@@ -844,7 +856,7 @@ __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const
                 else u = umajor_owner(off, g.n, c);
             }
             const int32_t v = bm_umajor_partner(seed, (uint64_t)c, u, g.n);
-            delta[c] = gain_swap_delta(g, G, u, v);
+            delta[c] = PIS ? gain_swap_delta_pis(g, G, pis, u, v) : gain_swap_delta(g, G, u, v);
         }
         return;
     }
@@ -858,7 +870,7 @@ __global__ void __launch_bounds__(256) k_gain_swap(BmDevGraph g, BmGain G, const
         } else {
             bm_uniform_pair(seed, (uint64_t)c, g.n, &u, &v);
         }
-        delta[c] = gain_swap_delta(g, G, u, v);
+        delta[c] = PIS ? gain_swap_delta_pis(g, G, pis, u, v) : gain_swap_delta(g, G, u, v);
     }
 }
 
@@ -929,7 +941,8 @@ __device__ __forceinline__ void best_partner_scan(const BmDevGraph& g, const BmG
             if (v > n) continue;
             int64_t d;
             if (mm[j]) {   /* a male edge between u and v may exist (rare): the full correction */
-                d = gain_swap_delta_from(g, G, u, v, f1, pv[j].x, true, gu[j], pu.y, tv[j], pv[j].y);
+                d = gain_swap_delta_from(g, G, u, v, f1, pv[j].x, true, gu[j], pu.y, tv[j], pv[j].y, pu.z, pv[j].z,
+                                         pu.w, pv[j].w);
             } else {       /* only self-loops, whose weights the per-vertex words carry */
                 d = (int64_t)gu[j] - pu.y + tv[j] - pv[j].y;
                 if (pu.z != BM_ABSENT) d += contrib(pu.z, pv[j].w) - (int64_t)contrib(pu.z, pu.w);
@@ -1648,19 +1661,25 @@ hipError_t bm_launch_gain_update(const BmDevGraph& g, const BmGain& G, int32_t* 
 }
 
 hipError_t bm_launch_gain_swap(const BmDevGraph& g, const BmGain& G, int mode, const int2* cand, const int64_t* off,
-                               uint64_t seed, int64_t k, int64_t* delta, hipStream_t s) {
+                               uint64_t seed, int64_t k, int64_t* delta, const int4* pis, hipStream_t s) {
     if (k <= 0) return hipSuccess;
     int64_t blocks = (k + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    if (mode == BM_CAND_LIST)
-        hipLaunchKernelGGL(k_gain_swap<BM_CAND_LIST>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, off, seed,
-                           k, delta);
-    else if (mode == BM_CAND_UMAJOR)
-        hipLaunchKernelGGL(k_gain_swap<BM_CAND_UMAJOR>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, off, seed,
-                           k, delta);
-    else
-        hipLaunchKernelGGL(k_gain_swap<BM_CAND_UNIFORM>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, off,
-                           seed, k, delta);
+#define BM_GAIN_SWAP_LAUNCH(M)                                                                                        \
+    if (pis)                                                                                                          \
+        hipLaunchKernelGGL((k_gain_swap<M, true>), dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, off, seed, k, \
+                           delta, pis);                                                                               \
+    else                                                                                                              \
+        hipLaunchKernelGGL((k_gain_swap<M, false>), dim3((unsigned)blocks), dim3(256), 0, s, g, G, cand, off, seed,  \
+                           k, delta, pis);
+    if (mode == BM_CAND_LIST) {
+        BM_GAIN_SWAP_LAUNCH(BM_CAND_LIST)
+    } else if (mode == BM_CAND_UMAJOR) {
+        BM_GAIN_SWAP_LAUNCH(BM_CAND_UMAJOR)
+    } else {
+        BM_GAIN_SWAP_LAUNCH(BM_CAND_UNIFORM)
+    }
+#undef BM_GAIN_SWAP_LAUNCH
     return hipGetLastError();
 }
 
