@@ -846,9 +846,10 @@ extern "C" int bm_batch_gen_changed_triples(bm_batch* b, const int32_t* changed,
     return BM_OK;
 }
 
-/* The per-vertex words for per-pair table lookups of k pairs: refreshed
- * (one light launch) and used when the batch has at least n pairs, where two
- * random table reads per pair saved outweigh it; NULL otherwise. */
+/* The per-vertex words for per-candidate table lookups of k pairs or
+ * triples: refreshed (one light launch) and used when the batch has at least
+ * n candidates, where the random table reads saved (two per pair, three per
+ * triple) outweigh it; NULL otherwise. */
 static const int4* pair_pis(bm_ctx* c, int64_t k, int* rc) {
     *rc = BM_OK;
     if (k < c->n) return nullptr;
@@ -1016,8 +1017,14 @@ extern "C" int bm_batch_eval_rot3(bm_batch* b) {
     KTiming t;
     if (c->gain_on) {
         timing_begin(c, BM_KERNEL_GAIN_ROT3, &t);
+        int rc = BM_OK;
+        const int4* pis = pair_pis(c, b->k, &rc);
+        if (rc != BM_OK) {
+            timing_end(c, &t);
+            return rc;
+        }
         HIPCHK(c, bm_launch_gain_rot3(devgraph(c), devgain(c), (const int4*)b->d_cand, b->k, b->d_delta, b->d_perm,
-                                      (int4*)b->d_cand, c->stream));
+                                      (int4*)b->d_cand, pis, c->stream));
     } else {
         timing_begin(c, BM_KERNEL_ROT3, &t);
         HIPCHK(c, bm_launch_rot3(devgraph(c), scan_launch(c, BM_KERNEL_ROT3, b->k), (const int4*)b->d_cand, b->k, b->d_delta,

@@ -154,7 +154,7 @@ hipError_t bm_launch_gain_best_swap(const BmDevGraph& g, const BmGain& G, const 
 hipError_t bm_launch_gain_best_partners(const BmDevGraph& g, const BmGain& G, const int32_t* GT, const int4* pis,
                                         int32_t u0, int32_t count, int4* out, int32_t num_cus, hipStream_t s);
 hipError_t bm_launch_gain_rot3(const BmDevGraph& g, const BmGain& G, const int4* cand, int64_t k, int64_t* best,
-                               uint8_t* perm, int4* cand_out_perm, hipStream_t s);
+                               uint8_t* perm, int4* cand_out_perm, const int4* pis, hipStream_t s);
 hipError_t bm_launch_moves_from_swap(const int32_t* pi_before, int32_t u, int32_t v, int4* moves, int64_t* count,
                                      hipStream_t s);
 hipError_t bm_launch_moves_from_perm3(const int32_t* pi_before, int32_t a, int32_t b, int32_t c, int32_t p,

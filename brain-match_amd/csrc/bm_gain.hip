@@ -1277,39 +1277,58 @@ __device__ __forceinline__ void rot3_from_gains(const int64_t (*Gv)[3], const in
     }
 }
 
+template <bool PIS>
 __global__ void __launch_bounds__(256) k_gain_rot3(BmDevGraph g, BmGain G, int4* __restrict__ cand, int64_t k,
                                                    int64_t* __restrict__ best, uint8_t* __restrict__ perm,
-                                                   int write_perm_into_cand) {
+                                                   int write_perm_into_cand, const int4* __restrict__ pis) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < k; c += stride) {
         const int4 q = cand[c];
         const int32_t vtx[3] = {q.x, q.y, q.z};
-        const int32_t f[3] = {g.pi[q.x], g.pi[q.y], g.pi[q.z]};
+        int32_t f[3], s[3], fs[3];
         int64_t Gv[3][3];   /* Gv[i][j] = G(vtx[i], f[j]) */
+        if (PIS) {   /* images, diagonal entries and self-loops from the per-vertex words */
+            const int4 p[3] = {pis[vtx[0]], pis[vtx[1]], pis[vtx[2]]};
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                f[i] = p[i].x;
+                Gv[i][i] = p[i].y;
+                s[i] = p[i].z;
+                fs[i] = p[i].w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                f[i] = g.pi[vtx[i]];
+                s[i] = G.m_self[vtx[i]];
+                fs[i] = G.f_self[f[i]];
+                Gv[i][i] = G.G[(int64_t)vtx[i] * G.stride + f[i]];
+            }
+        }
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const int32_t* Gr = G.G + (int64_t)vtx[i] * G.stride;
 #pragma unroll
-            for (int j = 0; j < 3; j++) Gv[i][j] = Gr[f[j]];
-        }
-        int32_t s[3], fs[3];
-        uint64_t md[3], fd[3];
-        bool have[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            s[i] = G.m_self[vtx[i]];
-            fs[i] = G.f_self[f[i]];
-            md[i] = G.m_out_desc[vtx[i]];
-            fd[i] = g.f_out_desc[f[i]];
+            for (int j = 0; j < 3; j++)
+                if (j != i) Gv[i][j] = Gr[f[j]];
         }
         const bool e01 = male_maybe(G, vtx[0], vtx[1]), e02 = male_maybe(G, vtx[0], vtx[2]),
                    e12 = male_maybe(G, vtx[1], vtx[2]);   /* either direction */
-        have[0][0] = have[1][1] = have[2][2] = false;
-        have[0][1] = have[1][0] = e01;
-        have[0][2] = have[2][0] = e02;
-        have[1][2] = have[2][1] = e12;
         int32_t w[3][3], F[3][3];
-        triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) w[i][j] = F[i][j] = BM_ABSENT;
+        if (e01 || e02 || e12) {   /* the hash-row descriptors only for a triple with an edge */
+            uint64_t md[3], fd[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                md[i] = G.m_out_desc[vtx[i]];
+                fd[i] = g.f_out_desc[f[i]];
+            }
+            const bool have[3][3] = {{false, e01, e02}, {e01, false, e12}, {e02, e12, false}};
+            triple_weights(g, G, vtx, md, f, fd, fs, have, w, F);
+        }
         int64_t bd;
         int bp;
         rot3_from_gains(Gv, w, F, s, fs, bd, bp);
@@ -1792,12 +1811,16 @@ hipError_t bm_launch_gain_rot3_amajor(const BmDevGraph& g, const BmGain& G, cons
 }
 
 hipError_t bm_launch_gain_rot3(const BmDevGraph& g, const BmGain& G, const int4* cand, int64_t k, int64_t* best,
-                               uint8_t* perm, int4* cand_out_perm, hipStream_t s) {
+                               uint8_t* perm, int4* cand_out_perm, const int4* pis, hipStream_t s) {
     if (k <= 0) return hipSuccess;
     int64_t blocks = (k + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_gain_rot3, dim3((unsigned)blocks), dim3(256), 0, s, g, G, const_cast<int4*>(cand), k, best,
-                       perm, cand_out_perm ? 1 : 0);
+    if (pis)
+        hipLaunchKernelGGL(k_gain_rot3<true>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, const_cast<int4*>(cand), k,
+                           best, perm, cand_out_perm ? 1 : 0, pis);
+    else
+        hipLaunchKernelGGL(k_gain_rot3<false>, dim3((unsigned)blocks), dim3(256), 0, s, g, G, const_cast<int4*>(cand), k,
+                           best, perm, cand_out_perm ? 1 : 0, pis);
     return hipGetLastError();
 }
 
