@@ -281,7 +281,9 @@ int bm_batch_eval_rot3(bm_batch* b);
  * same stream without materialising it.  With the gain table enabled and
  * n <= 18,975, one block per first vertex stages the two table rows it needs
  * and its adjacency bitmaps in LDS (the table's transpose is kept next to it,
- * doubling the table's memory); otherwise each pair is evaluated on its own. */
+ * doubling the table's memory); up to n of about 218,000 it stages only the
+ * bitmaps and gathers both table entries (no transpose); beyond that each
+ * pair is evaluated on its own. */
 int bm_batch_gen_umajor(bm_batch* b, uint64_t seed, int64_t k);
 int bm_batch_eval_swaps_umajor(bm_batch* b, uint64_t seed, int64_t k);
 
