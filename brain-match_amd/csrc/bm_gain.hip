@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(256) k_gain_edge_desc(BmDevGraph g, BmGain G, 
  * column pairs (a 32-bit word of every dense row each) and keeps their 16
  * rows' sums as packed 16-bit halves, one load of a dense-row word serving
  * 16 rows; a row whose weight is 0 is skipped (a scalar branch: about 40% of
- * the rows of a tile that uses a dense row do not).  The sums stay below
+ * a tile's rows do not use a given dense row).  The sums stay below
  * 2^16: each term min(w, fw) is at most fw, and the dense rows are chosen
  * so that every column's weights sum to at most 65535.  The result stays
  * packed: word w of the second half of G's row u holds columns 2w (low
@@ -395,17 +395,16 @@ __device__ __forceinline__ void dense_tile_words(const uint32_t* __restrict__ wt
             const uint32_t* wk = wt + k * kTileRows;
             const uint4 q0 = load_weights(wk), q1 = load_weights(wk + 4), q2 = load_weights(wk + 8),
                         q3 = load_weights(wk + 12);
-            const uint32_t any = q0.x | q0.y | q0.z | q0.w | q1.x | q1.y | q1.z | q1.w | q2.x | q2.y | q2.z |
-                                 q2.w | q3.x | q3.y | q3.z | q3.w;
-            if (any != 0u) {   /* some row of the tile uses dense row k */
-                const uint32_t w[kTileRows] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
-                                               q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
+            /* no test of the 16 weights together: nearly every tile uses every
+             * dense row, and the scalar unit is the pass's busiest (the test
+             * cost 3% of the build) */
+            const uint32_t w[kTileRows] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                           q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
 #pragma unroll
-                for (int r = 0; r < kTileRows; r++) {
-                    if (w[r] != 0u) {
-                        a0[r] = pk_min_add(a0[r], v0, w[r]);
-                        a1[r] = pk_min_add(a1[r], v1, w[r]);
-                    }
+            for (int r = 0; r < kTileRows; r++) {
+                if (w[r] != 0u) {
+                    a0[r] = pk_min_add(a0[r], v0, w[r]);
+                    a1[r] = pk_min_add(a1[r], v1, w[r]);
                 }
             }
             v0 = n0;
