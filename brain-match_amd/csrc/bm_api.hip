@@ -1516,14 +1516,14 @@ extern "C" int bm_gain_enable(bm_ctx* c, int on) {
             int2* d_rows = nullptr;
             int r2 = dalloc(c, &c->d_f_out_dense, (size_t)c->n + 1);
             if (r2 == BM_OK) r2 = dalloc(c, &c->d_f_in_dense, (size_t)c->n + 1);
-            if (r2 == BM_OK) r2 = dalloc(c, &c->d_dense, cand.size() * (size_t)(stride / 2));
+            if (r2 == BM_OK) r2 = dalloc(c, &c->d_dense, (cand.size() + 1) * (size_t)(stride / 2));   /* + a zero row */
             const size_t wcount = (size_t)((c->n + 16) / 16) * cand.size() * 16;
             if (r2 == BM_OK) r2 = dalloc(c, &c->d_dense_w, wcount);
             if (r2 == BM_OK) r2 = dalloc(c, &d_rows, rows.size());
             if (r2 == BM_OK) r2 = h2d(c, c->d_f_out_dense, hout.data(), hout.size());
             if (r2 == BM_OK) r2 = h2d(c, c->d_f_in_dense, hin.data(), hin.size());
             if (r2 == BM_OK) r2 = h2d(c, d_rows, rows.data(), rows.size());
-            if (r2 == BM_OK && hipMemsetAsync(c->d_dense, 0, cand.size() * (size_t)stride * 2, c->stream) != hipSuccess)
+            if (r2 == BM_OK && hipMemsetAsync(c->d_dense, 0, (cand.size() + 1) * (size_t)stride * 2, c->stream) != hipSuccess)
                 r2 = fail(c, BM_ERR_HIP, "bm_gain_enable: clearing the dense rows failed");
             if (r2 == BM_OK && bm_launch_dense_fill(c->d_f_out_ptr, c->d_f_out, c->d_f_in_ptr, c->d_f_in, d_rows,
                                                     (int32_t)rows.size(), stride, (uint16_t*)c->d_dense,

@@ -387,11 +387,18 @@ __device__ __forceinline__ void dense_tile_words(const uint32_t* __restrict__ wt
             a0[r] = 0u;
             a1[r] = 0u;
         }
-        /* dense row k's two words are loaded one iteration ahead */
-        uint32_t v0 = dense[l0], v1 = dense[l1];
+        /* dense row k's two words are loaded one iteration ahead (row nd is
+         * a zero row, so the look-ahead needs no clamp); byte offsets are
+         * loop-invariant and the row pointer advances by one add */
+        const uint32_t o0 = (uint32_t)l0 * 4u, o1 = (uint32_t)l1 * 4u;
+        auto word = [](const uint32_t* row, uint32_t off) {
+            return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(row) + off);
+        };
+        uint32_t v0 = word(dense, o0), v1 = word(dense, o1);
+        const uint32_t* dn = dense;
         for (int32_t k = 0; k < nd; k++) {
-            const uint32_t* dn = dense + (int64_t)(k + 1 < nd ? k + 1 : k) * hw;
-            const uint32_t n0 = dn[l0], n1 = dn[l1];
+            dn += hw;
+            const uint32_t n0 = word(dn, o0), n1 = word(dn, o1);
             const uint32_t* wk = wt + k * kTileRows;
             const uint4 q0 = load_weights(wk), q1 = load_weights(wk + 4), q2 = load_weights(wk + 8),
                         q3 = load_weights(wk + 12);
