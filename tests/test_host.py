@@ -167,3 +167,39 @@ def test_converters_round_trip(bm, tmp_path):
     assert (tmp_path / "sol_matching_generic.csv").read_text() == (tmp_path / "sol.csv").read_text()
     for f in ("sol.csv", "sol_matching.csv", "sol_matching_generic.csv"):
         assert np.array_equal(bm.load_mapping_csv(str(tmp_path / f), 50), pi)
+
+
+def test_bench_dense_row_rule():
+    """bench.py's copy of the library's dense-row rule (rows of at least a
+    quarter of the stride, longest first, each taken only while every
+    column's dense weights sum to at most 65,535): three female hubs of weight
+    21,845 to every vertex fill the columns to 65,535 exactly, so a fourth
+    hub of weight 1 is left out, and the split counts the three hubs' male
+    edges and their expansions."""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    n = 64
+    hubs = ((1, 21845), (2, 21845), (3, 21845), (4, 1))
+    src = np.array([h for h, _ in hubs for v in range(1, n + 1) if v != h], np.int32)
+    dst = np.array([v for h, _ in hubs for v in range(1, n + 1) if v != h], np.int32)
+    fw = np.array([w for h, w in hubs for v in range(1, n + 1) if v != h], np.int32)
+
+    class G:   # the fields dense_split reads
+        pass
+
+    female = G()
+    female.src, female.dst, female.w = src, dst, fw
+
+    class Ctx:
+        def __init__(self):
+            self.n = n
+
+        def degrees(self, female=False):   # the same edges in both graphs
+            return np.bincount(src, minlength=n + 1), np.bincount(dst, minlength=n + 1)
+
+    pi = np.arange(n + 1)
+    entries, pairs = bench.dense_split(Ctx(), pi, female, 3)
+    assert pairs == 3 * 63 and entries == 3 * 63 * 63
+    with pytest.raises(SystemExit):
+        bench.dense_split(Ctx(), pi, female, 4)
