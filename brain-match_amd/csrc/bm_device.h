@@ -57,9 +57,9 @@ struct BmGain {
     const int32_t* f_out_bnd;
     int32_t nwin;
     /* Dense copies of the longest female rows (compact graphs whose male
-     * weights are all >= 0): ndense rows of stride / 2 words, word i holding
-     * the weights of columns 2i (low half) and 2i + 1 (high half), 0 where
-     * the row has no entry.  f_out_dense / f_in_dense give a female vertex's
+     * weights are all >= 0): ndense rows of stride / 2 words and one zero row
+     * after them, word i holding the weights of columns 2i (low half) and
+     * 2i + 1 (high half), 0 where the row has no entry.  f_out_dense / f_in_dense give a female vertex's
      * row in it, or -1.  A separate pass sums the dense edges of 16 male
      * rows at a time and the row build starts from its result; their
      * work-list entries carry BM_EDESC_DENSE.  NULL / 0 when no row is
